@@ -1,0 +1,226 @@
+"""Benchmark: ms per Poisson solve to 1e-6 relative residual on one MI355X.
+
+Workload (default `--config c3`): the 5-level graded cut-cell octree around a
+torus (R=0.15, r=0.05, z axis, centre (0.5,0.5,0.5)) with finest spacing 1/1024
+(levels 3..7, finest band |d|<0.01 doubling per coarser level), 4x4 supersampled
+face fractions, Neumann solid and walls, Dirichlet 0 at y=1, RHS = divergence of
+N(0,1) face velocities (seed 2).  Plain MGPCG: PCG preconditioned by one FAS
+V-cycle of 2+2 damped-Jacobi sweeps (w=2/3) per level, FP64 direct solve on
+level 0, FP32 vectors, FP64 dot products; stop at 1e-6 relative residual.
+
+One step = one solve from x = 0 with the right-hand side already resident in
+HBM (restored on the device before each solve).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cirrus-amg_amd"))
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+CONFIGS = {
+    "c1": dict(name="C1 uniform 64^3 (512 tiles of 8^3), Dirichlet box, U[-1,1] seed 0 zero-mean",
+               target=("uniform", 3, 3), assembly="plain", rhs=("uniform", 0)),
+    "c2": dict(name="C2 3-level graded octree, finest 256^3, sphere r=0.2 Neumann solid, Dirichlet box, div N(0,1) seed 1",
+               target=("bands", 3, 5, "sphere", (0.5, 0.5, 0.5, 0.2), {5: 0.05, 4: 0.15}), assembly="solid",
+               rhs=("divergence", 1)),
+    "c3": dict(name="C3 5-level graded cut-cell octree, finest 1024^3, torus R=0.15 r=0.05, 4x4 supersampled faces, "
+                    "Dirichlet y=1, Neumann elsewhere, div N(0,1) seed 2",
+               target=("bands", 3, 7, "torus", (0.5, 0.5, 0.5, 0.15, 0.05), {7: 0.01, 6: 0.02, 5: 0.04, 4: 0.08}),
+               assembly="supersampled", rhs=("divergence", 2)),
+}
+
+
+def build(P, cfg):
+    t = cfg["target"]
+    if t[0] == "uniform":
+        target = P.make_target(P.TARGET_UNIFORM, t[1], t[2])
+    else:
+        sk = P.SDF_SPHERE if t[3] == "sphere" else P.SDF_TORUS
+        target = P.make_target(P.TARGET_SDF_BANDS, t[1], t[2], sk, t[4], t[5])
+    g = P.HADeviceGrid.test_grid(target)
+    if cfg["assembly"] == "plain":
+        g.prepare_types_and_coeffs(11, 0.5)
+    elif cfg["assembly"] == "solid":
+        g.assemble_cutcell(P.SDF_SPHERE, t[4], -1, 11, 0.5)
+    else:
+        g.set_wall_types(t[1], True)
+        g.assemble_cutcell(P.SDF_TORUS, t[4], 1, 11, 0.5)
+    kind, seed = cfg["rhs"]
+    if kind == "uniform":
+        g.rhs_uniform(1, seed, True)
+    else:
+        g.rhs_divergence(1, 5, seed)
+    return g
+
+
+def canonical_bytes_per_iter(lv, ty, max_level):
+    """B_iter = 81 N_leaf + sum_l n_l (21+29+29.5+8.5+2*29) (tile-dense, FP32 vectors)."""
+    import numpy as np
+    n_leaf = int((ty == 1).sum()) * 512
+    per_level = 21 + 29 + 29.5 + 8.5 + 2 * 29
+    levels = sum(int(((lv == l) & ((ty == 1) | (ty == 2))).sum()) * 512 * per_level for l in range(1, max_level + 1))
+    coarsest = 512 * 8 * 512 + 512 * 8   # dense FP64 inverse of level 0 + its vectors
+    del np
+    return 81.0 * n_leaf + levels + coarsest
+
+
+def cpu_baseline(cfg, gpu_iters, budget_s):
+    """The same operator assembled as FP64 CSR and solved by threaded CG on the host
+    (the oracle's OpenMP CG), timed on a bounded number of iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    t = cfg["target"]
+    if t[0] == "uniform":
+        tg = O.make_target(O.TARGET_UNIFORM, t[1], t[2])
+    else:
+        sk = O.SDF_SPHERE if t[3] == "sphere" else O.SDF_TORUS
+        tg = O.make_target(O.TARGET_SDF_BANDS, t[1], t[2], sk, t[4], t[5])
+    g = O.Grid(tg)
+    g.set_leaf_types_preset()
+    if cfg["assembly"] == "plain":
+        g.coarsen_types_and_coeffs(11, 0.5)
+    elif cfg["assembly"] == "solid":
+        g.cutcell_coeffs(O.SDF_SPHERE, t[4], -1, 11, 0.5)
+    else:
+        g.set_wall_types(t[1], True)
+        g.cutcell_coeffs(O.SDF_TORUS, t[4], 1, 11, 0.5)
+    kind, seed = cfg["rhs"]
+    if kind == "uniform":
+        g.rhs_uniform(1, seed, True)
+    else:
+        g.rhs_divergence(1, 5, seed)
+    A = g.assemble_csr(11)
+    b = g.channel(1).ravel()[A.cell].astype(np.float64)
+    threads = O.lib().og_num_threads()
+    # per-iteration cost of threaded CG on this CSR
+    t0 = time.time()
+    _, it, rr = A.cg(b, 1e-30, 5, threads)
+    t_iter = (time.time() - t0) / max(it, 1)
+    n_iter = max(5, int(budget_s / max(t_iter, 1e-6)))
+    t0 = time.time()
+    x, it, rr = A.cg(b, 1e-6, n_iter, threads)
+    dt = time.time() - t0
+    converged = rr < 1e-6
+    per_iter_ms = dt / max(it, 1) * 1e3
+    out = {"unit": "ms per solve", "cores": threads, "kind": "port", "threads_runtime": "OpenMP (libgomp)",
+           "per_iteration_ms": per_iter_ms, "cg_iterations_run": it, "cg_rel_res_reached": rr,
+           "csr_unknowns": A.n, "csr_nnz": int(A.nnz)}
+    if converged:
+        out["value"] = dt * 1e3
+        out["sample"] = "full FP64 CSR CG solve to 1e-6 (%d iterations)" % it
+    else:
+        # CG's error bound: iterations to 1e-6 from the observed geometric rate
+        rate = rr ** (1.0 / max(it, 1))
+        need = int(np.ceil(np.log(1e-6) / np.log(rate))) if rate < 1 else -1
+        out["value"] = per_iter_ms * need if need > 0 else None
+        out["sample"] = ("%d FP64 CSR CG iterations (%.1f s) reaching rel res %.2e; ms/solve = ms/iteration x "
+                         "%d iterations extrapolated from the observed rate" % (it, dt, rr, need))
+        out["extrapolated_iterations"] = need
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs the single-GPU solve only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import cirrus_amg as P
+
+    cfg = CONFIGS[args.config]
+    t_build = time.time()
+    g = build(P, cfg)
+    t_build = time.time() - t_build
+    lv, co, ty = g.tiles()
+    types = g.cell_types()
+    dof = int((types[ty == 1] == 2).sum())
+    g.copy_channel(1, 5)   # keep b resident
+
+    opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                          max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1)
+    solver = P.AMGSolver(g, 11, 0.5, 1.0, 1.0)
+
+    def step():
+        g.copy_channel(5, 1)
+        it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=opts)
+        return it, rel
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, rels, dev_ms = [], [], []
+    for _ in range(args.steps):
+        it, rel = step()
+        iters.append(it)
+        rels.append(rel)
+        dev_ms.append(solver.last_stats.ms)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    true_rel = solver.last_stats.true_rel_res
+
+    # dominant kernel: damped-Jacobi sweep on the finest level (4 per PCG iteration)
+    k_ms, k_tiles = solver.bench_kernel(1, 50)
+    alg_bytes = 29.0 * 512 * k_tiles          # x_in, b, 3 off-diagonals, diagonal (4 B each), type (1 B), x_out
+    k_gbs = alg_bytes / (k_ms * 1e-3) / 1e9
+    apply_ms, apply_tiles = solver.bench_kernel(0, 50)
+    apply_gbs = 25.0 * 512 * apply_tiles / (apply_ms * 1e-3) / 1e9
+
+    b_iter = canonical_bytes_per_iter(lv, ty, g.max_level)
+    solve_ms = float(np.mean(dev_ms))
+    it_mean = float(np.mean(iters))
+    solve_gbs = b_iter * it_mean / (solve_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "ms per Poisson solve to 1e-6 rel-residual (plain MGPCG)",
+        "value": ms_per_step,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 dots, f64 level-0 direct solve)",
+        "data": "synthetic (seeded)",
+        "config": {"workload": cfg["name"], "tiles": int(g.N), "leaf_tiles": int((ty == 1).sum()), "dof": dof,
+                   "levels": int(g.max_level) + 1, "solver": "PCG + FAS V(2,2) damped Jacobi w=2/3, direct level 0",
+                   "parallelism": "single GPU"},
+        "iterations": iters,
+        "rel_res": rels,
+        "true_rel_res_fp64": true_rel,
+        "device_ms_per_solve": solve_ms,
+        "dof_per_s": dof / (ms_per_step * 1e-3),
+        "solve_GBps_canonical": solve_gbs,
+        "solve_frac_of_hbm": solve_gbs / HBM_PEAK_GBS,
+        "canonical_bytes_per_iter": b_iter,
+        "roofline": {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k_gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
+                     "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles},
+        "apply_kernel": {"GBps": apply_gbs, "ms": apply_ms, "tiles": apply_tiles},
+        "build_s": t_build,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(cfg, it_mean, args.cpu_budget)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

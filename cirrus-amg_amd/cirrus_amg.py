@@ -1,0 +1,368 @@
+"""Thin ctypes tooling over libcirrus_amg.so (include/cirrus_amg.h).
+
+The solver itself is C++17 + HIP; this module only marshals numpy arrays
+across the C ABI.  Names follow the reference's classes:
+``HADeviceGrid`` (src/HAGrid.h) and ``AMGSolver`` (src/AMGSolver.h).
+There is no CPU fallback: every device call goes through the HIP library and
+raises ``CmgError`` when it fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcirrus_amg.so")
+
+LEAF, NONLEAF, GHOST = 1, 2, 4
+DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
+X_CHANNEL, B_CHANNEL, P_CHANNEL, AP_CHANNEL, Z_CHANNEL = 0, 1, 2, 3, 4   # src/PoissonTile.h:13-18
+
+TARGET_UNIFORM, TARGET_CENTER_POINT, TARGET_SPHERE_EMPTY, TARGET_SPHERE_SOLID = 0, 1, 2, 3
+TARGET_STAR_EMPTY, TARGET_STAR_SOLID, TARGET_SDF_BANDS, TARGET_CALLBACK = 4, 5, 6, 7
+SDF_SPHERE, SDF_TORUS, SDF_STAR = 0, 1, 2
+CYCLE_FAS_W, CYCLE_FAS_V = 0, 1
+SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
+PREC_FP32, PREC_FP64_REFINE = 0, 1
+
+STATUS = {0: "ok", 1: "bad argument", 2: "call out of order", 3: "HIP runtime error",
+          4: "non-finite value", 5: "CG breakdown", 6: "not converged", 7: "hash table full"}
+
+LEVEL_TARGET_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int32))
+
+
+class CmgError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        msg = "%s: %s (%d)" % (where, STATUS.get(status, "?"), status)
+        try:
+            detail = lib().cmg_last_error().decode()
+            if detail:
+                msg += " -- " + detail
+        except Exception:
+            pass
+        super().__init__(msg)
+
+
+class Target(C.Structure):
+    _fields_ = [("kind", C.c_int), ("min_level", C.c_int), ("max_level", C.c_int), ("sdf_kind", C.c_int),
+                ("sdf", C.c_float * 8), ("band", C.c_float * 12), ("fn", LEVEL_TARGET_FN), ("user", C.c_void_p)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("verbose", C.c_int), ("max_iters", C.c_int), ("relative_tolerance", C.c_double),
+                ("level_iters", C.c_int), ("coarsest_iters", C.c_int), ("sync_stride", C.c_int),
+                ("is_pure_neumann", C.c_int), ("cycle", C.c_int), ("smoother", C.c_int), ("omega", C.c_double),
+                ("coarsest_direct", C.c_int), ("precision", C.c_int), ("outer_tolerance", C.c_double),
+                ("inner_tolerance", C.c_double), ("warm_start", C.c_int), ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer_iters", C.c_int), ("rel_res", C.c_double),
+                ("true_rel_res", C.c_double), ("ms", C.c_double)]
+
+
+EXPORTS = [
+    "cmg_grid_create", "cmg_grid_destroy", "cmg_grid_set_tile", "cmg_grid_rebuild", "cmg_grid_iterative_refine",
+    "cmg_grid_finalize", "cmg_grid_num_tiles", "cmg_grid_max_level", "cmg_grid_level_starts", "cmg_grid_get_tiles",
+    "cmg_grid_get_slots", "cmg_grid_get_neighbors", "cmg_grid_get_parents", "cmg_grid_get_children",
+    "cmg_grid_set_cell_types", "cmg_grid_get_cell_types", "cmg_grid_get_host_channel",
+    "cmg_grid_set_leaf_types_preset", "cmg_grid_set_wall_types", "cmg_grid_set_channel", "cmg_grid_get_channel",
+    "cmg_grid_fill_channel", "cmg_prepare_types_and_coeffs", "cmg_assemble_cutcell", "cmg_solve_opts_default",
+    "cmg_solver_create", "cmg_solver_destroy", "cmg_solve", "cmg_solver_get_x64", "cmg_op_full_neg_lap",
+    "cmg_op_propagate_ghosts", "cmg_op_accumulate_leafs", "cmg_op_gs_color", "cmg_op_jacobi",
+    "cmg_op_residual_restrict", "cmg_op_fas_save", "cmg_op_fas_prolong", "cmg_op_dot", "cmg_op_cycle",
+    "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
+    "cmg_rhs_uniform", "cmg_rhs_divergence",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcirrus_amg.so is not built: run `make -C cirrus-amg_amd` "
+                               "or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, i, f, d, st = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_int
+        sig = {
+            "cmg_grid_create": (st, [f, i, vp, C.POINTER(vp)]), "cmg_grid_destroy": (None, [vp]),
+            "cmg_grid_set_tile": (st, [vp, i, i, i, i, C.c_uint8]), "cmg_grid_rebuild": (st, [vp]),
+            "cmg_grid_iterative_refine": (st, [vp, C.POINTER(Target)]), "cmg_grid_finalize": (st, [vp]),
+            "cmg_grid_num_tiles": (i, [vp]), "cmg_grid_max_level": (i, [vp]),
+            "cmg_grid_level_starts": (st, [vp, vp]), "cmg_grid_get_tiles": (st, [vp, vp, vp, vp]),
+            "cmg_grid_get_slots": (st, [vp, vp]), "cmg_grid_get_neighbors": (st, [vp, vp]),
+            "cmg_grid_get_parents": (st, [vp, vp]), "cmg_grid_get_children": (st, [vp, vp]),
+            "cmg_grid_set_cell_types": (st, [vp, vp]), "cmg_grid_get_cell_types": (st, [vp, vp]),
+            "cmg_grid_get_host_channel": (st, [vp, i, vp]),
+            "cmg_grid_set_leaf_types_preset": (st, [vp, C.POINTER(Target)]),
+            "cmg_grid_set_wall_types": (st, [vp, i, i]), "cmg_grid_set_channel": (st, [vp, i, vp]),
+            "cmg_grid_get_channel": (st, [vp, i, vp]), "cmg_grid_fill_channel": (st, [vp, i, f]),
+            "cmg_prepare_types_and_coeffs": (st, [vp, i, f]), "cmg_assemble_cutcell": (st, [vp, i, vp, i, i, f]),
+            "cmg_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+            "cmg_solver_create": (st, [vp, i, f, f, f, C.POINTER(vp)]), "cmg_solver_destroy": (None, [vp]),
+            "cmg_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+            "cmg_solver_get_x64": (st, [vp, vp]),
+            "cmg_op_full_neg_lap": (st, [vp, i, i, i]), "cmg_op_propagate_ghosts": (st, [vp, i]),
+            "cmg_op_accumulate_leafs": (st, [vp, i]), "cmg_op_gs_color": (st, [vp, i, i, i, i, i, d]),
+            "cmg_op_jacobi": (st, [vp, i, i, i, i, i, i, f]), "cmg_op_residual_restrict": (st, [vp, i, i, i, i, f]),
+            "cmg_op_fas_save": (st, [vp, i, i, i, i, i]), "cmg_op_fas_prolong": (st, [vp, i, i, i, f]),
+            "cmg_op_dot": (st, [vp, i, i, C.POINTER(d)]),
+            "cmg_op_cycle": (st, [vp, C.POINTER(SolveOpts), i, i, i]),
+            "cmg_status_string": (C.c_char_p, [st]), "cmg_last_error": (C.c_char_p, []),
+            "cmg_grid_copy_channel": (st, [vp, i, i]),
+            "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
+            "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(status, where):
+    if status != 0:
+        raise CmgError(status, where)
+
+
+DEFAULT_LOG2_HASH = (16, 16, 16, 16, 16, 16, 20, 20, 16, 16)   # tests/TestGrids.cuh:149
+
+
+def make_target(kind, min_level, max_level, sdf_kind=SDF_SPHERE, sdf=(), bands=None, fn=None):
+    t = Target()
+    t.kind, t.min_level, t.max_level, t.sdf_kind = kind, min_level, max_level, sdf_kind
+    for k, v in enumerate(sdf):
+        t.sdf[k] = v
+    for lvl, b in (bands or {}).items():
+        t.band[lvl] = b
+    if fn is not None:
+        t._cb = LEVEL_TARGET_FN(lambda user, level, c: int(fn(level, (c[0], c[1], c[2]))))
+        t.fn = t._cb
+    return t
+
+
+class HADeviceGrid:
+    """Octree of 8^3 tiles; construction mirrors HADeviceGrid (src/HAGrid.h:356-1326)."""
+
+    def __init__(self, h0=0.125, log2_hash=DEFAULT_LOG2_HASH):
+        self.h = C.c_void_p()
+        self.num_levels = len(log2_hash)
+        self.h0 = h0
+        arr = (C.c_uint32 * len(log2_hash))(*log2_hash)
+        _check(lib().cmg_grid_create(h0, len(log2_hash), arr, C.byref(self.h)), "cmg_grid_create")
+
+    def close(self):
+        if self.h:
+            lib().cmg_grid_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # construction
+    def setTileHost(self, level, coord, tile_type):
+        _check(lib().cmg_grid_set_tile(self.h, level, *coord, tile_type), "cmg_grid_set_tile")
+
+    def rebuild(self):
+        _check(lib().cmg_grid_rebuild(self.h), "cmg_grid_rebuild")
+
+    def iterativeRefine(self, target):
+        _check(lib().cmg_grid_iterative_refine(self.h, C.byref(target)), "cmg_grid_iterative_refine")
+
+    def finalize(self):
+        _check(lib().cmg_grid_finalize(self.h), "cmg_grid_finalize")
+
+    @classmethod
+    def test_grid(cls, target, h0=0.125, log2_hash=DEFAULT_LOG2_HASH):
+        """CreateTestGrid (tests/TestGrids.cuh:144-169): one LEAF root, rebuild, refine, LEAF cell types."""
+        g = cls(h0, log2_hash)
+        g.setTileHost(0, (0, 0, 0), LEAF)
+        g.rebuild()
+        g.iterativeRefine(target)
+        g.finalize()
+        g.set_leaf_types_preset(target)
+        return g
+
+    # topology views
+    @property
+    def N(self):
+        return lib().cmg_grid_num_tiles(self.h)
+
+    @property
+    def max_level(self):
+        return lib().cmg_grid_max_level(self.h)
+
+    def level_starts(self):
+        s = np.zeros(self.num_levels + 1, np.int32)
+        _check(lib().cmg_grid_level_starts(self.h, _ptr(s)), "level_starts")
+        return s
+
+    def tiles(self):
+        n = self.N
+        lv = np.zeros(n, np.int32); co = np.zeros((n, 3), np.int32); ty = np.zeros(n, np.uint8)
+        _check(lib().cmg_grid_get_tiles(self.h, _ptr(lv), _ptr(co), _ptr(ty)), "get_tiles")
+        return lv, co, ty
+
+    def slots(self):
+        s = np.zeros(self.N, np.int32); _check(lib().cmg_grid_get_slots(self.h, _ptr(s)), "get_slots"); return s
+
+    def neighbors(self):
+        a = np.zeros((self.N, 6), np.int32); _check(lib().cmg_grid_get_neighbors(self.h, _ptr(a)), "nbr"); return a
+
+    def parents(self):
+        a = np.zeros(self.N, np.int32); _check(lib().cmg_grid_get_parents(self.h, _ptr(a)), "parents"); return a
+
+    def children(self):
+        a = np.zeros((self.N, 8), np.int32); _check(lib().cmg_grid_get_children(self.h, _ptr(a)), "children"); return a
+
+    # cell data
+    def set_leaf_types_preset(self, target):
+        _check(lib().cmg_grid_set_leaf_types_preset(self.h, C.byref(target)), "set_leaf_types_preset")
+
+    def set_wall_types(self, chk_level, dirichlet_ytop):
+        _check(lib().cmg_grid_set_wall_types(self.h, chk_level, int(dirichlet_ytop)), "set_wall_types")
+
+    def cell_types(self):
+        a = np.zeros((self.N, 512), np.uint8)
+        _check(lib().cmg_grid_get_cell_types(self.h, _ptr(a)), "get_cell_types")
+        return a
+
+    def set_cell_types(self, types):
+        a = np.ascontiguousarray(types, np.uint8)
+        assert a.size == self.N * 512
+        _check(lib().cmg_grid_set_cell_types(self.h, _ptr(a)), "set_cell_types")
+
+    def host_channel(self, c):
+        a = np.zeros((self.N, 512), np.float32)
+        _check(lib().cmg_grid_get_host_channel(self.h, c, _ptr(a)), "get_host_channel")
+        return a
+
+    def set_channel(self, c, values):
+        a = np.ascontiguousarray(values, np.float32)
+        assert a.size == self.N * 512
+        _check(lib().cmg_grid_set_channel(self.h, c, _ptr(a)), "set_channel")
+
+    def channel(self, c):
+        a = np.zeros((self.N, 512), np.float32)
+        _check(lib().cmg_grid_get_channel(self.h, c, _ptr(a)), "get_channel")
+        return a
+
+    def copy_channel(self, src, dst):
+        _check(lib().cmg_grid_copy_channel(self.h, src, dst), "copy_channel")
+
+    def fill_channel(self, c, value):
+        _check(lib().cmg_grid_fill_channel(self.h, c, value), "fill_channel")
+
+    # inputs
+    def rhs_uniform(self, c, seed, zero_mean=True):
+        _check(lib().cmg_rhs_uniform(self.h, c, seed, int(zero_mean)), "rhs_uniform")
+
+    def rhs_divergence(self, c, u0, seed):
+        _check(lib().cmg_rhs_divergence(self.h, c, u0, seed), "rhs_divergence")
+
+    # assembly
+    def prepare_types_and_coeffs(self, coeff_channel, R_matrix):
+        _check(lib().cmg_prepare_types_and_coeffs(self.h, coeff_channel, R_matrix), "prepare_types_and_coeffs")
+
+    def assemble_cutcell(self, sdf_kind, sdf, face_rule, coeff_channel, R_matrix):
+        arr = (C.c_float * 8)(*sdf)
+        _check(lib().cmg_assemble_cutcell(self.h, sdf_kind, arr, face_rule, coeff_channel, R_matrix),
+               "assemble_cutcell")
+
+    # kernels
+    def full_neg_lap(self, x, c0, y):
+        _check(lib().cmg_op_full_neg_lap(self.h, x, c0, y), "full_neg_lap")
+
+    def propagate_ghosts(self, c):
+        _check(lib().cmg_op_propagate_ghosts(self.h, c), "propagate_ghosts")
+
+    def accumulate_leafs(self, c):
+        _check(lib().cmg_op_accumulate_leafs(self.h, c), "accumulate_leafs")
+
+    def gs_color(self, level, x, c0, rhs, color, omega):
+        _check(lib().cmg_op_gs_color(self.h, level, x, c0, rhs, color, omega), "gs_color")
+
+    def jacobi(self, level, x_in, x_out, ghost_ch, c0, rhs, omega):
+        _check(lib().cmg_op_jacobi(self.h, level, x_in, x_out, ghost_ch, c0, rhs, omega), "jacobi")
+
+    def residual_restrict(self, level, x, c0, rhs, R_restrict):
+        _check(lib().cmg_op_residual_restrict(self.h, level, x, c0, rhs, R_restrict), "residual_restrict")
+
+    def fas_save(self, level, x, c0, rhs, x0):
+        _check(lib().cmg_op_fas_save(self.h, level, x, c0, rhs, x0), "fas_save")
+
+    def fas_prolong(self, level, x, x0, P):
+        _check(lib().cmg_op_fas_prolong(self.h, level, x, x0, P), "fas_prolong")
+
+    def dot(self, a, b):
+        out = C.c_double()
+        _check(lib().cmg_op_dot(self.h, a, b, C.byref(out)), "dot")
+        return out.value
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().cmg_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class AMGSolver:
+    """AMGSolver(coeff_channel, R_matrix, R_restrict, P) (src/AMGSolver.h:19-74)."""
+
+    def __init__(self, grid, coeff_channel, R_matrix_coeff=1.0, R_restrict_coeff=1.0, P_coeff=1.0):
+        self.grid = grid
+        self.h = C.c_void_p()
+        _check(lib().cmg_solver_create(grid.h, coeff_channel, R_matrix_coeff, R_restrict_coeff, P_coeff,
+                                       C.byref(self.h)), "cmg_solver_create")
+        self.last_stats = None
+
+    def close(self):
+        if self.h:
+            lib().cmg_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, verbose=False, max_iters=100, relative_tolerance=1e-6, level_iters=2, coarsest_iters=10,
+              sync_stride=1, is_pure_neumann=False, opts=None, raise_on_nonconvergence=False):
+        """AMGSolver::solve (src/AMGSolver.cu:1399-1549): returns (iters, relative residual)."""
+        o = opts or default_opts()
+        o.verbose, o.max_iters, o.relative_tolerance = int(verbose), max_iters, relative_tolerance
+        o.level_iters, o.coarsest_iters, o.sync_stride = level_iters, coarsest_iters, sync_stride
+        o.is_pure_neumann = int(is_pure_neumann)
+        stats = SolveStats()
+        status = lib().cmg_solve(self.h, C.byref(o), C.byref(stats))
+        self.last_stats = stats
+        if status not in (0, 6) or (status == 6 and raise_on_nonconvergence):
+            raise CmgError(status, "cmg_solve")
+        return stats.iters, stats.rel_res
+
+    def x64(self):
+        a = np.zeros((self.grid.N, 512), np.float64)
+        _check(lib().cmg_solver_get_x64(self.h, _ptr(a)), "get_x64")
+        return a
+
+    def bench_kernel(self, kernel, reps):
+        ms, tiles = C.c_double(), C.c_int()
+        _check(lib().cmg_bench_kernel(self.h, kernel, reps, C.byref(ms), C.byref(tiles)), "bench_kernel")
+        return ms.value, tiles.value
+
+    def cycle(self, opts, x, rhs, x0):
+        _check(lib().cmg_op_cycle(self.h, C.byref(opts), x, rhs, x0), "cycle")

@@ -1,0 +1,445 @@
+// cmg_capi.cpp -- extern "C" boundary (include/cirrus_amg.h).
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "cmg_internal.hpp"
+
+namespace cmg {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace cmg
+
+using namespace cmg;
+
+#define CAPI_GUARD(...)                                    \
+    try {                                                  \
+        __VA_ARGS__;                                       \
+    } catch (const std::invalid_argument& ex) {            \
+        set_error(ex.what());                              \
+        return CMG_ERR_BAD_ARG;                            \
+    } catch (const std::exception& ex) {                   \
+        set_error(ex.what());                              \
+        std::string w = ex.what();                         \
+        if (w.find("hash table full") != std::string::npos) return CMG_ERR_HASH_FULL; \
+        return CMG_ERR_HIP;                                \
+    }
+
+namespace {
+
+cmg_status need_topology(const cmg_grid* g)
+{
+    if (!g || !g->topo) return CMG_ERR_BAD_ARG;
+    return CMG_OK;
+}
+
+cmg_status need_final(const cmg_grid* g)
+{
+    if (!g || !g->topo) return CMG_ERR_BAD_ARG;
+    if (!g->topo->finalized()) {
+        set_error("grid is not finalized (call cmg_grid_finalize)");
+        return CMG_ERR_STATE;
+    }
+    return CMG_OK;
+}
+
+// host cell types -> device (only once the device copy exists)
+void upload_types(cmg_grid* g)
+{
+    if (!g->uploaded) return;
+    auto& d = g->dev;
+    if (hipMemcpy(d.ctype, g->cells.type.data(), g->cells.type.size(), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::runtime_error("hipMemcpy cell types failed");
+    g->types_on_device = true;
+}
+
+void upload_channel(cmg_grid* g, int c)
+{
+    if (!g->uploaded) return;
+    auto& d = g->dev;
+    float* dst = dev_channel(d, c);
+    const auto& src = g->cells.ch[c];
+    if (hipMemcpy(dst, src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::runtime_error("hipMemcpy channel failed");
+}
+
+void ensure_types(cmg_grid* g)
+{
+    const size_t n = size_t(g->topo->num_tiles()) * kCells;
+    if (g->cells.type.size() != n) g->cells.type.assign(n, 0);
+}
+
+}  // namespace
+
+namespace cmg {
+void ensure_device(cmg_grid* g)
+{
+    if (g->uploaded) return;
+    dev_upload(g->dev, *g->topo);
+    g->uploaded = true;
+    upload_types(g);
+    for (int c = 0; c < kChannels; ++c)
+        if (!g->cells.ch[c].empty()) upload_channel(g, c);
+}
+}  // namespace cmg
+
+extern "C" {
+
+cmg_status cmg_grid_create(float h0, int num_levels, const uint32_t* log2_hash, cmg_grid** out)
+{
+    if (!out || !log2_hash || !(h0 > 0)) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        auto* g = new cmg_grid;
+        g->topo = std::make_unique<Topology>(h0, num_levels, log2_hash);
+        *out = g;
+    });
+    return CMG_OK;
+}
+
+void cmg_grid_destroy(cmg_grid* g)
+{
+    if (!g) return;
+    dev_free(g->dev);
+    delete g;
+}
+
+cmg_status cmg_grid_set_tile(cmg_grid* g, int level, int i, int j, int k, uint8_t type)
+{
+    if (auto st = need_topology(g)) return st;
+    if (type != CMG_LEAF && type != CMG_NONLEAF && type != CMG_GHOST) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        g->topo->set_tile(level, Coord{i, j, k}, type);
+        g->uploaded = false;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_rebuild(cmg_grid* g)
+{
+    if (auto st = need_topology(g)) return st;
+    CAPI_GUARD({
+        g->topo->compress();
+        g->topo->spawn_ghosts();
+        g->uploaded = false;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_iterative_refine(cmg_grid* g, const cmg_target* target)
+{
+    if (auto st = need_topology(g)) return st;
+    if (!target || (target->kind == CMG_TARGET_CALLBACK && !target->fn)) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        auto fn = make_target(*g->topo, *target);
+        g->topo->iterative_refine(fn);
+        g->uploaded = false;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_finalize(cmg_grid* g)
+{
+    if (auto st = need_topology(g)) return st;
+    CAPI_GUARD({
+        g->topo->finalize();
+        dev_free(g->dev);
+        g->uploaded = false;
+        g->cells = HostCells{};
+        ensure_types(g);
+        g->assembled_c0 = -1;
+    });
+    return CMG_OK;
+}
+
+int cmg_grid_num_tiles(const cmg_grid* g) { return (g && g->topo && g->topo->finalized()) ? g->topo->num_tiles() : 0; }
+int cmg_grid_max_level(const cmg_grid* g) { return (g && g->topo) ? g->topo->max_level() : -1; }
+
+cmg_status cmg_grid_level_starts(const cmg_grid* g, int32_t* starts)
+{
+    if (auto st = need_final(g)) return st;
+    if (!starts) return CMG_ERR_BAD_ARG;
+    std::memcpy(starts, g->topo->level_start.data(), g->topo->level_start.size() * sizeof(int32_t));
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_tiles(const cmg_grid* g, int32_t* level, int32_t* coord, uint8_t* type)
+{
+    if (auto st = need_final(g)) return st;
+    const auto& T = *g->topo;
+    for (int t = 0; t < T.num_tiles(); ++t) {
+        if (level) level[t] = T.t_level[t];
+        if (coord) for (int a = 0; a < 3; ++a) coord[3 * t + a] = T.t_coord[t][a];
+        if (type) type[t] = T.t_type[t];
+    }
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_slots(const cmg_grid* g, int32_t* slot)
+{
+    if (auto st = need_final(g)) return st;
+    if (!slot) return CMG_ERR_BAD_ARG;
+    std::memcpy(slot, g->topo->t_slot.data(), g->topo->t_slot.size() * sizeof(int32_t));
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_neighbors(const cmg_grid* g, int32_t* nbr)
+{
+    if (auto st = need_final(g)) return st;
+    if (!nbr) return CMG_ERR_BAD_ARG;
+    std::memcpy(nbr, g->topo->nbr.data(), g->topo->nbr.size() * sizeof(int32_t));
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_parents(const cmg_grid* g, int32_t* parent)
+{
+    if (auto st = need_final(g)) return st;
+    if (!parent) return CMG_ERR_BAD_ARG;
+    std::memcpy(parent, g->topo->parent.data(), g->topo->parent.size() * sizeof(int32_t));
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_children(const cmg_grid* g, int32_t* child)
+{
+    if (auto st = need_final(g)) return st;
+    if (!child) return CMG_ERR_BAD_ARG;
+    std::memcpy(child, g->topo->child.data(), g->topo->child.size() * sizeof(int32_t));
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_set_cell_types(cmg_grid* g, const uint8_t* types)
+{
+    if (auto st = need_final(g)) return st;
+    if (!types) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        std::memcpy(g->cells.type.data(), types, g->cells.type.size());
+        upload_types(g);
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_cell_types(cmg_grid* g, uint8_t* types)
+{
+    if (auto st = need_final(g)) return st;
+    if (!types) return CMG_ERR_BAD_ARG;
+    ensure_types(g);
+    std::memcpy(types, g->cells.type.data(), g->cells.type.size());
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_host_channel(cmg_grid* g, int channel, float* host)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || !host) return CMG_ERR_BAD_ARG;
+    const auto& c = g->cells.ch[channel];
+    if (c.empty()) {
+        set_error("channel has no host-side assembly data");
+        return CMG_ERR_STATE;
+    }
+    std::memcpy(host, c.data(), c.size() * sizeof(float));
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_set_leaf_types_preset(cmg_grid* g, const cmg_target* target)
+{
+    if (auto st = need_final(g)) return st;
+    if (!target) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        set_leaf_types_preset(*g->topo, g->cells, *target);
+        upload_types(g);
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_set_wall_types(cmg_grid* g, int chk_level, int dirichlet_ytop)
+{
+    if (auto st = need_final(g)) return st;
+    if (chk_level < 0) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        set_wall_types(*g->topo, g->cells, chk_level, dirichlet_ytop != 0);
+        upload_types(g);
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_set_channel(cmg_grid* g, int channel, const float* host)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || !host) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_device(g);
+        float* dst = dev_channel(g->dev, channel);
+        if (hipMemcpy(dst, host, size_t(g->dev.N) * kCells * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy set channel failed");
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_channel(cmg_grid* g, int channel, float* host)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || !host) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_device(g);
+        float* src = dev_channel(g->dev, channel);
+        if (hipMemcpy(host, src, size_t(g->dev.N) * kCells * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("hipMemcpy get channel failed");
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_fill_channel(cmg_grid* g, int channel, float value)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_device(g);
+        std::vector<float> h(size_t(g->dev.N) * kCells, value);
+        float* dst = dev_channel(g->dev, channel);
+        if (hipMemcpy(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy fill failed");
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_copy_channel(cmg_grid* g, int src, int dst) { return op_copy(g, src, dst); }
+
+cmg_status cmg_prepare_types_and_coeffs(cmg_grid* g, int c0, float R)
+{
+    if (auto st = need_final(g)) return st;
+    if (c0 < 0 || c0 + 3 >= kChannels) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        assemble(*g->topo, g->cells, c0, R, -1, 0, nullptr);
+        upload_types(g);
+        for (int a = 0; a < 4; ++a) upload_channel(g, c0 + a);
+        g->assembled_c0 = c0;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_assemble_cutcell(cmg_grid* g, int sdf_kind, const float* sdf, int face_rule, int c0, float R)
+{
+    if (auto st = need_final(g)) return st;
+    if (!sdf || c0 < 0 || c0 + 3 >= kChannels || (face_rule < -1 || face_rule > 1)) return CMG_ERR_BAD_ARG;
+    if (sdf_kind != CMG_SDF_SPHERE && sdf_kind != CMG_SDF_TORUS && sdf_kind != CMG_SDF_STAR) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        set_solid_types(*g->topo, g->cells, sdf_kind, sdf);
+        assemble(*g->topo, g->cells, c0, R, face_rule, sdf_kind, sdf);
+        upload_types(g);
+        for (int a = 0; a < 4; ++a) upload_channel(g, c0 + a);
+        g->assembled_c0 = c0;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_rhs_uniform(cmg_grid* g, int channel, uint64_t seed, int zero_mean)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        rhs_uniform(*g->topo, g->cells, channel, seed, zero_mean != 0);
+        upload_channel(g, channel);
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_rhs_divergence(cmg_grid* g, int channel, int u0, uint64_t seed)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || u0 < 0 || u0 + 2 >= kChannels) return CMG_ERR_BAD_ARG;
+    if (channel >= u0 && channel <= u0 + 2) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        rhs_divergence(*g->topo, g->cells, channel, u0, seed);
+        upload_channel(g, channel);
+        for (int a = 0; a < 3; ++a) upload_channel(g, u0 + a);
+    });
+    return CMG_OK;
+}
+
+void cmg_solve_opts_default(cmg_solve_opts* o)
+{
+    if (!o) return;
+    *o = cmg_solve_opts{};
+    // FluidEuler::project: solve(grid, false, 100, 1e-6, 2, 10, 1, pure_neumann)
+    o->verbose = 0;
+    o->max_iters = 100;
+    o->relative_tolerance = 1e-6;
+    o->level_iters = 2;
+    o->coarsest_iters = 10;
+    o->sync_stride = 1;
+    o->is_pure_neumann = 0;
+    o->cycle = CMG_CYCLE_FAS_W;
+    o->smoother = CMG_SMOOTH_RBGS;
+    o->omega = 1.5;
+    o->coarsest_direct = 0;
+    o->precision = CMG_PREC_FP32;
+    o->outer_tolerance = 0;
+    o->inner_tolerance = 0;
+    o->warm_start = 0;
+    o->use_graph = 1;
+}
+
+cmg_status cmg_solver_create(cmg_grid* g, int c0, float R_matrix, float R_restrict, float P, cmg_solver** out)
+{
+    return solver_create(g, c0, R_matrix, R_restrict, P, out);
+}
+
+void cmg_solver_destroy(cmg_solver* s) { solver_destroy(s); }
+
+cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
+{
+    return solver_solve(s, opts, stats);
+}
+
+cmg_status cmg_solver_get_x64(cmg_solver* s, double* host) { return solver_get_x64(s, host); }
+
+cmg_status cmg_op_full_neg_lap(cmg_grid* g, int x, int c0, int y) { return op_full_neg_lap(g, x, c0, y); }
+cmg_status cmg_op_propagate_ghosts(cmg_grid* g, int ch) { return op_propagate_ghosts(g, ch); }
+cmg_status cmg_op_accumulate_leafs(cmg_grid* g, int ch) { return op_accumulate_leafs(g, ch); }
+cmg_status cmg_op_gs_color(cmg_grid* g, int level, int x, int c0, int rhs, int color, double omega)
+{
+    return op_gs_color(g, level, x, c0, rhs, color, omega);
+}
+cmg_status cmg_op_jacobi(cmg_grid* g, int level, int x_in, int x_out, int gch, int c0, int rhs, float omega)
+{
+    return op_jacobi(g, level, x_in, x_out, gch, c0, rhs, omega);
+}
+cmg_status cmg_op_residual_restrict(cmg_grid* g, int level, int x, int c0, int rhs, float Rr)
+{
+    return op_residual_restrict(g, level, x, c0, rhs, Rr);
+}
+cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0) { return op_fas_save(g, level, x, c0, rhs, x0); }
+cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P) { return op_fas_prolong(g, level, x, x0, P); }
+cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out) { return op_dot(g, a, b, out); }
+cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0) { return solver_cycle(s, opts, x, rhs, x0); }
+
+cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
+{
+    return bench_kernel(s, kernel, reps, avg_ms, tiles);
+}
+
+const char* cmg_status_string(cmg_status st)
+{
+    switch (st) {
+    case CMG_OK: return "ok";
+    case CMG_ERR_BAD_ARG: return "bad argument";
+    case CMG_ERR_STATE: return "call out of order";
+    case CMG_ERR_HIP: return "HIP runtime error";
+    case CMG_ERR_NONFINITE: return "non-finite value";
+    case CMG_ERR_BREAKDOWN: return "CG breakdown";
+    case CMG_ERR_NOT_CONVERGED: return "not converged";
+    case CMG_ERR_HASH_FULL: return "hash table full";
+    }
+    return "unknown";
+}
+
+const char* cmg_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// cmg_internal.hpp -- state shared by the C ABI and the device code.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cmg_host.hpp"
+
+namespace cmg {
+
+// scalar slots of the solver's device scalar block
+enum Scalar : int {
+    S_DONE = 0,      // 0 running, 1 converged, 2 breakdown, 3 non-finite
+    S_ITER = 1,      // completed PCG iterations
+    S_GAMMA = 2, S_GAMMA_OLD = 3, S_ALPHA = 4, S_BETA = 5, S_FP = 6,
+    S_RR = 7, S_RHS2 = 8, S_THR2 = 9, S_MEAN = 10, S_TMP = 11,
+    S_MAXIT = 12,
+    S_COUNT = 16
+};
+
+// a launch list: tile ids in a spatially coherent order, padded for XCD
+// round-robin dispatch; entries < 0 are skipped by the kernels
+struct LaunchList {
+    int32_t* d = nullptr;
+    int n = 0;        // workgroups to launch
+    int tiles = 0;    // real tiles in the list
+};
+
+struct DevGrid {
+    int N = 0;
+    int num_levels = 0;
+    int max_level = -1;
+    float h0 = 0;
+    std::vector<int32_t> level_start;
+    hipStream_t stream = nullptr;
+    // topology
+    uint8_t* ttype = nullptr;     // N
+    uint8_t* toct = nullptr;      // N, bit a = coord[a] & 1
+    int32_t* tlevel = nullptr;    // N
+    int32_t* nbr = nullptr;       // 6N
+    int32_t* parent = nullptr;    // N
+    int32_t* child = nullptr;     // 8N
+    uint8_t* ctype = nullptr;     // 512N
+    float* ch[kChannels] = {};
+    // launch lists
+    LaunchList leafs;                   // absolute LEAF tiles
+    LaunchList all;                     // every tile
+    LaunchList subtree[kMaxLevels];     // LEAF | NONLEAF at the level
+    LaunchList nonleaf[kMaxLevels];
+    LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
+    LaunchList ghosts;                  // all GHOST tiles
+    int max_list = 0;
+    // reduction scratch for the op-level entry points
+    double* partial = nullptr;
+    double* scal = nullptr;
+};
+
+void dev_upload(DevGrid& d, const Topology& topo);
+void dev_free(DevGrid& d);
+float* dev_channel(DevGrid& d, int c);   // allocates (zeroed) on first use
+
+}  // namespace cmg
+
+struct cmg_grid {
+    std::unique_ptr<cmg::Topology> topo;
+    cmg::HostCells cells;
+    cmg::DevGrid dev;
+    bool uploaded = false;
+    bool types_on_device = false;
+    int assembled_c0 = -1;
+};
+
+struct cmg_solver;
+
+namespace cmg {
+void set_error(const std::string& msg);
+// uploads topology, cell types and host-assembled channels on first device use
+void ensure_device(cmg_grid* g);
+// device entry points used by the C ABI (cmg_device.hip)
+cmg_status op_full_neg_lap(cmg_grid* g, int x, int c0, int y);
+cmg_status op_propagate_ghosts(cmg_grid* g, int ch);
+cmg_status op_accumulate_leafs(cmg_grid* g, int ch);
+cmg_status op_gs_color(cmg_grid* g, int level, int x, int c0, int rhs, int color, double omega);
+cmg_status op_jacobi(cmg_grid* g, int level, int x_in, int x_out, int gch, int c0, int rhs, float omega);
+cmg_status op_residual_restrict(cmg_grid* g, int level, int x, int c0, int rhs, float Rr);
+cmg_status op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0);
+cmg_status op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P);
+cmg_status op_dot(cmg_grid* g, int a, int b, double* out);
+cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_solver** out);
+void solver_destroy(cmg_solver* s);
+cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
+cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0);
+cmg_status solver_get_x64(cmg_solver* s, double* host);
+cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
+cmg_status op_copy(cmg_grid* g, int src, int dst);
+}  // namespace cmg

@@ -1,0 +1,755 @@
+// cmg_kernels.hip -- gfx950 kernels of the octree multigrid Poisson solve.
+//
+// One 8^3 tile per 512-thread workgroup (8 wave64s, one x-plane per wave),
+// thread v <-> cell v = (i<<6)|(j<<3)|k.  The 10^3 halo'd tile and the three
+// negative-face coefficient arrays (+ the + side face from the neighbour)
+// are staged in LDS.  Workgroups walk Morton-ordered launch lists so that a
+// tile's face neighbours are processed by nearby workgroups on the same XCD
+// and their halo reads hit that XCD's L2.
+//
+// Arithmetic is written out exactly (fp-contract off, explicit fmaf) so that
+// each kernel reproduces the CPU restatement bit for bit:
+//   apply   s = x0*D ; s = fmaf(off, xn, s) for axis 0..2, side -,+
+//   ghost   vg = (float)((double)x_c + 0.5*(double)(V1 - avg8))
+//   RB-SOR  x = (float)((double)x + (double)((b - Ax)/D) * omega)
+//   Jacobi  x = x + w*((b - Ax)/D)
+#include <hip/hip_runtime.h>
+
+#include "cmg_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace cmg {
+namespace k {
+
+__device__ __forceinline__ int SX(int i, int j, int k) { return (i + 1) * 100 + (j + 1) * 10 + (k + 1); }
+__device__ __forceinline__ int cidx(int i, int j, int k) { return (i << 6) | (j << 3) | k; }
+
+template <typename T>
+struct Lds {
+    T x[1000];
+    float off[3][576];   // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
+};
+
+__device__ __forceinline__ bool stopped(const double* S) { return S && S[S_DONE] != 0.0; }
+
+// wave64 + workgroup reduction of one double per thread; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < int(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+}
+
+// LoadAMGLaplacianTileData (src/AMGSolver.cu:395-543).
+// FRESH: the halo is evaluated the way AMGFullNegativeLaplacianOnLeafs sees it
+//   after PropagateToChildren + AccumulateToParentsOneStep: a GHOST cell is its
+//   parent cell and a NONLEAF cell is the mean of its 8 children.
+// !FRESH: ghost values come from G, NONLEAF values from X (cycle state).
+template <bool FRESH, typename T>
+__device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, const T* __restrict__ X,
+                                          const T* __restrict__ G, const float* const* C, Lds<T>& s,
+                                          T& x_own, float c_own[3], float& d_own)
+{
+    const int v = threadIdx.x;
+    const int i = v >> 6, j = (v >> 3) & 7, kk = v & 7;
+    const size_t base = size_t(t) * 512;
+    const uint8_t ct = tp.ctype[base + v];
+    const T xr = X[base + v];
+    x_own = ct == INTERIOR ? xr : T(0);
+    s.x[SX(i, j, kk)] = x_own;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c_own[a] = C[a][base + v];
+        s.off[a][v] = c_own[a];
+    }
+    d_own = C[3][base + v];
+    __syncthreads();
+    if (v < 384) {
+        const int f = v >> 6, fi = v & 63;
+        const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
+        const int p = fi >> 3, q = fi & 7;
+        int fl[3], cl[3], nl[3];
+        const int axk = sgn < 0 ? -1 : 8;
+        if (axis == 0) { fl[0] = axk; fl[1] = p; fl[2] = q; }
+        else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
+        else { fl[0] = p; fl[1] = q; fl[2] = axk; }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { cl[a] = fl[a]; nl[a] = fl[a]; }
+        cl[axis] -= sgn;
+        nl[axis] = sgn < 0 ? 7 : 0;
+        const int nt = tp.nbr[size_t(t) * 6 + (sgn < 0 ? axis : axis + 3)];
+        const int nv = cidx(nl[0], nl[1], nl[2]);
+        T hv;
+        float cplus;
+        if (nt < 0) {
+            hv = T(0);
+            cplus = h;
+        } else {
+            const size_t ni = size_t(nt) * 512 + nv;
+            const uint8_t nty = tp.ttype[nt];
+            cplus = C[axis][ni];
+            if (nty & GHOST) {
+                T avg = 0;
+#pragma unroll
+                for (int di = 0; di < 2; ++di)
+#pragma unroll
+                    for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                        for (int dk = 0; dk < 2; ++dk) avg += s.x[SX(cl[0] ^ di, cl[1] ^ dj, cl[2] ^ dk)];
+                avg /= 8;
+                T V1;
+                if (FRESH) {
+                    const int pt = tp.parent[nt];
+                    const int o = tp.toct[nt];
+                    V1 = pt < 0 ? T(0)
+                                : X[size_t(pt) * 512 + cidx(4 * (o & 1) + (nl[0] >> 1), 4 * ((o >> 1) & 1) + (nl[1] >> 1),
+                                                             4 * ((o >> 2) & 1) + (nl[2] >> 1))];
+                } else {
+                    V1 = G[ni];
+                }
+                if constexpr (sizeof(T) == 4)
+                    hv = float(double(s.x[SX(cl[0], cl[1], cl[2])]) + 0.5 * double(V1 - avg));
+                else
+                    hv = s.x[SX(cl[0], cl[1], cl[2])] + 0.5 * (V1 - avg);
+            } else if (FRESH && (nty & NONLEAF)) {
+                hv = T(0);
+                if (tp.ctype[ni] == INTERIOR) {
+                    const int oct = (nl[0] >> 2) | ((nl[1] >> 2) << 1) | ((nl[2] >> 2) << 2);
+                    const int ctile = tp.child[size_t(nt) * 8 + oct];
+                    if (ctile >= 0 && (tp.ttype[ctile] & LEAF)) {
+                        T sum = 0;
+                        const size_t cb = size_t(ctile) * 512;
+#pragma unroll
+                        for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                                for (int k2 = 0; k2 < 2; ++k2)
+                                    sum += X[cb + cidx(2 * (nl[0] & 3) + ii, 2 * (nl[1] & 3) + jj, 2 * (nl[2] & 3) + k2)];
+                        hv = sum * T(0.125);
+                    } else {
+                        hv = X[ni];
+                    }
+                }
+            } else {
+                hv = tp.ctype[ni] == INTERIOR ? X[ni] : T(0);
+            }
+        }
+        s.x[SX(fl[0], fl[1], fl[2])] = hv;
+        if (sgn > 0) s.off[axis][512 + fi] = cplus;
+    }
+    __syncthreads();
+}
+
+// AMGLaplacianTileData::negativeLap (src/AMGSolver.cu:345-382) for the thread's cell
+template <typename T>
+__device__ __forceinline__ T neg_lap(const Lds<T>& s, T x0, const float c[3], float d)
+{
+    if (d == 0) return T(0);
+    const int v = threadIdx.x;
+    const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
+    const float cp0 = i < 7 ? s.off[0][v + 64] : s.off[0][512 + (j * 8 + k)];
+    const float cp1 = j < 7 ? s.off[1][v + 8] : s.off[1][512 + (i * 8 + k)];
+    const float cp2 = k < 7 ? s.off[2][v + 1] : s.off[2][512 + (i * 8 + j)];
+    T r = x0 * T(d);
+    r = fma(T(c[0]), s.x[SX(i - 1, j, k)], r);
+    r = fma(T(cp0), s.x[SX(i + 1, j, k)], r);
+    r = fma(T(c[1]), s.x[SX(i, j - 1, k)], r);
+    r = fma(T(cp1), s.x[SX(i, j + 1, k)], r);
+    r = fma(T(c[2]), s.x[SX(i, j, k - 1)], r);
+    r = fma(T(cp2), s.x[SX(i, j, k + 1)], r);
+    return r;
+}
+
+__device__ __forceinline__ float tile_h(const TopoView& tp, int t) { return tp.h0 / float(1 << tp.tlevel[t]); }
+
+// ---------------------------------------------------------------- apply ----
+// y = A x on LEAF tiles (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
+template <bool DOT>
+__global__ __launch_bounds__(512) void apply_leaf(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                  Coef C, float* __restrict__ Y, double* __restrict__ partial,
+                                                  const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) {
+        if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
+        return;
+    }
+    float x0, c[3], d;
+    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    const float y = tp.ctype[idx] == INTERIOR ? neg_lap(s, x0, c, d) : 0.0f;
+    Y[idx] = y;
+    if (DOT) {
+        double b = block_sum(double(X[idx]) * double(y), red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+}
+
+// r64 = b64 - A x64 on LEAF tiles in FP64 (FRESH halo), r32 = float(r64), partial |r64|^2
+__global__ __launch_bounds__(512) void residual64(TopoView tp, const int32_t* __restrict__ list, const double* __restrict__ X,
+                                                  Coef C, const double* __restrict__ B, double* __restrict__ R64,
+                                                  float* __restrict__ R32, double* __restrict__ partial)
+{
+    __shared__ Lds<double> s;
+    __shared__ double red[8];
+    const int t = list[blockIdx.x];
+    if (t < 0) {
+        if (threadIdx.x == 0) partial[blockIdx.x] = 0;
+        return;
+    }
+    double x0;
+    float c[3], d;
+    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    double r = 0;
+    if (tp.ctype[idx] == INTERIOR) r = B[idx] - neg_lap(s, x0, c, d);
+    R64[idx] = r;
+    R32[idx] = float(r);
+    double b = block_sum(r * r, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// ------------------------------------------------------------- smoothers ----
+// one colour of red-black SOR on subtree-LEAF tiles of a level, in place
+// (NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel, src/AMGSolver.cu:838-877)
+__global__ __launch_bounds__(512) void gs_color(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                Coef C, const float* __restrict__ B, int color, double omega,
+                                                const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    float x0, c[3], d;
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
+    const int v = threadIdx.x;
+    if ((((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) return;
+    const size_t idx = size_t(t) * 512 + v;
+    const float b = B[idx];
+    const float ax = neg_lap(s, x0, c, d);
+    const float delta = (tp.ctype[idx] & INTERIOR) ? (b - ax) / d : 0.0f;
+    X[idx] = float(double(x0) + double(delta) * omega);
+}
+
+// damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
+__global__ __launch_bounds__(512) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
+                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
+                                              const float* __restrict__ B, float w, const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    float x0, c[3], d;
+    load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, x0, c, d);
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    const float ax = neg_lap(s, x0, c, d);
+    const float delta = (tp.ctype[idx] & INTERIOR) ? (B[idx] - ax) / d : 0.0f;
+    Xout[idx] = x0 + w * delta;
+}
+
+// ------------------------------------------------------------- transfers ----
+// ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
+__global__ __launch_bounds__(512) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                         Coef C, float* __restrict__ B, float Rr,
+                                                         const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    __shared__ float res[512];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    float x0, c[3], d;
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
+    const int v = threadIdx.x;
+    res[v] = B[size_t(t) * 512 + v] - neg_lap(s, x0, c, d);
+    __syncthreads();
+    if (v >= 64) return;
+    const int pt = tp.parent[t];
+    if (pt < 0) return;
+    const int o = tp.toct[t];
+    const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
+    float rs = 0, xs = 0;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                rs += res[cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
+                xs += s.x[SX(fl0 + ii, fl1 + jj, fl2 + kk)];
+            }
+    const size_t pidx = size_t(pt) * 512 + cidx(4 * (o & 1) + (fl0 >> 1), 4 * ((o >> 1) & 1) + (fl1 >> 1), 4 * ((o >> 2) & 1) + (fl2 >> 1));
+    B[pidx] = rs * Rr;
+    X[pidx] = xs / 8;
+}
+
+// FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
+__global__ __launch_bounds__(512) void fas_save(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                Coef C, float* __restrict__ B, float* __restrict__ X0,
+                                                const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    float x0, c[3], d;
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    if (tp.ctype[idx] & INTERIOR) {
+        B[idx] = B[idx] + neg_lap(s, x0, c, d);
+        X0[idx] = X[idx];
+    } else {
+        B[idx] = 0;
+    }
+}
+
+// FASProlongateAndUpdateAMG128Kernel (src/AMGSolver.cu:1216-1255) on LEAF|NONLEAF|GHOST tiles of a level
+__global__ __launch_bounds__(512) void fas_prolong(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                   const float* __restrict__ X0, float P, int level,
+                                                   const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const int v = threadIdx.x;
+    const size_t idx = size_t(t) * 512 + v;
+    const int pt = tp.parent[t];
+    const int o = tp.toct[t];
+    const int pv = cidx(4 * (o & 1) + ((v >> 6) >> 1), 4 * ((o >> 1) & 1) + (((v >> 3) & 7) >> 1), 4 * ((o >> 2) & 1) + ((v & 7) >> 1));
+    const float cx = pt < 0 ? 0.0f : X[size_t(pt) * 512 + pv];
+    const float cx0 = pt < 0 ? 0.0f : X0[size_t(pt) * 512 + pv];
+    const bool interior = tp.ctype[idx] & INTERIOR;
+    if (tp.ttype[t] & GHOST) X[idx] = interior ? cx : 0.0f;
+    else X[idx] = interior ? X[idx] + P * (cx - cx0) : 0.0f;
+    (void)level;
+}
+
+// PropagateToChildren on GHOST tiles: ghost cell = parent cell (src/PoissonGrid.cu:377-405)
+__global__ __launch_bounds__(512) void propagate_ghosts(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const int v = threadIdx.x;
+    const int pt = tp.parent[t];
+    const int o = tp.toct[t];
+    const int pv = cidx(4 * (o & 1) + ((v >> 6) >> 1), 4 * ((o >> 1) & 1) + (((v >> 3) & 7) >> 1), 4 * ((o >> 2) & 1) + ((v & 7) >> 1));
+    X[size_t(t) * 512 + v] = pt < 0 ? 0.0f : X[size_t(pt) * 512 + pv];
+}
+
+// AccumulateToParentsOneStep(LEAF, 1/8) (src/PoissonGrid.cu:408-465)
+__global__ __launch_bounds__(64) void accumulate_leafs(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const int pt = tp.parent[t];
+    if (pt < 0) return;
+    const int v = threadIdx.x, o = tp.toct[t];
+    const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
+    float sum = 0;
+    for (int ii = 0; ii < 2; ++ii)
+        for (int jj = 0; jj < 2; ++jj)
+            for (int kk = 0; kk < 2; ++kk) sum += X[size_t(t) * 512 + cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
+    X[size_t(pt) * 512 + cidx(4 * (o & 1) + (fl0 >> 1), 4 * ((o >> 1) & 1) + (fl1 >> 1), 4 * ((o >> 2) & 1) + (fl2 >> 1))] = sum * 0.125f;
+}
+
+// ------------------------------------------------------------ PCG vectors ----
+__global__ __launch_bounds__(512) void fill_all(float* __restrict__ X, size_t n4, float value, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    float4* p = reinterpret_cast<float4*>(X);
+    const float4 v4 = make_float4(value, value, value, value);
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += size_t(gridDim.x) * blockDim.x) p[i] = v4;
+}
+
+__global__ __launch_bounds__(512) void fill_list(const int32_t* __restrict__ list, float* __restrict__ X, float value)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    X[size_t(t) * 512 + threadIdx.x] = value;
+}
+
+__global__ __launch_bounds__(512) void copy_list(const int32_t* __restrict__ list, const float* __restrict__ A, float* __restrict__ B,
+                                                 const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    B[idx] = A[idx];
+}
+
+// partial[wg] = sum over the tile of a*b (exact products, FP64 accumulation)
+__global__ __launch_bounds__(512) void dot_list(const int32_t* __restrict__ list, const float* __restrict__ A,
+                                                const float* __restrict__ B, double* __restrict__ partial,
+                                                const double* __restrict__ S)
+{
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    double v = 0;
+    if (t >= 0) {
+        const size_t idx = size_t(t) * 512 + threadIdx.x;
+        v = double(A[idx]) * double(B[idx]);
+    }
+    double b = block_sum(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// x += a p ; r -= a Ap on INTERIOR cells of LEAF tiles, partial |r|^2 (AMGSolver.cu:1462-1485)
+__global__ __launch_bounds__(512) void update_xr(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                 float* __restrict__ R, const float* __restrict__ P,
+                                                 const float* __restrict__ AP, double* __restrict__ partial,
+                                                 const double* __restrict__ S)
+{
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    double v = 0;
+    if (t >= 0) {
+        const size_t idx = size_t(t) * 512 + threadIdx.x;
+        const float a = float(S[S_ALPHA]);
+        float r = R[idx];
+        if (tp.ctype[idx] & INTERIOR) {
+            X[idx] = fmaf(a, P[idx], X[idx]);
+            r = fmaf(-a, AP[idx], r);
+            R[idx] = r;
+        }
+        v = double(r) * double(r);
+    }
+    double b = block_sum(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// p = z + b p on INTERIOR cells of LEAF tiles (AMGSolver.cu:1525-1536)
+__global__ __launch_bounds__(512) void update_p(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ P,
+                                                const float* __restrict__ Z, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    if (tp.ctype[idx] & INTERIOR) P[idx] = fmaf(float(S[S_BETA]), P[idx], Z[idx]);
+}
+
+// pure-Neumann recentring (ReCenterLeafCells, src/PoissonGrid.cu:889-921)
+__global__ __launch_bounds__(512) void sum_count(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                 double* __restrict__ psum, double* __restrict__ pcnt,
+                                                 const double* __restrict__ S)
+{
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    double v = 0, c = 0;
+    if (t >= 0) {
+        const size_t idx = size_t(t) * 512 + threadIdx.x;
+        if (tp.ctype[idx] & INTERIOR) { v = X[idx]; c = 1; }
+    }
+    double b = block_sum(v, red);
+    __syncthreads();
+    double bc = block_sum(c, red);
+    if (threadIdx.x == 0) { psum[blockIdx.x] = b; pcnt[blockIdx.x] = bc; }
+}
+
+__global__ __launch_bounds__(512) void sub_mean(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    if (tp.ctype[idx] & INTERIOR) X[idx] = float(double(X[idx]) - S[S_MEAN]);
+}
+
+// partial[wg] = sum over the tile of a*b in FP64
+__global__ __launch_bounds__(512) void dot64_list(const int32_t* __restrict__ list, const double* __restrict__ A,
+                                                  const double* __restrict__ B, double* __restrict__ partial)
+{
+    __shared__ double red[8];
+    const int t = list[blockIdx.x];
+    double v = 0;
+    if (t >= 0) {
+        const size_t idx = size_t(t) * 512 + threadIdx.x;
+        v = A[idx] * B[idx];
+    }
+    double b = block_sum(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// y = fmaf(a, x, y) on INTERIOR cells of the listed tiles
+__global__ __launch_bounds__(512) void axpy_list(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ Y,
+                                                 const float* __restrict__ X, float a)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    if (tp.ctype[idx] & INTERIOR) Y[idx] = fmaf(a, X[idx], Y[idx]);
+}
+
+// x64 += x32 on INTERIOR cells of LEAF tiles (FP64 solution update)
+__global__ __launch_bounds__(512) void update_x64(TopoView tp, const int32_t* __restrict__ list, double* __restrict__ X64,
+                                                  const float* __restrict__ E)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    if (tp.ctype[idx] & INTERIOR) X64[idx] += double(E[idx]);
+}
+
+__global__ __launch_bounds__(512) void to_double(const int32_t* __restrict__ list, const float* __restrict__ A, double* __restrict__ B,
+                                                 TopoView tp, int masked)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    B[idx] = (!masked || (tp.ctype[idx] & INTERIOR)) ? double(A[idx]) : 0.0;
+}
+
+__global__ __launch_bounds__(512) void to_float(const int32_t* __restrict__ list, const double* __restrict__ A, float* __restrict__ B)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const size_t idx = size_t(t) * 512 + threadIdx.x;
+    B[idx] = float(A[idx]);
+}
+
+// ------------------------------------------------------------ reductions ----
+// deterministic single-workgroup sum of partials, then the scalar step of the
+// PCG recurrence (alpha, convergence test, beta), replacing TernaryOnArray
+__global__ __launch_bounds__(1024) void finish(const double* __restrict__ partial, const double* __restrict__ partial2, int n,
+                                               double* __restrict__ S, int kind)
+{
+    __shared__ double red[16];
+    __shared__ double red2[16];
+    if (kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0) return;
+    double v = 0, v2 = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        v += partial[i];
+        if (partial2) v2 += partial2[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, 64);
+        v2 += __shfl_xor(v2, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[w] = v; red2[w] = v2; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s = 0, s2 = 0;
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) { s += red[i]; s2 += red2[i]; }
+    switch (kind) {
+    case FIN_SUM: S[S_TMP] = s; break;
+    case FIN_GAMMA0: S[S_GAMMA] = s; break;
+    case FIN_ALPHA:
+        S[S_FP] = s;
+        if (!isfinite(s)) S[S_DONE] = 3;
+        else if (s == 0.0) S[S_DONE] = 2;
+        else S[S_ALPHA] = S[S_GAMMA] / s;
+        break;
+    case FIN_RR:
+        S[S_RR] = s;
+        S[S_ITER] += 1;
+        if (!isfinite(s)) S[S_DONE] = 3;
+        else if (s < S[S_THR2]) S[S_DONE] = 1;
+        else if (S[S_ITER] >= S[S_MAXIT]) S[S_DONE] = 4;
+        break;
+    case FIN_BETA:
+        S[S_GAMMA_OLD] = S[S_GAMMA];
+        S[S_GAMMA] = s;
+        S[S_BETA] = s / S[S_GAMMA_OLD];
+        if (!isfinite(S[S_BETA])) S[S_DONE] = 3;
+        break;
+    case FIN_MEAN: S[S_MEAN] = s / s2; break;
+    }
+}
+
+// level-0 FP64 dense solve: x = A0^-1 b on the level's INTERIOR cells
+__global__ __launch_bounds__(1024) void coarse_direct(const double* __restrict__ inv, const int32_t* __restrict__ cells, int n,
+                                                      const int32_t* __restrict__ zero_cells, int nz, float* __restrict__ X,
+                                                      const float* __restrict__ B, const double* __restrict__ S)
+{
+    extern __shared__ double sb[];
+    if (stopped(S)) return;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = double(B[cells[i]]);
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const double* row = inv + size_t(r) * n;
+        double acc = 0;
+        for (int c = 0; c < n; ++c) acc = fma(row[c], sb[c], acc);
+        X[cells[r]] = float(acc);
+    }
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) X[zero_cells[i]] = 0.0f;
+}
+
+}  // namespace k
+
+// ---------------------------------------------------------------- launchers ---
+
+static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
+
+void launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    if (partial) k::apply_leaf<true><<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, C, Y, partial, S);
+    else k::apply_leaf<false><<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, C, Y, nullptr, S);
+}
+
+void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* B, double* R64, float* R32, double* partial,
+                       hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::residual64<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, C, B, R64, R32, partial);
+}
+
+void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
+                     hipStream_t st)
+{
+    const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return;
+    k::gs_color<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, B, color, omega, S);
+}
+
+void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
+                   const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return;
+    k::jacobi<<<l.n, 512, 0, st>>>(view(d), l.d, Xin, Xout, G, C, B, w, S);
+}
+
+void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return;
+    k::residual_restrict<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, B, Rr, S);
+}
+
+void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.nonleaf[level];
+    if (!nonempty(l)) return;
+    k::fas_save<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, B, X0, S);
+}
+
+void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.prolong[level];
+    if (!nonempty(l)) return;
+    k::fas_prolong<<<l.n, 512, 0, st>>>(view(d), l.d, X, X0, P, level, S);
+}
+
+void launch_propagate_ghosts(const DevGrid& d, float* X, hipStream_t st)
+{
+    if (!nonempty(d.ghosts)) return;
+    k::propagate_ghosts<<<d.ghosts.n, 512, 0, st>>>(view(d), d.ghosts.d, X);
+}
+
+void launch_accumulate_leafs(const DevGrid& d, float* X, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::accumulate_leafs<<<d.leafs.n, 64, 0, st>>>(view(d), d.leafs.d, X);
+}
+
+void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, hipStream_t st)
+{
+    const size_t n4 = size_t(d.N) * 128;
+    if (!n4) return;
+    int blocks = int(std::min<size_t>((n4 + 511) / 512, 8192));
+    k::fill_all<<<blocks, 512, 0, st>>>(X, n4, value, S);
+}
+
+void launch_fill_list(const LaunchList& l, float* X, float value, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::fill_list<<<l.n, 512, 0, st>>>(l.d, X, value);
+}
+
+void launch_copy_list(const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::copy_list<<<l.n, 512, 0, st>>>(l.d, A, B, S);
+}
+
+void launch_dot(const LaunchList& l, const float* A, const float* B, double* partial, const double* S, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::dot_list<<<l.n, 512, 0, st>>>(l.d, A, B, partial, S);
+}
+
+void launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
+                      hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::update_xr<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, R, P, AP, partial, S);
+}
+
+void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::update_p<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, P, Z, S);
+}
+
+void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::sum_count<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, psum, pcnt, S);
+}
+
+void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::sub_mean<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, S);
+}
+
+void launch_dot64(const LaunchList& l, const double* A, const double* B, double* partial, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::dot64_list<<<l.n, 512, 0, st>>>(l.d, A, B, partial);
+}
+
+void launch_axpy(const DevGrid& d, const LaunchList& l, float* Y, const float* X, float a, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::axpy_list<<<l.n, 512, 0, st>>>(view(d), l.d, Y, X, a);
+}
+
+void launch_update_x64(const DevGrid& d, double* X64, const float* E, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return;
+    k::update_x64<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X64, E);
+}
+
+void launch_to_double(const DevGrid& d, const LaunchList& l, const float* A, double* B, bool masked, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::to_double<<<l.n, 512, 0, st>>>(l.d, A, B, view(d), masked ? 1 : 0);
+}
+
+void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::to_float<<<l.n, 512, 0, st>>>(l.d, A, B);
+}
+
+void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st)
+{
+    k::finish<<<1, 1024, 0, st>>>(partial, partial2, n, S, kind);
+}
+
+void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
+                          const float* B, const double* S, hipStream_t st)
+{
+    k::coarse_direct<<<1, 1024, size_t(n) * sizeof(double), st>>>(inv, cells, n, zero_cells, nz, X, B, S);
+}
+
+}  // namespace cmg

@@ -1,0 +1,62 @@
+// cmg_kernels.hpp -- kernel-side views and launcher declarations.
+#pragma once
+
+#include "cmg_internal.hpp"
+
+namespace cmg {
+
+// read-only topology view passed by value to every kernel
+struct TopoView {
+    const uint8_t* ttype;
+    const uint8_t* toct;
+    const int32_t* tlevel;
+    const int32_t* nbr;
+    const int32_t* parent;
+    const int32_t* child;
+    const uint8_t* ctype;
+    float h0;
+};
+
+// the four operator channels c0..c0+3
+struct Coef {
+    const float* c[4];
+};
+
+inline TopoView view(const DevGrid& d)
+{
+    return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.h0};
+}
+
+enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5 };
+
+void launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st);
+void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* B, double* R64, float* R32, double* partial,
+                       hipStream_t st);
+void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
+                     hipStream_t st);
+void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
+                   const double* S, hipStream_t st);
+void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st);
+void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st);
+void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st);
+void launch_propagate_ghosts(const DevGrid& d, float* X, hipStream_t st);
+void launch_accumulate_leafs(const DevGrid& d, float* X, hipStream_t st);
+void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, hipStream_t st);
+void launch_fill_list(const LaunchList& l, float* X, float value, hipStream_t st);
+void launch_copy_list(const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
+void launch_dot(const LaunchList& l, const float* A, const float* B, double* partial, const double* S, hipStream_t st);
+void launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
+                      hipStream_t st);
+void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);
+void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st);
+void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st);
+void launch_dot64(const LaunchList& l, const double* A, const double* B, double* partial, hipStream_t st);
+void launch_axpy(const DevGrid& d, const LaunchList& l, float* Y, const float* X, float a, hipStream_t st);
+void launch_update_x64(const DevGrid& d, double* X64, const float* E, hipStream_t st);
+void launch_to_double(const DevGrid& d, const LaunchList& l, const float* A, double* B, bool masked, hipStream_t st);
+void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t st);
+void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st);
+void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
+                          const float* B, const double* S, hipStream_t st);
+
+}  // namespace cmg

@@ -1,0 +1,921 @@
+// cmg_solver.cpp -- device grid upload, launch lists, kernel-level operations and
+// the PCG / FAS multigrid driver (AMGSolver::solve, src/AMGSolver.cu:1399-1549).
+//
+// Everything of one grid runs on the grid's own stream.  One PCG iteration
+// (operator, reductions, scalar recurrence, the whole multigrid cycle) is
+// captured into a hipGraph and replayed; every kernel of the iteration exits
+// at once when the device-side convergence flag is set, so replays past
+// convergence are no-ops and the host only polls the flag.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+#include "cmg_kernels.hpp"
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) throw std::runtime_error(std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+struct cmg_solver {
+    cmg_grid* g = nullptr;
+    int c0 = 11;
+    float Rm = 0.5f, Rr = 1.0f, P = 1.0f;
+    double* S = nullptr;
+    double* hS = nullptr;          // pinned ring of scalar snapshots
+    double* partial = nullptr;
+    double* partial2 = nullptr;
+    float* tmp = nullptr;          // Jacobi ping-pong buffer
+    double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
+    // level-0 dense inverse
+    double* inv0 = nullptr;
+    int32_t* cells0 = nullptr;
+    int32_t* zero0 = nullptr;
+    int n0 = 0, nz0 = 0;
+    bool direct_ready = false;
+    bool direct_pure_neumann = false;
+    // captured iteration
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace cmg {
+
+namespace {
+
+template <typename T>
+T* dmalloc(size_t n)
+{
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+uint64_t spread3(uint32_t x)
+{
+    uint64_t v = x & 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Morton order (coordinates scaled to the finest level), then an XCD-aware
+// permutation: workgroup b runs on XCD b % 8, so XCD x gets the x-th contiguous
+// Morton chunk and neighbouring tiles share an L2.
+LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles)
+{
+    LaunchList l;
+    l.tiles = int(tiles.size());
+    if (tiles.empty()) return l;
+    const int L = topo.max_level();
+    std::vector<std::pair<uint64_t, int32_t>> key(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        const int t = tiles[i];
+        const int sh = L - topo.t_level[t];
+        const Coord& c = topo.t_coord[t];
+        uint64_t m = spread3(uint32_t(c[0]) << sh) | (spread3(uint32_t(c[1]) << sh) << 1) | (spread3(uint32_t(c[2]) << sh) << 2);
+        key[i] = {m, t};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> order;
+    const int n = int(key.size());
+    constexpr int kXcd = 8;
+    if (n < 64) {
+        for (auto& k : key) order.push_back(k.second);
+    } else {
+        const int chunk = (n + kXcd - 1) / kXcd;
+        order.assign(size_t(chunk) * kXcd, -1);
+        for (int b = 0; b < chunk * kXcd; ++b) {
+            const int pos = (b % kXcd) * chunk + b / kXcd;
+            order[b] = pos < n ? key[pos].second : -1;
+        }
+    }
+    l.n = int(order.size());
+    l.d = dmalloc<int32_t>(order.size());
+    HIPCHK(hipMemcpy(l.d, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return l;
+}
+
+void free_list(LaunchList& l)
+{
+    if (l.d) (void)hipFree(l.d);
+    l = LaunchList{};
+}
+
+Coef coef(DevGrid& d, int c0)
+{
+    Coef c;
+    for (int a = 0; a < 4; ++a) c.c[a] = dev_channel(d, c0 + a);
+    return c;
+}
+
+bool valid_channel(int c) { return c >= 0 && c < kChannels; }
+
+}  // namespace
+
+float* dev_channel(DevGrid& d, int c)
+{
+    if (!d.ch[c]) {
+        d.ch[c] = dmalloc<float>(size_t(d.N) * kCells);
+        HIPCHK(hipMemsetAsync(d.ch[c], 0, size_t(d.N) * kCells * sizeof(float), d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+    }
+    return d.ch[c];
+}
+
+void dev_free(DevGrid& d)
+{
+    for (void* p : {(void*)d.ttype, (void*)d.toct, (void*)d.tlevel, (void*)d.nbr, (void*)d.parent, (void*)d.child,
+                    (void*)d.ctype, (void*)d.partial, (void*)d.scal})
+        if (p) (void)hipFree(p);
+    for (auto& c : d.ch)
+        if (c) (void)hipFree(c), c = nullptr;
+    free_list(d.leafs);
+    free_list(d.all);
+    free_list(d.ghosts);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        free_list(d.subtree[l]);
+        free_list(d.nonleaf[l]);
+        free_list(d.prolong[l]);
+    }
+    if (d.stream) (void)hipStreamDestroy(d.stream);
+    d = DevGrid{};
+}
+
+void dev_upload(DevGrid& d, const Topology& topo)
+{
+    dev_free(d);
+    HIPCHK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    const int N = topo.num_tiles();
+    d.N = N;
+    d.num_levels = topo.num_levels();
+    d.max_level = topo.max_level();
+    d.h0 = topo.h0();
+    d.level_start = topo.level_start;
+    std::vector<uint8_t> oct(N);
+    for (int t = 0; t < N; ++t)
+        oct[t] = uint8_t((topo.t_coord[t][0] & 1) | ((topo.t_coord[t][1] & 1) << 1) | ((topo.t_coord[t][2] & 1) << 2));
+    auto up = [&](auto*& dst, const auto& src) {
+        using E = typename std::remove_reference<decltype(src)>::type::value_type;
+        dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(dmalloc<E>(src.size()));
+        if (!src.empty()) HIPCHK(hipMemcpy((void*)dst, src.data(), src.size() * sizeof(E), hipMemcpyHostToDevice));
+    };
+    up(d.ttype, topo.t_type);
+    up(d.toct, oct);
+    up(d.tlevel, topo.t_level);
+    up(d.nbr, topo.nbr);
+    up(d.parent, topo.parent);
+    up(d.child, topo.child);
+    d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
+    HIPCHK(hipMemset(d.ctype, 0, size_t(N) * kCells));
+    std::vector<int32_t> leafs, all, ghosts;
+    std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels];
+    for (int t = 0; t < N; ++t) {
+        const uint8_t ty = topo.t_type[t];
+        const int l = topo.t_level[t];
+        all.push_back(t);
+        if (ty & LEAF) leafs.push_back(t);
+        if (ty & GHOST) ghosts.push_back(t);
+        if (ty & (LEAF | NONLEAF)) sub[l].push_back(t);
+        if (ty & NONLEAF) nl[l].push_back(t);
+        pr[l].push_back(t);
+    }
+    d.leafs = make_list(topo, leafs);
+    d.all = make_list(topo, all);
+    d.ghosts = make_list(topo, ghosts);
+    d.max_list = std::max(d.leafs.n, d.all.n);
+    for (int l = 0; l < topo.num_levels(); ++l) {
+        d.subtree[l] = make_list(topo, sub[l]);
+        d.nonleaf[l] = make_list(topo, nl[l]);
+        d.prolong[l] = make_list(topo, pr[l]);
+    }
+    d.partial = dmalloc<double>(size_t(d.max_list) + 1);
+    d.scal = dmalloc<double>(S_COUNT);
+    HIPCHK(hipMemset(d.scal, 0, S_COUNT * sizeof(double)));
+    HIPCHK(hipDeviceSynchronize());
+}
+
+// ------------------------------------------------------------- single ops ---
+
+#define CMG_TRY(...)                                         \
+    try {                                                    \
+        __VA_ARGS__;                                         \
+    } catch (const std::exception& ex) {                     \
+        set_error(ex.what());                                \
+        return CMG_ERR_HIP;                                  \
+    }
+
+static cmg_status check_grid(cmg_grid* g)
+{
+    if (!g || !g->topo) return CMG_ERR_BAD_ARG;
+    if (!g->topo->finalized()) {
+        set_error("grid not finalized");
+        return CMG_ERR_STATE;
+    }
+    try {
+        ensure_device(g);
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return CMG_ERR_HIP;
+    }
+    return CMG_OK;
+}
+
+cmg_status op_full_neg_lap(cmg_grid* g, int x, int c0, int y)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(x) || !valid_channel(y) || c0 < 0 || c0 + 3 >= kChannels || x == y) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_apply(d, dev_channel(d, x), coef(d, c0), dev_channel(d, y), nullptr, nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_propagate_ghosts(cmg_grid* g, int ch)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(ch)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_propagate_ghosts(d, dev_channel(d, ch), d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_accumulate_leafs(cmg_grid* g, int ch)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(ch)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_accumulate_leafs(d, dev_channel(d, ch), d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+static bool valid_level(const cmg_grid* g, int level) { return level >= 0 && level < g->dev.num_levels; }
+
+cmg_status op_gs_color(cmg_grid* g, int level, int x, int c0, int rhs, int color, double omega)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_level(g, level) || !valid_channel(x) || !valid_channel(rhs) || c0 < 0 || c0 + 3 >= kChannels || (color & ~1))
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_gs_color(d, level, dev_channel(d, x), coef(d, c0), dev_channel(d, rhs), color, omega, nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_jacobi(cmg_grid* g, int level, int x_in, int x_out, int gch, int c0, int rhs, float omega)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_level(g, level) || !valid_channel(x_in) || !valid_channel(x_out) || !valid_channel(gch) || !valid_channel(rhs) ||
+        c0 < 0 || c0 + 3 >= kChannels || x_in == x_out)
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_jacobi(d, level, dev_channel(d, x_in), dev_channel(d, x_out), dev_channel(d, gch), coef(d, c0),
+                      dev_channel(d, rhs), omega, nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_residual_restrict(cmg_grid* g, int level, int x, int c0, int rhs, float Rr)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_level(g, level) || level == 0 || !valid_channel(x) || !valid_channel(rhs) || c0 < 0 || c0 + 3 >= kChannels)
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_residual_restrict(d, level, dev_channel(d, x), coef(d, c0), dev_channel(d, rhs), Rr, nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_level(g, level) || !valid_channel(x) || !valid_channel(rhs) || !valid_channel(x0) || c0 < 0 || c0 + 3 >= kChannels)
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_fas_save(d, level, dev_channel(d, x), coef(d, c0), dev_channel(d, rhs), dev_channel(d, x0), nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_level(g, level) || level == 0 || !valid_channel(x) || !valid_channel(x0)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_fas_prolong(d, level, dev_channel(d, x), dev_channel(d, x0), P, nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status op_dot(cmg_grid* g, int a, int b, double* out)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(a) || !valid_channel(b) || !out) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        launch_dot(d.leafs, dev_channel(d, a), dev_channel(d, b), d.partial, nullptr, d.stream);
+        launch_finish(d.partial, nullptr, d.leafs.n, d.scal, FIN_SUM, d.stream);
+        double hs[S_COUNT];
+        HIPCHK(hipMemcpyAsync(hs, d.scal, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+        *out = d.leafs.n ? hs[S_TMP] : 0.0;
+    });
+    return CMG_OK;
+}
+
+// ---------------------------------------------------------------- solver ----
+
+namespace {
+
+enum { CH_X = 0, CH_R = 1, CH_P = 2, CH_AP = 3, CH_Z = 4 };
+
+// dense FP64 inverse of the level-0 operator on its INTERIOR cells, probed
+// cell by cell with the same halo rules as the device stencil (level 0 has no
+// ghosts); Gauss-Jordan with partial pivoting.  For a pure-Neumann system the
+// constant null space is removed by adding mean(D)/n * 1 1^T.
+void build_direct(cmg_solver* s, bool pure_neumann)
+{
+    DevGrid& d = s->g->dev;
+    const int lo = d.level_start[0], hi = d.level_start[1];
+    const int nt = hi - lo;
+    const Topology& topo = *s->g->topo;
+    std::vector<float> C[4];
+    std::vector<uint8_t> T(size_t(d.N) * kCells);
+    HIPCHK(hipMemcpy(T.data(), d.ctype, T.size(), hipMemcpyDeviceToHost));
+    for (int a = 0; a < 4; ++a) {
+        C[a].resize(size_t(d.N) * kCells);
+        HIPCHK(hipMemcpy(C[a].data(), dev_channel(d, s->c0 + a), C[a].size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    std::vector<int> map(size_t(nt) * kCells, -1);
+    std::vector<int32_t> cells, zeros;
+    for (int t = lo; t < hi; ++t)
+        for (int v = 0; v < kCells; ++v) {
+            const size_t idx = size_t(t) * kCells + v;
+            if (T[idx] & INTERIOR) {
+                map[size_t(t - lo) * kCells + v] = int(cells.size());
+                cells.push_back(int32_t(idx));
+            } else {
+                zeros.push_back(int32_t(idx));
+            }
+        }
+    const int n = int(cells.size());
+    if (n > 4096) throw std::runtime_error("level-0 system too large for the dense direct solve");
+    std::vector<double> A(size_t(n) * n, 0.0);
+    auto unknown = [&](int t, int v) -> int {
+        if (t < lo || t >= hi) return -1;
+        return T[size_t(t) * kCells + v] == INTERIOR ? map[size_t(t - lo) * kCells + v] : -1;
+    };
+    for (int r = 0; r < n; ++r) {
+        const int t = cells[r] / kCells, v = cells[r] % kCells;
+        const double D = C[3][cells[r]];
+        if (D == 0) continue;
+        A[size_t(r) * n + r] += D;
+        const int l[3] = {v >> 6, (v >> 3) & 7, v & 7};
+        for (int a = 0; a < 3; ++a)
+            for (int sg = -1; sg <= 1; sg += 2) {
+                int nl[3] = {l[0], l[1], l[2]};
+                nl[a] += sg;
+                int tt = t;
+                if (nl[a] < 0 || nl[a] > 7) {
+                    tt = topo.nbr[size_t(t) * 6 + (sg < 0 ? a : a + 3)];
+                    nl[a] &= 7;
+                }
+                const int nv = cell_index(nl[0], nl[1], nl[2]);
+                double off;
+                if (sg < 0) off = C[a][cells[r]];
+                else if (tt < 0) off = 0;   // empty: x = 0
+                else off = C[a][size_t(tt) * kCells + nv];
+                if (tt < 0) continue;
+                const int c = unknown(tt, nv);
+                if (c >= 0) A[size_t(r) * n + c] += off;
+            }
+    }
+    if (pure_neumann && n > 0) {
+        double md = 0;
+        for (int r = 0; r < n; ++r) md += A[size_t(r) * n + r];
+        md /= double(n) * double(n);
+        for (auto& a : A) a += md;
+    }
+    // Gauss-Jordan inverse
+    std::vector<double> inv(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[size_t(i) * n + i] = 1.0;
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[size_t(i) * n + k]) > std::fabs(A[size_t(piv) * n + k])) piv = i;
+        if (A[size_t(piv) * n + k] == 0) throw std::runtime_error("singular level-0 operator");
+        if (piv != k)
+            for (int j = 0; j < n; ++j) {
+                std::swap(A[size_t(k) * n + j], A[size_t(piv) * n + j]);
+                std::swap(inv[size_t(k) * n + j], inv[size_t(piv) * n + j]);
+            }
+        const double dk = 1.0 / A[size_t(k) * n + k];
+        for (int j = 0; j < n; ++j) {
+            A[size_t(k) * n + j] *= dk;
+            inv[size_t(k) * n + j] *= dk;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const double f = A[size_t(i) * n + k];
+            if (f == 0) continue;
+            for (int j = 0; j < n; ++j) {
+                A[size_t(i) * n + j] -= f * A[size_t(k) * n + j];
+                inv[size_t(i) * n + j] -= f * inv[size_t(k) * n + j];
+            }
+        }
+    }
+    if (s->inv0) (void)hipFree(s->inv0);
+    if (s->cells0) (void)hipFree(s->cells0);
+    if (s->zero0) (void)hipFree(s->zero0);
+    s->n0 = n;
+    s->nz0 = int(zeros.size());
+    s->inv0 = dmalloc<double>(inv.size());
+    s->cells0 = dmalloc<int32_t>(cells.size());
+    s->zero0 = dmalloc<int32_t>(zeros.size());
+    if (n) {
+        HIPCHK(hipMemcpy(s->inv0, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(s->cells0, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    if (!zeros.empty()) HIPCHK(hipMemcpy(s->zero0, zeros.data(), zeros.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    s->direct_ready = true;
+    s->direct_pure_neumann = pure_neumann;
+}
+
+struct CycleCtx {
+    cmg_solver* s;
+    const cmg_solve_opts* o;
+    float *X, *B, *X0;
+    Coef C;
+    const double* S;
+};
+
+void smooth(const CycleCtx& c, int level, int iters, int order)
+{
+    DevGrid& d = c.s->g->dev;
+    hipStream_t st = d.stream;
+    if (c.o->smoother == CMG_SMOOTH_RBGS) {
+        for (int it = 0; it < iters; ++it) {
+            launch_gs_color(d, level, c.X, c.C, c.B, order, c.o->omega, c.S, st);
+            launch_gs_color(d, level, c.X, c.C, c.B, 1 - order, c.o->omega, c.S, st);
+        }
+        return;
+    }
+    float* cur = c.X;
+    float* other = c.s->tmp;
+    for (int it = 0; it < iters; ++it) {
+        launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st);
+        std::swap(cur, other);
+    }
+    if (cur != c.X) launch_copy_list(d.subtree[level], cur, c.X, c.S, st);
+}
+
+// FASMuCycleStep (src/AMGSolver.cu:1257-1277)
+void cycle_step(const CycleCtx& c, int level)
+{
+    DevGrid& d = c.s->g->dev;
+    hipStream_t st = d.stream;
+    if (level == 0) {
+        if (c.o->coarsest_direct) {
+            launch_coarse_direct(c.s->inv0, c.s->cells0, c.s->n0, c.s->zero0, c.s->nz0, c.X, c.B, c.S, st);
+        } else if (c.o->smoother == CMG_SMOOTH_RBGS) {
+            smooth(c, 0, c.o->coarsest_iters / 2, 0);
+            smooth(c, 0, c.o->coarsest_iters / 2, 1);
+        } else {
+            smooth(c, 0, c.o->coarsest_iters, 0);
+        }
+        return;
+    }
+    smooth(c, level, c.o->level_iters, 0);
+    launch_residual_restrict(d, level, c.X, c.C, c.B, c.s->Rr, c.S, st);
+    launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
+    const int mu = c.o->cycle == CMG_CYCLE_FAS_W ? 2 : 1;
+    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1);
+    launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
+    smooth(c, level, c.o->level_iters, 1);
+}
+
+// FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
+void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), coef(d, s->c0), S};
+    launch_fill_all(d, c.X, 0.0f, S, d.stream);
+    cycle_step(c, d.max_level);
+}
+
+void enqueue_recenter(cmg_solver* s, float* R)
+{
+    DevGrid& d = s->g->dev;
+    launch_sum_count(d, R, s->partial, s->partial2, s->S, d.stream);
+    launch_finish(s->partial, s->partial2, d.leafs.n, s->S, FIN_MEAN, d.stream);
+    launch_sub_mean(d, R, s->S, d.stream);
+}
+
+// one PCG iteration (AMGSolver.cu:1446-1537)
+void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
+{
+    DevGrid& d = s->g->dev;
+    hipStream_t st = d.stream;
+    float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *P = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
+          *Z = dev_channel(d, CH_Z);
+    const int n = d.leafs.n;
+    launch_apply(d, P, coef(d, s->c0), AP, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, n, s->S, FIN_ALPHA, st);
+    launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, n, s->S, FIN_RR, st);
+    if (o->is_pure_neumann) enqueue_recenter(s, R);
+    enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, s->S);
+    launch_dot(d.leafs, R, Z, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, n, s->S, FIN_BETA, st);
+    launch_update_p(d, P, Z, s->S, st);
+}
+
+std::string graph_key(const cmg_solve_opts* o)
+{
+    std::ostringstream k;
+    k << o->cycle << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
+      << o->coarsest_direct << ',' << o->is_pure_neumann;
+    return k.str();
+}
+
+void ensure_buffers(cmg_solver* s)
+{
+    DevGrid& d = s->g->dev;
+    if (!s->tmp) {
+        s->tmp = dmalloc<float>(size_t(d.N) * kCells);
+        HIPCHK(hipMemset(s->tmp, 0, size_t(d.N) * kCells * sizeof(float)));
+    }
+    for (int c : {CH_X, CH_R, CH_P, CH_AP, CH_Z}) dev_channel(d, c);
+    for (int a = 0; a < 4; ++a) dev_channel(d, s->c0 + a);
+}
+
+void ensure_fp64(cmg_solver* s)
+{
+    DevGrid& d = s->g->dev;
+    const size_t n = size_t(d.N) * kCells;
+    if (!s->x64) {
+        s->x64 = dmalloc<double>(n);
+        s->b64 = dmalloc<double>(n);
+        s->r64 = dmalloc<double>(n);
+        HIPCHK(hipMemset(s->x64, 0, n * sizeof(double)));
+        HIPCHK(hipMemset(s->b64, 0, n * sizeof(double)));
+        HIPCHK(hipMemset(s->r64, 0, n * sizeof(double)));
+    }
+}
+
+void set_scalars(cmg_solver* s, double tol2, int max_iters)
+{
+    double h[S_COUNT] = {};
+    h[S_THR2] = tol2;   // relative; FIN_RHS2 below turns it absolute
+    h[S_MAXIT] = max_iters;
+    DevGrid& d = s->g->dev;
+    HIPCHK(hipMemcpyAsync(s->S, h, sizeof(h), hipMemcpyHostToDevice, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));   // h lives on this stack frame
+}
+
+void read_scalars(cmg_solver* s, double* h)
+{
+    DevGrid& d = s->g->dev;
+    HIPCHK(hipMemcpyAsync(s->hS, s->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    std::memcpy(h, s->hS, S_COUNT * sizeof(double));
+}
+
+// one FP32 PCG solve on channels 0-4; returns the device status code
+int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, bool warm, int* iters, double* rel_res)
+{
+    DevGrid& d = s->g->dev;
+    hipStream_t st = d.stream;
+    float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *P = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
+          *Z = dev_channel(d, CH_Z);
+    set_scalars(s, tol * tol, max_iters);
+    if (!warm) {
+        launch_fill_list(d.leafs, X, 0.0f, st);
+    } else {
+        // r = b - A x (warm start; the reference always starts from 0, AMGSolver.cu:1407)
+        launch_apply(d, X, coef(d, s->c0), AP, nullptr, nullptr, st);
+        launch_axpy(d, d.leafs, R, AP, -1.0f, st);
+    }
+    // rhs2 = r.r, absolute threshold, done if zero (AMGSolver.cu:1410-1425)
+    launch_dot(d.leafs, R, R, s->partial, nullptr, st);
+    launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, st);
+    double h[S_COUNT];
+    read_scalars(s, h);
+    const double rhs2 = h[S_TMP];
+    *iters = 0;
+    *rel_res = 0;
+    if (rhs2 == 0) return 1;
+    if (!std::isfinite(rhs2)) return 3;
+    double thr2 = tol * tol * rhs2;
+    thr2 = std::max(thr2, 2.2250738585072014e-308);
+    double init[S_COUNT] = {};
+    init[S_THR2] = thr2;
+    init[S_MAXIT] = max_iters;
+    init[S_RHS2] = rhs2;
+    HIPCHK(hipMemcpyAsync(s->S, init, sizeof(init), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (o->is_pure_neumann) enqueue_recenter(s, R);
+    enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, s->S);
+    launch_copy_list(d.leafs, Z, P, s->S, st);
+    launch_dot(d.leafs, Z, R, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_GAMMA0, st);
+    if (max_iters <= 0) {
+        read_scalars(s, h);
+        *rel_res = 1.0;
+        return 4;
+    }
+    // capture (or reuse) the iteration graph
+    if (o->use_graph) {
+        const std::string key = graph_key(o);
+        if (!s->gexec || s->gkey != key) {
+            if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue_iteration(s, o);
+            HIPCHK(hipStreamEndCapture(st, &graph));
+            HIPCHK(hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            s->gkey = key;
+        }
+    }
+    const int stride = std::max(1, o->sync_stride);
+    int launched = 0;
+    for (;;) {
+        const int k = std::min(stride, max_iters - launched);
+        for (int i = 0; i < k; ++i) {
+            if (o->use_graph) HIPCHK(hipGraphLaunch(s->gexec, st));
+            else enqueue_iteration(s, o);
+        }
+        launched += k;
+        read_scalars(s, h);
+        if (h[S_DONE] != 0 || launched >= max_iters) break;
+    }
+    *iters = int(h[S_ITER]);
+    *rel_res = std::sqrt(h[S_RR] / rhs2);
+    const int done = int(h[S_DONE]);
+    if (done == 1) return 1;
+    if (done == 2 || done == 3) return done;
+    return 4;
+}
+
+cmg_status status_of(int done)
+{
+    switch (done) {
+    case 1: return CMG_OK;
+    case 2: return CMG_ERR_BREAKDOWN;
+    case 3: return CMG_ERR_NONFINITE;
+    default: return CMG_ERR_NOT_CONVERGED;
+    }
+}
+
+// squared FP64 residual norm ||b - A x||^2 (x from channel 0 when !from_x64)
+double residual_norm2(cmg_solver* s, bool from_x64)
+{
+    DevGrid& d = s->g->dev;
+    hipStream_t st = d.stream;
+    if (!from_x64) launch_to_double(d, d.leafs, dev_channel(d, CH_X), s->x64, true, st);
+    launch_residual64(d, s->x64, coef(d, s->c0), s->b64, s->r64, s->tmp, s->partial, st);
+    launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, st);
+    double h[S_COUNT];
+    read_scalars(s, h);
+    return h[S_TMP];
+}
+
+double b_norm2(cmg_solver* s)
+{
+    DevGrid& d = s->g->dev;
+    launch_dot64(d.leafs, s->b64, s->b64, s->partial, d.stream);
+    launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, d.stream);
+    double h[S_COUNT];
+    read_scalars(s, h);
+    return h[S_TMP];
+}
+
+}  // namespace
+
+cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_solver** out)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!out || c0 < 0 || c0 + 3 >= kChannels) return CMG_ERR_BAD_ARG;
+    auto* s = new cmg_solver;
+    s->g = g;
+    s->c0 = c0;
+    s->Rm = Rm;
+    s->Rr = Rr;
+    s->P = P;
+    try {
+        s->S = dmalloc<double>(S_COUNT);
+        HIPCHK(hipMemset(s->S, 0, S_COUNT * sizeof(double)));
+        HIPCHK(hipHostMalloc((void**)&s->hS, S_COUNT * sizeof(double), hipHostMallocDefault));
+        s->partial = dmalloc<double>(size_t(g->dev.max_list) + 1);
+        s->partial2 = dmalloc<double>(size_t(g->dev.max_list) + 1);
+        HIPCHK(hipEventCreate(&s->e0));
+        HIPCHK(hipEventCreate(&s->e1));
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        solver_destroy(s);
+        return CMG_ERR_HIP;
+    }
+    *out = s;
+    return CMG_OK;
+}
+
+void solver_destroy(cmg_solver* s)
+{
+    if (!s) return;
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->x64, (void*)s->b64,
+                    (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
+        if (p) (void)hipFree(p);
+    if (s->hS) (void)hipHostFree(s->hS);
+    if (s->e0) (void)hipEventDestroy(s->e0);
+    if (s->e1) (void)hipEventDestroy(s->e1);
+    delete s;
+}
+
+static cmg_status check_opts(const cmg_solve_opts* o)
+{
+    if (!o || o->max_iters < 0 || !(o->relative_tolerance >= 0) || o->level_iters < 0 || o->coarsest_iters < 0)
+        return CMG_ERR_BAD_ARG;
+    if (o->cycle != CMG_CYCLE_FAS_W && o->cycle != CMG_CYCLE_FAS_V) return CMG_ERR_BAD_ARG;
+    if (o->smoother != CMG_SMOOTH_RBGS && o->smoother != CMG_SMOOTH_JACOBI) return CMG_ERR_BAD_ARG;
+    if (o->precision != CMG_PREC_FP32 && o->precision != CMG_PREC_FP64_REFINE) return CMG_ERR_BAD_ARG;
+    return CMG_OK;
+}
+
+cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0)
+{
+    if (!s) return CMG_ERR_BAD_ARG;
+    if (auto st = check_opts(o)) return st;
+    if (!valid_channel(x) || !valid_channel(rhs) || !valid_channel(x0)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        ensure_buffers(s);
+        if (o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
+            build_direct(s, o->is_pure_neumann);
+        enqueue_cycle(s, o, x, rhs, x0, nullptr);
+        HIPCHK(hipStreamSynchronize(s->g->dev.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
+{
+    if (!s || !st) return CMG_ERR_BAD_ARG;
+    if (auto e = check_opts(o)) return e;
+    *st = cmg_solve_stats{};
+    int done = 1;
+    CMG_TRY({
+        DevGrid& d = s->g->dev;
+        hipStream_t stream = d.stream;
+        ensure_buffers(s);
+        ensure_fp64(s);
+        if (o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
+            build_direct(s, o->is_pure_neumann);
+        // keep b in FP64 for the true residual / refinement
+        launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipEventRecord(s->e0, stream));
+        if (o->precision == CMG_PREC_FP32) {
+            int iters = 0;
+            double rel = 0;
+            done = pcg32(s, o, o->relative_tolerance, o->max_iters, o->warm_start != 0, &iters, &rel);
+            st->iters = iters;
+            st->rel_res = rel;
+            HIPCHK(hipEventRecord(s->e1, stream));
+            HIPCHK(hipEventSynchronize(s->e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
+            st->ms = ms;
+            // true residual from the FP32 solution (outside the timed region)
+            const double rn2 = residual_norm2(s, false);
+            const double bn2 = b_norm2(s);
+            st->true_rel_res = bn2 > 0 ? std::sqrt(rn2 / bn2) : 0.0;
+        } else {
+            // FP64 iterative refinement around FP32 PCG
+            const size_t n = size_t(d.N) * kCells;
+            if (o->warm_start) launch_to_double(d, d.leafs, dev_channel(d, CH_X), s->x64, true, stream);
+            else HIPCHK(hipMemsetAsync(s->x64, 0, n * sizeof(double), stream));
+            const double bn2 = b_norm2(s);
+            double h[S_COUNT];
+            double rn2 = residual_norm2(s, true);
+            // the inner solves read their right-hand side from channel 1
+            launch_to_float(d.leafs, s->r64, dev_channel(d, CH_R), stream);
+            int total = 0, outer = 0;
+            done = 1;
+            const double otol = o->outer_tolerance > 0 ? o->outer_tolerance : o->relative_tolerance;
+            const double itol = o->inner_tolerance > 0 ? o->inner_tolerance : 1e-4;
+            while (bn2 > 0 && std::sqrt(rn2 / bn2) >= otol && outer < 50 && total < o->max_iters) {
+                int iters = 0;
+                double rel = 0;
+                int dd = pcg32(s, o, itol, o->max_iters - total, false, &iters, &rel);
+                total += iters;
+                ++outer;
+                if (dd == 2 || dd == 3) { done = dd; break; }
+                launch_update_x64(d, s->x64, dev_channel(d, CH_X), stream);
+                launch_residual64(d, s->x64, coef(d, s->c0), s->b64, s->r64, dev_channel(d, CH_R), s->partial, stream);
+                launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, stream);
+                read_scalars(s, h);
+                rn2 = h[S_TMP];
+                (void)n;
+                if (!std::isfinite(rn2)) { done = 3; break; }
+            }
+            if (done == 1 && bn2 > 0 && std::sqrt(rn2 / bn2) >= otol) done = 4;
+            launch_to_float(d.leafs, s->x64, dev_channel(d, CH_X), stream);
+            HIPCHK(hipEventRecord(s->e1, stream));
+            HIPCHK(hipEventSynchronize(s->e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
+            st->ms = ms;
+            st->iters = total;
+            st->outer_iters = outer;
+            st->rel_res = bn2 > 0 ? std::sqrt(rn2 / bn2) : 0.0;
+            st->true_rel_res = st->rel_res;
+        }
+    });
+    return status_of(done);
+}
+
+cmg_status op_copy(cmg_grid* g, int src, int dst)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(src) || !valid_channel(dst)) return CMG_ERR_BAD_ARG;
+    if (src == dst) return CMG_OK;
+    CMG_TRY({
+        auto& d = g->dev;
+        HIPCHK(hipMemcpyAsync(dev_channel(d, dst), dev_channel(d, src), size_t(d.N) * kCells * sizeof(float),
+                              hipMemcpyDeviceToDevice, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
+{
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 4) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        ensure_buffers(s);
+        DevGrid& d = s->g->dev;
+        hipStream_t st = d.stream;
+        const int L = d.max_level;
+        float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *Pp = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
+              *Z = dev_channel(d, CH_Z);
+        Coef C = coef(d, s->c0);
+        int nt = 0;
+        auto one = [&]() {
+            switch (kernel) {
+            case 0: launch_apply(d, Pp, C, AP, s->partial, nullptr, st); nt = d.leafs.tiles; break;
+            case 1: launch_jacobi(d, L, Z, s->tmp, Z, C, R, 2.0f / 3.0f, nullptr, st); nt = d.subtree[L].tiles; break;
+            case 2: launch_residual_restrict(d, L, Z, C, AP, 1.0f, nullptr, st); nt = d.subtree[L].tiles; break;
+            case 3: launch_update_xr(d, X, Pp, Z, AP, s->partial, nullptr, st); nt = d.leafs.tiles; break;
+            case 4: launch_fas_prolong(d, L, s->tmp, AP, 1.0f, nullptr, st); nt = d.prolong[L].tiles; break;
+            }
+        };
+        one();
+        HIPCHK(hipEventRecord(s->e0, st));
+        for (int i = 0; i < reps; ++i) one();
+        HIPCHK(hipEventRecord(s->e1, st));
+        HIPCHK(hipEventSynchronize(s->e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
+        *avg_ms = double(ms) / reps;
+        if (tiles) *tiles = nt;
+    });
+    return CMG_OK;
+}
+
+cmg_status solver_get_x64(cmg_solver* s, double* host)
+{
+    if (!s || !host) return CMG_ERR_BAD_ARG;
+    if (!s->x64) return CMG_ERR_STATE;
+    CMG_TRY({
+        HIPCHK(hipMemcpy(host, s->x64, size_t(s->g->dev.N) * kCells * sizeof(double), hipMemcpyDeviceToHost));
+    });
+    return CMG_OK;
+}
+
+}  // namespace cmg

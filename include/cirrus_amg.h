@@ -1,0 +1,230 @@
+/*
+ * cirrus_amg.h -- C ABI of the MI355X-native octree multigrid Poisson solve.
+ *
+ * Drop-in boundary for the pressure-projection solve of wang-mengdi/Cirrus-amg:
+ * the grid construction of HADeviceGrid (src/HAGrid.h), the operator assembly
+ * of AMGSolver / the cut-cell builder (src/AMGSolver.cu, tests/SolverTests.cu,
+ * cirrus_cutcell/MeshCutCell.cu), the matrix-free kernels, and the
+ * AMGSolver::solve(grid, ...) entry point (src/AMGSolver.h:62) that
+ * FluidEuler::project calls (cirrus_cutcell/FluidEuler.cu:702-706).
+ *
+ * Plain C types only.  All per-cell arrays are tile-major in the grid's
+ * all-tiles order (levels 0..L concatenated, each level in hash-slot order,
+ * exactly HADeviceGrid::dAllTiles), 512 cells per tile, local cell offset
+ * (i<<6)|(j<<3)|k with i along x (src/HAAccessor.h:32-42).
+ *
+ * Channels mirror PoissonTile<float,15> (src/PoissonTile.h:13-18,129):
+ * 0 = x, 1 = b / r, 2 = p, 3 = Ap (FAS x0), 4 = z, and the operator occupies
+ * four consecutive channels c0..c0+3 (3 negative-face coefficients, diagonal).
+ *
+ * Threading: a grid handle and its solvers are used from one host thread; all
+ * device work of a handle runs on the handle's own HIP stream.
+ */
+#ifndef CIRRUS_AMG_H
+#define CIRRUS_AMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cmg_status {
+    CMG_OK = 0,
+    CMG_ERR_BAD_ARG = 1,        /* invalid argument / channel / level */
+    CMG_ERR_STATE = 2,          /* call out of order (e.g. solve before assembly) */
+    CMG_ERR_HIP = 3,            /* a HIP runtime call failed */
+    CMG_ERR_NONFINITE = 4,      /* NaN/Inf met in a reduction */
+    CMG_ERR_BREAKDOWN = 5,      /* CG breakdown (p.Ap == 0) */
+    CMG_ERR_NOT_CONVERGED = 6,  /* max_iters reached above tolerance */
+    CMG_ERR_HASH_FULL = 7       /* a level's hash table is full */
+} cmg_status;
+
+enum { CMG_LEAF = 1, CMG_NONLEAF = 2, CMG_GHOST = 4 };            /* src/HATileInfo.h:10 */
+enum { CMG_DIRICHLET = 1, CMG_INTERIOR = 2, CMG_NEUMANN = 4 };    /* src/PoissonTile.h:7 */
+
+typedef struct cmg_grid cmg_grid;
+typedef struct cmg_solver cmg_solver;
+
+/* ---------------------------------------------------------------- grid ---- */
+
+/* HADeviceGrid(h0, levels_log2_hash) (src/HAGrid.h:374-399) */
+cmg_status cmg_grid_create(float h0, int num_levels, const uint32_t* log2_hash, cmg_grid** out);
+/* ~HADeviceGrid (src/HAGrid.h:453-465) */
+void cmg_grid_destroy(cmg_grid* g);
+/* setTileHost(level, b_ijk, Tile(), type) (src/HAGrid.h:526-548) */
+cmg_status cmg_grid_set_tile(cmg_grid* g, int level, int i, int j, int k, uint8_t type);
+/* compressHost + spawnGhostTiles, i.e. rebuild() (src/HAGrid.h:634-653,856-897) */
+cmg_status cmg_grid_rebuild(cmg_grid* g);
+
+/* level-target presets for iterativeRefine (tests/TestGrids.cuh:33-142 and the
+ * narrow-band SDF targets used by the benchmark configurations) */
+typedef enum cmg_target_kind {
+    CMG_TARGET_UNIFORM = 0, CMG_TARGET_CENTER_POINT = 1, CMG_TARGET_SPHERE_EMPTY = 2,
+    CMG_TARGET_SPHERE_SOLID = 3, CMG_TARGET_STAR_EMPTY = 4, CMG_TARGET_STAR_SOLID = 5,
+    CMG_TARGET_SDF_BANDS = 6, CMG_TARGET_CALLBACK = 7
+} cmg_target_kind;
+enum { CMG_SDF_SPHERE = 0, CMG_SDF_TORUS = 1, CMG_SDF_STAR = 2 };
+typedef int (*cmg_level_target_fn)(void* user, int level, const int32_t coord[3]);
+typedef struct cmg_target {
+    int kind;                 /* cmg_target_kind */
+    int min_level, max_level;
+    int sdf_kind;             /* CMG_SDF_*; sphere (cx,cy,cz,r), torus (cx,cy,cz,R,r) */
+    float sdf[8];
+    float band[12];           /* SDF_BANDS: tile goes to level l when |d|lower bound < band[l] */
+    cmg_level_target_fn fn;   /* CALLBACK: evaluated on each LEAF tile */
+    void* user;
+} cmg_target;
+
+/* iterativeRefine(level_target) (src/HAGrid.h:1173-1182; refineStep :1088-1171) */
+cmg_status cmg_grid_iterative_refine(cmg_grid* g, const cmg_target* target);
+/* freeze the topology: all-tiles arrays, CalculateNeighborTiles
+ * (src/PoissonGrid.cu:139-152), parent/child tables, device upload */
+cmg_status cmg_grid_finalize(cmg_grid* g);
+
+int cmg_grid_num_tiles(const cmg_grid* g);
+int cmg_grid_max_level(const cmg_grid* g);
+/* hNumTiles prefix sums: starts[l] .. starts[l+1] are the tiles of level l */
+cmg_status cmg_grid_level_starts(const cmg_grid* g, int32_t* starts /* num_levels+1 */);
+/* HATileInfo per tile in all-tiles order: level, coord (3), type */
+cmg_status cmg_grid_get_tiles(const cmg_grid* g, int32_t* level, int32_t* coord, uint8_t* type);
+/* hash slot of each tile (defines the order; src/HAGrid.h:638-642) */
+cmg_status cmg_grid_get_slots(const cmg_grid* g, int32_t* slot);
+/* PoissonTile::mNeighbors as tile indices, order x-,y-,z-,x+,y+,z+, -1 = empty */
+cmg_status cmg_grid_get_neighbors(const cmg_grid* g, int32_t* nbr);
+cmg_status cmg_grid_get_parents(const cmg_grid* g, int32_t* parent);
+cmg_status cmg_grid_get_children(const cmg_grid* g, int32_t* child /* 8 per tile */);
+
+/* ------------------------------------------------------- cell data --------- */
+
+/* PoissonTile::mCellType of every tile (N*512 bytes); the host copy is authoritative
+ * and mirrored to the device */
+cmg_status cmg_grid_set_cell_types(cmg_grid* g, const uint8_t* types);
+cmg_status cmg_grid_get_cell_types(cmg_grid* g, uint8_t* types);
+/* host-side result of the last assembly for a channel (no device needed) */
+cmg_status cmg_grid_get_host_channel(cmg_grid* g, int channel, float* host);
+/* GridCase::type on LEAF tiles (tests/TestGrids.cuh:161-167) */
+cmg_status cmg_grid_set_leaf_types_preset(cmg_grid* g, const cmg_target* target);
+/* boundary typing of tests/SolverTests.cu:1880-1897 */
+cmg_status cmg_grid_set_wall_types(cmg_grid* g, int chk_level, int dirichlet_ytop);
+/* one float channel of every tile (N*512 floats), host <-> device */
+cmg_status cmg_grid_set_channel(cmg_grid* g, int channel, const float* host);
+cmg_status cmg_grid_get_channel(cmg_grid* g, int channel, float* host);
+cmg_status cmg_grid_fill_channel(cmg_grid* g, int channel, float value);
+/* device-side copy of a whole channel (e.g. to restore a right-hand side) */
+cmg_status cmg_grid_copy_channel(cmg_grid* g, int src, int dst);
+
+/* ------------------------------------------------------- assembly ---------- */
+
+/* AMGSolver::prepareTypesAndCoeffs -> CoarsenTypesAndAMGCoeffs (src/AMGSolver.cu:256-318,1394-1397) */
+cmg_status cmg_prepare_types_and_coeffs(cmg_grid* g, int coeff_channel, float R_matrix);
+/* CreateLaplacianSystemWithSolidCut with an analytic SDF (tests/SolverTests.cu:1595-1714);
+ * cells with all 8 corners inside the SDF become NEUMANN; face_rule -1 = open faces
+ * (CoarsenTypesAndAMGCoeffs), 0 = corner FaceFluidRatio (cirrus_cutcell/MeshCutCell.cu:305-408),
+ * 1 = 4x4 supersampled open fraction (same 0.1 floor) */
+cmg_status cmg_assemble_cutcell(cmg_grid* g, int sdf_kind, const float* sdf, int face_rule,
+                                int coeff_channel, float R_matrix);
+
+/* ------------------------------------------------------- inputs ------------ */
+
+/* seeded right-hand sides on LEAF INTERIOR cells (SplitMix64 streams in all-tiles
+ * order): i.i.d. U[-1,1) with the mean removed, or the volume-weighted divergence of
+ * i.i.d. N(0,1) negative-face velocities written to channels u0..u0+2
+ * (AMGVolumeWeightedDivergenceWithoutCoeffOnLeafs, src/AMGSolver.cu:750-788).
+ * No reference counterpart for the generators themselves. */
+cmg_status cmg_rhs_uniform(cmg_grid* g, int channel, uint64_t seed, int zero_mean);
+cmg_status cmg_rhs_divergence(cmg_grid* g, int channel, int u0, uint64_t seed);
+
+/* ------------------------------------------------------- solver ------------ */
+
+typedef enum { CMG_CYCLE_FAS_W = 0, CMG_CYCLE_FAS_V = 1 } cmg_cycle_kind;
+typedef enum { CMG_SMOOTH_RBGS = 0, CMG_SMOOTH_JACOBI = 1 } cmg_smoother_kind;
+typedef enum { CMG_PREC_FP32 = 0, CMG_PREC_FP64_REFINE = 1 } cmg_precision;
+
+typedef struct cmg_solve_opts {
+    /* the eight parameters of AMGSolver::solve (src/AMGSolver.h:62) */
+    int verbose;
+    int max_iters;
+    double relative_tolerance;
+    int level_iters;          /* smoothing sweeps (RBGS iterations) per level visit */
+    int coarsest_iters;       /* RBGS: iterations split 1/2 + 1/2 ; Jacobi: sweeps */
+    int sync_stride;          /* convergence checked every sync_stride iterations on the device */
+    int is_pure_neumann;
+    /* extensions */
+    int cycle;                /* cmg_cycle_kind: FAS_W = reference (mu = 2) */
+    int smoother;             /* cmg_smoother_kind */
+    double omega;             /* RBGS 1.5 (reference) ; Jacobi 2/3 */
+    int coarsest_direct;      /* 1: FP64 dense solve of the level-0 system */
+    int precision;            /* cmg_precision */
+    double outer_tolerance;   /* FP64 refinement: stop at this true relative residual */
+    double inner_tolerance;   /* FP64 refinement: relative tolerance of each inner FP32 solve */
+    int warm_start;           /* 1: start from channel x (FP32) instead of 0 */
+    int use_graph;            /* 1: capture one PCG iteration in a hipGraph */
+} cmg_solve_opts;
+
+typedef struct cmg_solve_stats {
+    int iters;                /* PCG iterations (summed over refinement steps) */
+    int outer_iters;          /* FP64 refinement steps (0 in FP32 mode) */
+    double rel_res;           /* recursive relative residual, as AMGSolver::solve returns */
+    double true_rel_res;      /* ||b - A x|| / ||b|| recomputed in FP64 on the device */
+    double ms;                /* device time of the solve (hipEvents on the handle's stream) */
+} cmg_solve_stats;
+
+void cmg_solve_opts_default(cmg_solve_opts* o);   /* FluidEuler::project settings */
+
+/* AMGSolver(coeff_channel, R_matrix, R_restrict, P) (src/AMGSolver.h:32-42) */
+cmg_status cmg_solver_create(cmg_grid* g, int coeff_channel, float R_matrix, float R_restrict,
+                             float P, cmg_solver** out);
+void cmg_solver_destroy(cmg_solver* s);
+/* AMGSolver::solve (src/AMGSolver.cu:1399-1549): solves -lap(x) = b with b in
+ * channel 1 on LEAF INTERIOR cells; x lands in channel 0; channels 1-4 of
+ * every tile are clobbered (1 ends as the residual). */
+cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats);
+/* the FP64 solution of the last FP64-refinement solve (N*512 doubles) */
+cmg_status cmg_solver_get_x64(cmg_solver* s, double* host);
+
+/* ------------------------------------------------------- kernels ----------- */
+/* Individual device operations on channels; each restates one reference function. */
+
+/* AMGFullNegativeLaplacianOnLeafs (src/AMGSolver.cu:589-632); only y on LEAF tiles is defined */
+cmg_status cmg_op_full_neg_lap(cmg_grid* g, int x, int coeff_channel, int y);
+/* PropagateToChildren(GHOST, subtree) (src/PoissonGrid.cu:377-405) */
+cmg_status cmg_op_propagate_ghosts(cmg_grid* g, int channel);
+/* AccumulateToParentsOneStep(LEAF, 1/8) (src/PoissonGrid.cu:408-465) */
+cmg_status cmg_op_accumulate_leafs(cmg_grid* g, int channel);
+/* one colour of NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel (src/AMGSolver.cu:838-885) */
+cmg_status cmg_op_gs_color(cmg_grid* g, int level, int x, int coeff_channel, int rhs, int color, double omega);
+/* damped-Jacobi sweep on the same tiles and halo (no reference counterpart; GMGSolver::dampedJacobiSolve
+ * src/GMGSolver.cu:390-470 is the closest) */
+cmg_status cmg_op_jacobi(cmg_grid* g, int level, int x_in, int x_out, int ghost_channel, int coeff_channel,
+                         int rhs, float omega);
+/* ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964) */
+cmg_status cmg_op_residual_restrict(cmg_grid* g, int level, int x, int coeff_channel, int rhs, float R_restrict);
+/* FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) */
+cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int coeff_channel, int rhs, int x0);
+/* FASProlongateAndUpdateAMG128Kernel (src/AMGSolver.cu:1216-1255) */
+cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P);
+/* Dot over LEAF tiles (src/PoissonGrid.cu:156-214), FP64 */
+cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out);
+/* FASMuCycle (src/AMGSolver.cu:1279-1292) with the solver's cycle options */
+cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0);
+
+/* ------------------------------------------------------- measurement ------- */
+/* Launches one kernel `reps` times back to back on the grid's stream between two
+ * hipEvents and returns the average duration in ms, plus the number of tiles one
+ * launch processes.  Kernels: 0 = operator on LEAF tiles (+ p.Ap partials),
+ * 1 = damped-Jacobi sweep on the finest level, 2 = residual + restriction on the
+ * finest level, 3 = PCG x/r update (+ r.r partials), 4 = FAS prolongation on the
+ * finest level.  No reference counterpart (the reference times whole solves
+ * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
+cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
+
+/* ------------------------------------------------------- misc -------------- */
+const char* cmg_status_string(cmg_status st);
+/* last error message of this thread */
+const char* cmg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

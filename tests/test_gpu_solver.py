@@ -1,0 +1,173 @@
+"""Multigrid cycles and PCG solves on the GPU against the oracle and the FP64 CSR oracle."""
+import numpy as np
+import pytest
+
+import cirrus_amg as P
+import helpers as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(go, gp, seed):
+    go.rhs_uniform(1, seed, True)
+    gp.set_channel(1, go.channel(1))
+    return go.channel(1).copy()
+
+
+def _opts(kind):
+    if kind == "ref_w":      # FluidEuler::project settings (cirrus_cutcell/FluidEuler.cu:702-706)
+        return P.default_opts(), O.make_cycle(mu=2, smoother=0, omega=1.5, level_iters=2, coarsest_iters=10)
+    if kind == "v_jacobi_gs":  # V(2,2) damped Jacobi, RB-GS coarsest (deterministic end to end)
+        return (P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_iters=8),
+                O.make_cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_iters=8))
+    if kind == "v_jacobi_direct":
+        return (P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1),
+                O.make_cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_direct=1))
+    raise KeyError(kind)
+
+
+@pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c3_small"])
+@pytest.mark.parametrize("kind", ["ref_w", "v_jacobi_gs"])
+def test_cycle_bitexact(case, kind):
+    print("\ncycle", case, kind, flush=True)
+    go, gp = H.build_pair(case)
+    _rhs(go, gp, 5)
+    popts, ocyc = _opts(kind)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    go.fas_cycle(ocyc, 4, 1, 3, H.C0)
+    solver.cycle(popts, 4, 1, 3)
+    zo, zp = go.channel(4), gp.channel(4)
+    assert np.isfinite(zp).all()
+    assert np.array_equal(H.bits(zo), H.bits(zp)), (case, kind, np.abs(zo - zp).max())
+
+
+def test_cycle_direct_coarsest_close():
+    go, gp = H.build_pair("uniform3")
+    _rhs(go, gp, 6)
+    popts, ocyc = _opts("v_jacobi_direct")
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    go.fas_cycle(ocyc, 4, 1, 3, H.C0)
+    solver.cycle(popts, 4, 1, 3)
+    zo, zp = go.channel(4), gp.channel(4)
+    assert np.linalg.norm(zo - zp) <= 1e-5 * np.linalg.norm(zo)
+
+
+def _fp64_solution(go):
+    A = go.assemble_csr(H.C0)
+    b = go.channel(1).ravel()[A.cell].astype(np.float64)
+    x, it, rr = A.bicgstab(b, 1e-13, 20000)
+    assert rr < 1e-11
+    return A, b, x
+
+
+@pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c2_small", "c3_small"])
+@pytest.mark.parametrize("kind", ["ref_w", "v_jacobi_direct"])
+def test_pcg_matches_fp64_oracle(case, kind):
+    print("\npcg", case, kind, flush=True)
+    go, gp = H.build_pair(case)
+    _rhs(go, gp, 7)
+    A, b, x64 = _fp64_solution(go)
+    popts, ocyc = _opts(kind)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, level_iters=2,
+                           coarsest_iters=popts.coarsest_iters, opts=popts)
+    st = solver.last_stats
+    x = gp.channel(0).ravel()[A.cell].astype(np.float64)
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    # the oracle's own FP32 PCG for the iteration count
+    oit, orel = go.pcg(ocyc, H.C0, 100, 1e-6)
+    print(case, kind, "gpu iters", it, rel, "true", st.true_rel_res, "oracle iters", oit, "err", err, "ms", st.ms)
+    assert rel < 1e-6 and st.true_rel_res < 5e-6
+    assert abs(it - oit) <= 1
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("case", ["uniform3", "c3_small"])
+def test_fp64_refinement(case):
+    print("\nrefine", case, flush=True)
+    go, gp = H.build_pair(case)
+    _rhs(go, gp, 8)
+    A, b, x64 = _fp64_solution(go)
+    popts, _ = _opts("v_jacobi_direct")
+    popts.precision = P.PREC_FP64_REFINE
+    popts.outer_tolerance = 1e-10
+    popts.inner_tolerance = 1e-4
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.solve(max_iters=200, relative_tolerance=1e-10, opts=popts)
+    st = solver.last_stats
+    x = solver.x64().ravel()[A.cell]
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    res = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    print(case, "outer", st.outer_iters, "inner", it, "rel", rel, "csr res", res, "err", err)
+    assert rel < 1e-10 and res < 1e-9
+    assert err < 1e-8
+
+
+def test_uniform_dst_closed_form():
+    """Config-1 operator is h * (7-point Dirichlet Laplacian): DST-I gives the exact solution."""
+    from scipy.fft import dstn, idstn
+    go, gp = H.build_pair("uniform3")
+    b = _rhs(go, gp, 0)
+    lv, co, ty = go.tiles()
+    n, h = 64, go.hval(3)
+    leaf = np.where(ty == 1)[0]
+    v = np.arange(512)
+    gi = (co[leaf, 0, None] * 8 + (v >> 6)).ravel()
+    gj = (co[leaf, 1, None] * 8 + ((v >> 3) & 7)).ravel()
+    gk = (co[leaf, 2, None] * 8 + (v & 7)).ravel()
+    bb = np.zeros((n, n, n)); bb[gi, gj, gk] = b[leaf].ravel()
+    k = np.arange(1, n + 1); lam1 = 2 - 2 * np.cos(np.pi * k / (n + 1))
+    lam = h * (lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :])
+    xe = idstn(dstn(bb, type=1) / lam, type=1)
+    popts, _ = _opts("v_jacobi_direct")
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.solve(relative_tolerance=1e-6, opts=popts)
+    x = gp.channel(0)[leaf].ravel()
+    err = np.linalg.norm(x - xe[gi, gj, gk]) / np.linalg.norm(xe)
+    print("dst iters", it, "err", err)
+    assert err < 1e-5
+
+
+def test_warm_start_and_zero_rhs():
+    go, gp = H.build_pair("sphere_empty24")
+    b = _rhs(go, gp, 9)
+    popts, _ = _opts("v_jacobi_direct")
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it0, _ = solver.solve(relative_tolerance=1e-6, opts=popts)
+    x0 = gp.channel(0)
+    # a slightly different right-hand side from the previous solution
+    gp.set_channel(1, b * np.float32(1.001))
+    popts.warm_start = 1
+    it1, rel1 = solver.solve(relative_tolerance=1e-6, opts=popts)
+    assert it1 < it0 and rel1 < 1e-6
+    # zero right-hand side: zero iterations, x = 0 (AMGSolver.cu:1415-1420)
+    gp.set_channel(1, np.zeros_like(b))
+    popts.warm_start = 0
+    it2, rel2 = solver.solve(relative_tolerance=1e-6, opts=popts)
+    assert it2 == 0 and rel2 == 0 and not gp.channel(0).any()
+    del x0
+
+
+def test_pure_neumann():
+    """All-Neumann box (tests/SolverTests.cu:789-805): singular operator, recentred residual."""
+    go, gp = H.build_pair("uniform2")
+    types = go.cell_types().copy()
+    lv, co, ty = go.tiles()
+    for t in np.where(ty == 1)[0]:
+        for v in range(512):
+            g = co[t] * 8 + np.array([v >> 6, (v >> 3) & 7, v & 7])
+            if (g == 0).any() or (g == 31).any():
+                types[t, v] = 4
+    go2 = O.Grid(O.make_target(O.TARGET_UNIFORM, 2, 2))
+    go2.cell_types()[:] = types
+    go2.coarsen_types_and_coeffs(H.C0, H.R_MATRIX)
+    gp.set_cell_types(types)
+    gp.prepare_types_and_coeffs(H.C0, H.R_MATRIX)
+    go2.rhs_uniform(1, 3, True)
+    gp.set_channel(1, go2.channel(1))
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.solve(relative_tolerance=1e-6, is_pure_neumann=True)
+    oit, orel = go2.pcg(O.make_cycle(), H.C0, 100, 1e-6, True)
+    print("pure neumann gpu", it, rel, "oracle", oit, orel)
+    assert rel < 1e-6 and abs(it - oit) <= 1
