@@ -140,7 +140,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import cirrus_amg as P
 
     cfg = CONFIGS[args.config]
@@ -163,7 +162,7 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    P.device_synchronize()
     t0 = time.perf_counter()
     iters, rels, dev_ms = [], [], []
     for _ in range(args.steps):
@@ -171,7 +170,7 @@ def main():
         iters.append(it)
         rels.append(rel)
         dev_ms.append(solver.last_stats.ms)
-    torch.cuda.synchronize()
+    P.device_synchronize()
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     true_rel = solver.last_stats.true_rel_res
 

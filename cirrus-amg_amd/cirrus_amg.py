@@ -73,7 +73,7 @@ EXPORTS = [
     "cmg_op_propagate_ghosts", "cmg_op_accumulate_leafs", "cmg_op_gs_color", "cmg_op_jacobi",
     "cmg_op_residual_restrict", "cmg_op_fas_save", "cmg_op_fas_prolong", "cmg_op_dot", "cmg_op_cycle",
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
-    "cmg_rhs_uniform", "cmg_rhs_divergence",
+    "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize",
 ]
 
 _lib = None
@@ -113,6 +113,7 @@ def lib():
             "cmg_op_cycle": (st, [vp, C.POINTER(SolveOpts), i, i, i]),
             "cmg_status_string": (C.c_char_p, [st]), "cmg_last_error": (C.c_char_p, []),
             "cmg_grid_copy_channel": (st, [vp, i, i]),
+            "cmg_device_synchronize": (st, []),
             "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
             "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
         }
@@ -309,6 +310,10 @@ class HADeviceGrid:
         out = C.c_double()
         _check(lib().cmg_op_dot(self.h, a, b, C.byref(out)), "dot")
         return out.value
+
+
+def device_synchronize():
+    _check(lib().cmg_device_synchronize(), "cmg_device_synchronize")
 
 
 def default_opts(**kw):

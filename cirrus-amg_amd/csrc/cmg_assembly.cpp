@@ -372,6 +372,40 @@ void assemble(const Topology& topo, HostCells& cells, int c0, float R, int face_
             }
         }
     }
+    // (c) closed cells: a LEAF INTERIOR cell with all six faces closed has a zero
+    // diagonal (the reference asserts diag > 0, AMGSolver.cu:151); it is
+    // disconnected from the fluid and becomes NEUMANN.  Ghost types follow their
+    // parents and NONLEAF types are re-derived; coefficients do not change.
+    int closed = 0;
+    for (int t = 0; t < N; ++t) {
+        if (!(topo.t_type[t] & LEAF)) continue;
+        for (int v = 0; v < kCells; ++v) {
+            const size_t idx = size_t(t) * kCells + v;
+            if (T[idx] == INTERIOR && C[3][idx] == 0.0f) { T[idx] = NEUMANN; ++closed; }
+        }
+    }
+    if (!closed) return;
+    for (int t = 0; t < N; ++t) {
+        if (!(topo.t_type[t] & GHOST)) continue;
+        for (int v = 0; v < kCells; ++v) {
+            Vox p = parent_voxel(topo, t, v);
+            T[size_t(t) * kCells + v] = p.t >= 0 ? T[size_t(p.t) * kCells + p.v] : DIRICHLET;
+        }
+    }
+    for (int lvl = topo.num_levels() - 1; lvl >= 0; --lvl)
+        for (int t = topo.level_start[lvl]; t < topo.level_start[lvl + 1]; ++t) {
+            if (topo.t_type[t] & (LEAF | GHOST)) continue;
+            for (int v = 0; v < kCells; ++v) {
+                int cnt = 0;
+                for (int ci = 0; ci < 2; ++ci)
+                    for (int cj = 0; cj < 2; ++cj)
+                        for (int ck = 0; ck < 2; ++ck) {
+                            Vox cv = child_voxel(topo, t, v, ci, cj, ck);
+                            if (cv.t >= 0 && T[size_t(cv.t) * kCells + cv.v] == INTERIOR) ++cnt;
+                        }
+                T[size_t(t) * kCells + v] = cnt > 0 ? INTERIOR : DIRICHLET;
+            }
+        }
 }
 
 // ---------------------------------------------------------- synthetic inputs ---

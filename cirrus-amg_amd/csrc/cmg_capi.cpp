@@ -425,6 +425,15 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
     return bench_kernel(s, kernel, reps, avg_ms, tiles);
 }
 
+cmg_status cmg_device_synchronize(void)
+{
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_error("hipDeviceSynchronize failed");
+        return CMG_ERR_HIP;
+    }
+    return CMG_OK;
+}
+
 const char* cmg_status_string(cmg_status st)
 {
     switch (st) {

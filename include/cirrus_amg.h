@@ -219,6 +219,9 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 
+/* hipDeviceSynchronize on the current device (timing brackets) */
+cmg_status cmg_device_synchronize(void);
+
 /* ------------------------------------------------------- misc -------------- */
 const char* cmg_status_string(cmg_status st);
 /* last error message of this thread */

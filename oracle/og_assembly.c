@@ -190,7 +190,12 @@ void og_cutcell_coeffs(og_grid* g, int sdf_kind, const float* sdf, int face_rule
  * (b) levels finest..0: LEAF/GHOST INTERIOR diagonal = -sum of face terms
  *     (+ side from the neighbour, -h when it is missing); NONLEAF diagonal =
  *     R * (sum of child diagonals + twice the faces between INTERIOR children),
- *     NONLEAF type = INTERIOR iff any child is INTERIOR. */
+ *     NONLEAF type = INTERIOR iff any child is INTERIOR.
+ * (c) closed cells: a LEAF INTERIOR cell whose six faces are all closed ends with
+ *     a zero diagonal, which the reference rejects (CUDA_ASSERT diag > 0,
+ *     AMGSolver.cu:151).  Such a cell is disconnected from the fluid; it is
+ *     retyped NEUMANN, its ghost children follow, and NONLEAF types are
+ *     re-derived bottom-up.  Coefficients and diagonals do not change. */
 void og_prepare_system(og_grid* g, int c0, float R)
 {
     for (int i = g->max_level; i > 0; i--) {
@@ -273,4 +278,38 @@ void og_prepare_system(og_grid* g, int c0, float R)
             }
         }
     }
+    /* (c) closed cells */
+    int closed = 0;
+    for (int t = 0; t < g->N; t++) {
+        if (!(g->t_type[t] & OG_LEAF)) continue;
+        for (int v = 0; v < 512; v++) {
+            size_t idx = (size_t)t * 512 + v;
+            if (g->ctype[idx] == OG_INTERIOR && g->ch[c0 + 3][idx] == 0.0f) { g->ctype[idx] = OG_NEUMANN; closed++; }
+        }
+    }
+    if (!closed) return;
+    for (int t = 0; t < g->N; t++) {
+        if (!(g->t_type[t] & OG_GHOST)) continue;
+        int l = g->t_level[t];
+        const int32_t* c = &g->t_coord[3 * t];
+        for (int v = 0; v < 512; v++) {
+            int gi = c[0] * 8 + (v >> 6), gj = c[1] * 8 + ((v >> 3) & 7), gk = c[2] * 8 + (v & 7);
+            int off, p = og_voxel(g, l - 1, gi >> 1, gj >> 1, gk >> 1, &off);
+            g->ctype[(size_t)t * 512 + v] = p >= 0 ? g->ctype[(size_t)p * 512 + off] : OG_DIRICHLET;
+        }
+    }
+    for (int lvl = g->num_levels - 1; lvl >= 0; lvl--)
+        for (int t = g->lstart[lvl]; t < g->lstart[lvl + 1]; t++) {
+            if (g->t_type[t] & (OG_LEAF | OG_GHOST)) continue;
+            const int32_t* c = &g->t_coord[3 * t];
+            for (int v = 0; v < 512; v++) {
+                int gi[3] = { c[0] * 8 + (v >> 6), c[1] * 8 + ((v >> 3) & 7), c[2] * 8 + (v & 7) };
+                int cnt = 0;
+                for (int ci = 0; ci < 8; ci++) {
+                    int off, ct = og_voxel(g, lvl + 1, gi[0] * 2 + (ci >> 2), gi[1] * 2 + ((ci >> 1) & 1), gi[2] * 2 + (ci & 1), &off);
+                    if (ct >= 0 && g->ctype[(size_t)ct * 512 + off] == OG_INTERIOR) cnt++;
+                }
+                g->ctype[(size_t)t * 512 + v] = cnt > 0 ? OG_INTERIOR : OG_DIRICHLET;
+            }
+        }
 }

@@ -175,3 +175,20 @@ def test_oracle_fas_is_correction_scheme_on_uniform():
     leaf = g.tiles()[2] == 1
     d = outs[2][leaf] - outs[0][leaf] - outs[1][leaf]
     assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(outs[2][leaf])
+
+
+@pytest.mark.parametrize("name", ["c3_small", "c2_small", "sphere_solid24"])
+def test_no_closed_interior_cells(name):
+    """After assembly every LEAF INTERIOR cell has a positive diagonal and every
+    connected fluid region touches a Dirichlet boundary (the system is regular)."""
+    from scipy.sparse.csgraph import connected_components
+    go, _ = H.build_pair(name)
+    lv, co, ty = go.tiles()
+    T, D = go.cell_types(), go.channel(H.C0 + 3)
+    assert (D[(ty == 1)[:, None] & (T == 2)] > 0).all()
+    A = go.assemble_csr(H.C0).to_scipy()
+    n, lab = connected_components(A, directed=False)
+    rowsum = np.asarray(A.sum(axis=1)).ravel()
+    touches = np.zeros(n, bool)
+    np.logical_or.at(touches, lab, rowsum > 1e-7 * A.diagonal())
+    assert touches.all()
