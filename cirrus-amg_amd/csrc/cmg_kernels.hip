@@ -15,6 +15,9 @@
 //   Jacobi  x = x + w*((b - Ax)/D)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "cmg_kernels.hpp"
 
 #pragma clang fp contract(off)
@@ -46,113 +49,147 @@ __device__ __forceinline__ double block_sum(double v, double* red)
     return s;
 }
 
+// Per-thread state of the cells a thread owns: with NT threads per tile each
+// thread owns CPT = 512/NT cells v = tid + c*NT (CPT x-planes apart).
+template <typename T, int NT>
+struct Own {
+    static constexpr int CPT = 512 / NT;
+    T x[CPT];          // x masked to INTERIOR cells
+    float c[CPT][3];   // negative-face coefficients
+    float d[CPT];      // diagonal
+    uint8_t ct[CPT];   // cell type
+};
+
 // LoadAMGLaplacianTileData (src/AMGSolver.cu:395-543).
 // FRESH: the halo is evaluated the way AMGFullNegativeLaplacianOnLeafs sees it
 //   after PropagateToChildren + AccumulateToParentsOneStep: a GHOST cell is its
 //   parent cell and a NONLEAF cell is the mean of its 8 children.
 // !FRESH: ghost values come from G, NONLEAF values from X (cycle state).
-template <bool FRESH, typename T>
+// All global loads (own cells and the 384 halo cells) are issued before the
+// first barrier; only the ghost extrapolation waits for the tile's own values.
+template <bool FRESH, typename T, int NT>
 __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, const T* __restrict__ X,
-                                          const T* __restrict__ G, const float* const* C, Lds<T>& s,
-                                          T& x_own, float c_own[3], float& d_own)
+                                          const T* __restrict__ G, const float* const* C, Lds<T>& s, Own<T, NT>& o)
 {
-    const int v = threadIdx.x;
-    const int i = v >> 6, j = (v >> 3) & 7, kk = v & 7;
+    constexpr int CPT = 512 / NT;
+    constexpr int HPT = (384 + NT - 1) / NT;
+    const int tid = threadIdx.x;
     const size_t base = size_t(t) * 512;
-    const uint8_t ct = tp.ctype[base + v];
-    const T xr = X[base + v];
-    x_own = ct == INTERIOR ? xr : T(0);
-    s.x[SX(i, j, kk)] = x_own;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        c_own[a] = C[a][base + v];
-        s.off[a][v] = c_own[a];
+    for (int c = 0; c < CPT; ++c) {
+        const int v = tid + c * NT;
+        o.ct[c] = tp.ctype[base + v];
+        const T xr = X[base + v];
+        o.x[c] = o.ct[c] == INTERIOR ? xr : T(0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) o.c[c][a] = C[a][base + v];
+        o.d[c] = C[3][base + v];
     }
-    d_own = C[3][base + v];
-    __syncthreads();
-    if (v < 384) {
-        const int f = v >> 6, fi = v & 63;
+    T hraw[HPT];
+    float hcp[HPT];
+    bool hghost[HPT];
+#pragma unroll
+    for (int m = 0; m < HPT; ++m) {
+        const int k = tid + m * NT;
+        hraw[m] = T(0);
+        hcp[m] = h;
+        hghost[m] = false;
+        if (k >= 384) continue;
+        const int f = k >> 6, fi = k & 63;
         const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
         const int p = fi >> 3, q = fi & 7;
-        int fl[3], cl[3], nl[3];
+        int nl[3];
+        if (axis == 0) { nl[0] = sgn < 0 ? 7 : 0; nl[1] = p; nl[2] = q; }
+        else if (axis == 1) { nl[0] = p; nl[1] = sgn < 0 ? 7 : 0; nl[2] = q; }
+        else { nl[0] = p; nl[1] = q; nl[2] = sgn < 0 ? 7 : 0; }
+        const int nt = tp.nbr[size_t(t) * 6 + (sgn < 0 ? axis : axis + 3)];
+        if (nt < 0) continue;
+        const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
+        const uint8_t nty = tp.ttype[nt];
+        hcp[m] = C[axis][ni];
+        if (nty & GHOST) {
+            hghost[m] = true;
+            if (FRESH) {
+                const int pt = tp.parent[nt];
+                const int oc = tp.toct[nt];
+                hraw[m] = pt < 0 ? T(0)
+                                 : X[size_t(pt) * 512 + cidx(4 * (oc & 1) + (nl[0] >> 1), 4 * ((oc >> 1) & 1) + (nl[1] >> 1),
+                                                              4 * ((oc >> 2) & 1) + (nl[2] >> 1))];
+            } else {
+                hraw[m] = G[ni];
+            }
+        } else if (FRESH && (nty & NONLEAF)) {
+            if (tp.ctype[ni] == INTERIOR) {
+                const int oct = (nl[0] >> 2) | ((nl[1] >> 2) << 1) | ((nl[2] >> 2) << 2);
+                const int ctile = tp.child[size_t(nt) * 8 + oct];
+                if (ctile >= 0 && (tp.ttype[ctile] & LEAF)) {
+                    T sum = 0;
+                    const size_t cb = size_t(ctile) * 512;
+#pragma unroll
+                    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                            for (int k2 = 0; k2 < 2; ++k2)
+                                sum += X[cb + cidx(2 * (nl[0] & 3) + ii, 2 * (nl[1] & 3) + jj, 2 * (nl[2] & 3) + k2)];
+                    hraw[m] = sum * T(0.125);
+                } else {
+                    hraw[m] = X[ni];
+                }
+            }
+        } else {
+            const T xv = X[ni];
+            hraw[m] = tp.ctype[ni] == INTERIOR ? xv : T(0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int v = tid + c * NT;
+        s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) s.off[a][v] = o.c[c][a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < HPT; ++m) {
+        const int k = tid + m * NT;
+        if (k >= 384) continue;
+        const int f = k >> 6, fi = k & 63;
+        const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
+        const int p = fi >> 3, q = fi & 7;
+        int fl[3];
         const int axk = sgn < 0 ? -1 : 8;
         if (axis == 0) { fl[0] = axk; fl[1] = p; fl[2] = q; }
         else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
         else { fl[0] = p; fl[1] = q; fl[2] = axk; }
+        T hv = hraw[m];
+        if (hghost[m]) {
+            int cl[3] = {fl[0], fl[1], fl[2]};
+            cl[axis] -= sgn;
+            T avg = 0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { cl[a] = fl[a]; nl[a] = fl[a]; }
-        cl[axis] -= sgn;
-        nl[axis] = sgn < 0 ? 7 : 0;
-        const int nt = tp.nbr[size_t(t) * 6 + (sgn < 0 ? axis : axis + 3)];
-        const int nv = cidx(nl[0], nl[1], nl[2]);
-        T hv;
-        float cplus;
-        if (nt < 0) {
-            hv = T(0);
-            cplus = h;
-        } else {
-            const size_t ni = size_t(nt) * 512 + nv;
-            const uint8_t nty = tp.ttype[nt];
-            cplus = C[axis][ni];
-            if (nty & GHOST) {
-                T avg = 0;
+            for (int di = 0; di < 2; ++di)
 #pragma unroll
-                for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
-                    for (int dj = 0; dj < 2; ++dj)
-#pragma unroll
-                        for (int dk = 0; dk < 2; ++dk) avg += s.x[SX(cl[0] ^ di, cl[1] ^ dj, cl[2] ^ dk)];
-                avg /= 8;
-                T V1;
-                if (FRESH) {
-                    const int pt = tp.parent[nt];
-                    const int o = tp.toct[nt];
-                    V1 = pt < 0 ? T(0)
-                                : X[size_t(pt) * 512 + cidx(4 * (o & 1) + (nl[0] >> 1), 4 * ((o >> 1) & 1) + (nl[1] >> 1),
-                                                             4 * ((o >> 2) & 1) + (nl[2] >> 1))];
-                } else {
-                    V1 = G[ni];
-                }
-                if constexpr (sizeof(T) == 4)
-                    hv = float(double(s.x[SX(cl[0], cl[1], cl[2])]) + 0.5 * double(V1 - avg));
-                else
-                    hv = s.x[SX(cl[0], cl[1], cl[2])] + 0.5 * (V1 - avg);
-            } else if (FRESH && (nty & NONLEAF)) {
-                hv = T(0);
-                if (tp.ctype[ni] == INTERIOR) {
-                    const int oct = (nl[0] >> 2) | ((nl[1] >> 2) << 1) | ((nl[2] >> 2) << 2);
-                    const int ctile = tp.child[size_t(nt) * 8 + oct];
-                    if (ctile >= 0 && (tp.ttype[ctile] & LEAF)) {
-                        T sum = 0;
-                        const size_t cb = size_t(ctile) * 512;
-#pragma unroll
-                        for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-                            for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-                                for (int k2 = 0; k2 < 2; ++k2)
-                                    sum += X[cb + cidx(2 * (nl[0] & 3) + ii, 2 * (nl[1] & 3) + jj, 2 * (nl[2] & 3) + k2)];
-                        hv = sum * T(0.125);
-                    } else {
-                        hv = X[ni];
-                    }
-                }
-            } else {
-                hv = tp.ctype[ni] == INTERIOR ? X[ni] : T(0);
-            }
+                    for (int dk = 0; dk < 2; ++dk) avg += s.x[SX(cl[0] ^ di, cl[1] ^ dj, cl[2] ^ dk)];
+            avg /= 8;
+            if constexpr (sizeof(T) == 4)
+                hv = float(double(s.x[SX(cl[0], cl[1], cl[2])]) + 0.5 * double(hraw[m] - avg));
+            else
+                hv = s.x[SX(cl[0], cl[1], cl[2])] + 0.5 * (hraw[m] - avg);
         }
         s.x[SX(fl[0], fl[1], fl[2])] = hv;
-        if (sgn > 0) s.off[axis][512 + fi] = cplus;
+        if (sgn > 0) s.off[axis][512 + fi] = hcp[m];
     }
     __syncthreads();
 }
 
-// AMGLaplacianTileData::negativeLap (src/AMGSolver.cu:345-382) for the thread's cell
+// AMGLaplacianTileData::negativeLap (src/AMGSolver.cu:345-382) for cell v
 template <typename T>
-__device__ __forceinline__ T neg_lap(const Lds<T>& s, T x0, const float c[3], float d)
+__device__ __forceinline__ T neg_lap(const Lds<T>& s, int v, T x0, const float c[3], float d)
 {
     if (d == 0) return T(0);
-    const int v = threadIdx.x;
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
     const float cp0 = i < 7 ? s.off[0][v + 64] : s.off[0][512 + (j * 8 + k)];
     const float cp1 = j < 7 ? s.off[1][v + 8] : s.off[1][512 + (i * 8 + k)];
@@ -171,10 +208,10 @@ __device__ __forceinline__ float tile_h(const TopoView& tp, int t) { return tp.h
 
 // ---------------------------------------------------------------- apply ----
 // y = A x on LEAF tiles (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
-template <bool DOT>
-__global__ __launch_bounds__(512) void apply_leaf(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
-                                                  Coef C, float* __restrict__ Y, double* __restrict__ partial,
-                                                  const double* __restrict__ S)
+template <bool DOT, int NT>
+__global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                 Coef C, float* __restrict__ Y, double* __restrict__ partial,
+                                                 const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
@@ -184,21 +221,28 @@ __global__ __launch_bounds__(512) void apply_leaf(TopoView tp, const int32_t* __
         if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
-    float x0, c[3], d;
-    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
-    const size_t idx = size_t(t) * 512 + threadIdx.x;
-    const float y = tp.ctype[idx] == INTERIOR ? neg_lap(s, x0, c, d) : 0.0f;
-    Y[idx] = y;
+    Own<float, NT> o;
+    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const size_t idx = size_t(t) * 512 + v;
+        const float y = o.ct[c] == INTERIOR ? neg_lap(s, v, o.x[c], o.c[c], o.d[c]) : 0.0f;
+        Y[idx] = y;
+        if (DOT) acc += double(X[idx]) * double(y);
+    }
     if (DOT) {
-        double b = block_sum(double(X[idx]) * double(y), red);
+        double b = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
     }
 }
 
 // r64 = b64 - A x64 on LEAF tiles in FP64 (FRESH halo), r32 = float(r64), partial |r64|^2
-__global__ __launch_bounds__(512) void residual64(TopoView tp, const int32_t* __restrict__ list, const double* __restrict__ X,
-                                                  Coef C, const double* __restrict__ B, double* __restrict__ R64,
-                                                  float* __restrict__ R32, double* __restrict__ partial)
+template <int NT>
+__global__ __launch_bounds__(NT) void residual64(TopoView tp, const int32_t* __restrict__ list, const double* __restrict__ X,
+                                                 Coef C, const double* __restrict__ B, double* __restrict__ R64,
+                                                 float* __restrict__ R32, double* __restrict__ partial)
 {
     __shared__ Lds<double> s;
     __shared__ double red[8];
@@ -207,77 +251,116 @@ __global__ __launch_bounds__(512) void residual64(TopoView tp, const int32_t* __
         if (threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
-    double x0;
-    float c[3], d;
-    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
-    const size_t idx = size_t(t) * 512 + threadIdx.x;
-    double r = 0;
-    if (tp.ctype[idx] == INTERIOR) r = B[idx] - neg_lap(s, x0, c, d);
-    R64[idx] = r;
-    R32[idx] = float(r);
-    double b = block_sum(r * r, red);
+    Own<double, NT> o;
+    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const size_t idx = size_t(t) * 512 + v;
+        double r = 0;
+        if (o.ct[c] == INTERIOR) r = B[idx] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        R64[idx] = r;
+        R32[idx] = float(r);
+        acc += r * r;
+    }
+    double b = block_sum(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
 // ------------------------------------------------------------- smoothers ----
 // one colour of red-black SOR on subtree-LEAF tiles of a level, in place
 // (NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel, src/AMGSolver.cu:838-877)
-__global__ __launch_bounds__(512) void gs_color(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
-                                                Coef C, const float* __restrict__ B, int color, double omega,
-                                                const double* __restrict__ S)
+template <int NT>
+__global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X, Coef C,
+                                               const float* __restrict__ B, int color, double omega,
+                                               const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     if (stopped(S)) return;
     const int t = list[blockIdx.x];
     if (t < 0) return;
-    float x0, c[3], d;
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
-    const int v = threadIdx.x;
-    if ((((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) return;
-    const size_t idx = size_t(t) * 512 + v;
-    const float b = B[idx];
-    const float ax = neg_lap(s, x0, c, d);
-    const float delta = (tp.ctype[idx] & INTERIOR) ? (b - ax) / d : 0.0f;
-    X[idx] = float(double(x0) + double(delta) * omega);
+    Own<float, NT> o;
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        if ((((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) continue;
+        const size_t idx = size_t(t) * 512 + v;
+        const float b = B[idx];
+        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        const float delta = (o.ct[c] & INTERIOR) ? (b - ax) / o.d[c] : 0.0f;
+        X[idx] = float(double(o.x[c]) + double(delta) * omega);
+    }
 }
 
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
-__global__ __launch_bounds__(512) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
-                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
-                                              const float* __restrict__ B, float w, const double* __restrict__ S)
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
+                                             float* __restrict__ Xout, const float* __restrict__ G, Coef C,
+                                             const float* __restrict__ B, float w, const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     if (stopped(S)) return;
     const int t = list[blockIdx.x];
     if (t < 0) return;
-    float x0, c[3], d;
-    load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, x0, c, d);
+    Own<float, NT> o;
+    float b[512 / NT];
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
+    load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o);
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
+        Xout[size_t(t) * 512 + v] = o.x[c] + w * delta;
+    }
+}
+
+// first damped-Jacobi sweep from x = 0: with every x (own, neighbours, ghosts)
+// zero, A x = +0 and the sweep reduces to x = 0 + w (b / D) (bit-identical to
+// the general kernel); it reads b, D and the cell type only
+__global__ __launch_bounds__(512) void jacobi_zero(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ Xout,
+                                                   const float* __restrict__ D, const float* __restrict__ B, float w,
+                                                   const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
     const size_t idx = size_t(t) * 512 + threadIdx.x;
-    const float ax = neg_lap(s, x0, c, d);
-    const float delta = (tp.ctype[idx] & INTERIOR) ? (B[idx] - ax) / d : 0.0f;
-    Xout[idx] = x0 + w * delta;
+    const float delta = (tp.ctype[idx] & INTERIOR) ? (B[idx] - 0.0f) / D[idx] : 0.0f;
+    Xout[idx] = 0.0f + w * delta;
 }
 
 // ------------------------------------------------------------- transfers ----
 // ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
-__global__ __launch_bounds__(512) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
-                                                         Coef C, float* __restrict__ B, float Rr,
-                                                         const double* __restrict__ S)
+template <int NT>
+__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                        Coef C, float* __restrict__ B, float Rr,
+                                                        const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     __shared__ float res[512];
     if (stopped(S)) return;
     const int t = list[blockIdx.x];
     if (t < 0) return;
-    float x0, c[3], d;
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
-    const int v = threadIdx.x;
-    res[v] = B[size_t(t) * 512 + v] - neg_lap(s, x0, c, d);
+    Own<float, NT> o;
+    float b[512 / NT];
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        res[v] = b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+    }
     __syncthreads();
+    const int v = threadIdx.x;
     if (v >= 64) return;
     const int pt = tp.parent[t];
     if (pt < 0) return;
-    const int o = tp.toct[t];
+    const int oc = tp.toct[t];
     const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
     float rs = 0, xs = 0;
 #pragma unroll
@@ -289,28 +372,33 @@ __global__ __launch_bounds__(512) void residual_restrict(TopoView tp, const int3
                 rs += res[cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
                 xs += s.x[SX(fl0 + ii, fl1 + jj, fl2 + kk)];
             }
-    const size_t pidx = size_t(pt) * 512 + cidx(4 * (o & 1) + (fl0 >> 1), 4 * ((o >> 1) & 1) + (fl1 >> 1), 4 * ((o >> 2) & 1) + (fl2 >> 1));
+    const size_t pidx = size_t(pt) * 512 + cidx(4 * (oc & 1) + (fl0 >> 1), 4 * ((oc >> 1) & 1) + (fl1 >> 1), 4 * ((oc >> 2) & 1) + (fl2 >> 1));
     B[pidx] = rs * Rr;
     X[pidx] = xs / 8;
 }
 
 // FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
-__global__ __launch_bounds__(512) void fas_save(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
-                                                Coef C, float* __restrict__ B, float* __restrict__ X0,
-                                                const double* __restrict__ S)
+template <int NT>
+__global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                               Coef C, float* __restrict__ B, float* __restrict__ X0,
+                                               const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     if (stopped(S)) return;
     const int t = list[blockIdx.x];
     if (t < 0) return;
-    float x0, c[3], d;
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, x0, c, d);
-    const size_t idx = size_t(t) * 512 + threadIdx.x;
-    if (tp.ctype[idx] & INTERIOR) {
-        B[idx] = B[idx] + neg_lap(s, x0, c, d);
-        X0[idx] = X[idx];
-    } else {
-        B[idx] = 0;
+    Own<float, NT> o;
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const size_t idx = size_t(t) * 512 + v;
+        if (o.ct[c] & INTERIOR) {
+            B[idx] = B[idx] + neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+            X0[idx] = X[idx];
+        } else {
+            B[idx] = 0;
+        }
     }
 }
 
@@ -573,22 +661,27 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
     }
 }
 
-// level-0 FP64 dense solve: x = A0^-1 b on the level's INTERIOR cells
-__global__ __launch_bounds__(1024) void coarse_direct(const double* __restrict__ inv, const int32_t* __restrict__ cells, int n,
-                                                      const int32_t* __restrict__ zero_cells, int nz, float* __restrict__ X,
-                                                      const float* __restrict__ B, const double* __restrict__ S)
+// level-0 FP64 dense solve: x = A0^-1 b on the level's INTERIOR cells.
+// One wave per row: lanes stride the row (coalesced FP64 loads of the inverse),
+// then a shuffle reduction; 4 rows per 256-thread workgroup.
+__global__ __launch_bounds__(256) void coarse_direct(const double* __restrict__ inv, const int32_t* __restrict__ cells, int n,
+                                                     const int32_t* __restrict__ zero_cells, int nz, float* __restrict__ X,
+                                                     const float* __restrict__ B, const double* __restrict__ S)
 {
     extern __shared__ double sb[];
     if (stopped(S)) return;
     for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = double(B[cells[i]]);
     __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r < n) {
         const double* row = inv + size_t(r) * n;
         double acc = 0;
-        for (int c = 0; c < n; ++c) acc = fma(row[c], sb[c], acc);
-        X[cells[r]] = float(acc);
+        for (int c = lane; c < n; c += 64) acc = fma(row[c], sb[c], acc);
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) X[cells[r]] = float(acc);
     }
-    for (int i = threadIdx.x; i < nz; i += blockDim.x) X[zero_cells[i]] = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nz; i += gridDim.x * blockDim.x) X[zero_cells[i]] = 0.0f;
 }
 
 }  // namespace k
@@ -597,18 +690,50 @@ __global__ __launch_bounds__(1024) void coarse_direct(const double* __restrict__
 
 static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 
+// threads per tile of the stencil kernels (512 / 256 / 128: 1 / 2 / 4 cells per thread)
+static int stencil_threads()
+{
+    static int nt = [] {
+        const char* e = getenv("CMG_STENCIL_THREADS");
+        int v = e ? atoi(e) : 256;
+        return (v == 128 || v == 256 || v == 512) ? v : 256;
+    }();
+    return nt;
+}
+
+#define CMG_NT_DISPATCH(KERNEL, GRID, STREAM, ...)                                            \
+    do {                                                                                      \
+        switch (stencil_threads()) {                                                          \
+        case 128: KERNEL<128><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;             \
+        case 512: KERNEL<512><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;             \
+        default: KERNEL<256><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;             \
+        }                                                                                     \
+    } while (0)
+
 void launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
 {
     if (!nonempty(d.leafs)) return;
-    if (partial) k::apply_leaf<true><<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, C, Y, partial, S);
-    else k::apply_leaf<false><<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, C, Y, nullptr, S);
+    const TopoView tp = view(d);
+    if (partial) {
+        switch (stencil_threads()) {
+        case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+        case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+        default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+        }
+    } else {
+        switch (stencil_threads()) {
+        case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+        case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+        default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+        }
+    }
 }
 
 void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* B, double* R64, float* R32, double* partial,
                        hipStream_t st)
 {
     if (!nonempty(d.leafs)) return;
-    k::residual64<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, C, B, R64, R32, partial);
+    CMG_NT_DISPATCH(k::residual64, d.leafs.n, st, view(d), d.leafs.d, X, C, B, R64, R32, partial);
 }
 
 void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
@@ -616,7 +741,7 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    k::gs_color<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, B, color, omega, S);
+    CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
 }
 
 void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
@@ -624,21 +749,29 @@ void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, c
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    k::jacobi<<<l.n, 512, 0, st>>>(view(d), l.d, Xin, Xout, G, C, B, w, S);
+    CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S);
+}
+
+void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
+                        hipStream_t st)
+{
+    const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return;
+    k::jacobi_zero<<<l.n, 512, 0, st>>>(view(d), l.d, Xout, D, B, w, S);
 }
 
 void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st)
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    k::residual_restrict<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, B, Rr, S);
+    CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
 }
 
 void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st)
 {
     const LaunchList& l = d.nonleaf[level];
     if (!nonempty(l)) return;
-    k::fas_save<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, B, X0, S);
+    CMG_NT_DISPATCH(k::fas_save, l.n, st, view(d), l.d, X, C, B, X0, S);
 }
 
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)
@@ -749,7 +882,9 @@ void launch_finish(const double* partial, const double* partial2, int n, double*
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st)
 {
-    k::coarse_direct<<<1, 1024, size_t(n) * sizeof(double), st>>>(inv, cells, n, zero_cells, nz, X, B, S);
+    if (n == 0 && nz == 0) return;
+    const int blocks = std::max(1, (n + 3) / 4);
+    k::coarse_direct<<<blocks, 256, size_t(std::max(n, 1)) * sizeof(double), st>>>(inv, cells, n, zero_cells, nz, X, B, S);
 }
 
 }  // namespace cmg

@@ -36,6 +36,8 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
                      hipStream_t st);
 void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
                    const double* S, hipStream_t st);
+void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
+                        hipStream_t st);
 void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st);
 void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st);
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st);

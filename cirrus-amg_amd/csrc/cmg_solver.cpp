@@ -474,7 +474,8 @@ struct CycleCtx {
     const double* S;
 };
 
-void smooth(const CycleCtx& c, int level, int iters, int order)
+// `x_zero`: every x of the cycle is still zero (first sweep at the finest level)
+void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
@@ -488,7 +489,8 @@ void smooth(const CycleCtx& c, int level, int iters, int order)
     float* cur = c.X;
     float* other = c.s->tmp;
     for (int it = 0; it < iters; ++it) {
-        launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st);
+        if (it == 0 && x_zero) launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st);
+        else launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st);
         std::swap(cur, other);
     }
     if (cur != c.X) launch_copy_list(d.subtree[level], cur, c.X, c.S, st);
@@ -510,7 +512,7 @@ void cycle_step(const CycleCtx& c, int level)
         }
         return;
     }
-    smooth(c, level, c.o->level_iters, 0);
+    smooth(c, level, c.o->level_iters, 0, level == d.max_level);
     launch_residual_restrict(d, level, c.X, c.C, c.B, c.s->Rr, c.S, st);
     launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
     const int mu = c.o->cycle == CMG_CYCLE_FAS_W ? 2 : 1;
@@ -614,14 +616,7 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *P = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
           *Z = dev_channel(d, CH_Z);
     set_scalars(s, tol * tol, max_iters);
-    if (!warm) {
-        launch_fill_list(d.leafs, X, 0.0f, st);
-    } else {
-        // r = b - A x (warm start; the reference always starts from 0, AMGSolver.cu:1407)
-        launch_apply(d, X, coef(d, s->c0), AP, nullptr, nullptr, st);
-        launch_axpy(d, d.leafs, R, AP, -1.0f, st);
-    }
-    // rhs2 = r.r, absolute threshold, done if zero (AMGSolver.cu:1410-1425)
+    // |b|^2 fixes the tolerance (AMGSolver.cu:1410-1425 with x0 = 0)
     launch_dot(d.leafs, R, R, s->partial, nullptr, st);
     launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, st);
     double h[S_COUNT];
@@ -629,10 +624,28 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     const double rhs2 = h[S_TMP];
     *iters = 0;
     *rel_res = 0;
+    if (!warm) {
+        launch_fill_list(d.leafs, X, 0.0f, st);
+    } else if (rhs2 != 0) {
+        // r = b - A x (warm start; the reference always starts from 0, AMGSolver.cu:1407);
+        // the stopping test stays relative to |b|
+        launch_apply(d, X, coef(d, s->c0), AP, nullptr, nullptr, st);
+        launch_axpy(d, d.leafs, R, AP, -1.0f, st);
+    }
     if (rhs2 == 0) return 1;
     if (!std::isfinite(rhs2)) return 3;
     double thr2 = tol * tol * rhs2;
     thr2 = std::max(thr2, 2.2250738585072014e-308);
+    if (warm) {
+        // a warm start may already satisfy the tolerance: no iteration then
+        launch_dot(d.leafs, R, R, s->partial, nullptr, st);
+        launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, st);
+        read_scalars(s, h);
+        if (h[S_TMP] < thr2) {
+            *rel_res = std::sqrt(h[S_TMP] / rhs2);
+            return 1;
+        }
+    }
     double init[S_COUNT] = {};
     init[S_THR2] = thr2;
     init[S_MAXIT] = max_iters;
