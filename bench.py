@@ -69,11 +69,9 @@ def canonical_bytes_per_iter(lv, ty, max_level):
     return 81.0 * n_leaf + levels + coarsest
 
 
-def cpu_baseline(cfg, gpu_iters, budget_s):
-    """The same operator assembled as FP64 CSR and solved by threaded CG on the host
-    (the oracle's OpenMP CG), timed on a bounded number of iterations."""
+def oracle_grid(cfg):
+    """The same workload built by the oracle (C restatement, OpenMP)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle as O
     t = cfg["target"]
     if t[0] == "uniform":
@@ -95,34 +93,60 @@ def cpu_baseline(cfg, gpu_iters, budget_s):
         g.rhs_uniform(1, seed, True)
     else:
         g.rhs_divergence(1, 5, seed)
-    A = g.assemble_csr(11)
-    b = g.channel(1).ravel()[A.cell].astype(np.float64)
+    return O, g
+
+
+def cpu_baseline(cfg, max_iters):
+    """The same MGPCG solve (PCG + FAS V(2,2) damped Jacobi, FP64 direct level 0,
+    FP32 vectors, FP64 dots) run by the oracle's OpenMP C restatement on the host
+    cores, from x = 0 to 1e-6, timed around the solve only (assembly excluded)."""
+    O, g = oracle_grid(cfg)
     threads = O.lib().og_num_threads()
-    # per-iteration cost of threaded CG on this CSR
-    t0 = time.time()
-    _, it, rr = A.cg(b, 1e-30, 5, threads)
-    t_iter = (time.time() - t0) / max(it, 1)
-    n_iter = max(5, int(budget_s / max(t_iter, 1e-6)))
-    t0 = time.time()
-    x, it, rr = A.cg(b, 1e-6, n_iter, threads)
-    dt = time.time() - t0
-    converged = rr < 1e-6
-    per_iter_ms = dt / max(it, 1) * 1e3
-    out = {"unit": "ms per solve", "cores": threads, "kind": "port", "threads_runtime": "OpenMP (libgomp)",
-           "per_iteration_ms": per_iter_ms, "cg_iterations_run": it, "cg_rel_res_reached": rr,
-           "csr_unknowns": A.n, "csr_nnz": int(A.nnz)}
-    if converged:
-        out["value"] = dt * 1e3
-        out["sample"] = "full FP64 CSR CG solve to 1e-6 (%d iterations)" % it
-    else:
-        # CG's error bound: iterations to 1e-6 from the observed geometric rate
-        rate = rr ** (1.0 / max(it, 1))
-        need = int(np.ceil(np.log(1e-6) / np.log(rate))) if rate < 1 else -1
-        out["value"] = per_iter_ms * need if need > 0 else None
-        out["sample"] = ("%d FP64 CSR CG iterations (%.1f s) reaching rel res %.2e; ms/solve = ms/iteration x "
-                         "%d iterations extrapolated from the observed rate" % (it, dt, rr, need))
-        out["extrapolated_iterations"] = need
-    return out
+    cyc = O.Cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_iters=0, coarsest_direct=1,
+                  R_restrict=1.0, P=1.0, tmp_ch=6)
+    t0 = time.perf_counter()
+    it, rel = g.pcg(cyc, 11, max_iters, 1e-6)
+    dt = time.perf_counter() - t0
+    return {"value": dt * 1e3, "unit": "ms per solve", "cores": threads, "kind": "port",
+            "threads_runtime": "OpenMP (libgomp)",
+            "sample": "one full solve of the same workload from x = 0 to 1e-6 by the oracle's C restatement "
+                      "(%d PCG iterations, rel res %.3e), solve only" % (it, rel),
+            "iterations": it, "rel_res": rel}
+
+
+def load_traffic(name):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the factor a plain copy kernel of
+    known size gives); None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+KERNELS = {0: ("apply (y = A p on LEAF tiles + p.y partials)", 25.0),
+           1: ("damped-Jacobi sweep, finest level", 29.0),
+           2: ("residual + restrict, finest level", 29.5),
+           3: ("x += a p, r -= a Ap + r.r partials", 28.0),
+           4: ("FAS prolong + correct, finest level", 8.5),
+           5: ("p = z + b p", 12.0),
+           6: ("r.z partials", 8.0),
+           7: ("first Jacobi sweep from x = 0, finest level", 21.0),
+           8: ("plain copy", 8.0)}
+
+
+def profile_kernel(cfg, kernel, reps):
+    import cirrus_amg as P
+    g = build(P, cfg)
+    solver = P.AMGSolver(g, 11, 0.5, 1.0, 1.0)
+    ms, tiles = solver.bench_kernel(kernel, reps)
+    name, bpc = KERNELS[kernel]
+    print(json.dumps({"kernel": kernel, "name": name, "reps": reps, "ms": ms, "tiles": tiles,
+                      "bytes_per_launch": bpc * 512 * tiles, "GBps": bpc * 512 * tiles / (ms * 1e-3) / 1e9}))
 
 
 def main():
@@ -131,8 +155,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-refinement leg")
+    ap.add_argument("--profile-kernel", type=int, default=None, choices=sorted(KERNELS),
+                    help="only time one kernel (for rocprofv3 counter runs) and exit")
+    ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs the single-GPU solve only: this build has no multi-GPU path "
@@ -143,6 +170,9 @@ def main():
     import cirrus_amg as P
 
     cfg = CONFIGS[args.config]
+    if args.profile_kernel is not None:
+        profile_kernel(cfg, args.profile_kernel, args.reps)
+        return
     t_build = time.time()
     g = build(P, cfg)
     t_build = time.time() - t_build
@@ -173,11 +203,13 @@ def main():
     P.device_synchronize()
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     true_rel = solver.last_stats.true_rel_res
+    x32 = g.channel(0).astype(np.float64) if not args.no_fp64 else None
 
     # dominant kernel: damped-Jacobi sweep on the finest level (4 per PCG iteration)
     k_ms, k_tiles = solver.bench_kernel(1, 50)
     alg_bytes = 29.0 * 512 * k_tiles          # x_in, b, 3 off-diagonals, diagonal (4 B each), type (1 B), x_out
     k_gbs = alg_bytes / (k_ms * 1e-3) / 1e9
+    traffic = load_traffic("jacobi_finest_%s_pmc.txt" % args.config)
     apply_ms, apply_tiles = solver.bench_kernel(0, 50)
     apply_gbs = 25.0 * 512 * apply_tiles / (apply_ms * 1e-3) / 1e9
 
@@ -211,13 +243,31 @@ def main():
         "solve_frac_of_hbm": solve_gbs / HBM_PEAK_GBS,
         "canonical_bytes_per_iter": b_iter,
         "roofline": {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
+                     "traffic": traffic, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
                      "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles},
         "apply_kernel": {"GBps": apply_gbs, "ms": apply_ms, "tiles": apply_tiles},
         "build_s": t_build,
     }
+    if not args.no_fp64:
+        # mixed-precision form: FP32 MGPCG inner solves inside FP64 iterative
+        # refinement to 1e-10; its solution is the FP64 solution of the same
+        # discrete operator against which the FP32 1e-6 solve's error is stated
+        g.copy_channel(5, 1)
+        o64 = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                             precision=P.PREC_FP64_REFINE, outer_tolerance=1e-10, inner_tolerance=1e-4)
+        t64 = time.perf_counter()
+        it64, rel64 = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=o64)
+        P.device_synchronize()
+        t64 = (time.perf_counter() - t64) * 1e3
+        st = solver.last_stats
+        x64 = solver.x64()
+        m = (types == 2) & (ty == 1)[:, None]
+        err = float(np.linalg.norm((x32 - x64)[m]) / np.linalg.norm(x64[m]))
+        result["fp64_refinement"] = {"ms": t64, "outer_iters": int(st.outer_iters), "inner_iters_total": int(it64),
+                                     "true_rel_res_fp64": float(st.true_rel_res), "tolerance": 1e-10}
+        result["rel_l2_err_fp32_solve_vs_fp64"] = err
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(cfg, it_mean, args.cpu_budget)
+        result["cpu_baseline"] = cpu_baseline(cfg, 200)
     print(json.dumps(result))
 
 

@@ -53,6 +53,7 @@ struct DevGrid {
     LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
     LaunchList ghosts;                  // all GHOST tiles
     int max_list = 0;
+    int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU
     // reduction scratch for the op-level entry points
     double* partial = nullptr;
     double* scal = nullptr;

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "cmg_kernels.hpp"
 
@@ -206,35 +208,432 @@ __device__ __forceinline__ T neg_lap(const Lds<T>& s, int v, T x0, const float c
 
 __device__ __forceinline__ float tile_h(const TopoView& tp, int t) { return tp.h0 / float(1 << tp.tlevel[t]); }
 
-// ---------------------------------------------------------------- apply ----
-// y = A x on LEAF tiles (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
+// ------------------------------------------------- pipelined tile loops ----
+// The hot stencil kernels run persistent workgroups that walk their share of a
+// launch list (entries blockIdx.x, +gridDim.x, ...).  While one tile is staged
+// in LDS and computed, all global loads of the workgroup's next tile are
+// already in flight in a second register stage.
+
+// global loads of one tile, held in registers until the tile is processed
+template <typename T, int NT, bool WB>
+struct Stage {
+    static constexpr int CPT = 512 / NT;
+    static constexpr int HPT = (384 + NT - 1) / NT;
+    int t;
+    T xr[CPT];
+    uint8_t ct[CPT];
+    float c[CPT][3];
+    float d[CPT];
+    float b[WB ? CPT : 1];
+    T hraw[HPT];
+    float hcp[HPT];
+    bool hghost[HPT];
+};
+
+template <bool FRESH, typename T, int NT, bool WB>
+__device__ __forceinline__ void issue(Stage<T, NT, WB>& st, const TopoView& tp, int t, const T* __restrict__ X,
+                                      const T* __restrict__ G, const float* const* C, const float* __restrict__ B)
+{
+    constexpr int CPT = 512 / NT;
+    constexpr int HPT = (384 + NT - 1) / NT;
+    st.t = t;
+    if (t < 0) return;
+    const int tid = threadIdx.x;
+    const size_t base = size_t(t) * 512;
+    const float h = tile_h(tp, t);
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int v = tid + c * NT;
+        st.ct[c] = tp.ctype[base + v];
+        st.xr[c] = X[base + v];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) st.c[c][a] = C[a][base + v];
+        st.d[c] = C[3][base + v];
+        if (WB) st.b[c] = B[base + v];
+    }
+#pragma unroll
+    for (int m = 0; m < HPT; ++m) {
+        const int k = tid + m * NT;
+        st.hraw[m] = T(0);
+        st.hcp[m] = h;
+        st.hghost[m] = false;
+        if (k >= 384) continue;
+        const int f = k >> 6, fi = k & 63;
+        const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
+        const int p = fi >> 3, q = fi & 7;
+        int nl[3];
+        if (axis == 0) { nl[0] = sgn < 0 ? 7 : 0; nl[1] = p; nl[2] = q; }
+        else if (axis == 1) { nl[0] = p; nl[1] = sgn < 0 ? 7 : 0; nl[2] = q; }
+        else { nl[0] = p; nl[1] = q; nl[2] = sgn < 0 ? 7 : 0; }
+        const int nt = tp.nbr[size_t(t) * 6 + (sgn < 0 ? axis : axis + 3)];
+        if (nt < 0) continue;
+        const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
+        const uint8_t nty = tp.ttype[nt];
+        st.hcp[m] = C[axis][ni];
+        if (nty & GHOST) {
+            st.hghost[m] = true;
+            if (FRESH) {
+                const int pt = tp.parent[nt];
+                const int oc = tp.toct[nt];
+                st.hraw[m] = pt < 0 ? T(0)
+                                    : X[size_t(pt) * 512 + cidx(4 * (oc & 1) + (nl[0] >> 1), 4 * ((oc >> 1) & 1) + (nl[1] >> 1),
+                                                                 4 * ((oc >> 2) & 1) + (nl[2] >> 1))];
+            } else {
+                st.hraw[m] = G[ni];
+            }
+        } else if (FRESH && (nty & NONLEAF)) {
+            if (tp.ctype[ni] == INTERIOR) {
+                const int oct = (nl[0] >> 2) | ((nl[1] >> 2) << 1) | ((nl[2] >> 2) << 2);
+                const int ctile = tp.child[size_t(nt) * 8 + oct];
+                if (ctile >= 0 && (tp.ttype[ctile] & LEAF)) {
+                    T sum = 0;
+                    const size_t cb = size_t(ctile) * 512;
+#pragma unroll
+                    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                            for (int k2 = 0; k2 < 2; ++k2)
+                                sum += X[cb + cidx(2 * (nl[0] & 3) + ii, 2 * (nl[1] & 3) + jj, 2 * (nl[2] & 3) + k2)];
+                    st.hraw[m] = sum * T(0.125);
+                } else {
+                    st.hraw[m] = X[ni];
+                }
+            }
+        } else {
+            const T xv = X[ni];
+            st.hraw[m] = tp.ctype[ni] == INTERIOR ? xv : T(0);
+        }
+    }
+}
+
+// stage -> LDS (own cells, then the halo with the ghost extrapolation); the
+// caller guarantees the previous tile's LDS reads are finished
+template <typename T, int NT, bool WB>
+__device__ __forceinline__ void stage_lds(const Stage<T, NT, WB>& st, Lds<T>& s, Own<T, NT>& o)
+{
+    constexpr int CPT = 512 / NT;
+    constexpr int HPT = (384 + NT - 1) / NT;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int v = tid + c * NT;
+        o.ct[c] = st.ct[c];
+        o.x[c] = st.ct[c] == INTERIOR ? st.xr[c] : T(0);
+        o.d[c] = st.d[c];
+        s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            o.c[c][a] = st.c[c][a];
+            s.off[a][v] = st.c[c][a];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < HPT; ++m) {
+        const int k = tid + m * NT;
+        if (k >= 384) continue;
+        const int f = k >> 6, fi = k & 63;
+        const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
+        const int p = fi >> 3, q = fi & 7;
+        int fl[3];
+        const int axk = sgn < 0 ? -1 : 8;
+        if (axis == 0) { fl[0] = axk; fl[1] = p; fl[2] = q; }
+        else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
+        else { fl[0] = p; fl[1] = q; fl[2] = axk; }
+        T hv = st.hraw[m];
+        if (st.hghost[m]) {
+            int cl[3] = {fl[0], fl[1], fl[2]};
+            cl[axis] -= sgn;
+            T avg = 0;
+#pragma unroll
+            for (int di = 0; di < 2; ++di)
+#pragma unroll
+                for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                    for (int dk = 0; dk < 2; ++dk) avg += s.x[SX(cl[0] ^ di, cl[1] ^ dj, cl[2] ^ dk)];
+            avg /= 8;
+            if constexpr (sizeof(T) == 4)
+                hv = float(double(s.x[SX(cl[0], cl[1], cl[2])]) + 0.5 * double(st.hraw[m] - avg));
+            else
+                hv = s.x[SX(cl[0], cl[1], cl[2])] + 0.5 * (st.hraw[m] - avg);
+        }
+        s.x[SX(fl[0], fl[1], fl[2])] = hv;
+        if (sgn > 0) s.off[axis][512 + fi] = st.hcp[m];
+    }
+    __syncthreads();
+}
+
+// y = A x on LEAF tiles (persistent, pipelined); optional x.y partial per workgroup
 template <bool DOT, int NT>
-__global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
-                                                 Coef C, float* __restrict__ Y, double* __restrict__ partial,
-                                                 const double* __restrict__ S)
+__global__ __launch_bounds__(NT) void apply_p(TopoView tp, const int32_t* __restrict__ list, int n, const float* __restrict__ X,
+                                              Coef C, float* __restrict__ Y, double* __restrict__ partial,
+                                              const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) {
-        if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
-        return;
-    }
-    Own<float, NT> o;
-    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    int i = blockIdx.x;
+    Stage<float, NT, false> cur, nxt;
+    issue<true>(cur, tp, list[i], X, X, C.c, nullptr);
     double acc = 0;
+    for (;;) {
+        const int in = i + gridDim.x;
+        if (in < n) issue<true>(nxt, tp, list[in], X, X, C.c, nullptr);
+        if (cur.t >= 0) {
+            Own<float, NT> o;
+            stage_lds(cur, s, o);
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        const size_t idx = size_t(t) * 512 + v;
-        const float y = o.ct[c] == INTERIOR ? neg_lap(s, v, o.x[c], o.c[c], o.d[c]) : 0.0f;
-        Y[idx] = y;
-        if (DOT) acc += double(X[idx]) * double(y);
+            for (int c = 0; c < 512 / NT; ++c) {
+                const int v = threadIdx.x + c * NT;
+                const float y = o.ct[c] == INTERIOR ? neg_lap(s, v, o.x[c], o.c[c], o.d[c]) : 0.0f;
+                Y[size_t(cur.t) * 512 + v] = y;
+                if (DOT) acc += double(cur.xr[c]) * double(y);
+            }
+            __syncthreads();
+        }
+        if (in >= n) break;
+        cur = nxt;
+        i = in;
     }
     if (DOT) {
         double b = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+}
+
+// damped Jacobi sweep x_out = x_in + w (b - A x_in) / D (persistent, pipelined)
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi_p(TopoView tp, const int32_t* __restrict__ list, int n, const float* __restrict__ Xin,
+                                               float* __restrict__ Xout, const float* __restrict__ G, Coef C,
+                                               const float* __restrict__ B, float w, const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    if (stopped(S)) return;
+    int i = blockIdx.x;
+    Stage<float, NT, true> cur, nxt;
+    issue<false>(cur, tp, list[i], Xin, G, C.c, B);
+    for (;;) {
+        const int in = i + gridDim.x;
+        if (in < n) issue<false>(nxt, tp, list[in], Xin, G, C.c, B);
+        if (cur.t >= 0) {
+            Own<float, NT> o;
+            stage_lds(cur, s, o);
+#pragma unroll
+            for (int c = 0; c < 512 / NT; ++c) {
+                const int v = threadIdx.x + c * NT;
+                const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+                const float delta = (o.ct[c] & INTERIOR) ? (cur.b[c] - ax) / o.d[c] : 0.0f;
+                Xout[size_t(cur.t) * 512 + v] = o.x[c] + w * delta;
+            }
+            __syncthreads();
+        }
+        if (in >= n) break;
+        cur = nxt;
+        i = in;
+    }
+}
+
+// ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964), persistent and pipelined
+template <int NT>
+__global__ __launch_bounds__(NT) void restrict_p(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+                                                 Coef C, float* __restrict__ B, float Rr, const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    __shared__ float res[512];
+    if (stopped(S)) return;
+    int i = blockIdx.x;
+    Stage<float, NT, true> cur, nxt;
+    issue<false>(cur, tp, list[i], X, X, C.c, B);
+    for (;;) {
+        const int in = i + gridDim.x;
+        if (in < n) issue<false>(nxt, tp, list[in], X, X, C.c, B);
+        if (cur.t >= 0) {
+            Own<float, NT> o;
+            stage_lds(cur, s, o);
+#pragma unroll
+            for (int c = 0; c < 512 / NT; ++c) {
+                const int v = threadIdx.x + c * NT;
+                res[v] = cur.b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+            }
+            __syncthreads();
+            const int v = threadIdx.x;
+            const int pt = tp.parent[cur.t];
+            if (v < 64 && pt >= 0) {
+                const int oc = tp.toct[cur.t];
+                const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
+                float rs = 0, xs = 0;
+#pragma unroll
+                for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                        for (int kk = 0; kk < 2; ++kk) {
+                            rs += res[cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
+                            xs += s.x[SX(fl0 + ii, fl1 + jj, fl2 + kk)];
+                        }
+                const size_t pidx = size_t(pt) * 512 +
+                                    cidx(4 * (oc & 1) + (fl0 >> 1), 4 * ((oc >> 1) & 1) + (fl1 >> 1), 4 * ((oc >> 2) & 1) + (fl2 >> 1));
+                B[pidx] = rs * Rr;
+                X[pidx] = xs / 8;
+            }
+            __syncthreads();
+        }
+        if (in >= n) break;
+        cur = nxt;
+        i = in;
+    }
+}
+
+// ------------------------------------------------------ vector (float4) ----
+// Grid-stride loops over the tiles of a list, 128 threads x float4 per tile,
+// two tiles per 256-thread workgroup step.
+
+#define CMG_TILE_LOOP(list, n)                                                               \
+    for (int pos = blockIdx.x * 2 + (threadIdx.x >> 7); pos < (n); pos += gridDim.x * 2)   \
+        if (const int t_ = (list)[pos]; t_ >= 0)
+
+__device__ __forceinline__ size_t quad_index(int t) { return size_t(t) * 128 + (threadIdx.x & 127); }
+
+__global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+                                                   float* __restrict__ R, const float* __restrict__ P,
+                                                   const float* __restrict__ AP, double* __restrict__ partial,
+                                                   const double* __restrict__ S)
+{
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const float a = float(S[S_ALPHA]);
+    double acc = 0;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        float4 x = reinterpret_cast<const float4*>(X)[q];
+        float4 r = reinterpret_cast<const float4*>(R)[q];
+        const float4 p = reinterpret_cast<const float4*>(P)[q];
+        const float4 ap = reinterpret_cast<const float4*>(AP)[q];
+        if (ct.x & INTERIOR) { x.x = fmaf(a, p.x, x.x); r.x = fmaf(-a, ap.x, r.x); }
+        if (ct.y & INTERIOR) { x.y = fmaf(a, p.y, x.y); r.y = fmaf(-a, ap.y, r.y); }
+        if (ct.z & INTERIOR) { x.z = fmaf(a, p.z, x.z); r.z = fmaf(-a, ap.z, r.z); }
+        if (ct.w & INTERIOR) { x.w = fmaf(a, p.w, x.w); r.w = fmaf(-a, ap.w, r.w); }
+        reinterpret_cast<float4*>(X)[q] = x;
+        reinterpret_cast<float4*>(R)[q] = r;
+        acc += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z) + double(r.w) * double(r.w);
+    }
+    double b = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(256) void update_p_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ P,
+                                                  const float* __restrict__ Z, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const float b = float(S[S_BETA]);
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        float4 p = reinterpret_cast<const float4*>(P)[q];
+        const float4 z = reinterpret_cast<const float4*>(Z)[q];
+        if (ct.x & INTERIOR) p.x = fmaf(b, p.x, z.x);
+        if (ct.y & INTERIOR) p.y = fmaf(b, p.y, z.y);
+        if (ct.z & INTERIOR) p.z = fmaf(b, p.z, z.z);
+        if (ct.w & INTERIOR) p.w = fmaf(b, p.w, z.w);
+        reinterpret_cast<float4*>(P)[q] = p;
+    }
+}
+
+__global__ __launch_bounds__(256) void dot_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
+                                             const float* __restrict__ B, double* __restrict__ partial,
+                                             const double* __restrict__ S)
+{
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    double acc = 0;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const float4 a = reinterpret_cast<const float4*>(A)[q];
+        const float4 b = reinterpret_cast<const float4*>(B)[q];
+        acc += double(a.x) * double(b.x) + double(a.y) * double(b.y) + double(a.z) * double(b.z) + double(a.w) * double(b.w);
+    }
+    double b = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// first damped-Jacobi sweep from x = 0: with every x (own, neighbours, ghosts)
+// zero, A x = +0 and the sweep reduces to x = 0 + w (b / D), bit-identical to
+// the general sweep; it reads b, D and the cell type only
+__global__ __launch_bounds__(256) void jacobi_zero_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ Xout,
+                                                     const float* __restrict__ D, const float* __restrict__ B, float w,
+                                                     const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        const float4 b = reinterpret_cast<const float4*>(B)[q];
+        const float4 d = reinterpret_cast<const float4*>(D)[q];
+        float4 x;
+        x.x = 0.0f + w * ((ct.x & INTERIOR) ? (b.x - 0.0f) / d.x : 0.0f);
+        x.y = 0.0f + w * ((ct.y & INTERIOR) ? (b.y - 0.0f) / d.y : 0.0f);
+        x.z = 0.0f + w * ((ct.z & INTERIOR) ? (b.z - 0.0f) / d.z : 0.0f);
+        x.w = 0.0f + w * ((ct.w & INTERIOR) ? (b.w - 0.0f) / d.w : 0.0f);
+        reinterpret_cast<float4*>(Xout)[q] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
+                                              float* __restrict__ B, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        reinterpret_cast<float4*>(B)[q] = reinterpret_cast<const float4*>(A)[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_v(const int32_t* __restrict__ list, int n, float* __restrict__ X, float value)
+{
+    CMG_TILE_LOOP(list, n)
+    {
+        reinterpret_cast<float4*>(X)[quad_index(t_)] = make_float4(value, value, value, value);
+    }
+}
+
+// FASProlongateAndUpdateAMG128Kernel (src/AMGSolver.cu:1216-1255), four cells per thread
+__global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+                                                     const float* __restrict__ X0, float P, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const int v0 = int(threadIdx.x & 127) * 4;   // cells v0..v0+3 share i, j; k = 0..3 or 4..7
+        const int pt = tp.parent[t_];
+        const int oc = tp.toct[t_];
+        const bool ghost = tp.ttype[t_] & GHOST;
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        float4 x = reinterpret_cast<const float4*>(X)[q];
+        const int i = v0 >> 6, j = (v0 >> 3) & 7, k0 = v0 & 7;
+        const int pbase = cidx(4 * (oc & 1) + (i >> 1), 4 * ((oc >> 1) & 1) + (j >> 1), 4 * ((oc >> 2) & 1) + (k0 >> 1));
+        float cx[2] = {0.0f, 0.0f}, cx0[2] = {0.0f, 0.0f};
+        if (pt >= 0) {
+            const size_t pb = size_t(pt) * 512 + pbase;
+            cx[0] = X[pb]; cx[1] = X[pb + 1];
+            cx0[0] = X0[pb]; cx0[1] = X0[pb + 1];
+        }
+        const uint8_t cts[4] = {ct.x, ct.y, ct.z, ct.w};
+        float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool in = cts[e] & INTERIOR;
+            const float c1 = cx[e >> 1], c0 = cx0[e >> 1];
+            xs[e] = ghost ? (in ? c1 : 0.0f) : (in ? xs[e] + P * (c1 - c0) : 0.0f);
+        }
+        reinterpret_cast<float4*>(X)[q] = make_float4(xs[0], xs[1], xs[2], xs[3]);
     }
 }
 
@@ -294,89 +693,7 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __res
     }
 }
 
-// damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
-template <int NT>
-__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
-                                             float* __restrict__ Xout, const float* __restrict__ G, Coef C,
-                                             const float* __restrict__ B, float w, const double* __restrict__ S)
-{
-    __shared__ Lds<float> s;
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    Own<float, NT> o;
-    float b[512 / NT];
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o);
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
-        const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
-        Xout[size_t(t) * 512 + v] = o.x[c] + w * delta;
-    }
-}
-
-// first damped-Jacobi sweep from x = 0: with every x (own, neighbours, ghosts)
-// zero, A x = +0 and the sweep reduces to x = 0 + w (b / D) (bit-identical to
-// the general kernel); it reads b, D and the cell type only
-__global__ __launch_bounds__(512) void jacobi_zero(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ Xout,
-                                                   const float* __restrict__ D, const float* __restrict__ B, float w,
-                                                   const double* __restrict__ S)
-{
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    const size_t idx = size_t(t) * 512 + threadIdx.x;
-    const float delta = (tp.ctype[idx] & INTERIOR) ? (B[idx] - 0.0f) / D[idx] : 0.0f;
-    Xout[idx] = 0.0f + w * delta;
-}
-
 // ------------------------------------------------------------- transfers ----
-// ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
-template <int NT>
-__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
-                                                        Coef C, float* __restrict__ B, float Rr,
-                                                        const double* __restrict__ S)
-{
-    __shared__ Lds<float> s;
-    __shared__ float res[512];
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    Own<float, NT> o;
-    float b[512 / NT];
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        res[v] = b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
-    }
-    __syncthreads();
-    const int v = threadIdx.x;
-    if (v >= 64) return;
-    const int pt = tp.parent[t];
-    if (pt < 0) return;
-    const int oc = tp.toct[t];
-    const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
-    float rs = 0, xs = 0;
-#pragma unroll
-    for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-            for (int kk = 0; kk < 2; ++kk) {
-                rs += res[cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
-                xs += s.x[SX(fl0 + ii, fl1 + jj, fl2 + kk)];
-            }
-    const size_t pidx = size_t(pt) * 512 + cidx(4 * (oc & 1) + (fl0 >> 1), 4 * ((oc >> 1) & 1) + (fl1 >> 1), 4 * ((oc >> 2) & 1) + (fl2 >> 1));
-    B[pidx] = rs * Rr;
-    X[pidx] = xs / 8;
-}
-
 // FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
 template <int NT>
 __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
@@ -701,6 +1018,43 @@ static int stencil_threads()
     return nt;
 }
 
+// persistent stencil kernels run exactly as many workgroups as can be resident
+// (occupancy x CUs, a multiple of 8 so a workgroup stays on one XCD's share of
+// the list); vector kernels use up to 8 workgroups per CU
+static int resident_per_cu(const void* kern, int nt)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_pair(kern, nt);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, nt, 0) != hipSuccess || nb < 1) nb = 1;
+    cache[key] = nb;
+    return nb;
+}
+
+static inline int persistent_grid(const DevGrid& d, int n, const void* kern, int nt)
+{
+    int g = resident_per_cu(kern, nt) * (d.wg_cap / 8);
+    g = std::max(8, g - g % 8);
+    return std::max(1, std::min(n, g));
+}
+static inline int vector_grid(const DevGrid& d, int n) { return std::max(1, std::min((n + 1) / 2, d.wg_cap)); }
+
+#define CMG_NT_PERSISTENT(KERNEL, N, STREAM, ...)                                                                   \
+    do {                                                                                                          \
+        switch (stencil_threads()) {                                                                              \
+        case 128: { const int g_ = persistent_grid(d, (N), (const void*)KERNEL<128>, 128);                       \
+                    KERNEL<128><<<g_, 128, 0, (STREAM)>>>(__VA_ARGS__); } break;                                  \
+        case 512: { const int g_ = persistent_grid(d, (N), (const void*)KERNEL<512>, 512);                       \
+                    KERNEL<512><<<g_, 512, 0, (STREAM)>>>(__VA_ARGS__); } break;                                  \
+        default:  { const int g_ = persistent_grid(d, (N), (const void*)KERNEL<256>, 256);                       \
+                    KERNEL<256><<<g_, 256, 0, (STREAM)>>>(__VA_ARGS__); } break;                                  \
+        }                                                                                                         \
+    } while (0)
+
 #define CMG_NT_DISPATCH(KERNEL, GRID, STREAM, ...)                                            \
     do {                                                                                      \
         switch (stencil_threads()) {                                                          \
@@ -710,22 +1064,22 @@ static int stencil_threads()
         }                                                                                     \
     } while (0)
 
-void launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
+template <bool DOT, int NT>
+static int apply_launch(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
 {
-    if (!nonempty(d.leafs)) return;
-    const TopoView tp = view(d);
-    if (partial) {
-        switch (stencil_threads()) {
-        case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
-        case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
-        default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
-        }
-    } else {
-        switch (stencil_threads()) {
-        case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
-        case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
-        default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
-        }
+    const int n = d.leafs.n;
+    const int g = persistent_grid(d, n, (const void*)k::apply_p<DOT, NT>, NT);
+    k::apply_p<DOT, NT><<<g, NT, 0, st>>>(view(d), d.leafs.d, n, X, C, Y, partial, S);
+    return g;
+}
+
+int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return 0;
+    switch (stencil_threads()) {
+    case 128: return partial ? apply_launch<true, 128>(d, X, C, Y, partial, S, st) : apply_launch<false, 128>(d, X, C, Y, nullptr, S, st);
+    case 512: return partial ? apply_launch<true, 512>(d, X, C, Y, partial, S, st) : apply_launch<false, 512>(d, X, C, Y, nullptr, S, st);
+    default: return partial ? apply_launch<true, 256>(d, X, C, Y, partial, S, st) : apply_launch<false, 256>(d, X, C, Y, nullptr, S, st);
     }
 }
 
@@ -749,7 +1103,7 @@ void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, c
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S);
+    CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, view(d), l.d, l.n, Xin, Xout, G, C, B, w, S);
 }
 
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
@@ -757,14 +1111,14 @@ void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    k::jacobi_zero<<<l.n, 512, 0, st>>>(view(d), l.d, Xout, D, B, w, S);
+    k::jacobi_zero_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, Xout, D, B, w, S);
 }
 
 void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st)
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
+    CMG_NT_PERSISTENT(k::restrict_p, l.n, st, view(d), l.d, l.n, X, C, B, Rr, S);
 }
 
 void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st)
@@ -778,7 +1132,7 @@ void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, 
 {
     const LaunchList& l = d.prolong[level];
     if (!nonempty(l)) return;
-    k::fas_prolong<<<l.n, 512, 0, st>>>(view(d), l.d, X, X0, P, level, S);
+    k::fas_prolong_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, X, X0, P, S);
 }
 
 void launch_propagate_ghosts(const DevGrid& d, float* X, hipStream_t st)
@@ -801,35 +1155,40 @@ void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, h
     k::fill_all<<<blocks, 512, 0, st>>>(X, n4, value, S);
 }
 
-void launch_fill_list(const LaunchList& l, float* X, float value, hipStream_t st)
+void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st)
 {
     if (!nonempty(l)) return;
-    k::fill_list<<<l.n, 512, 0, st>>>(l.d, X, value);
+    k::fill_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, X, value);
 }
 
-void launch_copy_list(const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st)
+void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st)
 {
     if (!nonempty(l)) return;
-    k::copy_list<<<l.n, 512, 0, st>>>(l.d, A, B, S);
+    k::copy_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, A, B, S);
 }
 
-void launch_dot(const LaunchList& l, const float* A, const float* B, double* partial, const double* S, hipStream_t st)
+int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const float* B, double* partial, const double* S,
+               hipStream_t st)
 {
-    if (!nonempty(l)) return;
-    k::dot_list<<<l.n, 512, 0, st>>>(l.d, A, B, partial, S);
+    if (!nonempty(l)) return 0;
+    const int g = vector_grid(d, l.n);
+    k::dot_v<<<g, 256, 0, st>>>(l.d, l.n, A, B, partial, S);
+    return g;
 }
 
-void launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
-                      hipStream_t st)
+int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
+                     hipStream_t st)
 {
-    if (!nonempty(d.leafs)) return;
-    k::update_xr<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, R, P, AP, partial, S);
+    if (!nonempty(d.leafs)) return 0;
+    const int g = vector_grid(d, d.leafs.n);
+    k::update_xr_v<<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S);
+    return g;
 }
 
 void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st)
 {
     if (!nonempty(d.leafs)) return;
-    k::update_p<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, P, Z, S);
+    k::update_p_v<<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, P, Z, S);
 }
 
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st)

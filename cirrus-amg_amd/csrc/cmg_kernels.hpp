@@ -29,7 +29,8 @@ inline TopoView view(const DevGrid& d)
 
 enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5 };
 
-void launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st);
+// launchers that write per-workgroup partials return how many they wrote
+int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st);
 void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* B, double* R64, float* R32, double* partial,
                        hipStream_t st);
 void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
@@ -44,11 +45,12 @@ void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, 
 void launch_propagate_ghosts(const DevGrid& d, float* X, hipStream_t st);
 void launch_accumulate_leafs(const DevGrid& d, float* X, hipStream_t st);
 void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, hipStream_t st);
-void launch_fill_list(const LaunchList& l, float* X, float value, hipStream_t st);
-void launch_copy_list(const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
-void launch_dot(const LaunchList& l, const float* A, const float* B, double* partial, const double* S, hipStream_t st);
-void launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
-                      hipStream_t st);
+void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st);
+void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
+int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const float* B, double* partial, const double* S,
+               hipStream_t st);
+int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
+                     hipStream_t st);
 void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st);
 void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st);

@@ -196,6 +196,12 @@ void dev_upload(DevGrid& d, const Topology& topo)
         d.nonleaf[l] = make_list(topo, nl[l]);
         d.prolong[l] = make_list(topo, pr[l]);
     }
+    {
+        int dev = 0, cus = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        d.wg_cap = std::max(8, 8 * cus);
+    }
     d.partial = dmalloc<double>(size_t(d.max_list) + 1);
     d.scal = dmalloc<double>(S_COUNT);
     HIPCHK(hipMemset(d.scal, 0, S_COUNT * sizeof(double)));
@@ -338,8 +344,8 @@ cmg_status op_dot(cmg_grid* g, int a, int b, double* out)
     if (!valid_channel(a) || !valid_channel(b) || !out) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         auto& d = g->dev;
-        launch_dot(d.leafs, dev_channel(d, a), dev_channel(d, b), d.partial, nullptr, d.stream);
-        launch_finish(d.partial, nullptr, d.leafs.n, d.scal, FIN_SUM, d.stream);
+        const int np = launch_dot(d, d.leafs, dev_channel(d, a), dev_channel(d, b), d.partial, nullptr, d.stream);
+        launch_finish(d.partial, nullptr, np, d.scal, FIN_SUM, d.stream);
         double hs[S_COUNT];
         HIPCHK(hipMemcpyAsync(hs, d.scal, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
         HIPCHK(hipStreamSynchronize(d.stream));
@@ -493,7 +499,7 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
         else launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st);
         std::swap(cur, other);
     }
-    if (cur != c.X) launch_copy_list(d.subtree[level], cur, c.X, c.S, st);
+    if (cur != c.X) launch_copy_list(d, d.subtree[level], cur, c.X, c.S, st);
 }
 
 // FASMuCycleStep (src/AMGSolver.cu:1257-1277)
@@ -545,15 +551,14 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     hipStream_t st = d.stream;
     float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *P = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
           *Z = dev_channel(d, CH_Z);
-    const int n = d.leafs.n;
-    launch_apply(d, P, coef(d, s->c0), AP, s->partial, s->S, st);
-    launch_finish(s->partial, nullptr, n, s->S, FIN_ALPHA, st);
-    launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
-    launch_finish(s->partial, nullptr, n, s->S, FIN_RR, st);
+    const int n1 = launch_apply(d, P, coef(d, s->c0), AP, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, n1, s->S, FIN_ALPHA, st);
+    const int n2 = launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
     enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, s->S);
-    launch_dot(d.leafs, R, Z, s->partial, s->S, st);
-    launch_finish(s->partial, nullptr, n, s->S, FIN_BETA, st);
+    const int n3 = launch_dot(d, d.leafs, R, Z, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, n3, s->S, FIN_BETA, st);
     launch_update_p(d, P, Z, s->S, st);
 }
 
@@ -617,15 +622,15 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
           *Z = dev_channel(d, CH_Z);
     set_scalars(s, tol * tol, max_iters);
     // |b|^2 fixes the tolerance (AMGSolver.cu:1410-1425 with x0 = 0)
-    launch_dot(d.leafs, R, R, s->partial, nullptr, st);
-    launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, st);
+    const int nb = launch_dot(d, d.leafs, R, R, s->partial, nullptr, st);
+    launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, st);
     double h[S_COUNT];
     read_scalars(s, h);
     const double rhs2 = h[S_TMP];
     *iters = 0;
     *rel_res = 0;
     if (!warm) {
-        launch_fill_list(d.leafs, X, 0.0f, st);
+        launch_fill_list(d, d.leafs, X, 0.0f, st);
     } else if (rhs2 != 0) {
         // r = b - A x (warm start; the reference always starts from 0, AMGSolver.cu:1407);
         // the stopping test stays relative to |b|
@@ -638,8 +643,8 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     thr2 = std::max(thr2, 2.2250738585072014e-308);
     if (warm) {
         // a warm start may already satisfy the tolerance: no iteration then
-        launch_dot(d.leafs, R, R, s->partial, nullptr, st);
-        launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_SUM, st);
+        const int nr = launch_dot(d, d.leafs, R, R, s->partial, nullptr, st);
+        launch_finish(s->partial, nullptr, nr, s->S, FIN_SUM, st);
         read_scalars(s, h);
         if (h[S_TMP] < thr2) {
             *rel_res = std::sqrt(h[S_TMP] / rhs2);
@@ -654,9 +659,9 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     HIPCHK(hipStreamSynchronize(st));
     if (o->is_pure_neumann) enqueue_recenter(s, R);
     enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, s->S);
-    launch_copy_list(d.leafs, Z, P, s->S, st);
-    launch_dot(d.leafs, Z, R, s->partial, s->S, st);
-    launch_finish(s->partial, nullptr, d.leafs.n, s->S, FIN_GAMMA0, st);
+    launch_copy_list(d, d.leafs, Z, P, s->S, st);
+    const int ng = launch_dot(d, d.leafs, Z, R, s->partial, s->S, st);
+    launch_finish(s->partial, nullptr, ng, s->S, FIN_GAMMA0, st);
     if (max_iters <= 0) {
         read_scalars(s, h);
         *rel_res = 1.0;
@@ -889,7 +894,7 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
-    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 4) return CMG_ERR_BAD_ARG;
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 8) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         ensure_buffers(s);
         DevGrid& d = s->g->dev;
@@ -906,6 +911,10 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
             case 2: launch_residual_restrict(d, L, Z, C, AP, 1.0f, nullptr, st); nt = d.subtree[L].tiles; break;
             case 3: launch_update_xr(d, X, Pp, Z, AP, s->partial, nullptr, st); nt = d.leafs.tiles; break;
             case 4: launch_fas_prolong(d, L, s->tmp, AP, 1.0f, nullptr, st); nt = d.prolong[L].tiles; break;
+            case 5: launch_update_p(d, Pp, Z, nullptr, st); nt = d.leafs.tiles; break;
+            case 6: launch_dot(d, d.leafs, R, Z, s->partial, nullptr, st); nt = d.leafs.tiles; break;
+            case 7: launch_jacobi_zero(d, L, s->tmp, C.c[3], R, 2.0f / 3.0f, nullptr, st); nt = d.subtree[L].tiles; break;
+            case 8: launch_copy_list(d, d.leafs, Z, s->tmp, nullptr, st); nt = d.leafs.tiles; break;
             }
         };
         one();

@@ -215,7 +215,8 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * launch processes.  Kernels: 0 = operator on LEAF tiles (+ p.Ap partials),
  * 1 = damped-Jacobi sweep on the finest level, 2 = residual + restriction on the
  * finest level, 3 = PCG x/r update (+ r.r partials), 4 = FAS prolongation on the
- * finest level.  No reference counterpart (the reference times whole solves
+ * finest level, 5 = PCG p update, 6 = r.z partials, 7 = first Jacobi sweep from
+ * zero on the finest level, 8 = copy of one vector on LEAF tiles.  No reference counterpart (the reference times whole solves
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 
