@@ -73,7 +73,8 @@ EXPORTS = [
     "cmg_op_propagate_ghosts", "cmg_op_accumulate_leafs", "cmg_op_gs_color", "cmg_op_jacobi",
     "cmg_op_residual_restrict", "cmg_op_fas_save", "cmg_op_fas_prolong", "cmg_op_dot", "cmg_op_cycle",
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
-    "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize",
+    "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize", "cmg_set_kernel_forms",
+    "cmg_get_kernel_forms",
 ]
 
 _lib = None
@@ -114,6 +115,8 @@ def lib():
             "cmg_status_string": (C.c_char_p, [st]), "cmg_last_error": (C.c_char_p, []),
             "cmg_grid_copy_channel": (st, [vp, i, i]),
             "cmg_device_synchronize": (st, []),
+            "cmg_set_kernel_forms": (st, [i, i]),
+            "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
             "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
             "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
         }
@@ -314,6 +317,17 @@ class HADeviceGrid:
 
 def device_synchronize():
     _check(lib().cmg_device_synchronize(), "cmg_device_synchronize")
+
+
+def set_kernel_forms(persistent_mask, stencil_threads):
+    """Stencil kernel forms (bit 0 operator, bit 1 Jacobi, bit 2 restrict persistent; threads per tile)."""
+    _check(lib().cmg_set_kernel_forms(persistent_mask, stencil_threads), "cmg_set_kernel_forms")
+
+
+def get_kernel_forms():
+    m, t = C.c_int(), C.c_int()
+    _check(lib().cmg_get_kernel_forms(C.byref(m), C.byref(t)), "cmg_get_kernel_forms")
+    return m.value, t.value
 
 
 def default_opts(**kw):

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "cmg_internal.hpp"
+#include "cmg_kernels.hpp"
 
 namespace cmg {
 namespace {
@@ -423,6 +424,22 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
     return bench_kernel(s, kernel, reps, avg_ms, tiles);
+}
+
+cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads)
+{
+    if (!set_kernel_forms(persistent_mask, stencil_threads)) {
+        set_error("kernel forms: mask must be 0..7 and threads 128, 256 or 512");
+        return CMG_ERR_BAD_ARG;
+    }
+    return CMG_OK;
+}
+
+cmg_status cmg_get_kernel_forms(int* persistent_mask, int* stencil_threads)
+{
+    if (!persistent_mask || !stencil_threads) return CMG_ERR_BAD_ARG;
+    get_kernel_forms(persistent_mask, stencil_threads);
+    return CMG_OK;
 }
 
 cmg_status cmg_device_synchronize(void)

@@ -16,9 +16,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <stdexcept>
 
 #include "cmg_kernels.hpp"
 
@@ -637,6 +639,105 @@ __global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int32_t*
     }
 }
 
+// ------------------------------------------------- one tile per workgroup ----
+// y = A x on LEAF tiles, one tile per workgroup (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
+template <bool DOT, int NT>
+__global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                 Coef C, float* __restrict__ Y, double* __restrict__ partial,
+                                                 const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) {
+        if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
+        return;
+    }
+    Own<float, NT> o;
+    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const size_t idx = size_t(t) * 512 + v;
+        const float y = o.ct[c] == INTERIOR ? neg_lap(s, v, o.x[c], o.c[c], o.d[c]) : 0.0f;
+        Y[idx] = y;
+        if (DOT) acc += double(X[idx]) * double(y);
+    }
+    if (DOT) {
+        double b = block_sum(acc, red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+}
+
+// damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
+                                             float* __restrict__ Xout, const float* __restrict__ G, Coef C,
+                                             const float* __restrict__ B, float w, const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    Own<float, NT> o;
+    float b[512 / NT];
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
+    load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o);
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
+        Xout[size_t(t) * 512 + v] = o.x[c] + w * delta;
+    }
+}
+
+// ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
+template <int NT>
+__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                        Coef C, float* __restrict__ B, float Rr,
+                                                        const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    __shared__ float res[512];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    Own<float, NT> o;
+    float b[512 / NT];
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
+    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        res[v] = b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+    }
+    __syncthreads();
+    const int v = threadIdx.x;
+    if (v >= 64) return;
+    const int pt = tp.parent[t];
+    if (pt < 0) return;
+    const int oc = tp.toct[t];
+    const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
+    float rs = 0, xs = 0;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                rs += res[cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
+                xs += s.x[SX(fl0 + ii, fl1 + jj, fl2 + kk)];
+            }
+    const size_t pidx = size_t(pt) * 512 + cidx(4 * (oc & 1) + (fl0 >> 1), 4 * ((oc >> 1) & 1) + (fl1 >> 1), 4 * ((oc >> 2) & 1) + (fl2 >> 1));
+    B[pidx] = rs * Rr;
+    X[pidx] = xs / 8;
+}
+
 // r64 = b64 - A x64 on LEAF tiles in FP64 (FRESH halo), r32 = float(r64), partial |r64|^2
 template <int NT>
 __global__ __launch_bounds__(NT) void residual64(TopoView tp, const int32_t* __restrict__ list, const double* __restrict__ X,
@@ -719,27 +820,6 @@ __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __res
     }
 }
 
-// FASProlongateAndUpdateAMG128Kernel (src/AMGSolver.cu:1216-1255) on LEAF|NONLEAF|GHOST tiles of a level
-__global__ __launch_bounds__(512) void fas_prolong(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
-                                                   const float* __restrict__ X0, float P, int level,
-                                                   const double* __restrict__ S)
-{
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    const int v = threadIdx.x;
-    const size_t idx = size_t(t) * 512 + v;
-    const int pt = tp.parent[t];
-    const int o = tp.toct[t];
-    const int pv = cidx(4 * (o & 1) + ((v >> 6) >> 1), 4 * ((o >> 1) & 1) + (((v >> 3) & 7) >> 1), 4 * ((o >> 2) & 1) + ((v & 7) >> 1));
-    const float cx = pt < 0 ? 0.0f : X[size_t(pt) * 512 + pv];
-    const float cx0 = pt < 0 ? 0.0f : X0[size_t(pt) * 512 + pv];
-    const bool interior = tp.ctype[idx] & INTERIOR;
-    if (tp.ttype[t] & GHOST) X[idx] = interior ? cx : 0.0f;
-    else X[idx] = interior ? X[idx] + P * (cx - cx0) : 0.0f;
-    (void)level;
-}
-
 // PropagateToChildren on GHOST tiles: ghost cell = parent cell (src/PoissonGrid.cu:377-405)
 __global__ __launch_bounds__(512) void propagate_ghosts(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X)
 {
@@ -775,76 +855,6 @@ __global__ __launch_bounds__(512) void fill_all(float* __restrict__ X, size_t n4
     float4* p = reinterpret_cast<float4*>(X);
     const float4 v4 = make_float4(value, value, value, value);
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += size_t(gridDim.x) * blockDim.x) p[i] = v4;
-}
-
-__global__ __launch_bounds__(512) void fill_list(const int32_t* __restrict__ list, float* __restrict__ X, float value)
-{
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    X[size_t(t) * 512 + threadIdx.x] = value;
-}
-
-__global__ __launch_bounds__(512) void copy_list(const int32_t* __restrict__ list, const float* __restrict__ A, float* __restrict__ B,
-                                                 const double* __restrict__ S)
-{
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    const size_t idx = size_t(t) * 512 + threadIdx.x;
-    B[idx] = A[idx];
-}
-
-// partial[wg] = sum over the tile of a*b (exact products, FP64 accumulation)
-__global__ __launch_bounds__(512) void dot_list(const int32_t* __restrict__ list, const float* __restrict__ A,
-                                                const float* __restrict__ B, double* __restrict__ partial,
-                                                const double* __restrict__ S)
-{
-    __shared__ double red[8];
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    double v = 0;
-    if (t >= 0) {
-        const size_t idx = size_t(t) * 512 + threadIdx.x;
-        v = double(A[idx]) * double(B[idx]);
-    }
-    double b = block_sum(v, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
-}
-
-// x += a p ; r -= a Ap on INTERIOR cells of LEAF tiles, partial |r|^2 (AMGSolver.cu:1462-1485)
-__global__ __launch_bounds__(512) void update_xr(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
-                                                 float* __restrict__ R, const float* __restrict__ P,
-                                                 const float* __restrict__ AP, double* __restrict__ partial,
-                                                 const double* __restrict__ S)
-{
-    __shared__ double red[8];
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    double v = 0;
-    if (t >= 0) {
-        const size_t idx = size_t(t) * 512 + threadIdx.x;
-        const float a = float(S[S_ALPHA]);
-        float r = R[idx];
-        if (tp.ctype[idx] & INTERIOR) {
-            X[idx] = fmaf(a, P[idx], X[idx]);
-            r = fmaf(-a, AP[idx], r);
-            R[idx] = r;
-        }
-        v = double(r) * double(r);
-    }
-    double b = block_sum(v, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
-}
-
-// p = z + b p on INTERIOR cells of LEAF tiles (AMGSolver.cu:1525-1536)
-__global__ __launch_bounds__(512) void update_p(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ P,
-                                                const float* __restrict__ Z, const double* __restrict__ S)
-{
-    if (stopped(S)) return;
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    const size_t idx = size_t(t) * 512 + threadIdx.x;
-    if (tp.ctype[idx] & INTERIOR) P[idx] = fmaf(float(S[S_BETA]), P[idx], Z[idx]);
 }
 
 // pure-Neumann recentring (ReCenterLeafCells, src/PoissonGrid.cu:889-921)
@@ -1007,15 +1017,47 @@ __global__ __launch_bounds__(256) void coarse_direct(const double* __restrict__ 
 
 static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 
-// threads per tile of the stencil kernels (512 / 256 / 128: 1 / 2 / 4 cells per thread)
+// Kernel forms of the stencil families.  Threads per tile: 512 / 256 / 128
+// (1 / 2 / 4 cells per thread).  Persistence mask: bit 0 apply, bit 1 Jacobi,
+// bit 2 residual + restrict; a set bit selects the persistent pipelined kernel,
+// a clear bit the one-tile-per-workgroup kernel.  Defaults may be overridden by
+// the environment (CMG_STENCIL_THREADS, CMG_PERSISTENT) or set_kernel_forms.
+enum { PERSIST_APPLY = 1, PERSIST_JACOBI = 2, PERSIST_RESTRICT = 4 };
+static std::atomic<int> g_threads{0}, g_mask{-1};
+
 static int stencil_threads()
 {
-    static int nt = [] {
-        const char* e = getenv("CMG_STENCIL_THREADS");
-        int v = e ? atoi(e) : 256;
-        return (v == 128 || v == 256 || v == 512) ? v : 256;
-    }();
-    return nt;
+    int v = g_threads.load(std::memory_order_relaxed);
+    if (v) return v;
+    const char* e = getenv("CMG_STENCIL_THREADS");
+    v = e ? atoi(e) : 256;
+    v = (v == 128 || v == 256 || v == 512) ? v : 256;
+    g_threads.store(v, std::memory_order_relaxed);
+    return v;
+}
+
+static int persistent_mask()
+{
+    int m = g_mask.load(std::memory_order_relaxed);
+    if (m >= 0) return m;
+    const char* e = getenv("CMG_PERSISTENT");
+    m = e ? (atoi(e) & 7) : int(PERSIST_APPLY | PERSIST_JACOBI | PERSIST_RESTRICT);
+    g_mask.store(m, std::memory_order_relaxed);
+    return m;
+}
+
+bool set_kernel_forms(int mask, int threads)
+{
+    if (mask < 0 || mask > 7 || (threads != 128 && threads != 256 && threads != 512)) return false;
+    g_mask.store(mask, std::memory_order_relaxed);
+    g_threads.store(threads, std::memory_order_relaxed);
+    return true;
+}
+
+void get_kernel_forms(int* mask, int* threads)
+{
+    if (mask) *mask = persistent_mask();
+    if (threads) *threads = stencil_threads();
 }
 
 // persistent stencil kernels run exactly as many workgroups as can be resident
@@ -1076,6 +1118,23 @@ static int apply_launch(const DevGrid& d, const float* X, Coef C, float* Y, doub
 int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
 {
     if (!nonempty(d.leafs)) return 0;
+    if (!(persistent_mask() & PERSIST_APPLY)) {
+        const TopoView tp = view(d);
+        if (partial) {
+            switch (stencil_threads()) {
+            case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+            case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+            default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+            }
+        } else {
+            switch (stencil_threads()) {
+            case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+            case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+            default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+            }
+        }
+        return d.leafs.n;
+    }
     switch (stencil_threads()) {
     case 128: return partial ? apply_launch<true, 128>(d, X, C, Y, partial, S, st) : apply_launch<false, 128>(d, X, C, Y, nullptr, S, st);
     case 512: return partial ? apply_launch<true, 512>(d, X, C, Y, partial, S, st) : apply_launch<false, 512>(d, X, C, Y, nullptr, S, st);
@@ -1103,7 +1162,8 @@ void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, c
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, view(d), l.d, l.n, Xin, Xout, G, C, B, w, S);
+    if (!(persistent_mask() & PERSIST_JACOBI)) CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S);
+    else CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, view(d), l.d, l.n, Xin, Xout, G, C, B, w, S);
 }
 
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
@@ -1118,7 +1178,8 @@ void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, flo
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    CMG_NT_PERSISTENT(k::restrict_p, l.n, st, view(d), l.d, l.n, X, C, B, Rr, S);
+    if (!(persistent_mask() & PERSIST_RESTRICT)) CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
+    else CMG_NT_PERSISTENT(k::restrict_p, l.n, st, view(d), l.d, l.n, X, C, B, Rr, S);
 }
 
 void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st)
@@ -1179,6 +1240,7 @@ int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const floa
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
                      hipStream_t st)
 {
+    if (!S) throw std::invalid_argument("update_xr needs the solver scalars (alpha)");
     if (!nonempty(d.leafs)) return 0;
     const int g = vector_grid(d, d.leafs.n);
     k::update_xr_v<<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S);
@@ -1187,6 +1249,7 @@ int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const
 
 void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st)
 {
+    if (!S) throw std::invalid_argument("update_p needs the solver scalars (beta)");
     if (!nonempty(d.leafs)) return;
     k::update_p_v<<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, P, Z, S);
 }
@@ -1199,6 +1262,7 @@ void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pc
 
 void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st)
 {
+    if (!S) throw std::invalid_argument("sub_mean needs the solver scalars (mean)");
     if (!nonempty(d.leafs)) return;
     k::sub_mean<<<d.leafs.n, 512, 0, st>>>(view(d), d.leafs.d, X, S);
 }
@@ -1235,6 +1299,7 @@ void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t
 
 void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st)
 {
+    if (!S || !partial) throw std::invalid_argument("finish needs the partials and the solver scalars");
     k::finish<<<1, 1024, 0, st>>>(partial, partial2, n, S, kind);
 }
 

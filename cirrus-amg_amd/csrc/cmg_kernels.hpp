@@ -60,6 +60,10 @@ void launch_update_x64(const DevGrid& d, double* X64, const float* E, hipStream_
 void launch_to_double(const DevGrid& d, const LaunchList& l, const float* A, double* B, bool masked, hipStream_t st);
 void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t st);
 void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st);
+// stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
+bool set_kernel_forms(int mask, int threads);
+void get_kernel_forms(int* mask, int* threads);
+
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st);
 

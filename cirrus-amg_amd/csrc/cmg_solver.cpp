@@ -567,6 +567,9 @@ std::string graph_key(const cmg_solve_opts* o)
     std::ostringstream k;
     k << o->cycle << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann;
+    int mask = 0, threads = 0;
+    get_kernel_forms(&mask, &threads);
+    k << ',' << mask << ',' << threads;
     return k.str();
 }
 
@@ -903,18 +906,24 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
         float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *Pp = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
               *Z = dev_channel(d, CH_Z);
         Coef C = coef(d, s->c0);
+        // live scalars for the kernels that read alpha / beta: not stopped, alpha = beta = 1/2
+        double h[S_COUNT] = {};
+        h[S_ALPHA] = 0.5;
+        h[S_BETA] = 0.5;
+        HIPCHK(hipMemcpy(s->S, h, sizeof(h), hipMemcpyHostToDevice));
+        const double* S = s->S;
         int nt = 0;
         auto one = [&]() {
             switch (kernel) {
-            case 0: launch_apply(d, Pp, C, AP, s->partial, nullptr, st); nt = d.leafs.tiles; break;
-            case 1: launch_jacobi(d, L, Z, s->tmp, Z, C, R, 2.0f / 3.0f, nullptr, st); nt = d.subtree[L].tiles; break;
-            case 2: launch_residual_restrict(d, L, Z, C, AP, 1.0f, nullptr, st); nt = d.subtree[L].tiles; break;
-            case 3: launch_update_xr(d, X, Pp, Z, AP, s->partial, nullptr, st); nt = d.leafs.tiles; break;
-            case 4: launch_fas_prolong(d, L, s->tmp, AP, 1.0f, nullptr, st); nt = d.prolong[L].tiles; break;
-            case 5: launch_update_p(d, Pp, Z, nullptr, st); nt = d.leafs.tiles; break;
-            case 6: launch_dot(d, d.leafs, R, Z, s->partial, nullptr, st); nt = d.leafs.tiles; break;
-            case 7: launch_jacobi_zero(d, L, s->tmp, C.c[3], R, 2.0f / 3.0f, nullptr, st); nt = d.subtree[L].tiles; break;
-            case 8: launch_copy_list(d, d.leafs, Z, s->tmp, nullptr, st); nt = d.leafs.tiles; break;
+            case 0: launch_apply(d, Pp, C, AP, s->partial, S, st); nt = d.leafs.tiles; break;
+            case 1: launch_jacobi(d, L, Z, s->tmp, Z, C, R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
+            case 2: launch_residual_restrict(d, L, Z, C, AP, 1.0f, S, st); nt = d.subtree[L].tiles; break;
+            case 3: launch_update_xr(d, X, Pp, Z, AP, s->partial, S, st); nt = d.leafs.tiles; break;
+            case 4: launch_fas_prolong(d, L, s->tmp, AP, 1.0f, S, st); nt = d.prolong[L].tiles; break;
+            case 5: launch_update_p(d, Pp, Z, S, st); nt = d.leafs.tiles; break;
+            case 6: launch_dot(d, d.leafs, R, Z, s->partial, S, st); nt = d.leafs.tiles; break;
+            case 7: launch_jacobi_zero(d, L, s->tmp, C.c[3], R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
+            case 8: launch_copy_list(d, d.leafs, Z, s->tmp, S, st); nt = d.leafs.tiles; break;
             }
         };
         one();

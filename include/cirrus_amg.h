@@ -216,9 +216,21 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * 1 = damped-Jacobi sweep on the finest level, 2 = residual + restriction on the
  * finest level, 3 = PCG x/r update (+ r.r partials), 4 = FAS prolongation on the
  * finest level, 5 = PCG p update, 6 = r.z partials, 7 = first Jacobi sweep from
- * zero on the finest level, 8 = copy of one vector on LEAF tiles.  No reference counterpart (the reference times whole solves
+ * zero on the finest level, 8 = copy of one vector on LEAF tiles.  Overwrites the solver's work channels
+ * (0-4) and scalars.  No reference counterpart (the reference times whole solves
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
+
+/* Forms of the stencil kernels (operator, Jacobi sweep, residual + restriction),
+ * process-wide.  persistent_mask: bit 0 operator, bit 1 Jacobi, bit 2 residual +
+ * restriction; a set bit selects a persistent pipelined kernel (each workgroup
+ * walks several tiles, prefetching the next tile's loads), a clear bit one tile
+ * per workgroup.  stencil_threads: 128, 256 or 512 threads per tile.  Results
+ * are bit-identical across forms.  No reference counterpart (the reference
+ * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
+ * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
+cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);
+cmg_status cmg_get_kernel_forms(int* persistent_mask, int* stencil_threads);
 
 /* hipDeviceSynchronize on the current device (timing brackets) */
 cmg_status cmg_device_synchronize(void);

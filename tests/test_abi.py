@@ -49,3 +49,16 @@ def test_hash_full_is_reported():
     with pytest.raises(P.CmgError) as e:
         g.iterativeRefine(P.make_target(P.TARGET_UNIFORM, 2, 2))
     assert e.value.status == 7
+
+
+def test_kernel_forms_validation():
+    saved = P.get_kernel_forms()
+    try:
+        for bad in ((8, 256), (-1, 256), (7, 64), (0, 1024)):
+            with pytest.raises(P.CmgError) as e:
+                P.set_kernel_forms(*bad)
+            assert e.value.status == P.lib().cmg_set_kernel_forms(*bad)
+        P.set_kernel_forms(5, 128)
+        assert P.get_kernel_forms() == (5, 128)
+    finally:
+        P.set_kernel_forms(*saved)

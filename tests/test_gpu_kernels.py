@@ -112,3 +112,45 @@ def test_dot_fp64(pair):
     ref = float(np.sum(a[leaf].astype(np.float64) * b[leaf].astype(np.float64)))
     got = gp.dot(2, 3)
     assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref)) * 100, (got, ref)
+
+
+FORMS = [(mask, threads) for mask in (0, 7) for threads in (128, 256, 512)]
+
+
+@pytest.fixture
+def restore_forms():
+    P = H.P
+    saved = P.get_kernel_forms()
+    yield P
+    P.set_kernel_forms(*saved)
+
+
+@pytest.mark.parametrize("forms", FORMS, ids=["mask%d_nt%d" % f for f in FORMS])
+def test_kernel_forms_bitexact(pair, forms, restore_forms):
+    """Every form of the stencil kernels (persistent or one tile per workgroup,
+    128/256/512 threads per tile) gives the oracle's bits."""
+    name, go, gp = pair
+    P = restore_forms
+    P.set_kernel_forms(*forms)
+    assert P.get_kernel_forms() == forms
+    lib = __import__("oracle").lib()
+    _load(go, gp, 0, H.random_fill(go.N, 101))
+    go.full_neg_lap(0, H.C0, 3)
+    gp.full_neg_lap(0, H.C0, 3)
+    leaf = _leaf_rows(go)
+    assert np.array_equal(H.bits(go.channel(3)[leaf]), H.bits(gp.channel(3)[leaf])), ("apply", name, forms)
+    for level in range(go.max_level + 1):
+        _load(go, gp, 1, H.random_fill(go.N, 110 + level))
+        _load(go, gp, 5, H.random_fill(go.N, 120 + level))
+        _load(go, gp, 6, H.random_fill(go.N, 130 + level))
+        lib.og_jacobi(go.h, level, 5, 6, 0, H.C0, 1, np.float32(2.0 / 3.0))
+        gp.jacobi(level, 5, 6, 0, H.C0, 1, np.float32(2.0 / 3.0))
+        assert np.array_equal(H.bits(go.channel(6)), H.bits(gp.channel(6))), ("jacobi", name, level, forms)
+    for level in range(1, go.max_level + 1):
+        for ch, seed in ((0, 140), (1, 141)):
+            _load(go, gp, ch, H.random_fill(go.N, seed + level))
+        lib.og_residual_restrict(go.h, level, 0, H.C0, 1, 1.0)
+        gp.residual_restrict(level, 0, H.C0, 1, 1.0)
+        assert np.array_equal(H.bits(go.channel(1)), H.bits(gp.channel(1))), ("restrict b", name, level, forms)
+        assert np.array_equal(H.bits(go.channel(0)), H.bits(gp.channel(0))), ("restrict x", name, level, forms)
+

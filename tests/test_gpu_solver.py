@@ -171,3 +171,27 @@ def test_pure_neumann():
     oit, orel = go2.pcg(O.make_cycle(), H.C0, 100, 1e-6, True)
     print("pure neumann gpu", it, rel, "oracle", oit, orel)
     assert rel < 1e-6 and abs(it - oit) <= 1
+
+
+@pytest.mark.parametrize("forms", [(0, 256), (7, 512), (2, 128)], ids=["onetile_nt256", "persistent_nt512", "mixed_nt128"])
+def test_pcg_same_across_kernel_forms(forms):
+    """The kernel forms change only the grouping of the FP64 partial sums: the
+    same solve converges in the same number of iterations to the same x."""
+    print("\nforms", forms, flush=True)
+    go, gp = H.build_pair("c3_small")
+    b = _rhs(go, gp, 9)
+    opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1)
+    saved = P.get_kernel_forms()
+    try:
+        solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+        it0, rel0 = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
+        x0 = gp.channel(0).astype(np.float64)
+        P.set_kernel_forms(*forms)
+        gp.set_channel(1, b)
+        it1, rel1 = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
+        x1 = gp.channel(0).astype(np.float64)
+    finally:
+        P.set_kernel_forms(*saved)
+    assert rel0 < 1e-6 and rel1 < 1e-6
+    assert abs(it1 - it0) <= 1, (it0, it1)
+    assert np.linalg.norm(x1 - x0) <= 1e-5 * np.linalg.norm(x0)
