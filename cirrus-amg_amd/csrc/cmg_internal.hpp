@@ -18,6 +18,7 @@ enum Scalar : int {
     S_GAMMA = 2, S_GAMMA_OLD = 3, S_ALPHA = 4, S_BETA = 5, S_FP = 6,
     S_RR = 7, S_RHS2 = 8, S_THR2 = 9, S_MEAN = 10, S_TMP = 11,
     S_MAXIT = 12,
+    S_STRIDE = 13,   // convergence tested when S_ITER % S_STRIDE == 0; 0 = never
     S_COUNT = 16
 };
 

@@ -975,7 +975,7 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
         S[S_RR] = s;
         S[S_ITER] += 1;
         if (!isfinite(s)) S[S_DONE] = 3;
-        else if (s < S[S_THR2]) S[S_DONE] = 1;
+        else if (S[S_STRIDE] > 0 && s < S[S_THR2] && fmod(S[S_ITER], S[S_STRIDE]) == 0.0) S[S_DONE] = 1;
         else if (S[S_ITER] >= S[S_MAXIT]) S[S_DONE] = 4;
         break;
     case FIN_BETA:

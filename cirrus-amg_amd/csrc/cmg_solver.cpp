@@ -617,7 +617,10 @@ void read_scalars(cmg_solver* s, double* h)
 }
 
 // one FP32 PCG solve on channels 0-4; returns the device status code
-int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, bool warm, int* iters, double* rel_res)
+// stride > 0: the stopping test runs after iterations stride, 2 stride, ...
+// (AMGSolver.cu:1487-1497); stride <= 0: exactly max_iters iterations, no test
+int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, bool warm, int stride, int* iters,
+          double* rel_res)
 {
     DevGrid& d = s->g->dev;
     hipStream_t st = d.stream;
@@ -658,6 +661,7 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     init[S_THR2] = thr2;
     init[S_MAXIT] = max_iters;
     init[S_RHS2] = rhs2;
+    init[S_STRIDE] = stride > 0 ? stride : 0;
     HIPCHK(hipMemcpyAsync(s->S, init, sizeof(init), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     if (o->is_pure_neumann) enqueue_recenter(s, R);
@@ -684,10 +688,10 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
             s->gkey = key;
         }
     }
-    const int stride = std::max(1, o->sync_stride);
+    const int poll = stride > 0 ? stride : std::max(1, max_iters);
     int launched = 0;
     for (;;) {
-        const int k = std::min(stride, max_iters - launched);
+        const int k = std::min(poll, max_iters - launched);
         for (int i = 0; i < k; ++i) {
             if (o->use_graph) HIPCHK(hipGraphLaunch(s->gexec, st));
             else enqueue_iteration(s, o);
@@ -699,6 +703,11 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     *iters = int(h[S_ITER]);
     *rel_res = std::sqrt(h[S_RR] / rhs2);
     const int done = int(h[S_DONE]);
+    if (stride <= 0) {
+        // no synchronised dot: the reference returns (max_iters, -1)
+        *rel_res = -1;
+        return done == 4 ? 1 : done;
+    }
     if (done == 1) return 1;
     if (done == 2 || done == 3) return done;
     return 4;
@@ -824,7 +833,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         if (o->precision == CMG_PREC_FP32) {
             int iters = 0;
             double rel = 0;
-            done = pcg32(s, o, o->relative_tolerance, o->max_iters, o->warm_start != 0, &iters, &rel);
+            done = pcg32(s, o, o->relative_tolerance, o->max_iters, o->warm_start != 0, o->sync_stride, &iters, &rel);
             st->iters = iters;
             st->rel_res = rel;
             HIPCHK(hipEventRecord(s->e1, stream));
@@ -853,7 +862,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
             while (bn2 > 0 && std::sqrt(rn2 / bn2) >= otol && outer < 50 && total < o->max_iters) {
                 int iters = 0;
                 double rel = 0;
-                int dd = pcg32(s, o, itol, o->max_iters - total, false, &iters, &rel);
+                int dd = pcg32(s, o, itol, o->max_iters - total, false, std::max(1, o->sync_stride), &iters, &rel);
                 total += iters;
                 ++outer;
                 if (dd == 2 || dd == 3) { done = dd; break; }

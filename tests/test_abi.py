@@ -2,6 +2,7 @@
 import ctypes as C
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -62,3 +63,17 @@ def test_kernel_forms_validation():
         assert P.get_kernel_forms() == (5, 128)
     finally:
         P.set_kernel_forms(*saved)
+
+
+def _wrapper_binary():
+    here = os.path.dirname(os.path.abspath(__file__))
+    subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], check=True)
+    return os.path.join(here, "cpp", "_build", "test_amg_wrapper")
+
+
+def test_cpp_wrapper_cpu_part():
+    """A C++ caller written against the reference's class API (HADeviceGrid,
+    AMGSolver) builds and runs its host part through include/cirrus_amg.hpp."""
+    r = subprocess.run([_wrapper_binary(), "cpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "cpu part OK" in r.stdout

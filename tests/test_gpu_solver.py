@@ -195,3 +195,42 @@ def test_pcg_same_across_kernel_forms(forms):
     assert rel0 < 1e-6 and rel1 < 1e-6
     assert abs(it1 - it0) <= 1, (it0, it1)
     assert np.linalg.norm(x1 - x0) <= 1e-5 * np.linalg.norm(x0)
+
+
+def test_sync_stride_semantics():
+    """AMGSolver::solve's sync_stride (src/AMGSolver.cu:1410-1549): the stopping
+    test runs after iterations stride, 2*stride, ...; stride <= 0 runs exactly
+    max_iters iterations and returns (max_iters, -1)."""
+    go, gp = H.build_pair("sphere_empty24")
+    b = _rhs(go, gp, 12)
+    opts = P.default_opts()
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it1, rel1 = solver.solve(max_iters=100, relative_tolerance=1e-6, sync_stride=1, opts=opts)
+    assert rel1 < 1e-6
+    for stride in (2, 3, 5):
+        gp.set_channel(1, b)
+        it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, sync_stride=stride, opts=opts)
+        assert it % stride == 0 and it1 <= it < it1 + stride and rel < 1e-6, (stride, it, it1)
+    gp.set_channel(1, b)
+    it, rel = solver.solve(max_iters=4, relative_tolerance=1e-6, sync_stride=-1, opts=opts)
+    assert (it, rel) == (4, -1.0)
+    x_nosync = gp.channel(0).copy()
+    gp.set_channel(1, b)
+    it, rel = solver.solve(max_iters=4, relative_tolerance=0.0, sync_stride=1, opts=opts)
+    assert it == 4
+    assert np.array_equal(H.bits(x_nosync), H.bits(gp.channel(0)))
+
+
+def test_cpp_wrapper_solves():
+    """The C++ caller (include/cirrus_amg.hpp) runs FluidEuler::project's solve
+    call, the V-cycle form and the unsynchronised form on the 64^3 grid."""
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(here, "cpp", "_build", "test_amg_wrapper")
+    assert os.path.exists(exe), "build tests/cpp first (make -C tests/cpp)"
+    print("\ncpp wrapper gpu", flush=True)
+    r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=100)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "gpu part OK" in r.stdout
