@@ -21,7 +21,7 @@ X_CHANNEL, B_CHANNEL, P_CHANNEL, AP_CHANNEL, Z_CHANNEL = 0, 1, 2, 3, 4   # src/P
 TARGET_UNIFORM, TARGET_CENTER_POINT, TARGET_SPHERE_EMPTY, TARGET_SPHERE_SOLID = 0, 1, 2, 3
 TARGET_STAR_EMPTY, TARGET_STAR_SOLID, TARGET_SDF_BANDS, TARGET_CALLBACK = 4, 5, 6, 7
 SDF_SPHERE, SDF_TORUS, SDF_STAR = 0, 1, 2
-CYCLE_FAS_W, CYCLE_FAS_V = 0, 1
+CYCLE_FAS_W, CYCLE_FAS_V, CYCLE_FAS_F, CYCLE_VCYCLE, CYCLE_MU = 0, 1, 2, 3, 4
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 PREC_FP32, PREC_FP64_REFINE = 0, 1
 
@@ -54,7 +54,7 @@ class SolveOpts(C.Structure):
                 ("level_iters", C.c_int), ("coarsest_iters", C.c_int), ("sync_stride", C.c_int),
                 ("is_pure_neumann", C.c_int), ("cycle", C.c_int), ("smoother", C.c_int), ("omega", C.c_double),
                 ("coarsest_direct", C.c_int), ("precision", C.c_int), ("outer_tolerance", C.c_double),
-                ("inner_tolerance", C.c_double), ("warm_start", C.c_int), ("use_graph", C.c_int)]
+                ("inner_tolerance", C.c_double), ("warm_start", C.c_int), ("use_graph", C.c_int), ("mu", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -74,7 +74,7 @@ EXPORTS = [
     "cmg_op_residual_restrict", "cmg_op_fas_save", "cmg_op_fas_prolong", "cmg_op_dot", "cmg_op_cycle",
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
     "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize", "cmg_set_kernel_forms",
-    "cmg_get_kernel_forms",
+    "cmg_get_kernel_forms", "cmg_mg_solve",
 ]
 
 _lib = None
@@ -116,6 +116,7 @@ def lib():
             "cmg_grid_copy_channel": (st, [vp, i, i]),
             "cmg_device_synchronize": (st, []),
             "cmg_set_kernel_forms": (st, [i, i]),
+            "cmg_mg_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
             "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
             "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
@@ -371,6 +372,17 @@ class AMGSolver:
         self.last_stats = stats
         if status not in (0, 6) or (status == 6 and raise_on_nonconvergence):
             raise CmgError(status, "cmg_solve")
+        return stats.iters, stats.rel_res
+
+    def mg_solve(self, opts, max_iters=100, relative_tolerance=1e-6):
+        """AMGSolver::FASMuCycleSolve (src/AMGSolver.cu:1311-1356): (loop index at exit, |r|/|b|)."""
+        o = opts
+        o.max_iters, o.relative_tolerance = max_iters, relative_tolerance
+        stats = SolveStats()
+        status = lib().cmg_mg_solve(self.h, C.byref(o), C.byref(stats))
+        self.last_stats = stats
+        if status not in (0, 6):
+            raise CmgError(status, "cmg_mg_solve")
         return stats.iters, stats.rel_res
 
     def x64(self):

@@ -421,6 +421,11 @@ cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P) { 
 cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out) { return op_dot(g, a, b, out); }
 cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0) { return solver_cycle(s, opts, x, rhs, x0); }
 
+cmg_status cmg_mg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
+{
+    return solver_mg_solve(s, opts, stats);
+}
+
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
     return bench_kernel(s, kernel, reps, avg_ms, tiles);

@@ -51,6 +51,7 @@ struct DevGrid {
     LaunchList all;                     // every tile
     LaunchList subtree[kMaxLevels];     // LEAF | NONLEAF at the level
     LaunchList nonleaf[kMaxLevels];
+    LaunchList leafat[kMaxLevels];      // LEAF tiles at the level
     LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
     LaunchList ghosts;                  // all GHOST tiles
     int max_list = 0;
@@ -95,6 +96,7 @@ cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_s
 void solver_destroy(cmg_solver* s);
 cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
 cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0);
+cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
 cmg_status solver_get_x64(cmg_solver* s, double* host);
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 cmg_status op_copy(cmg_grid* g, int src, int dst);

@@ -597,8 +597,10 @@ __global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, 
     }
 }
 
-__global__ __launch_bounds__(256) void fill_v(const int32_t* __restrict__ list, int n, float* __restrict__ X, float value)
+__global__ __launch_bounds__(256) void fill_v(const int32_t* __restrict__ list, int n, float* __restrict__ X, float value,
+                                              const double* __restrict__ S)
 {
+    if (stopped(S)) return;
     CMG_TILE_LOOP(list, n)
     {
         reinterpret_cast<float4*>(X)[quad_index(t_)] = make_float4(value, value, value, value);
@@ -1013,6 +1015,151 @@ __global__ __launch_bounds__(256) void coarse_direct(const double* __restrict__ 
 
 }  // namespace k
 
+namespace k {
+
+// ------------------------------------------------- non-FAS transfers ----
+
+// RestrictSourceToCoarse (src/AMGSolver.cu:1295-1307) = AccumulateToParentsOneStep
+// over LEAF|NONLEAF tiles of a level, INTERIOR cells: parent = coeff * sum8
+__global__ __launch_bounds__(64) void restrict_source(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+                                                      float coeff, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const int pt = tp.parent[t];
+    if (pt < 0) return;
+    const int v = threadIdx.x, oc = tp.toct[t];
+    const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
+    float sum = 0;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const size_t idx = size_t(t) * 512 + cidx(fl0 + ii, fl1 + jj, fl2 + kk);
+                sum += (tp.ctype[idx] & INTERIOR) ? X[idx] : 0.0f;
+            }
+    X[size_t(pt) * 512 + cidx(4 * (oc & 1) + (fl0 >> 1), 4 * ((oc >> 1) & 1) + (fl1 >> 1), 4 * ((oc >> 2) & 1) + (fl2 >> 1))] =
+        sum * coeff;
+}
+
+// ProlongateAndUpdateAMG128Kernel (src/AMGSolver.cu:1022-1053), four cells per thread:
+// GHOST x = parent x, others x += P * parent x (non-INTERIOR cells -> 0)
+__global__ __launch_bounds__(256) void prolong_add_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+                                                     float P, const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const int v0 = int(threadIdx.x & 127) * 4;
+        const int pt = tp.parent[t_];
+        const int oc = tp.toct[t_];
+        const bool ghost = tp.ttype[t_] & GHOST;
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        const float4 x = reinterpret_cast<const float4*>(X)[q];
+        const int i = v0 >> 6, j = (v0 >> 3) & 7, k0 = v0 & 7;
+        const int pbase = cidx(4 * (oc & 1) + (i >> 1), 4 * ((oc >> 1) & 1) + (j >> 1), 4 * ((oc >> 2) & 1) + (k0 >> 1));
+        float cx[2] = {0.0f, 0.0f};
+        if (pt >= 0) {
+            const size_t pb = size_t(pt) * 512 + pbase;
+            cx[0] = X[pb];
+            cx[1] = X[pb + 1];
+        }
+        const uint8_t cts[4] = {ct.x, ct.y, ct.z, ct.w};
+        float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool in = cts[e] & INTERIOR;
+            const float c1 = cx[e >> 1];
+            xs[e] = ghost ? (in ? c1 : 0.0f) : (in ? xs[e] + P * c1 : 0.0f);
+        }
+        reinterpret_cast<float4*>(X)[q] = make_float4(xs[0], xs[1], xs[2], xs[3]);
+    }
+}
+
+// DeductFluxOnAbsLeafs128Kernel (src/AMGSolver.cu:966-993) on LEAF tiles of a
+// level: rhs = f - negativeLapOuterNonLeaf(x) on INTERIOR cells, else 0.  Only
+// halo cells in NONLEAF face neighbours carry x; every other term is off * 0,
+// kept so that the FMA chain (and the sign of a zero) is the operator's.
+__global__ __launch_bounds__(512) void deduct_flux(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                   Coef C, const float* __restrict__ F, float* __restrict__ RHS,
+                                                   const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const int v = threadIdx.x;
+    const size_t idx = size_t(t) * 512 + v;
+    if (!(tp.ctype[idx] & INTERIOR)) {
+        RHS[idx] = 0.0f;
+        return;
+    }
+    const float h = tile_h(tp, t);
+    const int l[3] = {v >> 6, (v >> 3) & 7, v & 7};
+    const int stride[3] = {64, 8, 1};
+    const float d = C.c[3][idx];
+    float sum = 0.0f;
+    if (d != 0) {
+        sum = 0.0f * d;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                float off, x1 = 0.0f;
+                const bool inside = sg < 0 ? l[a] > 0 : l[a] < 7;
+                if (sg < 0) off = C.c[a][idx];
+                else if (inside) off = C.c[a][idx + stride[a]];
+                else off = h;
+                if (!inside) {
+                    const int nt = tp.nbr[size_t(t) * 6 + (sg < 0 ? a : a + 3)];
+                    if (nt >= 0) {
+                        const size_t ni = size_t(nt) * 512 + (v + (sg < 0 ? 7 : -7) * stride[a]);
+                        if (sg > 0) off = C.c[a][ni];
+                        if (tp.ttype[nt] == NONLEAF) x1 = tp.ctype[ni] == INTERIOR ? X[ni] : 0.0f;
+                    }
+                }
+                sum = fmaf(off, x1, sum);
+            }
+        }
+    }
+    RHS[idx] = F[idx] - sum;
+}
+
+// initial launch of VCycle / muCycle (src/AMGSolver.cu:1062-1078): x = 0 on every
+// tile, rhs = f on LEAF|NONLEAF tiles and 0 on GHOST tiles
+__global__ __launch_bounds__(256) void nonfas_init_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+                                                     const float* __restrict__ F, float* __restrict__ RHS,
+                                                     const double* __restrict__ S)
+{
+    if (stopped(S)) return;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        reinterpret_cast<float4*>(X)[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        reinterpret_cast<float4*>(RHS)[q] =
+            (tp.ttype[t_] & GHOST) ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : reinterpret_cast<const float4*>(F)[q];
+    }
+}
+
+// Y = B - Y on the tiles of a list (the residual of FASMuCycleSolve, src/AMGSolver.cu:1340-1344)
+__global__ __launch_bounds__(256) void sub_from_v(const int32_t* __restrict__ list, int n, const float* __restrict__ B,
+                                                  float* __restrict__ Y)
+{
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const float4 b = reinterpret_cast<const float4*>(B)[q];
+        float4 y = reinterpret_cast<const float4*>(Y)[q];
+        y.x = b.x - y.x; y.y = b.y - y.y; y.z = b.z - y.z; y.w = b.w - y.w;
+        reinterpret_cast<float4*>(Y)[q] = y;
+    }
+}
+
+}  // namespace k
+
 // ---------------------------------------------------------------- launchers ---
 
 static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
@@ -1216,10 +1363,10 @@ void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, h
     k::fill_all<<<blocks, 512, 0, st>>>(X, n4, value, S);
 }
 
-void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st)
+void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st, const double* S)
 {
     if (!nonempty(l)) return;
-    k::fill_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, X, value);
+    k::fill_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, X, value, S);
 }
 
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st)
@@ -1309,6 +1456,41 @@ void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const 
     if (n == 0 && nz == 0) return;
     const int blocks = std::max(1, (n + 3) / 4);
     k::coarse_direct<<<blocks, 256, size_t(std::max(n, 1)) * sizeof(double), st>>>(inv, cells, n, zero_cells, nz, X, B, S);
+}
+
+void launch_restrict_source(const DevGrid& d, int level, float* X, float coeff, const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return;
+    k::restrict_source<<<l.n, 64, 0, st>>>(view(d), l.d, X, coeff, S);
+}
+
+void launch_prolong_add(const DevGrid& d, int level, float* X, float P, const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.prolong[level];
+    if (!nonempty(l)) return;
+    k::prolong_add_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, X, P, S);
+}
+
+void launch_deduct_flux(const DevGrid& d, int level, const float* X, Coef C, const float* F, float* RHS, const double* S,
+                        hipStream_t st)
+{
+    const LaunchList& l = d.leafat[level];
+    if (!nonempty(l)) return;
+    k::deduct_flux<<<l.n, 512, 0, st>>>(view(d), l.d, X, C, F, RHS, S);
+}
+
+void launch_nonfas_init(const DevGrid& d, float* X, const float* F, float* RHS, const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.all;
+    if (!nonempty(l)) return;
+    k::nonfas_init_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, X, F, RHS, S);
+}
+
+void launch_sub_from(const DevGrid& d, const LaunchList& l, const float* B, float* Y, hipStream_t st)
+{
+    if (!nonempty(l)) return;
+    k::sub_from_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, B, Y);
 }
 
 }  // namespace cmg

@@ -45,7 +45,8 @@ void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, 
 void launch_propagate_ghosts(const DevGrid& d, float* X, hipStream_t st);
 void launch_accumulate_leafs(const DevGrid& d, float* X, hipStream_t st);
 void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, hipStream_t st);
-void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st);
+void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st,
+                      const double* S = nullptr);
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
 int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const float* B, double* partial, const double* S,
                hipStream_t st);
@@ -60,6 +61,14 @@ void launch_update_x64(const DevGrid& d, double* X64, const float* E, hipStream_
 void launch_to_double(const DevGrid& d, const LaunchList& l, const float* A, double* B, bool masked, hipStream_t st);
 void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t st);
 void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st);
+// non-FAS cycles and FASFCycle / FASMuCycleSolve
+void launch_restrict_source(const DevGrid& d, int level, float* X, float coeff, const double* S, hipStream_t st);
+void launch_prolong_add(const DevGrid& d, int level, float* X, float P, const double* S, hipStream_t st);
+void launch_deduct_flux(const DevGrid& d, int level, const float* X, Coef C, const float* F, float* RHS, const double* S,
+                        hipStream_t st);
+void launch_nonfas_init(const DevGrid& d, float* X, const float* F, float* RHS, const double* S, hipStream_t st);
+void launch_sub_from(const DevGrid& d, const LaunchList& l, const float* B, float* Y, hipStream_t st);
+
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
 bool set_kernel_forms(int mask, int threads);
 void get_kernel_forms(int* mask, int* threads);

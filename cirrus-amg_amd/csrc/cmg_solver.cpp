@@ -144,6 +144,7 @@ void dev_free(DevGrid& d)
         free_list(d.subtree[l]);
         free_list(d.nonleaf[l]);
         free_list(d.prolong[l]);
+        free_list(d.leafat[l]);
     }
     if (d.stream) (void)hipStreamDestroy(d.stream);
     d = DevGrid{};
@@ -176,7 +177,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
     HIPCHK(hipMemset(d.ctype, 0, size_t(N) * kCells));
     std::vector<int32_t> leafs, all, ghosts;
-    std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels];
+    std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels], la[kMaxLevels];
     for (int t = 0; t < N; ++t) {
         const uint8_t ty = topo.t_type[t];
         const int l = topo.t_level[t];
@@ -185,6 +186,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
         if (ty & GHOST) ghosts.push_back(t);
         if (ty & (LEAF | NONLEAF)) sub[l].push_back(t);
         if (ty & NONLEAF) nl[l].push_back(t);
+        if (ty & LEAF) la[l].push_back(t);
         pr[l].push_back(t);
     }
     d.leafs = make_list(topo, leafs);
@@ -195,6 +197,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
         d.subtree[l] = make_list(topo, sub[l]);
         d.nonleaf[l] = make_list(topo, nl[l]);
         d.prolong[l] = make_list(topo, pr[l]);
+        d.leafat[l] = make_list(topo, la[l]);
     }
     {
         int dev = 0, cus = 0;
@@ -502,38 +505,125 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
     if (cur != c.X) launch_copy_list(d, d.subtree[level], cur, c.X, c.S, st);
 }
 
-// FASMuCycleStep (src/AMGSolver.cu:1257-1277)
-void cycle_step(const CycleCtx& c, int level)
+// repetitions of the coarse visit (AMGSolver::mu_cycle_repeat_times)
+int cycle_mu(const cmg_solve_opts* o)
+{
+    if (o->mu > 0) return o->mu;
+    return o->cycle == CMG_CYCLE_FAS_V ? 1 : 2;
+}
+
+void coarsest(const CycleCtx& c)
+{
+    DevGrid& d = c.s->g->dev;
+    if (c.o->coarsest_direct) {
+        launch_coarse_direct(c.s->inv0, c.s->cells0, c.s->n0, c.s->zero0, c.s->nz0, c.X, c.B, c.S, d.stream);
+    } else if (c.o->smoother == CMG_SMOOTH_RBGS) {
+        smooth(c, 0, c.o->coarsest_iters / 2, 0);
+        smooth(c, 0, c.o->coarsest_iters / 2, 1);
+    } else {
+        smooth(c, 0, c.o->coarsest_iters, 0);
+    }
+}
+
+// FASMuCycleStep (src/AMGSolver.cu:1257-1277); x_zero: x is still 0 everywhere
+void cycle_step(const CycleCtx& c, int level, bool x_zero)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
     if (level == 0) {
-        if (c.o->coarsest_direct) {
-            launch_coarse_direct(c.s->inv0, c.s->cells0, c.s->n0, c.s->zero0, c.s->nz0, c.X, c.B, c.S, st);
-        } else if (c.o->smoother == CMG_SMOOTH_RBGS) {
-            smooth(c, 0, c.o->coarsest_iters / 2, 0);
-            smooth(c, 0, c.o->coarsest_iters / 2, 1);
-        } else {
-            smooth(c, 0, c.o->coarsest_iters, 0);
-        }
+        coarsest(c);
         return;
     }
-    smooth(c, level, c.o->level_iters, 0, level == d.max_level);
+    smooth(c, level, c.o->level_iters, 0, x_zero);
     launch_residual_restrict(d, level, c.X, c.C, c.B, c.s->Rr, c.S, st);
     launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
-    const int mu = c.o->cycle == CMG_CYCLE_FAS_W ? 2 : 1;
-    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1);
+    const int mu = cycle_mu(c.o);
+    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false);
     launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
     smooth(c, level, c.o->level_iters, 1);
 }
 
-// FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
+// AMGSolver::FASFCycle (src/AMGSolver.cu:1358-1387)
+void fas_fcycle(const CycleCtx& c)
+{
+    DevGrid& d = c.s->g->dev;
+    hipStream_t st = d.stream;
+    launch_fill_all(d, c.X, 0.0f, c.S, st);
+    for (int l = d.max_level; l > 0; --l) launch_restrict_source(d, l, c.B, c.s->Rr, c.S, st);
+    coarsest(c);
+    for (int l = 1; l <= d.max_level; ++l) {
+        launch_prolong_add(d, l, c.X, c.s->P, c.S, st);
+        cycle_step(c, l, false);
+    }
+}
+
+// AMGSolver::VCycle (src/AMGSolver.cu:1059-1128): non-FAS; c.X0 holds f, c.B is the working rhs
+void vcycle(const CycleCtx& c)
+{
+    DevGrid& d = c.s->g->dev;
+    hipStream_t st = d.stream;
+    launch_nonfas_init(d, c.X, c.X0, c.B, c.S, st);
+    for (int l = d.max_level; l > 0; --l) {
+        smooth(c, l, c.o->level_iters, 0, l == d.max_level);
+        launch_residual_restrict(d, l, c.X, c.C, c.B, c.s->Rr, c.S, st);
+    }
+    coarsest(c);
+    for (int l = 1; l <= d.max_level; ++l) {
+        launch_prolong_add(d, l, c.X, c.s->P, c.S, st);
+        smooth(c, l, c.o->level_iters, 1);
+    }
+}
+
+// AMGSolver::muCycleStep (src/AMGSolver.cu:1130-1152): non-FAS with flux correction
+void mu_step(const CycleCtx& c, int level, bool x_zero)
+{
+    DevGrid& d = c.s->g->dev;
+    hipStream_t st = d.stream;
+    if (level == 0) {
+        coarsest(c);
+        return;
+    }
+    smooth(c, level, c.o->level_iters, 0, x_zero);
+    launch_residual_restrict(d, level, c.X, c.C, c.B, c.s->Rr, c.S, st);
+    launch_deduct_flux(d, level - 1, c.X, c.C, c.X0, c.B, c.S, st);
+    launch_fill_list(d, d.nonleaf[level - 1], c.X, 0.0f, st, c.S);
+    const int mu = cycle_mu(c.o);
+    for (int i = 0; i < mu; ++i) mu_step(c, level - 1, false);
+    launch_prolong_add(d, level, c.X, c.s->P, c.S, st);
+    smooth(c, level, c.o->level_iters, 1);
+}
+
+// the preconditioner z ~ A^-1 r of the options' cycle kind.  FAS kinds: x, rhs
+// = r (its NONLEAF tiles become coarse right-hand sides), x0 = FAS scratch.
+// Non-FAS kinds: x, f = r (kept), x0 = working right-hand side.
 void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0, const double* S)
 {
     DevGrid& d = s->g->dev;
-    CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), coef(d, s->c0), S};
-    launch_fill_all(d, c.X, 0.0f, S, d.stream);
-    cycle_step(c, d.max_level);
+    const Coef C = coef(d, s->c0);
+    switch (o->cycle) {
+    case CMG_CYCLE_VCYCLE:
+    case CMG_CYCLE_MU: {
+        const CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, x0), dev_channel(d, rhs), C, S};
+        if (o->cycle == CMG_CYCLE_VCYCLE) {
+            vcycle(c);
+        } else {
+            launch_nonfas_init(d, c.X, c.X0, c.B, S, d.stream);
+            mu_step(c, d.max_level, true);
+        }
+        break;
+    }
+    case CMG_CYCLE_FAS_F: {
+        const CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), C, S};
+        fas_fcycle(c);
+        break;
+    }
+    default: {
+        // FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
+        const CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), C, S};
+        launch_fill_all(d, c.X, 0.0f, S, d.stream);
+        cycle_step(c, d.max_level, true);
+    }
+    }
 }
 
 void enqueue_recenter(cmg_solver* s, float* R)
@@ -565,7 +655,7 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
 std::string graph_key(const cmg_solve_opts* o)
 {
     std::ostringstream k;
-    k << o->cycle << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
+    k << o->cycle << ',' << cycle_mu(o) << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
@@ -792,7 +882,7 @@ static cmg_status check_opts(const cmg_solve_opts* o)
 {
     if (!o || o->max_iters < 0 || !(o->relative_tolerance >= 0) || o->level_iters < 0 || o->coarsest_iters < 0)
         return CMG_ERR_BAD_ARG;
-    if (o->cycle != CMG_CYCLE_FAS_W && o->cycle != CMG_CYCLE_FAS_V) return CMG_ERR_BAD_ARG;
+    if (o->cycle < CMG_CYCLE_FAS_W || o->cycle > CMG_CYCLE_MU || o->mu < 0) return CMG_ERR_BAD_ARG;
     if (o->smoother != CMG_SMOOTH_RBGS && o->smoother != CMG_SMOOTH_JACOBI) return CMG_ERR_BAD_ARG;
     if (o->precision != CMG_PREC_FP32 && o->precision != CMG_PREC_FP64_REFINE) return CMG_ERR_BAD_ARG;
     return CMG_OK;
@@ -888,6 +978,65 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         }
     });
     return status_of(done);
+}
+
+// AMGSolver::FASMuCycleSolve (src/AMGSolver.cu:1311-1356)
+cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
+{
+    if (!s || !st) return CMG_ERR_BAD_ARG;
+    if (auto e = check_opts(o)) return e;
+    if (o->cycle != CMG_CYCLE_FAS_W && o->cycle != CMG_CYCLE_FAS_V) return CMG_ERR_BAD_ARG;
+    *st = cmg_solve_stats{};
+    bool converged = false;
+    CMG_TRY({
+        DevGrid& d = s->g->dev;
+        hipStream_t stream = d.stream;
+        ensure_buffers(s);
+        if (o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
+            build_direct(s, o->is_pure_neumann);
+        float *X = dev_channel(d, CH_X), *B = dev_channel(d, CH_R), *X0 = dev_channel(d, CH_P);
+        const Coef C = coef(d, s->c0);
+        set_scalars(s, 0.0, o->max_iters);
+        double h[S_COUNT];
+        HIPCHK(hipEventRecord(s->e0, stream));
+        launch_fill_all(d, X, 0.0f, nullptr, stream);
+        int nb = launch_dot(d, d.leafs, B, B, s->partial, nullptr, stream);
+        launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, stream);
+        read_scalars(s, h);
+        const double rhs_norm = std::sqrt(h[S_TMP]);
+        const double thr = std::max(o->relative_tolerance * rhs_norm, 2.2250738585072014e-308);
+        double res = rhs_norm;
+        const CycleCtx c{s, o, X, B, X0, C, nullptr};
+        int i = 0;
+        for (i = 0; i < o->max_iters; ++i) {
+            cycle_step(c, d.max_level, i == 0);
+            // AMGFullNegativeLaplacianOnLeafs(x -> x0) leaves the propagated ghost
+            // and accumulated NONLEAF values in x, as the next cycle sees them
+            launch_propagate_ghosts(d, X, stream);
+            launch_accumulate_leafs(d, X, stream);
+            launch_apply(d, X, C, X0, nullptr, nullptr, stream);
+            launch_sub_from(d, d.leafs, B, X0, stream);
+            nb = launch_dot(d, d.leafs, X0, X0, s->partial, nullptr, stream);
+            launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, stream);
+            read_scalars(s, h);
+            res = std::sqrt(h[S_TMP]);
+            if (!std::isfinite(res)) break;
+            if (res < thr) {
+                converged = true;
+                break;
+            }
+        }
+        HIPCHK(hipEventRecord(s->e1, stream));
+        HIPCHK(hipEventSynchronize(s->e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
+        st->ms = ms;
+        st->iters = i;
+        st->rel_res = rhs_norm > 0 ? res / rhs_norm : 0.0;
+        st->true_rel_res = st->rel_res;
+        if (!std::isfinite(res)) return CMG_ERR_NONFINITE;
+    });
+    return converged ? CMG_OK : CMG_ERR_NOT_CONVERGED;
 }
 
 cmg_status op_copy(cmg_grid* g, int src, int dst)

@@ -137,7 +137,12 @@ cmg_status cmg_rhs_divergence(cmg_grid* g, int channel, int u0, uint64_t seed);
 
 /* ------------------------------------------------------- solver ------------ */
 
-typedef enum { CMG_CYCLE_FAS_W = 0, CMG_CYCLE_FAS_V = 1 } cmg_cycle_kind;
+/* preconditioner cycles of AMGSolver: FAS mu-cycle (FASMuCycle, src/AMGSolver.cu:1257-1292)
+ * as W (mu = 2, the reference default) or V (mu = 1); FAS F-cycle (FASFCycle, :1358-1387);
+ * the non-FAS VCycle (:1059-1128) and muCycle (:1130-1174) */
+typedef enum {
+    CMG_CYCLE_FAS_W = 0, CMG_CYCLE_FAS_V = 1, CMG_CYCLE_FAS_F = 2, CMG_CYCLE_VCYCLE = 3, CMG_CYCLE_MU = 4
+} cmg_cycle_kind;
 typedef enum { CMG_SMOOTH_RBGS = 0, CMG_SMOOTH_JACOBI = 1 } cmg_smoother_kind;
 typedef enum { CMG_PREC_FP32 = 0, CMG_PREC_FP64_REFINE = 1 } cmg_precision;
 
@@ -160,6 +165,7 @@ typedef struct cmg_solve_opts {
     double inner_tolerance;   /* FP64 refinement: relative tolerance of each inner FP32 solve */
     int warm_start;           /* 1: start from channel x (FP32) instead of 0 */
     int use_graph;            /* 1: capture one PCG iteration in a hipGraph */
+    int mu;                   /* coarse-visit repetitions (mu_cycle_repeat_times); 0: 1 for FAS_V, else 2 */
 } cmg_solve_opts;
 
 typedef struct cmg_solve_stats {
@@ -180,6 +186,13 @@ void cmg_solver_destroy(cmg_solver* s);
  * channel 1 on LEAF INTERIOR cells; x lands in channel 0; channels 1-4 of
  * every tile are clobbered (1 ends as the residual). */
 cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats);
+/* AMGSolver::FASMuCycleSolve (src/AMGSolver.cu:1311-1356): FAS mu-cycles as a
+ * stand-alone iteration from x = 0 (channel 0) on b (channel 1, kept), residual
+ * in channel 2, until |r| < relative_tolerance |b| or max_iters cycles.  Uses
+ * opts' cycle (mu), smoother, omega, level_iters, coarsest_iters /
+ * coarsest_direct.  stats->iters is the reference's return value: the loop
+ * index at exit (cycles run - 1 when converged). */
+cmg_status cmg_mg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats);
 /* the FP64 solution of the last FP64-refinement solve (N*512 doubles) */
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host);
 

@@ -333,6 +333,109 @@ void og_fas_prolong(og_grid* g, int level, int x, int x0, float P)
     }
 }
 
+/* RestrictSourceToCoarse (AMGSolver.cu:1295-1307) =
+ * AccumulateToParentsOneStepKernel (PoissonGrid.cu:408-451) over the LEAF
+ * subtree tiles of `level`, INTERIOR cells, overwriting the parent */
+void og_restrict_source(og_grid* g, int level, int ch, float coeff)
+{
+    #pragma omp parallel for schedule(dynamic, 8)
+    for (int t = g->lstart[level]; t < g->lstart[level + 1]; t++) {
+        if (!(og_subtree_type(g->t_type[t], level, level) & OG_LEAF)) continue;
+        const int32_t* c = &g->t_coord[3 * t];
+        for (int q = 0; q < 64; q++) {
+            int fl[3] = { q / 16 * 2, (q / 4) % 4 * 2, q % 4 * 2 };
+            int off, p = og_voxel(g, level - 1, (c[0] * 8 + fl[0]) >> 1, (c[1] * 8 + fl[1]) >> 1, (c[2] * 8 + fl[2]) >> 1, &off);
+            if (p < 0) continue;
+            float sum = 0;
+            for (int ii = 0; ii < 2; ii++)
+                for (int jj = 0; jj < 2; jj++)
+                    for (int kk = 0; kk < 2; kk++) {
+                        size_t idx = (size_t)t * 512 + og_vi(fl[0] + ii, fl[1] + jj, fl[2] + kk);
+                        sum += (g->ctype[idx] & OG_INTERIOR) ? g->ch[ch][idx] : 0.0f;
+                    }
+            g->ch[ch][(size_t)p * 512 + off] = sum * coeff;
+        }
+    }
+}
+
+/* ProlongateAndUpdateAMG128Kernel (AMGSolver.cu:1022-1053) on LEAF|GHOST
+ * subtree tiles of `level`: GHOST x = parent x, others x += P * parent x */
+void og_prolong_add(og_grid* g, int level, int x, float P)
+{
+    #pragma omp parallel for schedule(dynamic, 8)
+    for (int t = g->lstart[level]; t < g->lstart[level + 1]; t++) {
+        uint8_t ft = og_subtree_type(g->t_type[t], level, level);
+        if (!(ft & (OG_LEAF | OG_GHOST))) continue;
+        const int32_t* c = &g->t_coord[3 * t];
+        for (int v = 0; v < 512; v++) {
+            size_t idx = (size_t)t * 512 + v;
+            int gi = c[0] * 8 + (v >> 6), gj = c[1] * 8 + ((v >> 3) & 7), gk = c[2] * 8 + (v & 7);
+            int off, p = og_voxel(g, level - 1, gi >> 1, gj >> 1, gk >> 1, &off);
+            float cx = p < 0 ? 0.0f : g->ch[x][(size_t)p * 512 + off];
+            float fx = g->ch[x][idx];
+            uint8_t ct = g->ctype[idx];
+            if (ft & OG_GHOST) g->ch[x][idx] = (ct & OG_INTERIOR) ? cx : 0.0f;
+            else g->ch[x][idx] = (ct & OG_INTERIOR) ? fx + P * cx : 0.0f;
+        }
+    }
+}
+
+/* DeductFluxOnAbsLeafs128Kernel (AMGSolver.cu:966-993) on LEAF tiles of
+ * `level`: rhs = f - negativeLapOuterNonLeaf(x) on INTERIOR cells, else 0.
+ * negativeLapOuterNonLeaf (AMGSolver.cu:383-403) keeps only the terms whose
+ * cell lies in a NONLEAF tile; the own tile is LEAF, so only halo cells of
+ * NONLEAF face neighbours contribute. */
+void og_deduct_flux(og_grid* g, int level, int x, int c0, int f, int rhs)
+{
+    #pragma omp parallel
+    {
+        og_lds* s = (og_lds*)malloc(sizeof(og_lds));
+        #pragma omp for schedule(dynamic, 8)
+        for (int t = g->lstart[level]; t < g->lstart[level + 1]; t++) {
+            if (!(g->t_type[t] & OG_LEAF)) continue;
+            load_tile(g, t, x, x, c0, s);
+            /* keep only halo values of NONLEAF neighbours */
+            for (int i = -1; i <= 8; i++)
+                for (int j = -1; j <= 8; j++)
+                    for (int k = -1; k <= 8; k++) {
+                        int out = (i < 0) + (i > 7) + (j < 0) + (j > 7) + (k < 0) + (k > 7);
+                        if (out != 1) { XV(s, i, j, k) = 0.0f; continue; }
+                        int a = (i < 0 || i > 7) ? 0 : (j < 0 || j > 7) ? 1 : 2;
+                        int l = a == 0 ? i : a == 1 ? j : k;
+                        int nt = g->nbr[6 * t + (l < 0 ? a : a + 3)];
+                        if (nt < 0 || g->t_type[nt] != OG_NONLEAF) XV(s, i, j, k) = 0.0f;
+                    }
+            for (int v = 0; v < 512; v++) {
+                size_t idx = (size_t)t * 512 + v;
+                if (!(g->ctype[idx] & OG_INTERIOR)) { g->ch[rhs][idx] = 0.0f; continue; }
+                int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
+                float d = s->diag[i][j][k];
+                float sum = 0.0f;
+                if (d != 0) {
+                    sum = 0.0f * d;
+                    const int l[3] = { i, j, k };
+                    for (int a = 0; a < 3; a++)
+                        for (int sg = -1; sg <= 1; sg += 2) {
+                            int n[3] = { l[0], l[1], l[2] }, u[3] = { l[0], l[1], l[2] };
+                            n[a] += sg;
+                            u[a] += sg == -1 ? 0 : 1;
+                            sum = fmaf(*offp(s, a, u[0], u[1], u[2]), XV(s, n[0], n[1], n[2]), sum);
+                        }
+                }
+                g->ch[rhs][idx] = g->ch[f][idx] - sum;
+            }
+        }
+        free(s);
+    }
+}
+
+/* SetLevelZeroAbsTiles128Kernel (AMGSolver.cu:995-1016) */
+void og_set_zero_tiles(og_grid* g, int level, int tile_types, int x)
+{
+    for (int t = g->lstart[level]; t < g->lstart[level + 1]; t++)
+        if (g->t_type[t] & tile_types) memset(&g->ch[x][(size_t)t * 512], 0, 512 * sizeof(float));
+}
+
 double og_dot(og_grid* g, int a, int b)
 {
     double s = 0;
@@ -490,6 +593,111 @@ void og_fas_cycle(og_grid* g, const og_cycle* cyc, int x, int rhs, int x0, int c
     og_fas_cycle_step(g, cyc, g->max_level, x, rhs, x0, c0);
 }
 
+static void coarsest_solve(og_grid* g, const og_cycle* cyc, int x, int c0, int rhs)
+{
+    if (cyc->coarsest_direct) coarsest_direct(g, x, c0, rhs);
+    else if (cyc->smoother == 0) {
+        og_gauss_seidel(g, cyc->coarsest_iters / 2, 0, 0, x, c0, rhs, cyc->omega);
+        og_gauss_seidel(g, cyc->coarsest_iters / 2, 1, 0, x, c0, rhs, cyc->omega);
+    } else smooth(g, cyc, 0, cyc->coarsest_iters, 0, x, c0, rhs);
+}
+
+/* x = 0 on every tile; rhs = f on LEAF|NONLEAF tiles, 0 on GHOST tiles
+ * (the initial launch of VCycle / muCycle, AMGSolver.cu:1062-1078) */
+static void nonfas_init(og_grid* g, int x, int f, int rhs)
+{
+    for (int t = 0; t < g->N; t++) {
+        size_t b = (size_t)t * 512;
+        memset(&g->ch[x][b], 0, 512 * sizeof(float));
+        if (g->t_type[t] & OG_GHOST) memset(&g->ch[rhs][b], 0, 512 * sizeof(float));
+        else memcpy(&g->ch[rhs][b], &g->ch[f][b], 512 * sizeof(float));
+    }
+}
+
+void og_vcycle(og_grid* g, const og_cycle* cyc, int x, int f, int rhs, int c0)
+{
+    nonfas_init(g, x, f, rhs);
+    for (int l = g->max_level; l > 0; l--) {
+        smooth(g, cyc, l, cyc->level_iters, 0, x, c0, rhs);
+        og_residual_restrict(g, l, x, c0, rhs, cyc->R_restrict);
+    }
+    coarsest_solve(g, cyc, x, c0, rhs);
+    for (int l = 1; l <= g->max_level; l++) {
+        og_prolong_add(g, l, x, cyc->P);
+        smooth(g, cyc, l, cyc->level_iters, 1, x, c0, rhs);
+    }
+}
+
+static void mu_step(og_grid* g, const og_cycle* cyc, int level, int x, int f, int rhs, int c0)
+{
+    if (level == 0) {
+        coarsest_solve(g, cyc, x, c0, rhs);
+        return;
+    }
+    smooth(g, cyc, level, cyc->level_iters, 0, x, c0, rhs);
+    og_residual_restrict(g, level, x, c0, rhs, cyc->R_restrict);
+    og_deduct_flux(g, level - 1, x, c0, f, rhs);
+    og_set_zero_tiles(g, level - 1, OG_NONLEAF, x);
+    for (int i = 0; i < cyc->mu; i++) mu_step(g, cyc, level - 1, x, f, rhs, c0);
+    og_prolong_add(g, level, x, cyc->P);
+    smooth(g, cyc, level, cyc->level_iters, 1, x, c0, rhs);
+}
+
+void og_mu_cycle(og_grid* g, const og_cycle* cyc, int x, int f, int rhs, int c0)
+{
+    nonfas_init(g, x, f, rhs);
+    mu_step(g, cyc, g->max_level, x, f, rhs, c0);
+}
+
+void og_fas_fcycle(og_grid* g, const og_cycle* cyc, int x, int rhs, int x0, int c0)
+{
+    memset(g->ch[x], 0, sizeof(float) * (size_t)g->N * 512);
+    for (int l = g->max_level; l > 0; l--) og_restrict_source(g, l, rhs, cyc->R_restrict);
+    coarsest_solve(g, cyc, x, c0, rhs);
+    for (int l = 1; l <= g->max_level; l++) {
+        og_prolong_add(g, l, x, cyc->P);
+        og_fas_cycle_step(g, cyc, l, x, rhs, x0, c0);
+    }
+}
+
+void og_precondition(og_grid* g, const og_cycle* cyc, int z, int r, int s, int c0)
+{
+    switch (cyc->kind) {
+    case 1: og_fas_fcycle(g, cyc, z, r, s, c0); break;
+    case 2: og_vcycle(g, cyc, z, r, s, c0); break;
+    case 3: og_mu_cycle(g, cyc, z, r, s, c0); break;
+    default: og_fas_cycle(g, cyc, z, r, s, c0); break;
+    }
+}
+
+og_solve_out og_fas_mu_solve(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol)
+{
+    enum { X = 0, B = 1, X0 = 2 };
+    og_solve_out out = { 0, 0.0 };
+    memset(g->ch[X], 0, sizeof(float) * (size_t)g->N * 512);
+    double rhs_norm = sqrt(og_dot(g, B, B));
+    double thr = tol * rhs_norm;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    double res = rhs_norm;
+    int i;
+    for (i = 0; i < max_iters; i++) {
+        og_fas_cycle_step(g, cyc, g->max_level, X, B, X0, c0);
+        og_full_neg_lap(g, X, c0, X0);
+        for (int t = 0; t < g->N; t++) {
+            if (!(g->t_type[t] & OG_LEAF)) continue;
+            for (int v = 0; v < 512; v++) {
+                size_t idx = (size_t)t * 512 + v;
+                g->ch[X0][idx] = g->ch[B][idx] - g->ch[X0][idx];
+            }
+        }
+        res = sqrt(og_dot(g, X0, X0));
+        if (res < thr) break;
+    }
+    out.iters = i;
+    out.rel_res = res / rhs_norm;
+    return out;
+}
+
 /* AMGSolver::solve (AMGSolver.cu:1399-1549), sync_stride = 1 */
 og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann)
 {
@@ -502,7 +710,7 @@ og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, doub
     double thr2 = tol * tol * rhs2;
     if (thr2 < 2.2250738585072014e-308) thr2 = 2.2250738585072014e-308;
     if (pure_neumann) og_recenter(g, R);
-    og_fas_cycle(g, cyc, Z, R, AP, c0);
+    og_precondition(g, cyc, Z, R, AP, c0);
     for (int t = 0; t < g->N; t++)
         if (g->t_type[t] & OG_LEAF) memcpy(&g->ch[P][(size_t)t * 512], &g->ch[Z][(size_t)t * 512], 512 * sizeof(float));
     double gamma = og_dot(g, Z, R);
@@ -525,7 +733,7 @@ og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, doub
         rr = og_dot(g, R, R);
         if (rr < thr2) { out.iters = i + 1; out.rel_res = sqrt(rr / rhs2); return out; }
         if (pure_neumann) og_recenter(g, R);
-        og_fas_cycle(g, cyc, Z, R, AP, c0);
+        og_precondition(g, cyc, Z, R, AP, c0);
         double gamma_old = gamma;
         gamma = og_dot(g, R, Z);
         double beta = gamma / gamma_old;

@@ -132,7 +132,26 @@ typedef struct og_cycle {
     int coarsest_direct;   /* 1 = FP64 dense solve at level 0 */
     float R_restrict, P;
     int tmp_ch;            /* scratch channel for Jacobi ping-pong */
+    int kind;              /* preconditioner: 0 FAS mu-cycle, 1 FAS F-cycle, 2 VCycle, 3 muCycle */
 } og_cycle;
+
+/* RestrictSourceToCoarse (AMGSolver.cu:1295-1307): parent = coeff * sum of the
+ * INTERIOR children of the LEAF/NONLEAF tiles of `level` */
+void og_restrict_source(og_grid* g, int level, int ch, float coeff);
+/* ProlongateAndUpdateAMG128Kernel (AMGSolver.cu:1022-1057): x += P * parent x */
+void og_prolong_add(og_grid* g, int level, int x, float P);
+/* DeductFluxOnAbsLeafs128Kernel (AMGSolver.cu:966-993) */
+void og_deduct_flux(og_grid* g, int level, int x, int c0, int f, int rhs);
+/* SetLevelZeroAbsTiles128Kernel (AMGSolver.cu:995-1016) */
+void og_set_zero_tiles(og_grid* g, int level, int tile_types, int x);
+/* AMGSolver::VCycle (AMGSolver.cu:1059-1128), non-FAS; f is kept, rhs is scratch */
+void og_vcycle(og_grid* g, const og_cycle* cyc, int x, int f, int rhs, int c0);
+/* AMGSolver::muCycle / muCycleStep (AMGSolver.cu:1130-1174), non-FAS */
+void og_mu_cycle(og_grid* g, const og_cycle* cyc, int x, int f, int rhs, int c0);
+/* AMGSolver::FASFCycle (AMGSolver.cu:1358-1387) */
+void og_fas_fcycle(og_grid* g, const og_cycle* cyc, int x, int rhs, int x0, int c0);
+/* the preconditioner named by cyc->kind: z ~ A^-1 r with scratch channel s */
+void og_precondition(og_grid* g, const og_cycle* cyc, int z, int r, int s, int c0);
 
 /* FASMuCycle (AMGSolver.cu:1279-1292): x = 0 on all tiles, then FASMuCycleStep(maxLevel) */
 void og_fas_cycle(og_grid* g, const og_cycle* cyc, int x, int rhs, int x0, int c0);
@@ -141,6 +160,10 @@ void og_fas_cycle_step(og_grid* g, const og_cycle* cyc, int level, int x, int rh
 typedef struct og_solve_out { int iters; double rel_res; } og_solve_out;
 /* AMGSolver::solve (AMGSolver.cu:1399-1549); channels x=0, r=1, p=2, Ap=3, z=4 */
 og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann);
+/* AMGSolver::FASMuCycleSolve (AMGSolver.cu:1311-1356): x = 0 (channel 0), b in
+ * channel 1, residual and FAS x0 in channel 2; returns the reference's
+ * (loop index at exit, |r| / |b|) */
+og_solve_out og_fas_mu_solve(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol);
 
 /* ---- FP64 assembled oracle of the composite LEAF operator ----------------- */
 typedef struct og_csr {

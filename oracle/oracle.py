@@ -28,7 +28,7 @@ class Target(C.Structure):
 class Cycle(C.Structure):
     _fields_ = [("mu", C.c_int), ("smoother", C.c_int), ("omega", C.c_double),
                 ("level_iters", C.c_int), ("coarsest_iters", C.c_int), ("coarsest_direct", C.c_int),
-                ("R_restrict", C.c_float), ("P", C.c_float), ("tmp_ch", C.c_int)]
+                ("R_restrict", C.c_float), ("P", C.c_float), ("tmp_ch", C.c_int), ("kind", C.c_int)]
 
 
 class SolveOut(C.Structure):
@@ -79,6 +79,11 @@ def lib():
             "og_dot": (d, [vp, i, i]), "og_recenter": (None, [vp, i]),
             "og_fas_cycle": (None, [vp, vp, i, i, i, i]),
             "og_pcg": (SolveOut, [vp, vp, i, i, d, i]),
+            "og_fas_mu_solve": (SolveOut, [vp, vp, i, i, d]),
+            "og_restrict_source": (None, [vp, i, i, f]), "og_prolong_add": (None, [vp, i, i, f]),
+            "og_deduct_flux": (None, [vp, i, i, i, i, i]), "og_set_zero_tiles": (None, [vp, i, i, i]),
+            "og_vcycle": (None, [vp, vp, i, i, i, i]), "og_mu_cycle": (None, [vp, vp, i, i, i, i]),
+            "og_fas_fcycle": (None, [vp, vp, i, i, i, i]), "og_precondition": (None, [vp, vp, i, i, i, i]),
             "og_assemble_csr": (C.POINTER(CSR), [vp, i]), "og_csr_free": (None, [C.POINTER(CSR)]),
             "og_csr_matvec": (None, [C.POINTER(CSR), vp, vp]),
             "og_bicgstab": (i, [C.POINTER(CSR), vp, vp, d, i, vp, i]),
@@ -114,9 +119,12 @@ def make_target(kind, min_level, max_level, sdf_kind=SDF_SPHERE, sdf=(), bands=N
     return t
 
 
+CYCLE_FAS_MU, CYCLE_FAS_F, CYCLE_VCYCLE, CYCLE_MU = 0, 1, 2, 3
+
+
 def make_cycle(mu=2, smoother=0, omega=1.5, level_iters=2, coarsest_iters=10, coarsest_direct=0,
-               R_restrict=1.0, P=1.0, tmp_ch=5):
-    return Cycle(mu, smoother, omega, level_iters, coarsest_iters, coarsest_direct, R_restrict, P, tmp_ch)
+               R_restrict=1.0, P=1.0, tmp_ch=5, kind=CYCLE_FAS_MU):
+    return Cycle(mu, smoother, omega, level_iters, coarsest_iters, coarsest_direct, R_restrict, P, tmp_ch, kind)
 
 
 class Grid:
@@ -212,6 +220,13 @@ class Grid:
 
     def fas_cycle(self, cycle, x, rhs, x0, c0=11):
         lib().og_fas_cycle(self.h, C.byref(cycle), x, rhs, x0, c0)
+
+    def precondition(self, cycle, z, r, s, c0=11):
+        lib().og_precondition(self.h, C.byref(cycle), z, r, s, c0)
+
+    def fas_mu_solve(self, cycle, c0=11, max_iters=100, tol=1e-6):
+        out = lib().og_fas_mu_solve(self.h, C.byref(cycle), c0, max_iters, tol)
+        return out.iters, out.rel_res
 
     def rhs_uniform(self, ch, seed, zero_mean=True):
         lib().og_rhs_uniform(self.h, ch, seed, int(zero_mean))

@@ -234,3 +234,62 @@ def test_cpp_wrapper_solves():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "gpu part OK" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c3_small"])
+@pytest.mark.parametrize("kind,smoother", [(P.CYCLE_FAS_F, 0), (P.CYCLE_VCYCLE, 0), (P.CYCLE_MU, 0),
+                                           (P.CYCLE_FAS_F, 1), (P.CYCLE_MU, 1)])
+def test_cycle_kinds_bitexact(case, kind, smoother):
+    """FASFCycle, VCycle and muCycle (src/AMGSolver.cu:1059-1174,1358-1387) on the
+    device give the oracle's bits."""
+    print("\ncycle kind", case, kind, smoother, flush=True)
+    go, gp = H.build_pair(case)
+    rng = np.random.default_rng(31)
+    for ch in (1, 3, 4, 5):
+        v = rng.uniform(-1, 1, (go.N, 512)).astype(np.float32)
+        go.channel(ch)[:] = v
+        gp.set_channel(ch, v)
+    okind = {P.CYCLE_FAS_F: O.CYCLE_FAS_F, P.CYCLE_VCYCLE: O.CYCLE_VCYCLE, P.CYCLE_MU: O.CYCLE_MU}[kind]
+    omega = 1.5 if smoother == 0 else 2.0 / 3.0
+    popts = P.default_opts(cycle=kind, smoother=smoother, omega=omega, coarsest_iters=8)
+    ocyc = O.make_cycle(mu=2, smoother=smoother, omega=omega, level_iters=2, coarsest_iters=8, kind=okind)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    go.precondition(ocyc, 4, 1, 3, H.C0)
+    solver.cycle(popts, 4, 1, 3)
+    for ch in (4, 3, 1):
+        a, b = go.channel(ch), gp.channel(ch)
+        assert np.array_equal(H.bits(a), H.bits(b)), (case, kind, smoother, ch, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("kind", [P.CYCLE_FAS_F, P.CYCLE_VCYCLE, P.CYCLE_MU])
+def test_pcg_each_cycle_kind_matches_fp64_oracle(kind):
+    go, gp = H.build_pair("c3_small")
+    _rhs(go, gp, 14)
+    A, _, x64 = _fp64_solution(go)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, opts=P.default_opts(cycle=kind))
+    okind = {P.CYCLE_FAS_F: O.CYCLE_FAS_F, P.CYCLE_VCYCLE: O.CYCLE_VCYCLE, P.CYCLE_MU: O.CYCLE_MU}[kind]
+    go2, _ = H.build_pair("c3_small", with_device=False)
+    go2.channel(1)[:] = go.channel(1)
+    oit, orel = go2.pcg(O.make_cycle(kind=okind), H.C0, 200, 1e-6)
+    x = gp.channel(0).ravel()[A.cell]
+    assert rel < 1e-6
+    assert abs(it - oit) <= 1, (it, oit)
+    assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)
+
+
+@pytest.mark.parametrize("case", ["uniform3", "c3_small"])
+def test_mg_solve_matches_oracle(case):
+    """FASMuCycleSolve (src/AMGSolver.cu:1311-1356): the same cycles, the same
+    reference return value, the same solution bits."""
+    print("\nmg solve", case, flush=True)
+    go, gp = H.build_pair(case)
+    _rhs(go, gp, 21)
+    opts = P.default_opts()
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.mg_solve(opts, max_iters=100, relative_tolerance=1e-6)
+    oit, orel = go.fas_mu_solve(O.make_cycle(mu=2), H.C0, 100, 1e-6)
+    assert rel < 1e-6 and it == oit, (it, oit, rel, orel)
+    assert abs(rel - orel) <= 1e-6 * orel
+    leaf = go.tiles()[2] == 1
+    assert np.array_equal(H.bits(go.channel(0)[leaf]), H.bits(gp.channel(0)[leaf]))

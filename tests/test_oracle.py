@@ -192,3 +192,172 @@ def test_no_closed_interior_cells(name):
     touches = np.zeros(n, bool)
     np.logical_or.at(touches, lab, rowsum > 1e-7 * A.diagonal())
     assert touches.all()
+
+
+# ---- non-FAS transfers, VCycle / muCycle / FASFCycle, FASMuCycleSolve ---------
+
+def _graded(name="sphere_empty24"):
+    go, _ = H.build_pair(name, with_device=False)
+    return go
+
+
+def _centres(go, level_filter=None):
+    """global cell index (i, j, k) of every cell: (N, 512, 3)"""
+    lv, co, ty = go.tiles()
+    v = np.arange(512)
+    loc = np.stack([v >> 6, (v >> 3) & 7, v & 7], axis=1)
+    return co[:, None, :] * 8 + loc[None, :, :]
+
+
+def test_restrict_source_sums_interior_children():
+    """parent = R * (sum of the INTERIOR children): a field linear in the global
+    index restricts to 8 R times its mean over the 2^3 block, i.e. the linear
+    function at the parent's centre (catches swapped axes or octants)."""
+    go = _graded("c2_small")
+    lv, co, ty = go.tiles()
+    gi = _centres(go).astype(np.float64)
+    T = go.cell_types()
+    lin = lambda g: 3.0 * g[..., 0] - 2.0 * g[..., 1] + 0.5 * g[..., 2] + 1.0
+    level = go.max_level
+    go.channel(1)[:] = lin(gi).astype(np.float32)
+    go.channel(1)[T != 2] = 0   # only INTERIOR cells contribute; keep the test exact
+    before = go.channel(1).copy()
+    R = 0.5
+    O.lib().og_restrict_source(go.h, level, 1, np.float32(R))
+    after = go.channel(1)
+    par = go.parents()
+    kids = np.where((lv == level) & ((ty == 1) | (ty == 2)))[0]
+    checked = 0
+    for t in kids[:40]:
+        p = par[t]
+        for q in range(0, 512, 37):
+            c = gi[t, q].astype(int)
+            blk = [(c[0] & ~1) + a for a in (0, 1)], [(c[1] & ~1) + b for b in (0, 1)], [(c[2] & ~1) + e for e in (0, 1)]
+            loc = [np.where((gi[t] == (x, y, z)).all(axis=1))[0][0] for x in blk[0] for y in blk[1] for z in blk[2]]
+            if not (T[t, loc] == 2).all():
+                continue
+            pc = c >> 1
+            pv = np.where((gi[p] == pc).all(axis=1))[0][0]
+            want = R * 8 * lin(np.array([c[0] & ~1, c[1] & ~1, c[2] & ~1]) + 0.5)
+            assert abs(after[p, pv] - want) <= 1e-5 * max(1.0, abs(want)), (t, q)
+            checked += 1
+    assert checked > 50
+    # tiles of other levels are untouched
+    other = lv != level - 1
+    assert np.array_equal(after[other], before[other])
+
+
+def test_prolong_add_injects_parent_values():
+    """x += P * parent x: with fine x = 0 every INTERIOR fine cell gets P times
+    the value of the parent cell containing it; GHOST cells get the parent value."""
+    go = _graded("sphere_empty24")
+    lv, co, ty = go.tiles()
+    gi = _centres(go)
+    T = go.cell_types()
+    level = go.max_level
+    f = lambda g: (7.0 * g[..., 0] + 3.0 * g[..., 1] - g[..., 2]).astype(np.float32)
+    x = np.where(lv[:, None] == level - 1, f(gi), 0).astype(np.float32)
+    go.channel(0)[:] = x
+    O.lib().og_prolong_add(go.h, level, 0, np.float32(2.0))
+    out = go.channel(0)
+    sel = lv == level
+    par_val = f(gi >> 1)
+    is_ghost = (ty == 4)[:, None]
+    want = np.where(T == 2, np.where(is_ghost, par_val, 2.0 * par_val), 0)
+    has_parent = go.parents() >= 0
+    m = sel & has_parent
+    assert np.array_equal(out[m], want[m].astype(np.float32))
+
+
+def test_deduct_flux_is_the_operator_on_nonleaf_values():
+    """DeductFlux with f = 0 equals minus the full operator applied to an x that
+    is nonzero only on NONLEAF tiles of the level (the outer-NONLEAF terms)."""
+    go = _graded("c2_small")
+    lv, co, ty = go.tiles()
+    level = go.max_level - 1
+    rng = np.random.default_rng(5)
+    # x on the LEAF children of NONLEAF tiles at `level`; accumulate gives the NONLEAF means
+    par = go.parents()
+    x = np.zeros((go.N, 512), np.float32)
+    kids = (lv == level + 1) & (ty == 1) & (ty[par] == 2)
+    x[kids] = rng.uniform(-1, 1, (int(kids.sum()), 512)).astype(np.float32)
+    go.channel(0)[:] = x
+    go.full_neg_lap(0, H.C0, 3)   # propagate + accumulate + operator on LEAF tiles
+    y = go.channel(3).copy()
+    xs = go.channel(0).copy()      # now holds the NONLEAF means
+    go.channel(0)[:] = xs
+    go.channel(5)[:] = 0
+    O.lib().og_deduct_flux(go.h, level, 0, H.C0, 5, 6)
+    rhs = go.channel(6)
+    leafs = (lv == level) & (ty == 1)
+    assert np.array_equal(rhs[leafs], -y[leafs])
+    nz = np.count_nonzero(rhs[leafs])
+    assert nz > 100
+
+
+def test_deduct_flux_without_nonleaf_values_is_identity():
+    go = _graded("c2_small")
+    lv, co, ty = go.tiles()
+    level = go.max_level - 1
+    f = np.random.default_rng(6).uniform(-1, 1, (go.N, 512)).astype(np.float32)
+    go.channel(0)[:] = np.where((ty == 2)[:, None], 0, np.random.default_rng(7).uniform(-1, 1, (go.N, 512))).astype(np.float32)
+    go.channel(5)[:] = f
+    O.lib().og_deduct_flux(go.h, level, 0, H.C0, 5, 6)
+    T = go.cell_types()
+    leafs = (lv == level) & (ty == 1)
+    want = np.where(T == 2, f, 0)
+    assert np.array_equal(go.channel(6)[leafs], want[leafs])
+
+
+def test_mu_cycle_equals_fas_on_uniform():
+    """Without coarse leaves the non-FAS muCycle (coarse x zeroed, no flux terms)
+    is the correction scheme, which a linear FAS cycle equals up to rounding."""
+    g = O.Grid(O.make_target(O.TARGET_UNIFORM, 3, 3))
+    g.set_leaf_types_preset()
+    g.coarsen_types_and_coeffs(H.C0, H.R_MATRIX)
+    g.rhs_uniform(1, 3, True)
+    r = g.channel(1).copy()
+    outs = []
+    for kind in (O.CYCLE_FAS_MU, O.CYCLE_MU):
+        g.channel(1)[:] = r
+        cyc = O.make_cycle(mu=2, smoother=0, omega=1.5, level_iters=2, coarsest_iters=10, kind=kind)
+        g.precondition(cyc, 4, 1, 3, H.C0)
+        outs.append(g.channel(4).astype(np.float64).copy())
+    leaf = g.tiles()[2] == 1
+    d = outs[1][leaf] - outs[0][leaf]
+    assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(outs[0][leaf])
+
+
+@pytest.mark.parametrize("kind", [O.CYCLE_FAS_F, O.CYCLE_VCYCLE, O.CYCLE_MU])
+def test_pcg_with_each_cycle_kind(kind):
+    g = O.Grid(O.make_target(O.TARGET_SPHERE_EMPTY, 2, 4))
+    g.set_leaf_types_preset()
+    g.coarsen_types_and_coeffs(H.C0, H.R_MATRIX)
+    g.rhs_uniform(1, 4, True)
+    A = g.assemble_csr(H.C0)
+    b = g.channel(1).ravel()[A.cell].astype(np.float64)
+    x64, it, rr = A.bicgstab(b, 1e-13)
+    iters, rel = g.pcg(O.make_cycle(kind=kind), H.C0, 200, 1e-6)
+    x = g.channel(0).ravel()[A.cell]
+    assert rel < 1e-6, (kind, iters, rel)
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-5
+
+
+def test_fas_mu_solve_converges_to_the_solution():
+    g = O.Grid(O.make_target(O.TARGET_SPHERE_EMPTY, 2, 4))
+    g.set_leaf_types_preset()
+    g.coarsen_types_and_coeffs(H.C0, H.R_MATRIX)
+    g.rhs_uniform(1, 8, True)
+    A = g.assemble_csr(H.C0)
+    b = g.channel(1).ravel()[A.cell].astype(np.float64)
+    x64, it, rr = A.bicgstab(b, 1e-13)
+    iters, rel = g.fas_mu_solve(O.make_cycle(mu=2), H.C0, 100, 1e-6)
+    assert rel < 1e-6 and 0 < iters < 100
+    x = g.channel(0).ravel()[A.cell]
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-5
+    # the reference returns the loop index at exit: one cycle fewer than it ran
+    g.channel(1)[:] = 0
+    g.channel(1).ravel()[A.cell] = b.astype(np.float32)
+    # with max_iters = iters the loop ends without breaking: index = iters, not converged
+    it2, rel2 = g.fas_mu_solve(O.make_cycle(mu=2), H.C0, iters, 1e-6)
+    assert it2 == iters and rel2 >= 1e-6
