@@ -24,6 +24,7 @@ SDF_SPHERE, SDF_TORUS, SDF_STAR = 0, 1, 2
 CYCLE_FAS_W, CYCLE_FAS_V, CYCLE_FAS_F, CYCLE_VCYCLE, CYCLE_MU = 0, 1, 2, 3, 4
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 PREC_FP32, PREC_FP64_REFINE = 0, 1
+SOLVER_AMG, SOLVER_GMG, SOLVER_CMG = 0, 1, 2
 
 STATUS = {0: "ok", 1: "bad argument", 2: "call out of order", 3: "HIP runtime error",
           4: "non-finite value", 5: "CG breakdown", 6: "not converged", 7: "hash table full"}
@@ -74,7 +75,8 @@ EXPORTS = [
     "cmg_op_residual_restrict", "cmg_op_fas_save", "cmg_op_fas_prolong", "cmg_op_dot", "cmg_op_cycle",
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
     "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize", "cmg_set_kernel_forms",
-    "cmg_get_kernel_forms", "cmg_mg_solve",
+    "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
+    "cmg_solver_apply",
 ]
 
 _lib = None
@@ -117,6 +119,9 @@ def lib():
             "cmg_device_synchronize": (st, []),
             "cmg_set_kernel_forms": (st, [i, i]),
             "cmg_mg_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+            "cmg_geo_solver_create": (st, [vp, i, C.c_float, C.c_float, C.POINTER(vp)]),
+            "cmg_gmg_damped_jacobi_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+            "cmg_solver_apply": (st, [vp, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
             "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
             "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
@@ -390,6 +395,10 @@ class AMGSolver:
         _check(lib().cmg_solver_get_x64(self.h, _ptr(a)), "get_x64")
         return a
 
+    def apply(self, x, y):
+        """y = A x with this solver's operator on LEAF tiles."""
+        _check(lib().cmg_solver_apply(self.h, x, y), "cmg_solver_apply")
+
     def bench_kernel(self, kernel, reps):
         ms, tiles = C.c_double(), C.c_int()
         _check(lib().cmg_bench_kernel(self.h, kernel, reps, C.byref(ms), C.byref(tiles)), "bench_kernel")
@@ -397,3 +406,34 @@ class AMGSolver:
 
     def cycle(self, opts, x, rhs, x0):
         _check(lib().cmg_op_cycle(self.h, C.byref(opts), x, rhs, x0), "cycle")
+
+
+class _GeoSolver(AMGSolver):
+    FAMILY = None
+
+    def __init__(self, grid, one_over_alpha=1.0, prolong_coeff=1.0):
+        self.grid = grid
+        self.h = C.c_void_p()
+        _check(lib().cmg_geo_solver_create(grid.h, self.FAMILY, one_over_alpha, prolong_coeff, C.byref(self.h)),
+               "cmg_geo_solver_create")
+        self.last_stats = None
+
+
+class GMGSolver(_GeoSolver):
+    """GMGSolver(one_over_alpha, prolong_coeff) (src/GMGSolver.h:14-50)."""
+    FAMILY = SOLVER_GMG
+
+    def damped_jacobi_solve(self, max_iters, relative_tolerance, omega):
+        """GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:390-470): (loop index at exit, |r|/|b|)."""
+        o = default_opts(max_iters=max_iters, relative_tolerance=relative_tolerance, omega=omega)
+        stats = SolveStats()
+        status = lib().cmg_gmg_damped_jacobi_solve(self.h, C.byref(o), C.byref(stats))
+        self.last_stats = stats
+        if status not in (0, 6):
+            raise CmgError(status, "cmg_gmg_damped_jacobi_solve")
+        return stats.iters, stats.rel_res
+
+
+class CMGSolver(_GeoSolver):
+    """CMGSolver(one_over_alpha, prolong_coeff) (src/CMGSolver.h:12-47)."""
+    FAMILY = SOLVER_CMG

@@ -394,6 +394,18 @@ cmg_status cmg_solver_create(cmg_grid* g, int c0, float R_matrix, float R_restri
 
 void cmg_solver_destroy(cmg_solver* s) { solver_destroy(s); }
 
+cmg_status cmg_geo_solver_create(cmg_grid* g, int family, float one_over_alpha, float prolong_coeff, cmg_solver** out)
+{
+    return solver_create_geo(g, family, one_over_alpha, prolong_coeff, out);
+}
+
+cmg_status cmg_gmg_damped_jacobi_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
+{
+    return solver_damped_jacobi(s, opts, stats);
+}
+
+cmg_status cmg_solver_apply(cmg_solver* s, int x, int y) { return solver_apply(s, x, y); }
+
 cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
 {
     return solver_solve(s, opts, stats);

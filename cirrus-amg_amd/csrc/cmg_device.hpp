@@ -1,0 +1,41 @@
+// cmg_device.hpp -- device helpers shared by the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cmg_kernels.hpp"
+
+namespace cmg {
+namespace k {
+
+// LDS index of a cell of the 10^3 halo'd tile (local coords -1..8)
+__device__ __forceinline__ int SX(int i, int j, int k) { return (i + 1) * 100 + (j + 1) * 10 + (k + 1); }
+__device__ __forceinline__ int cidx(int i, int j, int k) { return (i << 6) | (j << 3) | k; }
+
+__device__ __forceinline__ bool stopped(const double* S) { return S && S[S_DONE] != 0.0; }
+
+// wave64 + workgroup reduction of one double per thread; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < int(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+}
+
+__device__ __forceinline__ float tile_h(const TopoView& tp, int t) { return tp.h0 / float(1 << tp.tlevel[t]); }
+
+// Grid-stride loops over the tiles of a list, 128 threads x float4 per tile,
+// two tiles per 256-thread workgroup step.
+#define CMG_TILE_LOOP(list, n)                                                               \
+    for (int pos = blockIdx.x * 2 + (threadIdx.x >> 7); pos < (n); pos += gridDim.x * 2)   \
+        if (const int t_ = (list)[pos]; t_ >= 0)
+
+__device__ __forceinline__ size_t quad_index(int t) { return size_t(t) * 128 + (threadIdx.x & 127); }
+
+}  // namespace k
+}  // namespace cmg

@@ -54,6 +54,7 @@ struct DevGrid {
     LaunchList leafat[kMaxLevels];      // LEAF tiles at the level
     LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
     LaunchList ghosts;                  // all GHOST tiles
+    LaunchList leafghost;               // all LEAF and GHOST tiles
     int max_list = 0;
     int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU
     // reduction scratch for the op-level entry points
@@ -97,6 +98,9 @@ void solver_destroy(cmg_solver* s);
 cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
 cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0);
 cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
+cmg_status solver_create_geo(cmg_grid* g, int family, float one_over_alpha, float P, cmg_solver** out);
+cmg_status solver_damped_jacobi(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
+cmg_status solver_apply(cmg_solver* s, int x, int y);
 cmg_status solver_get_x64(cmg_solver* s, double* host);
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 cmg_status op_copy(cmg_grid* g, int src, int dst);

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <stdexcept>
 
+#include "cmg_device.hpp"
 #include "cmg_kernels.hpp"
 
 #pragma clang fp contract(off)
@@ -29,8 +30,6 @@
 namespace cmg {
 namespace k {
 
-__device__ __forceinline__ int SX(int i, int j, int k) { return (i + 1) * 100 + (j + 1) * 10 + (k + 1); }
-__device__ __forceinline__ int cidx(int i, int j, int k) { return (i << 6) | (j << 3) | k; }
 
 template <typename T>
 struct Lds {
@@ -38,20 +37,7 @@ struct Lds {
     float off[3][576];   // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
 };
 
-__device__ __forceinline__ bool stopped(const double* S) { return S && S[S_DONE] != 0.0; }
 
-// wave64 + workgroup reduction of one double per thread; result valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double* red)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    double s = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < int(blockDim.x >> 6); ++i) s += red[i];
-    return s;
-}
 
 // Per-thread state of the cells a thread owns: with NT threads per tile each
 // thread owns CPT = 512/NT cells v = tid + c*NT (CPT x-planes apart).
@@ -208,7 +194,6 @@ __device__ __forceinline__ T neg_lap(const Lds<T>& s, int v, T x0, const float c
     return r;
 }
 
-__device__ __forceinline__ float tile_h(const TopoView& tp, int t) { return tp.h0 / float(1 << tp.tlevel[t]); }
 
 // ------------------------------------------------- pipelined tile loops ----
 // The hot stencil kernels run persistent workgroups that walk their share of a
@@ -488,14 +473,7 @@ __global__ __launch_bounds__(NT) void restrict_p(TopoView tp, const int32_t* __r
 }
 
 // ------------------------------------------------------ vector (float4) ----
-// Grid-stride loops over the tiles of a list, 128 threads x float4 per tile,
-// two tiles per 256-thread workgroup step.
-
-#define CMG_TILE_LOOP(list, n)                                                               \
-    for (int pos = blockIdx.x * 2 + (threadIdx.x >> 7); pos < (n); pos += gridDim.x * 2)   \
-        if (const int t_ = (list)[pos]; t_ >= 0)
-
-__device__ __forceinline__ size_t quad_index(int t) { return size_t(t) * 128 + (threadIdx.x & 127); }
+// (CMG_TILE_LOOP walks the tiles of a list, 128 threads x float4 per tile)
 
 __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
                                                    float* __restrict__ R, const float* __restrict__ P,
@@ -1465,9 +1443,9 @@ void launch_restrict_source(const DevGrid& d, int level, float* X, float coeff, 
     k::restrict_source<<<l.n, 64, 0, st>>>(view(d), l.d, X, coeff, S);
 }
 
-void launch_prolong_add(const DevGrid& d, int level, float* X, float P, const double* S, hipStream_t st)
+void launch_prolong_add(const DevGrid& d, int level, float* X, float P, const double* S, hipStream_t st, bool leaf_only)
 {
-    const LaunchList& l = d.prolong[level];
+    const LaunchList& l = leaf_only ? d.subtree[level] : d.prolong[level];
     if (!nonempty(l)) return;
     k::prolong_add_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, X, P, S);
 }

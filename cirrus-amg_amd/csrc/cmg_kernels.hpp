@@ -63,7 +63,19 @@ void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t
 void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st);
 // non-FAS cycles and FASFCycle / FASMuCycleSolve
 void launch_restrict_source(const DevGrid& d, int level, float* X, float coeff, const double* S, hipStream_t st);
-void launch_prolong_add(const DevGrid& d, int level, float* X, float P, const double* S, hipStream_t st);
+// leaf_only: LEAF|NONLEAF tiles of the level (CMG), else GHOST tiles too
+void launch_prolong_add(const DevGrid& d, int level, float* X, float P, const double* S, hipStream_t st,
+                        bool leaf_only = false);
+
+// geometric solvers (cmg_geo.hip); mode 0 = GMG, 1 = CMG
+void launch_geo_apply(const DevGrid& d, int mode, const LaunchList& l, int subtree_level, const float* X, float* Y, bool diag,
+                      const double* S, hipStream_t st);
+void launch_geo_gs(const DevGrid& d, int mode, int level, float* X, const float* B, int color, const double* S, hipStream_t st);
+void launch_geo_restrict(const DevGrid& d, int mode, int level, const float* X, float* B, float one_over_alpha,
+                         const double* S, hipStream_t st);
+void launch_accumulate_ghosts_add(const DevGrid& d, float* Y, float coeff, const double* S, hipStream_t st);
+void launch_residual_interior(const DevGrid& d, const float* B, float* AX, hipStream_t st);
+void launch_djacobi_update(const DevGrid& d, float* X, const float* R, const float* D, float omega, hipStream_t st);
 void launch_deduct_flux(const DevGrid& d, int level, const float* X, Coef C, const float* F, float* RHS, const double* S,
                         hipStream_t st);
 void launch_nonfas_init(const DevGrid& d, float* X, const float* F, float* RHS, const double* S, hipStream_t st);

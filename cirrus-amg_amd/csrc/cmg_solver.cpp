@@ -22,6 +22,7 @@
 
 struct cmg_solver {
     cmg_grid* g = nullptr;
+    int family = CMG_SOLVER_AMG;   // AMGSolver, GMGSolver or CMGSolver
     int c0 = 11;
     float Rm = 0.5f, Rr = 1.0f, P = 1.0f;
     double* S = nullptr;
@@ -140,6 +141,7 @@ void dev_free(DevGrid& d)
     free_list(d.leafs);
     free_list(d.all);
     free_list(d.ghosts);
+    free_list(d.leafghost);
     for (int l = 0; l < kMaxLevels; ++l) {
         free_list(d.subtree[l]);
         free_list(d.nonleaf[l]);
@@ -176,7 +178,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     up(d.child, topo.child);
     d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
     HIPCHK(hipMemset(d.ctype, 0, size_t(N) * kCells));
-    std::vector<int32_t> leafs, all, ghosts;
+    std::vector<int32_t> leafs, all, ghosts, leafghost;
     std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels], la[kMaxLevels];
     for (int t = 0; t < N; ++t) {
         const uint8_t ty = topo.t_type[t];
@@ -184,6 +186,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
         all.push_back(t);
         if (ty & LEAF) leafs.push_back(t);
         if (ty & GHOST) ghosts.push_back(t);
+        if (ty & (LEAF | GHOST)) leafghost.push_back(t);
         if (ty & (LEAF | NONLEAF)) sub[l].push_back(t);
         if (ty & NONLEAF) nl[l].push_back(t);
         if (ty & LEAF) la[l].push_back(t);
@@ -192,6 +195,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.leafs = make_list(topo, leafs);
     d.all = make_list(topo, all);
     d.ghosts = make_list(topo, ghosts);
+    d.leafghost = make_list(topo, leafghost);
     d.max_list = std::max(d.leafs.n, d.all.n);
     for (int l = 0; l < topo.num_levels(); ++l) {
         d.subtree[l] = make_list(topo, sub[l]);
@@ -593,12 +597,76 @@ void mu_step(const CycleCtx& c, int level, bool x_zero)
     smooth(c, level, c.o->level_iters, 1);
 }
 
+// ---- geometric solvers (GMGSolver, CMGSolver) ------------------------------
+
+int geo_mode(const cmg_solver* s) { return s->family == CMG_SOLVER_GMG ? 0 : 1; }
+
+void geo_gauss_seidel(cmg_solver* s, int iters, int order, int level, float* X, const float* B, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    for (int it = 0; it < iters; ++it) {
+        launch_geo_gs(d, geo_mode(s), level, X, B, order, S, d.stream);
+        launch_geo_gs(d, geo_mode(s), level, X, B, 1 - order, S, d.stream);
+    }
+}
+
+// GMGSolver::VCycle (src/GMGSolver.cu:310-387) / CMGSolver::VCycle (src/CMGSolver.cu:372-424)
+void geo_vcycle(cmg_solver* s, const cmg_solve_opts* o, float* X, const float* F, float* B, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    hipStream_t st = d.stream;
+    const int mode = geo_mode(s);
+    launch_nonfas_init(d, X, F, B, S, st);
+    for (int l = d.max_level; l > 0; --l) {
+        geo_gauss_seidel(s, o->level_iters, 0, l, X, B, S);
+        launch_geo_restrict(d, mode, l, X, B, s->Rr, S, st);
+    }
+    geo_gauss_seidel(s, o->coarsest_iters / 2, 0, 0, X, B, S);
+    geo_gauss_seidel(s, o->coarsest_iters / 2, 1, 0, X, B, S);
+    for (int l = 1; l <= d.max_level; ++l) {
+        // GMG prolongs into LEAF|NONLEAF|GHOST tiles, CMG into LEAF|NONLEAF only
+        launch_prolong_add(d, l, X, s->P, S, st, mode == 1);
+        geo_gauss_seidel(s, o->level_iters, 1, l, X, B, S);
+    }
+}
+
+// FullNegativeLaplacian (src/GMGSolver.cu:167-171) / ConservativeFullNegativeLaplacian
+// (src/CMGSolver.cu:209-225): y = A x on LEAF tiles (GMG also fills GHOST tiles
+// and adds them into their parents); the propagated ghost and accumulated
+// NONLEAF values stay in x
+void geo_full_apply(cmg_solver* s, float* X, float* Y, bool diag, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    hipStream_t st = d.stream;
+    launch_propagate_ghosts(d, X, st);
+    if (s->family == CMG_SOLVER_GMG) {
+        launch_geo_apply(d, 0, d.leafghost, -1, X, Y, diag, S, st);
+        launch_accumulate_ghosts_add(d, Y, 1.0f, S, st);
+    } else {
+        launch_accumulate_leafs(d, X, st);
+        launch_geo_apply(d, 1, d.leafs, -1, X, Y, false, S, st);
+    }
+}
+
+// y = A x with the solver's operator; returns the number of x.y partials when `partial`
+int enqueue_apply(cmg_solver* s, float* X, float* Y, double* partial, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    if (s->family == CMG_SOLVER_AMG) return launch_apply(d, X, coef(d, s->c0), Y, partial, S, d.stream);
+    geo_full_apply(s, X, Y, false, S);
+    return partial ? launch_dot(d, d.leafs, X, Y, partial, S, d.stream) : 0;
+}
+
 // the preconditioner z ~ A^-1 r of the options' cycle kind.  FAS kinds: x, rhs
 // = r (its NONLEAF tiles become coarse right-hand sides), x0 = FAS scratch.
 // Non-FAS kinds: x, f = r (kept), x0 = working right-hand side.
 void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0, const double* S)
 {
     DevGrid& d = s->g->dev;
+    if (s->family != CMG_SOLVER_AMG) {
+        geo_vcycle(s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), S);
+        return;
+    }
     const Coef C = coef(d, s->c0);
     switch (o->cycle) {
     case CMG_CYCLE_VCYCLE:
@@ -641,7 +709,7 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     hipStream_t st = d.stream;
     float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *P = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
           *Z = dev_channel(d, CH_Z);
-    const int n1 = launch_apply(d, P, coef(d, s->c0), AP, s->partial, s->S, st);
+    const int n1 = enqueue_apply(s, P, AP, s->partial, s->S);
     launch_finish(s->partial, nullptr, n1, s->S, FIN_ALPHA, st);
     const int n2 = launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
@@ -671,7 +739,8 @@ void ensure_buffers(cmg_solver* s)
         HIPCHK(hipMemset(s->tmp, 0, size_t(d.N) * kCells * sizeof(float)));
     }
     for (int c : {CH_X, CH_R, CH_P, CH_AP, CH_Z}) dev_channel(d, c);
-    for (int a = 0; a < 4; ++a) dev_channel(d, s->c0 + a);
+    if (s->family == CMG_SOLVER_AMG)
+        for (int a = 0; a < 4; ++a) dev_channel(d, s->c0 + a);
 }
 
 void ensure_fp64(cmg_solver* s)
@@ -730,7 +799,7 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     } else if (rhs2 != 0) {
         // r = b - A x (warm start; the reference always starts from 0, AMGSolver.cu:1407);
         // the stopping test stays relative to |b|
-        launch_apply(d, X, coef(d, s->c0), AP, nullptr, nullptr, st);
+        enqueue_apply(s, X, AP, nullptr, nullptr);
         launch_axpy(d, d.leafs, R, AP, -1.0f, st);
     }
     if (rhs2 == 0) return 1;
@@ -865,6 +934,76 @@ cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_s
     return CMG_OK;
 }
 
+cmg_status solver_create_geo(cmg_grid* g, int family, float one_over_alpha, float P, cmg_solver** out)
+{
+    if (family != CMG_SOLVER_GMG && family != CMG_SOLVER_CMG) return CMG_ERR_BAD_ARG;
+    cmg_status st = solver_create(g, 11, 1.0f, one_over_alpha, P, out);
+    if (st == CMG_OK) (*out)->family = family;
+    return st;
+}
+
+cmg_status solver_apply(cmg_solver* s, int x, int y)
+{
+    if (!s || !valid_channel(x) || !valid_channel(y) || x == y) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        ensure_buffers(s);
+        DevGrid& d = s->g->dev;
+        enqueue_apply(s, dev_channel(d, x), dev_channel(d, y), nullptr, nullptr);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+// GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:390-470)
+cmg_status solver_damped_jacobi(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
+{
+    if (!s || !o || !st || s->family != CMG_SOLVER_GMG || o->max_iters < 0) return CMG_ERR_BAD_ARG;
+    *st = cmg_solve_stats{};
+    bool converged = false;
+    CMG_TRY({
+        DevGrid& d = s->g->dev;
+        hipStream_t stream = d.stream;
+        ensure_buffers(s);
+        float *X = dev_channel(d, CH_X), *B = dev_channel(d, CH_R), *AX = dev_channel(d, CH_P), *D = dev_channel(d, CH_AP);
+        set_scalars(s, 0.0, o->max_iters);
+        double h[S_COUNT];
+        HIPCHK(hipEventRecord(s->e0, stream));
+        launch_fill_all(d, X, 0.0f, nullptr, stream);
+        int nb = launch_dot(d, d.all, B, B, s->partial, nullptr, stream);   // |b| over every tile
+        launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, stream);
+        read_scalars(s, h);
+        const double rhs_norm = std::sqrt(h[S_TMP]);
+        const double thr = std::max(o->relative_tolerance * rhs_norm, 2.2250738585072014e-308);
+        double res = rhs_norm;
+        int i = 0;
+        for (i = 0; i < o->max_iters; ++i) {
+            geo_full_apply(s, X, AX, false, nullptr);
+            launch_residual_interior(d, B, AX, stream);
+            nb = launch_dot(d, d.leafs, AX, AX, s->partial, nullptr, stream);
+            launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, stream);
+            read_scalars(s, h);
+            res = std::sqrt(h[S_TMP]);
+            if (!std::isfinite(res)) break;
+            if (res < thr) {
+                converged = true;
+                break;
+            }
+            geo_full_apply(s, X, D, true, nullptr);
+            launch_djacobi_update(d, X, AX, D, float(o->omega), stream);
+        }
+        HIPCHK(hipEventRecord(s->e1, stream));
+        HIPCHK(hipEventSynchronize(s->e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
+        st->ms = ms;
+        st->iters = i;
+        st->rel_res = rhs_norm > 0 ? res / rhs_norm : 0.0;
+        st->true_rel_res = st->rel_res;
+        if (!std::isfinite(res)) return CMG_ERR_NONFINITE;
+    });
+    return converged ? CMG_OK : CMG_ERR_NOT_CONVERGED;
+}
+
 void solver_destroy(cmg_solver* s)
 {
     if (!s) return;
@@ -895,7 +1034,8 @@ cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, 
     if (!valid_channel(x) || !valid_channel(rhs) || !valid_channel(x0)) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         ensure_buffers(s);
-        if (o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
+        if (s->family == CMG_SOLVER_AMG && o->coarsest_direct &&
+            (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
             build_direct(s, o->is_pure_neumann);
         enqueue_cycle(s, o, x, rhs, x0, nullptr);
         HIPCHK(hipStreamSynchronize(s->g->dev.stream));
@@ -907,17 +1047,19 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
 {
     if (!s || !st) return CMG_ERR_BAD_ARG;
     if (auto e = check_opts(o)) return e;
+    if (s->family != CMG_SOLVER_AMG && o->precision != CMG_PREC_FP32) return CMG_ERR_BAD_ARG;
     *st = cmg_solve_stats{};
     int done = 1;
     CMG_TRY({
         DevGrid& d = s->g->dev;
         hipStream_t stream = d.stream;
         ensure_buffers(s);
-        ensure_fp64(s);
-        if (o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
+        const bool amg = s->family == CMG_SOLVER_AMG;
+        if (amg) ensure_fp64(s);
+        if (amg && o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
             build_direct(s, o->is_pure_neumann);
         // keep b in FP64 for the true residual / refinement
-        launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
+        if (amg) launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipEventRecord(s->e0, stream));
         if (o->precision == CMG_PREC_FP32) {
@@ -932,9 +1074,13 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
             HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
             st->ms = ms;
             // true residual from the FP32 solution (outside the timed region)
-            const double rn2 = residual_norm2(s, false);
-            const double bn2 = b_norm2(s);
-            st->true_rel_res = bn2 > 0 ? std::sqrt(rn2 / bn2) : 0.0;
+            if (amg) {
+                const double rn2 = residual_norm2(s, false);
+                const double bn2 = b_norm2(s);
+                st->true_rel_res = bn2 > 0 ? std::sqrt(rn2 / bn2) : 0.0;
+            } else {
+                st->true_rel_res = st->rel_res;
+            }
         } else {
             // FP64 iterative refinement around FP32 PCG
             const size_t n = size_t(d.N) * kCells;
@@ -986,6 +1132,7 @@ cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_sta
     if (!s || !st) return CMG_ERR_BAD_ARG;
     if (auto e = check_opts(o)) return e;
     if (o->cycle != CMG_CYCLE_FAS_W && o->cycle != CMG_CYCLE_FAS_V) return CMG_ERR_BAD_ARG;
+    if (s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     *st = cmg_solve_stats{};
     bool converged = false;
     CMG_TRY({
@@ -1055,7 +1202,7 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
-    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 8) return CMG_ERR_BAD_ARG;
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 8 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         ensure_buffers(s);
         DevGrid& d = s->g->dev;

@@ -178,9 +178,22 @@ typedef struct cmg_solve_stats {
 
 void cmg_solve_opts_default(cmg_solve_opts* o);   /* FluidEuler::project settings */
 
+/* solver families: AMGSolver (assembled coefficients), GMGSolver (src/GMGSolver.h:14-50),
+ * CMGSolver (src/CMGSolver.h:12-47) */
+enum { CMG_SOLVER_AMG = 0, CMG_SOLVER_GMG = 1, CMG_SOLVER_CMG = 2 };
+
 /* AMGSolver(coeff_channel, R_matrix, R_restrict, P) (src/AMGSolver.h:32-42) */
 cmg_status cmg_solver_create(cmg_grid* g, int coeff_channel, float R_matrix, float R_restrict,
                              float P, cmg_solver** out);
+/* GMGSolver(one_over_alpha, prolong_coeff) / CMGSolver(one_over_alpha, prolong_coeff)
+ * (src/GMGSolver.h:18-26, src/CMGSolver.h:16-24): family = CMG_SOLVER_GMG or CMG_SOLVER_CMG.
+ * Their operators come from cell and tile types alone (no coefficient channels).
+ * cmg_solve on such a solver runs GMGSolver::solve / CMGSolver::solve
+ * (src/GMGSolver.cu:472-607, src/CMGSolver.cu:428-601): PCG with their V-cycle
+ * of red-black Gauss-Seidel (level_iters, coarsest_iters); cycle, smoother,
+ * omega, coarsest_direct and FP64 refinement do not apply (CMG_ERR_BAD_ARG for
+ * precision != FP32), and true_rel_res repeats rel_res. */
+cmg_status cmg_geo_solver_create(cmg_grid* g, int family, float one_over_alpha, float prolong_coeff, cmg_solver** out);
 void cmg_solver_destroy(cmg_solver* s);
 /* AMGSolver::solve (src/AMGSolver.cu:1399-1549): solves -lap(x) = b with b in
  * channel 1 on LEAF INTERIOR cells; x lands in channel 0; channels 1-4 of
@@ -193,6 +206,16 @@ cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats*
  * coarsest_direct.  stats->iters is the reference's return value: the loop
  * index at exit (cycles run - 1 when converged). */
 cmg_status cmg_mg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats);
+/* GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:390-470) on a GMG solver: x = 0
+ * (channel 0), b (channel 1, kept), residual channel 2, diagonal channel 3,
+ * x += omega r / D until |r| < relative_tolerance |b| (|b| over every tile);
+ * opts' max_iters, relative_tolerance and omega are used; stats->iters is the
+ * loop index at exit, as the reference returns it. */
+cmg_status cmg_gmg_damped_jacobi_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats);
+/* y = A x with the solver's operator on LEAF tiles (AMG: AMGFullNegativeLaplacianOnLeafs;
+ * GMG: FullNegativeLaplacian, src/GMGSolver.cu:167-171; CMG: ConservativeFullNegativeLaplacian,
+ * src/CMGSolver.cu:209-225); x keeps the propagated ghost / accumulated parent values */
+cmg_status cmg_solver_apply(cmg_solver* s, int x, int y);
 /* the FP64 solution of the last FP64-refinement solve (N*512 doubles) */
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host);
 
@@ -219,7 +242,9 @@ cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int coeff_channel, int
 cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P);
 /* Dot over LEAF tiles (src/PoissonGrid.cu:156-214), FP64 */
 cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out);
-/* FASMuCycle (src/AMGSolver.cu:1279-1292) with the solver's cycle options */
+/* the solver's preconditioner cycle on channels (x, rhs, x0): AMG FAS kinds use rhs as
+ * the right-hand side and x0 as FAS scratch; the non-FAS kinds and the GMG / CMG
+ * V-cycles keep rhs (f) and use x0 as the working right-hand side */
 cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0);
 
 /* ------------------------------------------------------- measurement ------- */

@@ -165,6 +165,22 @@ og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, doub
  * (loop index at exit, |r| / |b|) */
 og_solve_out og_fas_mu_solve(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol);
 
+/* ---- geometric multigrid solvers (src/GMGSolver.cu, src/CMGSolver.cu) ------ */
+enum { OG_GEO_GMG = 0, OG_GEO_CMG = 1 };
+/* the family's operator on every tile whose subtree type at subtree_level (-1:
+ * own type) is in `types` (and at that level when subtree_level >= 0) */
+void og_geo_neg_lap_tiles(og_grid* g, int mode, int subtree_level, int types, int x, int y, int diag);
+/* FullNegativeLaplacian (GMGSolver.cu:167-171) / ConservativeFullNegativeLaplacian (CMGSolver.cu:209-225) */
+void og_geo_full_neg_lap(og_grid* g, int mode, int x, int y, int diag);
+void og_geo_gs_color(og_grid* g, int mode, int level, int x, int rhs, int color);          /* GMG :173-206, CMG :228-264 */
+void og_geo_residual_restrict(og_grid* g, int mode, int level, int x, int rhs, float one_over_alpha); /* :215-275 / :273-330 */
+void og_geo_prolong(og_grid* g, int mode, int level, int x, float P);                       /* :277-307 / :338-369 */
+void og_geo_vcycle(og_grid* g, int mode, int x, int f, int rhs, int level_iters, int coarsest_iters,
+                   float one_over_alpha, float P);                                          /* :310-387 / :372-424 */
+og_solve_out og_geo_pcg(og_grid* g, int mode, int level_iters, int coarsest_iters, float one_over_alpha, float P,
+                        int max_iters, double tol, int pure_neumann);                       /* :472-607 / :428-601 */
+og_solve_out og_gmg_damped_jacobi(og_grid* g, int max_iters, double tol, float omega);     /* GMGSolver.cu:390-470 */
+
 /* ---- FP64 assembled oracle of the composite LEAF operator ----------------- */
 typedef struct og_csr {
     int n;                 /* number of unknowns = LEAF INTERIOR cells            */

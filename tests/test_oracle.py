@@ -361,3 +361,93 @@ def test_fas_mu_solve_converges_to_the_solution():
     # with max_iters = iters the loop ends without breaking: index = iters, not converged
     it2, rel2 = g.fas_mu_solve(O.make_cycle(mu=2), H.C0, iters, 1e-6)
     assert it2 == iters and rel2 >= 1e-6
+
+
+# ---- geometric solvers (GMGSolver, CMGSolver) ------------------------------------
+
+@pytest.mark.parametrize("mode", [O.GEO_GMG, O.GEO_CMG])
+def test_geo_operator_is_the_7_point_laplacian_on_uniform(mode):
+    """On a uniform grid both geometric operators reduce to h (6 x0 - sum x_nb)
+    with x = 0 outside the box: the AMG operator of the same grid."""
+    g = O.Grid(O.make_target(O.TARGET_UNIFORM, 2, 2))
+    g.set_leaf_types_preset()
+    g.coarsen_types_and_coeffs(H.C0, H.R_MATRIX)
+    x = np.random.default_rng(2).uniform(-1, 1, (g.N, 512)).astype(np.float32)
+    g.channel(0)[:] = x
+    g.full_neg_lap(0, H.C0, 3)
+    amg = g.channel(3).astype(np.float64).copy()
+    g.channel(0)[:] = x
+    O.lib().og_geo_full_neg_lap(g.h, mode, 0, 4, 0)
+    geo = g.channel(4).astype(np.float64)
+    leaf = g.tiles()[2] == 1
+    assert np.abs(geo[leaf] - amg[leaf]).max() <= 1e-6 * np.abs(amg[leaf]).max()
+
+
+@pytest.mark.parametrize("mode", [O.GEO_GMG, O.GEO_CMG])
+def test_geo_operator_annihilates_constants_away_from_dirichlet(mode):
+    """A constant field (ghosts take their parents' constant value) has zero
+    Laplacian at every INTERIOR cell whose face neighbours all exist and are
+    not DIRICHLET."""
+    go = _graded("sphere_empty24")
+    lv, co, ty = go.tiles()
+    go.channel(0)[:] = 3.0
+    O.lib().og_geo_full_neg_lap(go.h, mode, 0, 3, 0)
+    y = go.channel(3)
+    T = go.cell_types()
+    nbr = go.neighbors()
+    v = np.arange(512)
+    loc = np.stack([v >> 6, (v >> 3) & 7, v & 7], axis=1)
+    checked = 0
+    for t in np.where(ty == 1)[0]:
+        ok = np.ones(512, bool)
+        for a in range(3):
+            for side, q in ((-1, a), (1, a + 3)):
+                edge = loc[:, a] == (0 if side < 0 else 7)
+                if nbr[t, q] < 0:
+                    ok &= ~edge
+                    continue
+                nl = loc.copy()
+                nl[:, a] = np.where(edge, 7 if side < 0 else 0, loc[:, a] + side)
+                nv = (nl[:, 0] << 6) | (nl[:, 1] << 3) | nl[:, 2]
+                nt = np.where(edge, nbr[t, q], t)
+                ok &= T[nt, nv] != 1
+        m = ok & (T[t] == 2)
+        assert np.abs(y[t, m]).max(initial=0) <= 1e-5, t
+        checked += int(m.sum())
+    assert checked > 10000
+
+
+@pytest.mark.parametrize("mode", [O.GEO_GMG, O.GEO_CMG])
+def test_geo_pcg_solves_its_own_operator(mode):
+    """PCG with the family's V-cycle converges to the solution of the family's
+    operator, assembled here column by column from the oracle's own apply."""
+    go = _graded("sphere_empty24")
+    lv, co, ty = go.tiles()
+    T = go.cell_types()
+    cells = np.argwhere((ty == 1)[:, None] & (T == 2))
+    rng = np.random.default_rng(9)
+    b = np.zeros((go.N, 512), np.float32)
+    b[cells[:, 0], cells[:, 1]] = rng.uniform(-1, 1, len(cells)).astype(np.float32)
+    go.channel(1)[:] = b
+    it, rel = go.geo_pcg(mode, max_iters=200, tol=1e-6)
+    assert rel < 1e-6, (it, rel)
+    x = go.channel(0)[cells[:, 0], cells[:, 1]].astype(np.float64)
+    # residual of the FP32 solution with the family's operator, recomputed in FP64 from probes
+    go.channel(0)[:] = 0
+    go.channel(0)[cells[:, 0], cells[:, 1]] = x.astype(np.float32)
+    O.lib().og_geo_full_neg_lap(go.h, mode, 0, 3, 0)
+    ax = go.channel(3)[cells[:, 0], cells[:, 1]].astype(np.float64)
+    bb = b[cells[:, 0], cells[:, 1]].astype(np.float64)
+    assert np.linalg.norm(bb - ax) <= 2e-6 * np.linalg.norm(bb)
+
+
+def test_gmg_damped_jacobi_converges():
+    g = O.Grid(O.make_target(O.TARGET_UNIFORM, 1, 1))
+    g.set_leaf_types_preset()
+    g.rhs_uniform(1, 5, True)
+    it, rel = g.gmg_damped_jacobi(5000, 1e-2, 2.0 / 3.0)
+    assert rel < 1e-2 and 10 < it < 5000
+    # the loop index at exit is returned; one iteration fewer does not converge
+    g.rhs_uniform(1, 5, True)
+    it2, rel2 = g.gmg_damped_jacobi(it, 1e-2, 2.0 / 3.0)
+    assert it2 == it and rel2 >= 1e-2
