@@ -76,7 +76,7 @@ EXPORTS = [
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
     "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize", "cmg_set_kernel_forms",
     "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
-    "cmg_solver_apply",
+    "cmg_solver_apply", "cmg_op_norm",
 ]
 
 _lib = None
@@ -122,6 +122,7 @@ def lib():
             "cmg_geo_solver_create": (st, [vp, i, C.c_float, C.c_float, C.POINTER(vp)]),
             "cmg_gmg_damped_jacobi_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_solver_apply": (st, [vp, i, i]),
+            "cmg_op_norm": (st, [vp, i, i, i, i, C.POINTER(d)]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
             "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
             "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
@@ -314,6 +315,12 @@ class HADeviceGrid:
 
     def fas_prolong(self, level, x, x0, P):
         _check(lib().cmg_op_fas_prolong(self.h, level, x, x0, P), "fas_prolong")
+
+    def norm(self, order, channel, volume_weighted=False, cell_types=2):
+        """NormSync (src/PoissonGrid.cu:322-375) over LEAF tiles."""
+        out = C.c_double()
+        _check(lib().cmg_op_norm(self.h, order, channel, int(volume_weighted), cell_types, C.byref(out)), "cmg_op_norm")
+        return out.value
 
     def dot(self, a, b):
         out = C.c_double()

@@ -431,6 +431,10 @@ cmg_status cmg_op_residual_restrict(cmg_grid* g, int level, int x, int c0, int r
 cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0) { return op_fas_save(g, level, x, c0, rhs, x0); }
 cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P) { return op_fas_prolong(g, level, x, x0, P); }
 cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out) { return op_dot(g, a, b, out); }
+cmg_status cmg_op_norm(cmg_grid* g, int order, int channel, int volume_weighted, int cell_types, double* out)
+{
+    return op_norm(g, order, channel, volume_weighted, cell_types, out);
+}
 cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0) { return solver_cycle(s, opts, x, rhs, x0); }
 
 cmg_status cmg_mg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)

@@ -93,6 +93,7 @@ cmg_status op_residual_restrict(cmg_grid* g, int level, int x, int c0, int rhs, 
 cmg_status op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0);
 cmg_status op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P);
 cmg_status op_dot(cmg_grid* g, int a, int b, double* out);
+cmg_status op_norm(cmg_grid* g, int order, int ch, int volume_weighted, int cell_types, double* out);
 cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_solver** out);
 void solver_destroy(cmg_solver* s);
 cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);

@@ -995,6 +995,57 @@ __global__ __launch_bounds__(256) void coarse_direct(const double* __restrict__ 
 
 namespace k {
 
+// ChannelPowerSumKernel128 (src/PoissonGrid.cu:223-319) on one tile per 128-thread
+// workgroup: per tile the sum of |x|^order (order -1: max |x|) over the cells
+// whose type is in `cell_types`, and their count, both times h^3 when volume
+// weighted; FP64 throughout
+__global__ __launch_bounds__(128) void power_sum(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+                                                 int order, int volume_weighted, int cell_types, double* __restrict__ vsum,
+                                                 double* __restrict__ wsum)
+{
+    __shared__ double red[2][2];
+    const int t = list[blockIdx.x];
+    double v = 0, c = 0;
+    if (t >= 0) {
+        const size_t q = size_t(t) * 128 + threadIdx.x;
+        const float4 x = reinterpret_cast<const float4*>(X)[q];
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        const float xs[4] = {x.x, x.y, x.z, x.w};
+        const uint8_t cs[4] = {ct.x, ct.y, ct.z, ct.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (!(cs[e] & cell_types)) continue;
+            const double a = fabs(double(xs[e]));
+            c += 1;
+            if (order == -1) v = fmax(v, a);
+            else if (order == 1) v += a;
+            else if (order == 2) v += a * a;
+            else v += pow(a, double(order));
+        }
+    }
+    // wave64 reductions, then the two waves
+    for (int o = 32; o > 0; o >>= 1) {
+        const double vo = __shfl_xor(v, o, 64);
+        v = order == -1 ? fmax(v, vo) : v + vo;
+        c += __shfl_xor(c, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6][0] = v;
+        red[threadIdx.x >> 6][1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double bv = order == -1 ? fmax(red[0][0], red[1][0]) : red[0][0] + red[1][0];
+    double bc = red[0][1] + red[1][1];
+    if (t >= 0 && volume_weighted && order != -1) {
+        const double h = double(tile_h(tp, t));
+        bv *= h * h * h;
+        bc *= h * h * h;
+    }
+    vsum[blockIdx.x] = bv;
+    wsum[blockIdx.x] = bc;
+}
+
 // ------------------------------------------------- non-FAS transfers ----
 
 // RestrictSourceToCoarse (src/AMGSolver.cu:1295-1307) = AccumulateToParentsOneStep
@@ -1469,6 +1520,14 @@ void launch_sub_from(const DevGrid& d, const LaunchList& l, const float* B, floa
 {
     if (!nonempty(l)) return;
     k::sub_from_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, B, Y);
+}
+
+int launch_power_sum(const DevGrid& d, const float* X, int order, bool volume_weighted, int cell_types, double* vsum,
+                     double* wsum, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return 0;
+    k::power_sum<<<d.leafs.n, 128, 0, st>>>(view(d), d.leafs.d, X, order, int(volume_weighted), cell_types, vsum, wsum);
+    return d.leafs.n;
 }
 
 }  // namespace cmg

@@ -81,6 +81,10 @@ void launch_deduct_flux(const DevGrid& d, int level, const float* X, Coef C, con
 void launch_nonfas_init(const DevGrid& d, float* X, const float* F, float* RHS, const double* S, hipStream_t st);
 void launch_sub_from(const DevGrid& d, const LaunchList& l, const float* B, float* Y, hipStream_t st);
 
+// per-LEAF-tile partials of NormSync (src/PoissonGrid.cu:223-375); returns the partial count
+int launch_power_sum(const DevGrid& d, const float* X, int order, bool volume_weighted, int cell_types, double* vsum,
+                     double* wsum, hipStream_t st);
+
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
 bool set_kernel_forms(int mask, int threads);
 void get_kernel_forms(int* mask, int* threads);

@@ -345,6 +345,45 @@ cmg_status op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P)
     return CMG_OK;
 }
 
+// NormSync (src/PoissonGrid.cu:322-375): order -1 = max |x|; order p > 0 =
+// (sum |x|^p w / sum w)^(1/p) over LEAF cells whose type is in cell_types, w = 1
+// or the cell volume; tile partials are summed on the host in tile order
+cmg_status op_norm(cmg_grid* g, int order, int ch, int volume_weighted, int cell_types, double* out)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!out || !valid_channel(ch) || order == 0 || order < -1 || (order == -1 && volume_weighted)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        double* w = dmalloc<double>(size_t(d.max_list) + 1);
+        std::vector<double> hv, hw;
+        try {
+            const int n = launch_power_sum(d, dev_channel(d, ch), order, volume_weighted != 0, cell_types, d.partial, w, d.stream);
+            hv.resize(size_t(n));
+            hw.resize(size_t(n));
+            if (n) {
+                HIPCHK(hipMemcpyAsync(hv.data(), d.partial, n * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+                HIPCHK(hipMemcpyAsync(hw.data(), w, n * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+            }
+            HIPCHK(hipStreamSynchronize(d.stream));
+        } catch (...) {
+            (void)hipFree(w);
+            throw;
+        }
+        (void)hipFree(w);
+        double vs = 0, ws = 0;
+        for (size_t i = 0; i < hv.size(); ++i) {
+            vs = order == -1 ? std::max(vs, hv[i]) : vs + hv[i];
+            ws += hw[i];
+        }
+        if (order == -1) *out = vs;
+        else if (ws == 0) *out = 0;
+        else if (order == 1) *out = vs / ws;
+        else if (order == 2) *out = std::sqrt(vs / ws);
+        else *out = std::pow(vs / ws, 1.0 / order);
+    });
+    return CMG_OK;
+}
+
 cmg_status op_dot(cmg_grid* g, int a, int b, double* out)
 {
     if (auto st = check_grid(g)) return st;

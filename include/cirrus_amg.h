@@ -242,6 +242,10 @@ cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int coeff_channel, int
 cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P);
 /* Dot over LEAF tiles (src/PoissonGrid.cu:156-214), FP64 */
 cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out);
+/* NormSync (src/PoissonGrid.cu:322-375) over LEAF tiles: order -1 = max |x| (not volume
+ * weighted), order p >= 1 = (sum |x|^p w / sum w)^(1/p) over the cells whose type is in
+ * cell_types, w = 1 or the cell volume h^3; sums in FP64 */
+cmg_status cmg_op_norm(cmg_grid* g, int order, int channel, int volume_weighted, int cell_types, double* out);
 /* the solver's preconditioner cycle on channels (x, rhs, x0): AMG FAS kinds use rhs as
  * the right-hand side and x0 as FAS scratch; the non-FAS kinds and the GMG / CMG
  * V-cycles keep rhs (f) and use x0 as the working right-hand side */

@@ -452,6 +452,32 @@ double og_dot(og_grid* g, int a, int b)
     return s;
 }
 
+/* NormSync (PoissonGrid.cu:322-375) with ChannelPowerSumKernel128 (:223-319):
+ * over LEAF tiles, cells whose type is in cell_types, |x|; order -1 = max,
+ * else (sum |x|^p w / sum w)^(1/p), w = h^3 when volume weighted; FP64 */
+double og_norm(og_grid* g, int order, int ch, int volume_weighted, int cell_types)
+{
+    double vs = 0, ws = 0;
+    for (int t = 0; t < g->N; t++) {
+        if (!(g->t_type[t] & OG_LEAF)) continue;
+        double w = 1;
+        if (volume_weighted) { double h = og_h(g, g->t_level[t]); w = h * h * h; }
+        for (int v = 0; v < 512; v++) {
+            size_t idx = (size_t)t * 512 + v;
+            if (!(g->ctype[idx] & cell_types)) continue;
+            double a = fabs((double)g->ch[ch][idx]);
+            if (order == -1) { if (a > vs) vs = a; continue; }
+            vs += (order == 1 ? a : order == 2 ? a * a : pow(a, order)) * w;
+            ws += w;
+        }
+    }
+    if (order == -1) return vs;
+    if (ws == 0) return 0;
+    if (order == 1) return vs / ws;
+    if (order == 2) return sqrt(vs / ws);
+    return pow(vs / ws, 1.0 / order);
+}
+
 /* ReCenterLeafCells (PoissonGrid.cu:889-921): subtract the pointwise mean over
  * INTERIOR cells of LEAF tiles */
 void og_recenter(og_grid* g, int ch)

@@ -121,6 +121,7 @@ void og_fas_save(og_grid* g, int level, int x, int c0, int rhs, int x0);        
 void og_fas_prolong(og_grid* g, int level, int x, int x0, float P);                   /* :1216-1255 */
 double og_dot(og_grid* g, int a, int b);                       /* PoissonGrid.cu:156-214, LEAF tiles */
 void og_recenter(og_grid* g, int ch);                          /* PoissonGrid.cu:889-921 */
+double og_norm(og_grid* g, int order, int ch, int volume_weighted, int cell_types); /* PoissonGrid.cu:322-375 */
 
 /* multigrid cycle options */
 typedef struct og_cycle {

@@ -80,6 +80,7 @@ def lib():
             "og_fas_cycle": (None, [vp, vp, i, i, i, i]),
             "og_pcg": (SolveOut, [vp, vp, i, i, d, i]),
             "og_fas_mu_solve": (SolveOut, [vp, vp, i, i, d]),
+            "og_norm": (d, [vp, i, i, i, i]),
             "og_geo_neg_lap_tiles": (None, [vp, i, i, i, i, i, i]), "og_geo_full_neg_lap": (None, [vp, i, i, i, i]),
             "og_geo_gs_color": (None, [vp, i, i, i, i, i]), "og_geo_residual_restrict": (None, [vp, i, i, i, i, f]),
             "og_geo_prolong": (None, [vp, i, i, i, f]), "og_geo_vcycle": (None, [vp, i, i, i, i, i, i, f, f]),
@@ -126,6 +127,7 @@ def make_target(kind, min_level, max_level, sdf_kind=SDF_SPHERE, sdf=(), bands=N
 
 CYCLE_FAS_MU, CYCLE_FAS_F, CYCLE_VCYCLE, CYCLE_MU = 0, 1, 2, 3
 GEO_GMG, GEO_CMG = 0, 1
+OG_INTERIOR_T = 2
 
 
 def make_cycle(mu=2, smoother=0, omega=1.5, level_iters=2, coarsest_iters=10, coarsest_direct=0,
@@ -219,6 +221,9 @@ class Grid:
 
     def dot(self, a, b):
         return lib().og_dot(self.h, a, b)
+
+    def norm(self, order, ch, volume_weighted=False, cell_types=OG_INTERIOR_T):
+        return lib().og_norm(self.h, order, ch, int(volume_weighted), cell_types)
 
     def pcg(self, cycle, c0=11, max_iters=100, tol=1e-6, pure_neumann=False):
         out = lib().og_pcg(self.h, C.byref(cycle), c0, max_iters, tol, int(pure_neumann))

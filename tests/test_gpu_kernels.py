@@ -155,3 +155,12 @@ def test_kernel_forms_bitexact(pair, forms, restore_forms):
         assert np.array_equal(H.bits(go.channel(1)), H.bits(gp.channel(1))), ("restrict b", name, level, forms)
         assert np.array_equal(H.bits(go.channel(0)), H.bits(gp.channel(0))), ("restrict x", name, level, forms)
 
+
+
+@pytest.mark.parametrize("order,vw,types", [(-1, False, 2), (1, True, 2), (2, True, 2), (2, False, 7), (3, False, 2)])
+def test_norm_matches_oracle(pair, order, vw, types):
+    name, go, gp = pair
+    _load(go, gp, 2, H.random_fill(go.N, 150))
+    ref = go.norm(order, 2, vw, types)
+    got = gp.norm(order, 2, vw, types)
+    assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref)), (name, order, vw, got, ref)

@@ -451,3 +451,26 @@ def test_gmg_damped_jacobi_converges():
     g.rhs_uniform(1, 5, True)
     it2, rel2 = g.gmg_damped_jacobi(it, 1e-2, 2.0 / 3.0)
     assert it2 == it and rel2 >= 1e-2
+
+
+def test_norm_closed_forms():
+    """NormSync: a constant c on the selected cells has every norm |c|; a single
+    nonzero value v gives Linf |v|, L1 |v| w / sum w and L2 |v| sqrt(w / sum w)."""
+    go = _graded("sphere_empty24")
+    lv, co, ty = go.tiles()
+    T = go.cell_types()
+    sel = (ty == 1)[:, None] & (T == 2)
+    go.channel(2)[:] = np.where(sel, -2.5, 7.0).astype(np.float32)
+    for order in (-1, 1, 2, 3):
+        for vw in ((False, True) if order != -1 else (False,)):
+            assert abs(go.norm(order, 2, vw) - 2.5) <= 1e-12 * 2.5, (order, vw)
+    go.channel(2)[:] = 0
+    t, v = np.argwhere(sel)[len(np.argwhere(sel)) // 2]
+    go.channel(2)[t, v] = -4.0
+    h = 0.125 / 2 ** lv
+    w = (h ** 3)[:, None] * np.ones((1, 512))
+    W = w[sel].sum()
+    assert go.norm(-1, 2) == 4.0
+    assert abs(go.norm(1, 2, True) - 4.0 * w[t, v] / W) <= 1e-12
+    assert abs(go.norm(2, 2, True) - 4.0 * np.sqrt(w[t, v] / W)) <= 1e-12
+    assert abs(go.norm(2, 2, False) - 4.0 / np.sqrt(sel.sum())) <= 1e-12
