@@ -541,6 +541,33 @@ __global__ __launch_bounds__(256) void dot_v(const int32_t* __restrict__ list, i
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
+// two dot products sharing the first operand: partial = A.B, partial2 = A.C
+__global__ __launch_bounds__(256) void dot2_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
+                                              const float* __restrict__ B, const float* __restrict__ Cv,
+                                              double* __restrict__ partial, double* __restrict__ partial2,
+                                              const double* __restrict__ S)
+{
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    double acc = 0, acc2 = 0;
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const float4 a = reinterpret_cast<const float4*>(A)[q];
+        const float4 b = reinterpret_cast<const float4*>(B)[q];
+        const float4 c = reinterpret_cast<const float4*>(Cv)[q];
+        acc += double(a.x) * double(b.x) + double(a.y) * double(b.y) + double(a.z) * double(b.z) + double(a.w) * double(b.w);
+        acc2 += double(a.x) * double(c.x) + double(a.y) * double(c.y) + double(a.z) * double(c.z) + double(a.w) * double(c.w);
+    }
+    double b = block_sum(acc, red);
+    __syncthreads();
+    double b2 = block_sum(acc2, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = b;
+        partial2[blockIdx.x] = b2;
+    }
+}
+
 // first damped-Jacobi sweep from x = 0: with every x (own, neighbours, ghosts)
 // zero, A x = +0 and the sweep reduces to x = 0 + w (b / D), bit-identical to
 // the general sweep; it reads b, D and the cell type only
@@ -965,6 +992,13 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
         if (!isfinite(S[S_BETA])) S[S_DONE] = 3;
         break;
     case FIN_MEAN: S[S_MEAN] = s / s2; break;
+    case FIN_BETA_FLEX:
+        // flexible CG (Polak-Ribiere): beta = z.(r - r_old) / gamma_old = -alpha z.Ap / gamma_old
+        S[S_GAMMA_OLD] = S[S_GAMMA];
+        S[S_GAMMA] = s;
+        S[S_BETA] = -S[S_ALPHA] * s2 / S[S_GAMMA_OLD];
+        if (!isfinite(S[S_BETA])) S[S_DONE] = 3;
+        break;
     }
 }
 
@@ -1410,6 +1444,15 @@ int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const floa
     if (!nonempty(l)) return 0;
     const int g = vector_grid(d, l.n);
     k::dot_v<<<g, 256, 0, st>>>(l.d, l.n, A, B, partial, S);
+    return g;
+}
+
+int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const float* B, const float* C, double* partial,
+                double* partial2, const double* S, hipStream_t st)
+{
+    if (!nonempty(l)) return 0;
+    const int g = vector_grid(d, l.n);
+    k::dot2_v<<<g, 256, 0, st>>>(l.d, l.n, A, B, C, partial, partial2, S);
     return g;
 }
 

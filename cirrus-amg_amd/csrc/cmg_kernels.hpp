@@ -27,7 +27,7 @@ inline TopoView view(const DevGrid& d)
     return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.h0};
 }
 
-enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5 };
+enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5, FIN_BETA_FLEX = 6 };
 
 // launchers that write per-workgroup partials return how many they wrote
 int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st);
@@ -50,6 +50,9 @@ void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float val
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
 int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const float* B, double* partial, const double* S,
                hipStream_t st);
+// partial = sum A.B, partial2 = sum A.C per workgroup (one pass over A)
+int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const float* B, const float* C, double* partial,
+                double* partial2, const double* S, hipStream_t st);
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
                      hipStream_t st);
 void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);

@@ -30,6 +30,7 @@ struct cmg_solver {
     double* partial = nullptr;
     double* partial2 = nullptr;
     float* tmp = nullptr;          // Jacobi ping-pong buffer
+    float* scratch = nullptr;      // cycle scratch of flexible CG (keeps Ap)
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -699,18 +700,18 @@ int enqueue_apply(cmg_solver* s, float* X, float* Y, double* partial, const doub
 // the preconditioner z ~ A^-1 r of the options' cycle kind.  FAS kinds: x, rhs
 // = r (its NONLEAF tiles become coarse right-hand sides), x0 = FAS scratch.
 // Non-FAS kinds: x, f = r (kept), x0 = working right-hand side.
-void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0, const double* S)
+void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float* RHSp, float* X0p, const double* S)
 {
     DevGrid& d = s->g->dev;
     if (s->family != CMG_SOLVER_AMG) {
-        geo_vcycle(s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), S);
+        geo_vcycle(s, o, Xp, RHSp, X0p, S);
         return;
     }
     const Coef C = coef(d, s->c0);
     switch (o->cycle) {
     case CMG_CYCLE_VCYCLE:
     case CMG_CYCLE_MU: {
-        const CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, x0), dev_channel(d, rhs), C, S};
+        const CycleCtx c{s, o, Xp, X0p, RHSp, C, S};
         if (o->cycle == CMG_CYCLE_VCYCLE) {
             vcycle(c);
         } else {
@@ -720,17 +721,39 @@ void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x
         break;
     }
     case CMG_CYCLE_FAS_F: {
-        const CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), C, S};
+        const CycleCtx c{s, o, Xp, RHSp, X0p, C, S};
         fas_fcycle(c);
         break;
     }
     default: {
         // FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
-        const CycleCtx c{s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), C, S};
+        const CycleCtx c{s, o, Xp, RHSp, X0p, C, S};
         launch_fill_all(d, c.X, 0.0f, S, d.stream);
         cycle_step(c, d.max_level, true);
     }
     }
+}
+
+void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    enqueue_cycle_ptr(s, o, dev_channel(d, x), dev_channel(d, rhs), dev_channel(d, x0), S);
+}
+
+// the preconditioner of one PCG step, z ~ A^-1 r; flexible CG keeps Ap, so
+// the cycle's scratch is then a solver-private buffer
+void enqueue_precondition(cmg_solver* s, const cmg_solve_opts* o, const double* S)
+{
+    DevGrid& d = s->g->dev;
+    if (!o->flexible) {
+        enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, S);
+        return;
+    }
+    if (!s->scratch) {
+        s->scratch = dmalloc<float>(size_t(d.N) * kCells);
+        HIPCHK(hipMemset(s->scratch, 0, size_t(d.N) * kCells * sizeof(float)));
+    }
+    enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), s->scratch, S);
 }
 
 void enqueue_recenter(cmg_solver* s, float* R)
@@ -753,16 +776,22 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     const int n2 = launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
-    enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, s->S);
-    const int n3 = launch_dot(d, d.leafs, R, Z, s->partial, s->S, st);
-    launch_finish(s->partial, nullptr, n3, s->S, FIN_BETA, st);
+    enqueue_precondition(s, o, s->S);
+    if (o->flexible) {
+        // flexible CG: beta = z.(r - r_old) / gamma_old (Polak-Ribiere; AMGSolver.cu:1520 uses Fletcher-Reeves)
+        const int n3 = launch_dot2(d, d.leafs, Z, R, AP, s->partial, s->partial2, s->S, st);
+        launch_finish(s->partial, s->partial2, n3, s->S, FIN_BETA_FLEX, st);
+    } else {
+        const int n3 = launch_dot(d, d.leafs, R, Z, s->partial, s->S, st);
+        launch_finish(s->partial, nullptr, n3, s->S, FIN_BETA, st);
+    }
     launch_update_p(d, P, Z, s->S, st);
 }
 
 std::string graph_key(const cmg_solve_opts* o)
 {
     std::ostringstream k;
-    k << o->cycle << ',' << cycle_mu(o) << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
+    k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
@@ -863,7 +892,7 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     HIPCHK(hipMemcpyAsync(s->S, init, sizeof(init), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     if (o->is_pure_neumann) enqueue_recenter(s, R);
-    enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, s->S);
+    enqueue_precondition(s, o, s->S);
     launch_copy_list(d, d.leafs, Z, P, s->S, st);
     const int ng = launch_dot(d, d.leafs, Z, R, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, ng, s->S, FIN_GAMMA0, st);
@@ -1047,7 +1076,7 @@ void solver_destroy(cmg_solver* s)
 {
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->x64, (void*)s->b64,
+    for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->x64, (void*)s->b64,
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
     if (s->hS) (void)hipHostFree(s->hS);

@@ -166,6 +166,7 @@ typedef struct cmg_solve_opts {
     int warm_start;           /* 1: start from channel x (FP32) instead of 0 */
     int use_graph;            /* 1: capture one PCG iteration in a hipGraph */
     int mu;                   /* coarse-visit repetitions (mu_cycle_repeat_times); 0: 1 for FAS_V, else 2 */
+    int flexible;             /* 1: flexible CG (Polak-Ribiere beta) for a varying preconditioner */
 } cmg_solve_opts;
 
 typedef struct cmg_solve_stats {

@@ -724,8 +724,11 @@ og_solve_out og_fas_mu_solve(og_grid* g, const og_cycle* cyc, int c0, int max_it
     return out;
 }
 
-/* AMGSolver::solve (AMGSolver.cu:1399-1549), sync_stride = 1 */
-og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann)
+/* AMGSolver::solve (AMGSolver.cu:1399-1549), sync_stride = 1.  flexible: flexible
+ * CG, beta = z.(r - r_old) / gamma_old = -alpha z.Ap / gamma_old (Polak-Ribiere;
+ * the reference uses Fletcher-Reeves, AMGSolver.cu:1520); the cycle's scratch
+ * is then channel 7 so that Ap survives it */
+og_solve_out og_pcg_ex(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann, int flexible)
 {
     enum { X = 0, R = 1, P = 2, AP = 3, Z = 4 };
     og_solve_out out = { 0, 0.0 };
@@ -735,8 +738,9 @@ og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, doub
     if (rhs2 == 0) return out;
     double thr2 = tol * tol * rhs2;
     if (thr2 < 2.2250738585072014e-308) thr2 = 2.2250738585072014e-308;
+    const int SCR = flexible ? 7 : AP;
     if (pure_neumann) og_recenter(g, R);
-    og_precondition(g, cyc, Z, R, AP, c0);
+    og_precondition(g, cyc, Z, R, SCR, c0);
     for (int t = 0; t < g->N; t++)
         if (g->t_type[t] & OG_LEAF) memcpy(&g->ch[P][(size_t)t * 512], &g->ch[Z][(size_t)t * 512], 512 * sizeof(float));
     double gamma = og_dot(g, Z, R);
@@ -759,10 +763,16 @@ og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, doub
         rr = og_dot(g, R, R);
         if (rr < thr2) { out.iters = i + 1; out.rel_res = sqrt(rr / rhs2); return out; }
         if (pure_neumann) og_recenter(g, R);
-        og_precondition(g, cyc, Z, R, AP, c0);
+        og_precondition(g, cyc, Z, R, SCR, c0);
         double gamma_old = gamma;
-        gamma = og_dot(g, R, Z);
-        double beta = gamma / gamma_old;
+        double beta;
+        if (flexible) {
+            gamma = og_dot(g, Z, R);
+            beta = -alpha * og_dot(g, Z, AP) / gamma_old;
+        } else {
+            gamma = og_dot(g, R, Z);
+            beta = gamma / gamma_old;
+        }
         float b = (float)beta;
         for (int t = 0; t < g->N; t++) {
             if (!(g->t_type[t] & OG_LEAF)) continue;
@@ -777,4 +787,9 @@ og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, doub
     out.iters = i;
     out.rel_res = sqrt(rr / rhs2);
     return out;
+}
+
+og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann)
+{
+    return og_pcg_ex(g, cyc, c0, max_iters, tol, pure_neumann, 0);
 }

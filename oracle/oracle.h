@@ -161,6 +161,8 @@ void og_fas_cycle_step(og_grid* g, const og_cycle* cyc, int level, int x, int rh
 typedef struct og_solve_out { int iters; double rel_res; } og_solve_out;
 /* AMGSolver::solve (AMGSolver.cu:1399-1549); channels x=0, r=1, p=2, Ap=3, z=4 */
 og_solve_out og_pcg(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann);
+/* og_pcg with flexible CG (flexible = 1): beta = -alpha z.Ap / gamma_old */
+og_solve_out og_pcg_ex(og_grid* g, const og_cycle* cyc, int c0, int max_iters, double tol, int pure_neumann, int flexible);
 /* AMGSolver::FASMuCycleSolve (AMGSolver.cu:1311-1356): x = 0 (channel 0), b in
  * channel 1, residual and FAS x0 in channel 2; returns the reference's
  * (loop index at exit, |r| / |b|) */

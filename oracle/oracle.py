@@ -79,6 +79,7 @@ def lib():
             "og_dot": (d, [vp, i, i]), "og_recenter": (None, [vp, i]),
             "og_fas_cycle": (None, [vp, vp, i, i, i, i]),
             "og_pcg": (SolveOut, [vp, vp, i, i, d, i]),
+            "og_pcg_ex": (SolveOut, [vp, vp, i, i, d, i, i]),
             "og_fas_mu_solve": (SolveOut, [vp, vp, i, i, d]),
             "og_norm": (d, [vp, i, i, i, i]),
             "og_geo_neg_lap_tiles": (None, [vp, i, i, i, i, i, i]), "og_geo_full_neg_lap": (None, [vp, i, i, i, i]),
@@ -225,8 +226,8 @@ class Grid:
     def norm(self, order, ch, volume_weighted=False, cell_types=OG_INTERIOR_T):
         return lib().og_norm(self.h, order, ch, int(volume_weighted), cell_types)
 
-    def pcg(self, cycle, c0=11, max_iters=100, tol=1e-6, pure_neumann=False):
-        out = lib().og_pcg(self.h, C.byref(cycle), c0, max_iters, tol, int(pure_neumann))
+    def pcg(self, cycle, c0=11, max_iters=100, tol=1e-6, pure_neumann=False, flexible=False):
+        out = lib().og_pcg_ex(self.h, C.byref(cycle), c0, max_iters, tol, int(pure_neumann), int(flexible))
         return out.iters, out.rel_res
 
     def fas_cycle(self, cycle, x, rhs, x0, c0=11):

@@ -302,3 +302,20 @@ def test_mg_solve_matches_oracle(case):
     assert abs(rel - orel) <= 1e-6 * orel
     leaf = go.tiles()[2] == 1
     assert np.array_equal(H.bits(go.channel(0)[leaf]), H.bits(gp.channel(0)[leaf]))
+
+
+@pytest.mark.parametrize("case", ["sphere_empty24", "c3_small"])
+def test_flexible_cg_matches_oracle(case):
+    go, gp = H.build_pair(case)
+    _rhs(go, gp, 17)
+    A, _, x64 = _fp64_solution(go)
+    go2, _ = H.build_pair(case, with_device=False)
+    go2.channel(1)[:] = go.channel(1)
+    cyc = O.make_cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_direct=1)
+    oit, orel = go2.pcg(cyc, H.C0, 100, 1e-6, flexible=True)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1, flexible=1)
+    it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
+    assert rel < 1e-6 and abs(it - oit) <= 1, (it, oit)
+    x = gp.channel(0).ravel()[A.cell]
+    assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)

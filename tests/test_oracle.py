@@ -474,3 +474,20 @@ def test_norm_closed_forms():
     assert abs(go.norm(1, 2, True) - 4.0 * w[t, v] / W) <= 1e-12
     assert abs(go.norm(2, 2, True) - 4.0 * np.sqrt(w[t, v] / W)) <= 1e-12
     assert abs(go.norm(2, 2, False) - 4.0 / np.sqrt(sel.sum())) <= 1e-12
+
+
+def test_flexible_cg_equals_pcg_with_a_fixed_preconditioner():
+    """With a fixed (linear) preconditioner flexible CG's Polak-Ribiere beta equals
+    Fletcher-Reeves in exact arithmetic: same iteration count (+-1), same answer."""
+    out = []
+    for flexible in (False, True):
+        g = O.Grid(O.make_target(O.TARGET_SPHERE_EMPTY, 2, 4))
+        g.set_leaf_types_preset()
+        g.coarsen_types_and_coeffs(H.C0, H.R_MATRIX)
+        g.rhs_uniform(1, 4, True)
+        cyc = O.make_cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_direct=1)
+        it, rel = g.pcg(cyc, H.C0, 100, 1e-8, flexible=flexible)
+        out.append((it, rel, g.channel(0).astype(np.float64).copy()))
+    (i0, r0, x0), (i1, r1, x1) = out
+    assert r0 < 1e-8 and r1 < 1e-8 and abs(i0 - i1) <= 1
+    assert np.linalg.norm(x1 - x0) <= 1e-6 * np.linalg.norm(x0)
