@@ -77,7 +77,7 @@ EXPORTS = [
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
     "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize", "cmg_set_kernel_forms",
     "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
-    "cmg_solver_apply", "cmg_op_norm",
+    "cmg_solver_apply", "cmg_op_norm", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
 ]
 
 _lib = None
@@ -124,6 +124,8 @@ def lib():
             "cmg_gmg_damped_jacobi_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_solver_apply": (st, [vp, i, i]),
             "cmg_op_norm": (st, [vp, i, i, i, i, C.POINTER(d)]),
+            "cmg_divergence": (st, [vp, i, i, i]), "cmg_add_gradient": (st, [vp, i, i, i, i]),
+            "cmg_mix_velocity": (st, [vp, i, i, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
             "cmg_rhs_uniform": (st, [vp, i, C.c_uint64, i]), "cmg_rhs_divergence": (st, [vp, i, i, C.c_uint64]),
             "cmg_bench_kernel": (st, [vp, i, i, C.POINTER(d), C.POINTER(i)]),
@@ -316,6 +318,15 @@ class HADeviceGrid:
 
     def fas_prolong(self, level, x, x0, P):
         _check(lib().cmg_op_fas_prolong(self.h, level, x, x0, P), "fas_prolong")
+
+    def divergence(self, u0, channel, coeff_channel=-1):
+        _check(lib().cmg_divergence(self.h, u0, channel, coeff_channel), "cmg_divergence")
+
+    def add_gradient(self, x, coeff_channel, u0, weighted=False):
+        _check(lib().cmg_add_gradient(self.h, x, coeff_channel, u0, int(weighted)), "cmg_add_gradient")
+
+    def mix_velocity(self, coeff_channel, u_fluid, u_solid, u_mix):
+        _check(lib().cmg_mix_velocity(self.h, coeff_channel, u_fluid, u_solid, u_mix), "cmg_mix_velocity")
 
     def norm(self, order, channel, volume_weighted=False, cell_types=2):
         """NormSync (src/PoissonGrid.cu:322-375) over LEAF tiles."""

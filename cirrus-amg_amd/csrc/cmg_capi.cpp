@@ -435,6 +435,18 @@ cmg_status cmg_op_norm(cmg_grid* g, int order, int channel, int volume_weighted,
 {
     return op_norm(g, order, channel, volume_weighted, cell_types, out);
 }
+cmg_status cmg_divergence(cmg_grid* g, int u0, int channel, int coeff_channel)
+{
+    return op_divergence(g, u0, channel, coeff_channel);
+}
+cmg_status cmg_add_gradient(cmg_grid* g, int x, int coeff_channel, int u0, int weighted)
+{
+    return op_add_gradient(g, x, coeff_channel, u0, weighted);
+}
+cmg_status cmg_mix_velocity(cmg_grid* g, int coeff_channel, int u_fluid, int u_solid, int u_mix)
+{
+    return op_mix_velocity(g, coeff_channel, u_fluid, u_solid, u_mix);
+}
 cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0) { return solver_cycle(s, opts, x, rhs, x0); }
 
 cmg_status cmg_mg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)

@@ -88,6 +88,15 @@ void launch_sub_from(const DevGrid& d, const LaunchList& l, const float* B, floa
 int launch_power_sum(const DevGrid& d, const float* X, int order, bool volume_weighted, int cell_types, double* vsum,
                      double* wsum, hipStream_t st);
 
+// projection wrappers (cmg_project.hip)
+void launch_faces_to_parents(const DevGrid& d, float* U0, float* U1, float* U2, hipStream_t st);
+void launch_divergence(const DevGrid& d, const float* U0, const float* U1, const float* U2, const Coef* C, float* B,
+                       hipStream_t st);
+void launch_add_gradient(const DevGrid& d, const float* X, Coef C, float* U0, float* U1, float* U2, bool weighted,
+                         hipStream_t st);
+void launch_mix_velocity(const DevGrid& d, Coef C, const float* const F[3], const float* const S[3], float* const M[3],
+                         hipStream_t st);
+
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
 bool set_kernel_forms(int mask, int threads);
 void get_kernel_forms(int* mask, int* threads);

@@ -346,6 +346,65 @@ cmg_status op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P)
     return CMG_OK;
 }
 
+// ---- projection wrappers --------------------------------------------------------
+
+static bool valid_vec(int c) { return c >= 0 && c + 2 < kChannels; }
+
+// AMGVolumeWeightedDivergence[WithoutCoeff]OnLeafs (src/AMGSolver.cu:750-835):
+// ghost faces take their parents' values, NONLEAF faces a quarter of their four
+// child faces (both stay in u), then b on LEAF tiles; c0 < 0: without coefficients
+cmg_status op_divergence(cmg_grid* g, int u0, int ch, int c0)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_vec(u0) || !valid_channel(ch) || (c0 >= 0 && c0 + 3 >= kChannels) || (ch >= u0 && ch <= u0 + 2))
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        float* U[3] = {dev_channel(d, u0), dev_channel(d, u0 + 1), dev_channel(d, u0 + 2)};
+        for (int a = 0; a < 3; ++a) launch_propagate_ghosts(d, U[a], d.stream);
+        launch_faces_to_parents(d, U[0], U[1], U[2], d.stream);
+        Coef C{};
+        if (c0 >= 0) C = coef(d, c0);
+        launch_divergence(d, U[0], U[1], U[2], c0 >= 0 ? &C : nullptr, dev_channel(d, ch), d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+// AMGAddGradientToFace / AMGAddFaceWeightedGradientToFace (src/AMGSolver.cu:634-748)
+cmg_status op_add_gradient(cmg_grid* g, int x, int c0, int u0, int weighted)
+{
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(x) || c0 < 0 || c0 + 3 >= kChannels || !valid_vec(u0) || (x >= u0 && x <= u0 + 2))
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        float* X = dev_channel(d, x);
+        launch_propagate_ghosts(d, X, d.stream);
+        launch_accumulate_leafs(d, X, d.stream);
+        launch_add_gradient(d, X, coef(d, c0), dev_channel(d, u0), dev_channel(d, u0 + 1), dev_channel(d, u0 + 2),
+                            weighted != 0, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+// FluidEuler::mixFluidAndSolidVelocityOnFaces (cirrus_cutcell/FluidEuler.cu:531-552)
+cmg_status op_mix_velocity(cmg_grid* g, int c0, int uf0, int us0, int um0)
+{
+    if (auto st = check_grid(g)) return st;
+    if (c0 < 0 || c0 + 3 >= kChannels || !valid_vec(uf0) || !valid_vec(us0) || !valid_vec(um0)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        auto& d = g->dev;
+        const float* F[3] = {dev_channel(d, uf0), dev_channel(d, uf0 + 1), dev_channel(d, uf0 + 2)};
+        const float* Sv[3] = {dev_channel(d, us0), dev_channel(d, us0 + 1), dev_channel(d, us0 + 2)};
+        float* M[3] = {dev_channel(d, um0), dev_channel(d, um0 + 1), dev_channel(d, um0 + 2)};
+        launch_mix_velocity(d, coef(d, c0), F, Sv, M, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
 // NormSync (src/PoissonGrid.cu:322-375): order -1 = max |x|; order p > 0 =
 // (sum |x|^p w / sum w)^(1/p) over LEAF cells whose type is in cell_types, w = 1
 // or the cell volume; tile partials are summed on the host in tile order

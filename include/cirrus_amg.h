@@ -135,6 +135,22 @@ cmg_status cmg_assemble_cutcell(cmg_grid* g, int sdf_kind, const float* sdf, int
 cmg_status cmg_rhs_uniform(cmg_grid* g, int channel, uint64_t seed, int zero_mean);
 cmg_status cmg_rhs_divergence(cmg_grid* g, int channel, int u0, uint64_t seed);
 
+/* ------------------------------------------------------- projection -------- */
+/* Face velocities: channel u + a holds the velocity through each cell's negative
+ * a-face.  Divergence on LEAF tiles into `channel` after ghost faces take their
+ * parents' values and NONLEAF faces a quarter of their four child faces (left in
+ * u): coeff_channel < 0 = AMGVolumeWeightedDivergenceWithoutCoeffOnLeafs
+ * (src/AMGSolver.cu:750-788), else AMGVolumeWeightedDivergenceOnLeafs (:790-835). */
+cmg_status cmg_divergence(cmg_grid* g, int u0, int channel, int coeff_channel);
+/* u += grad(x) on LEAF faces: weighted = 0 AMGAddGradientToFace (src/AMGSolver.cu:634-688,
+ * open faces only), 1 AMGAddFaceWeightedGradientToFace (:690-748, times the fluid ratio);
+ * x keeps its propagated ghost / accumulated parent values */
+cmg_status cmg_add_gradient(cmg_grid* g, int x, int coeff_channel, int u0, int weighted);
+/* FluidEuler::mixFluidAndSolidVelocityOnFaces (cirrus_cutcell/FluidEuler.cu:531-552) on LEAF
+ * tiles; the solid face-centre velocity is read from channels u_solid..+2 (the reference
+ * evaluates its scene function there) */
+cmg_status cmg_mix_velocity(cmg_grid* g, int coeff_channel, int u_fluid, int u_solid, int u_mix);
+
 /* ------------------------------------------------------- solver ------------ */
 
 /* preconditioner cycles of AMGSolver: FAS mu-cycle (FASMuCycle, src/AMGSolver.cu:1257-1292)

@@ -436,6 +436,63 @@ void og_set_zero_tiles(og_grid* g, int level, int tile_types, int x)
         if (g->t_type[t] & tile_types) memset(&g->ch[x][(size_t)t * 512], 0, 512 * sizeof(float));
 }
 
+/* AMGAddGradientToFace (AMGSolver.cu:634-688), weighted = 0: u += (p - p_down) / h
+ * on faces whose fluid ratio -c/h is positive; AMGAddFaceWeightedGradientToFace
+ * (:690-748), weighted = 1: u += (p - p_down) / h * (-c / h).  On LEAF tiles; x is
+ * read through the operator's loader after ghosts take their parents' values and
+ * NONLEAF cells the mean of their children (x keeps those values). */
+void og_add_gradient(og_grid* g, int x, int c0, int u0, int weighted)
+{
+    og_propagate_ghosts(g, x);
+    og_accumulate_leafs(g, x);
+    #pragma omp parallel
+    {
+        og_lds* s = (og_lds*)malloc(sizeof(og_lds));
+        #pragma omp for schedule(dynamic, 8)
+        for (int t = 0; t < g->N; t++) {
+            if (!(g->t_type[t] & OG_LEAF)) continue;
+            load_tile(g, t, x, x, c0, s);
+            float h = og_h(g, g->t_level[t]);
+            for (int v = 0; v < 512; v++) {
+                size_t idx = (size_t)t * 512 + v;
+                int l[3] = { v >> 6, (v >> 3) & 7, v & 7 };
+                for (int a = 0; a < 3; a++) {
+                    int d[3] = { l[0], l[1], l[2] };
+                    d[a] -= 1;
+                    float pu = XV(s, l[0], l[1], l[2]), pd = XV(s, d[0], d[1], d[2]);
+                    float ratio = -*offp(s, a, l[0], l[1], l[2]) / h;
+                    float* u = &g->ch[u0 + a][idx];
+                    if (weighted) *u = *u + (pu - pd) / h * ratio;
+                    else if (ratio > 0) *u = *u + (pu - pd) / h;
+                }
+            }
+        }
+        free(s);
+    }
+}
+
+/* FluidEuler::mixFluidAndSolidVelocityOnFaces (cirrus_cutcell/FluidEuler.cu:531-552)
+ * with the solid face velocity given in channels s0..s0+2: on LEAF tiles
+ * u_mix = r u_fluid (r > 0) + (1 - r) u_solid (1 - r > 0), r = -c/h */
+void og_mix_velocity(og_grid* g, int c0, int uf0, int us0, int um0)
+{
+    for (int t = 0; t < g->N; t++) {
+        if (!(g->t_type[t] & OG_LEAF)) continue;
+        float h = og_h(g, g->t_level[t]);
+        for (int v = 0; v < 512; v++) {
+            size_t idx = (size_t)t * 512 + v;
+            for (int a = 0; a < 3; a++) {
+                float fr = -g->ch[c0 + a][idx] / h;
+                float sr = 1 - fr;
+                float mix = 0;
+                if (fr > 0) mix += fr * g->ch[uf0 + a][idx];
+                if (sr > 0) mix += sr * g->ch[us0 + a][idx];
+                g->ch[um0 + a][idx] = mix;
+            }
+        }
+    }
+}
+
 double og_dot(og_grid* g, int a, int b)
 {
     double s = 0;

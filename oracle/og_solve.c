@@ -69,20 +69,11 @@ void og_rhs_uniform(og_grid* g, int ch, uint64_t seed, int zero_mean)
     }
 }
 
-/* AMGVolumeWeightedDivergenceWithoutCoeffOnLeafs (AMGSolver.cu:750-788):
- * ghost faces = parent faces, NONLEAF faces = 1/4 sum of the 4 child faces
- * (AccumulateFacesToParentsOneStep, PoissonGrid.cu:523-586), then
- * b = sum over faces of (-u0 h h on the - side, u1 h h on the + side) */
-void og_rhs_divergence(og_grid* g, int ch, int u0, uint64_t seed)
+/* AccumulateFacesToParentsOneStep(LEAF, 1/4, overwrite, all cell types)
+ * (PoissonGrid.cu:523-586): the parent face of each 2x2 block of LEAF child
+ * faces = 1/4 of their sum, summed (jj, kk) over the rotated face axes */
+static void faces_to_parents(og_grid* g, int u0)
 {
-    uint64_t s = seed;
-    for (int a = 0; a < 3; a++) memset(g->ch[u0 + a], 0, sizeof(float) * (size_t)g->N * 512);
-    for (int t = 0; t < g->N; t++) {
-        if (!(g->t_type[t] & OG_LEAF)) continue;
-        for (int v = 0; v < 512; v++)
-            for (int a = 0; a < 3; a++) g->ch[u0 + a][(size_t)t * 512 + v] = og_normal(&s);
-    }
-    for (int a = 0; a < 3; a++) og_propagate_ghosts(g, u0 + a);
     for (int t = 0; t < g->N; t++) {
         if (!(g->t_type[t] & OG_LEAF)) continue;
         int lvl = g->t_level[t];
@@ -104,8 +95,19 @@ void og_rhs_divergence(og_grid* g, int ch, int u0, uint64_t seed)
             }
         }
     }
+}
+
+/* AMGVolumeWeightedDivergenceWithoutCoeffOnLeafs (AMGSolver.cu:750-788), c0 < 0:
+ *   b = sum over faces of (-u0 h h on the - side, u1 h h on the + side);
+ * AMGVolumeWeightedDivergenceOnLeafs (AMGSolver.cu:790-835), c0 >= 0:
+ *   b = sum of (u0 c0 h on the - side, -u1 c1 h on the + side), c from the face's cell;
+ * ghost faces = parent faces, NONLEAF faces = 1/4 sum of the 4 child faces;
+ * faces in the order x-, x+, y-, y+, z-, z+; b = 0 off INTERIOR cells */
+void og_divergence(og_grid* g, int u0, int ch, int c0)
+{
+    for (int a = 0; a < 3; a++) og_propagate_ghosts(g, u0 + a);
+    faces_to_parents(g, u0);
     float* B = g->ch[ch];
-    memset(B, 0, sizeof(float) * (size_t)g->N * 512);
     for (int t = 0; t < g->N; t++) {
         if (!(g->t_type[t] & OG_LEAF)) continue;
         int lvl = g->t_level[t];
@@ -122,11 +124,32 @@ void og_rhs_divergence(og_grid* g, int ch, int u0, uint64_t seed)
                     n[a] += sg;
                     int off, nt = og_voxel(g, lvl, n[0], n[1], n[2], &off);
                     float uu1 = nt < 0 ? 0.0f : g->ch[u0 + a][(size_t)nt * 512 + off];
-                    sum += sg == -1 ? (-uu0 * h) * h : (uu1 * h) * h;
+                    if (c0 < 0) {
+                        sum += sg == -1 ? (-uu0 * h) * h : (uu1 * h) * h;
+                    } else {
+                        float k0 = g->ch[c0 + a][idx];
+                        float k1 = nt < 0 ? 0.0f : g->ch[c0 + a][(size_t)nt * 512 + off];
+                        sum += sg == -1 ? (uu0 * k0) * h : (-uu1 * k1) * h;
+                    }
                 }
             B[idx] = (g->ctype[idx] & OG_INTERIOR) ? sum : 0.0f;
         }
     }
+}
+
+/* b = volume-weighted divergence of i.i.d. N(0,1) negative-face velocities
+ * written to channels u0..u0+2 on LEAF tiles (0 elsewhere) */
+void og_rhs_divergence(og_grid* g, int ch, int u0, uint64_t seed)
+{
+    uint64_t s = seed;
+    for (int a = 0; a < 3; a++) memset(g->ch[u0 + a], 0, sizeof(float) * (size_t)g->N * 512);
+    for (int t = 0; t < g->N; t++) {
+        if (!(g->t_type[t] & OG_LEAF)) continue;
+        for (int v = 0; v < 512; v++)
+            for (int a = 0; a < 3; a++) g->ch[u0 + a][(size_t)t * 512 + v] = og_normal(&s);
+    }
+    memset(g->ch[ch], 0, sizeof(float) * (size_t)g->N * 512);
+    og_divergence(g, u0, ch, -1);
 }
 
 /* ---- FP64 CSR of the composite operator ------------------------------------- */

@@ -122,6 +122,14 @@ void og_fas_prolong(og_grid* g, int level, int x, int x0, float P);             
 double og_dot(og_grid* g, int a, int b);                       /* PoissonGrid.cu:156-214, LEAF tiles */
 void og_recenter(og_grid* g, int ch);                          /* PoissonGrid.cu:889-921 */
 double og_norm(og_grid* g, int order, int ch, int volume_weighted, int cell_types); /* PoissonGrid.cu:322-375 */
+/* projection wrappers: divergence of face velocities u0..u0+2 into ch (c0 < 0:
+ * AMGVolumeWeightedDivergenceWithoutCoeffOnLeafs, AMGSolver.cu:750-788; c0 >= 0:
+ * AMGVolumeWeightedDivergenceOnLeafs, :790-835), gradient of x added to faces
+ * (AMGAddGradientToFace :634-688 / AMGAddFaceWeightedGradientToFace :690-748),
+ * fluid/solid face velocity mix (FluidEuler.cu:531-552, solid velocity from channels) */
+void og_divergence(og_grid* g, int u0, int ch, int c0);
+void og_add_gradient(og_grid* g, int x, int c0, int u0, int weighted);
+void og_mix_velocity(og_grid* g, int c0, int uf0, int us0, int um0);
 
 /* multigrid cycle options */
 typedef struct og_cycle {

@@ -82,6 +82,8 @@ def lib():
             "og_pcg_ex": (SolveOut, [vp, vp, i, i, d, i, i]),
             "og_fas_mu_solve": (SolveOut, [vp, vp, i, i, d]),
             "og_norm": (d, [vp, i, i, i, i]),
+            "og_divergence": (None, [vp, i, i, i]), "og_add_gradient": (None, [vp, i, i, i, i]),
+            "og_mix_velocity": (None, [vp, i, i, i, i]),
             "og_geo_neg_lap_tiles": (None, [vp, i, i, i, i, i, i]), "og_geo_full_neg_lap": (None, [vp, i, i, i, i]),
             "og_geo_gs_color": (None, [vp, i, i, i, i, i]), "og_geo_residual_restrict": (None, [vp, i, i, i, i, f]),
             "og_geo_prolong": (None, [vp, i, i, i, f]), "og_geo_vcycle": (None, [vp, i, i, i, i, i, i, f, f]),
@@ -222,6 +224,15 @@ class Grid:
 
     def dot(self, a, b):
         return lib().og_dot(self.h, a, b)
+
+    def divergence(self, u0, ch, c0=-1):
+        lib().og_divergence(self.h, u0, ch, c0)
+
+    def add_gradient(self, x, c0, u0, weighted=False):
+        lib().og_add_gradient(self.h, x, c0, u0, int(weighted))
+
+    def mix_velocity(self, c0, uf0, us0, um0):
+        lib().og_mix_velocity(self.h, c0, uf0, us0, um0)
 
     def norm(self, order, ch, volume_weighted=False, cell_types=OG_INTERIOR_T):
         return lib().og_norm(self.h, order, ch, int(volume_weighted), cell_types)

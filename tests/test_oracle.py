@@ -491,3 +491,72 @@ def test_flexible_cg_equals_pcg_with_a_fixed_preconditioner():
     (i0, r0, x0), (i1, r1, x1) = out
     assert r0 < 1e-8 and r1 < 1e-8 and abs(i0 - i1) <= 1
     assert np.linalg.norm(x1 - x0) <= 1e-6 * np.linalg.norm(x0)
+
+
+# ---- projection wrappers ----------------------------------------------------------
+
+def _inner_cells(go):
+    """LEAF INTERIOR cells none of whose faces lies on the domain boundary"""
+    lv, co, ty = go.tiles()
+    T = go.cell_types()
+    nbr = go.neighbors()
+    v = np.arange(512)
+    loc = np.stack([v >> 6, (v >> 3) & 7, v & 7], 1)
+    inner = np.zeros((go.N, 512), bool)
+    for t in np.where(ty == 1)[0]:
+        ok = T[t] == 2
+        for a in range(3):
+            ok = ok & ~((loc[:, a] == 7) & (nbr[t, a + 3] < 0)) & ~((loc[:, a] == 0) & (nbr[t, a] < 0))
+        inner[t] = ok
+    return inner
+
+
+@pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c2_small"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_projection_removes_divergence(case, weighted):
+    """div(u + grad p) = div u - A p: after solving A p = div u the divergence
+    vanishes to rounding on every cell away from the box faces (whose outer
+    faces carry no velocity), on graded and cut-cell grids alike."""
+    go = _graded(case)
+    go.rhs_divergence(1, 5, 7)
+    b = go.channel(1).copy()
+    A = go.assemble_csr(H.C0)
+    x, it, rr = A.bicgstab(b.ravel()[A.cell].astype(np.float64), 1e-12, 20000)
+    go.channel(0)[:] = 0
+    go.channel(0).ravel()[A.cell] = x.astype(np.float32)
+    go.add_gradient(0, H.C0, 5, weighted)
+    go.divergence(5, 2, -1)
+    inner = _inner_cells(go)
+    before, after = np.abs(b[inner]).max(), np.abs(go.channel(2)[inner]).max()
+    assert after <= 1e-5 * before, (case, before, after)
+
+
+def test_divergence_with_coefficients_of_a_gradient_is_minus_the_operator():
+    """On the uniform grid div_c(grad_w x) = -A x on cells away from the box."""
+    go = _graded("uniform3")
+    x = np.random.default_rng(3).uniform(-1, 1, (go.N, 512)).astype(np.float32)
+    go.channel(0)[:] = x
+    go.full_neg_lap(0, H.C0, 3)
+    ax = go.channel(3).astype(np.float64).copy()
+    for a in range(3):
+        go.channel(5 + a)[:] = 0
+    go.channel(0)[:] = x
+    go.add_gradient(0, H.C0, 5, True)
+    go.divergence(5, 2, H.C0)
+    inner = _inner_cells(go)
+    d = go.channel(2).astype(np.float64)
+    # sum (u0 c0 h - u1 c1 h) with u = grad x, c = -h  ->  -h (6 x - sum x_nb) ... = -A x
+    assert np.abs(d[inner] + ax[inner]).max() <= 1e-5 * np.abs(ax[inner]).max()
+
+
+def test_mix_velocity_keeps_a_common_velocity():
+    """r u + (1 - r) u = u for every fluid ratio r (the two branches of the mix)."""
+    go = _graded("c2_small")
+    lv, co, ty = go.tiles()
+    for a in range(3):
+        go.channel(5 + a)[:] = 1.5
+        go.channel(8 + a)[:] = 1.5
+    go.mix_velocity(H.C0, 5, 8, 2)
+    for a in range(3):
+        m = go.channel(2 + a)[ty == 1]
+        assert np.abs(m - 1.5).max() <= 1e-6
