@@ -135,7 +135,7 @@ KERNELS = {0: ("apply (y = A p on LEAF tiles + p.y partials)", 25.0),
            4: ("FAS prolong + correct, finest level", 8.5),
            5: ("p = z + b p", 12.0),
            6: ("r.z partials", 8.0),
-           7: ("first Jacobi sweep from x = 0, finest level", 21.0),
+           7: ("first Jacobi sweep from x = 0, finest level", 13.0),
            8: ("plain copy", 8.0)}
 
 

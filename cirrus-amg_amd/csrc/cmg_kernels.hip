@@ -1251,7 +1251,9 @@ static int persistent_mask()
     int m = g_mask.load(std::memory_order_relaxed);
     if (m >= 0) return m;
     const char* e = getenv("CMG_PERSISTENT");
-    m = e ? (atoi(e) & 7) : int(PERSIST_APPLY | PERSIST_JACOBI | PERSIST_RESTRICT);
+    // one tile per workgroup everywhere: faster than the persistent forms on MI355X at
+    // every thread count (the persistent forms hold two tiles' loads in registers)
+    m = e ? (atoi(e) & 7) : 0;
     g_mask.store(m, std::memory_order_relaxed);
     return m;
 }

@@ -285,7 +285,7 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * restriction; a set bit selects a persistent pipelined kernel (each workgroup
  * walks several tiles, prefetching the next tile's loads), a clear bit one tile
  * per workgroup.  stencil_threads: 128, 256 or 512 threads per tile.  Results
- * are bit-identical across forms.  No reference counterpart (the reference
+ * are bit-identical across forms.  Default: mask 0, 256 threads.  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
 cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);
