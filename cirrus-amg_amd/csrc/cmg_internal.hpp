@@ -57,6 +57,7 @@ struct DevGrid {
     LaunchList leafghost;               // all LEAF and GHOST tiles
     int max_list = 0;
     int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU
+    int derive = 0;                     // set by a solve whose operator passed the diagonal check
     // reduction scratch for the op-level entry points
     double* partial = nullptr;
     double* scal = nullptr;

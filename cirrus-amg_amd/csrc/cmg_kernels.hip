@@ -35,7 +35,28 @@ template <typename T>
 struct Lds {
     T x[1000];
     float off[3][576];   // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
+    uint8_t empty[6];    // face f = 2 axis + (side > 0) has no neighbour tile
 };
+
+// diagonal of a LEAF INTERIOR cell from its face coefficients, in the order the
+// assembly sums them (CoarsenTypesAndAMGCoeffs, src/AMGSolver.cu:129-160):
+// d = 0 - c-(x) - c+(x) - c-(y) - c+(y) - c-(z) - c+(z), c+ = -h without neighbour
+template <typename T>
+__device__ __forceinline__ float derived_diag(const Lds<T>& s, int v, const float c[3], float h)
+{
+    const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
+    const float cp0 = i < 7 ? s.off[0][v + 64] : (s.empty[1] ? -h : s.off[0][512 + (j * 8 + k)]);
+    const float cp1 = j < 7 ? s.off[1][v + 8] : (s.empty[3] ? -h : s.off[1][512 + (i * 8 + k)]);
+    const float cp2 = k < 7 ? s.off[2][v + 1] : (s.empty[5] ? -h : s.off[2][512 + (i * 8 + j)]);
+    float d = 0.0f;
+    d = d - c[0];
+    d = d - cp0;
+    d = d - c[1];
+    d = d - cp1;
+    d = d - c[2];
+    d = d - cp2;
+    return d;
+}
 
 
 
@@ -65,6 +86,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     constexpr int HPT = (384 + NT - 1) / NT;
     const int tid = threadIdx.x;
     const size_t base = size_t(t) * 512;
+    const bool dv = tp.derive && (tp.ttype[t] & LEAF);
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         const int v = tid + c * NT;
@@ -73,7 +95,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         o.x[c] = o.ct[c] == INTERIOR ? xr : T(0);
 #pragma unroll
         for (int a = 0; a < 3; ++a) o.c[c][a] = C[a][base + v];
-        o.d[c] = C[3][base + v];
+        o.d[c] = dv ? 0.0f : C[3][base + v];
     }
     T hraw[HPT];
     float hcp[HPT];
@@ -93,6 +115,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         else if (axis == 1) { nl[0] = p; nl[1] = sgn < 0 ? 7 : 0; nl[2] = q; }
         else { nl[0] = p; nl[1] = q; nl[2] = sgn < 0 ? 7 : 0; }
         const int nt = tp.nbr[size_t(t) * 6 + (sgn < 0 ? axis : axis + 3)];
+        if (fi == 0) s.empty[f] = nt < 0;
         if (nt < 0) continue;
         const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
         const uint8_t nty = tp.ttype[nt];
@@ -173,6 +196,13 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (sgn > 0) s.off[axis][512 + fi] = hcp[m];
     }
     __syncthreads();
+    if (dv) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int v = tid + c * NT;
+            o.d[c] = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
+        }
+    }
 }
 
 // AMGLaplacianTileData::negativeLap (src/AMGSolver.cu:345-382) for cell v
@@ -676,6 +706,34 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __r
         double b = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
     }
+}
+
+// per tile: the number of cells whose stored diagonal is not bitwise the derived one
+template <int NT>
+__global__ __launch_bounds__(NT) void diag_check(TopoView tp, const int32_t* __restrict__ list, Coef C,
+                                                 double* __restrict__ partial)
+{
+    __shared__ Lds<float> s;
+    __shared__ double red[8];
+    const int t = list[blockIdx.x];
+    double bad = 0;
+    if (t >= 0) {
+        Own<float, NT> o;
+        tp.derive = 0;
+        const float h = tile_h(tp, t);
+        load_tile<true>(tp, t, h, C.c[0], C.c[0], C.c, s, o);
+#pragma unroll
+        for (int c = 0; c < 512 / NT; ++c) {
+            const int v = threadIdx.x + c * NT;
+            const float d = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
+            bad += __float_as_uint(d) != __float_as_uint(o.d[c]) ? 1.0 : 0.0;
+        }
+    } else {
+        __syncthreads();   // matches the barriers of load_tile
+        __syncthreads();
+    }
+    const double b = block_sum(bad, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
@@ -1230,7 +1288,8 @@ static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 // Kernel forms of the stencil families.  Threads per tile: 512 / 256 / 128
 // (1 / 2 / 4 cells per thread).  Persistence mask: bit 0 apply, bit 1 Jacobi,
 // bit 2 residual + restrict; a set bit selects the persistent pipelined kernel,
-// a clear bit the one-tile-per-workgroup kernel.  Defaults may be overridden by
+// a clear bit the one-tile-per-workgroup kernel.  Bit 3 keeps the stored LEAF
+// diagonals even where they could be derived from the face coefficients.  Defaults may be overridden by
 // the environment (CMG_STENCIL_THREADS, CMG_PERSISTENT) or set_kernel_forms.
 enum { PERSIST_APPLY = 1, PERSIST_JACOBI = 2, PERSIST_RESTRICT = 4 };
 static std::atomic<int> g_threads{0}, g_mask{-1};
@@ -1253,14 +1312,14 @@ static int persistent_mask()
     const char* e = getenv("CMG_PERSISTENT");
     // one tile per workgroup everywhere: faster than the persistent forms on MI355X at
     // every thread count (the persistent forms hold two tiles' loads in registers)
-    m = e ? (atoi(e) & 7) : 0;
+    m = e ? (atoi(e) & 15) : 0;
     g_mask.store(m, std::memory_order_relaxed);
     return m;
 }
 
 bool set_kernel_forms(int mask, int threads)
 {
-    if (mask < 0 || mask > 7 || (threads != 128 && threads != 256 && threads != 512)) return false;
+    if (mask < 0 || mask > 15 || (threads != 128 && threads != 256 && threads != 512)) return false;
     g_mask.store(mask, std::memory_order_relaxed);
     g_threads.store(threads, std::memory_order_relaxed);
     return true;
@@ -1574,5 +1633,14 @@ int launch_power_sum(const DevGrid& d, const float* X, int order, bool volume_we
     k::power_sum<<<d.leafs.n, 128, 0, st>>>(view(d), d.leafs.d, X, order, int(volume_weighted), cell_types, vsum, wsum);
     return d.leafs.n;
 }
+
+int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st)
+{
+    if (!nonempty(d.leafs)) return 0;
+    k::diag_check<256><<<d.leafs.n, 256, 0, st>>>(view(d), d.leafs.d, C, partial);
+    return d.leafs.n;
+}
+
+bool derive_allowed() { return !(persistent_mask() & 8); }
 
 }  // namespace cmg

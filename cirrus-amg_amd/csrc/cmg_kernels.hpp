@@ -15,6 +15,7 @@ struct TopoView {
     const int32_t* child;
     const uint8_t* ctype;
     float h0;
+    int derive;   // LEAF-tile diagonals are derived from the face coefficients (verified equal)
 };
 
 // the four operator channels c0..c0+3
@@ -24,7 +25,7 @@ struct Coef {
 
 inline TopoView view(const DevGrid& d)
 {
-    return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.h0};
+    return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.h0, d.derive};
 }
 
 enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5, FIN_BETA_FLEX = 6 };
@@ -97,9 +98,15 @@ void launch_add_gradient(const DevGrid& d, const float* X, Coef C, float* U0, fl
 void launch_mix_velocity(const DevGrid& d, Coef C, const float* const F[3], const float* const S[3], float* const M[3],
                          hipStream_t st);
 
+// number of LEAF cells whose stored diagonal differs (bitwise) from the one the
+// face coefficients give (derived = 0 - sum of the six faces, -h for a missing
+// + neighbour, 0 off INTERIOR); partials reduced by the caller
+int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st);
+
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
 bool set_kernel_forms(int mask, int threads);
 void get_kernel_forms(int* mask, int* threads);
+bool derive_allowed();   // forms bit 3 clear
 
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st);

@@ -847,14 +847,14 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     launch_update_p(d, P, Z, s->S, st);
 }
 
-std::string graph_key(const cmg_solve_opts* o)
+std::string graph_key(const cmg_solve_opts* o, int derive)
 {
     std::ostringstream k;
     k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
-    k << ',' << mask << ',' << threads;
+    k << ',' << mask << ',' << threads << ',' << derive;
     return k.str();
 }
 
@@ -962,7 +962,7 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     }
     // capture (or reuse) the iteration graph
     if (o->use_graph) {
-        const std::string key = graph_key(o);
+        const std::string key = graph_key(o, d.derive);
         if (!s->gexec || s->gkey != key) {
             if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;
             hipGraph_t graph;
@@ -1170,6 +1170,25 @@ cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, 
     return CMG_OK;
 }
 
+// the stencils may derive LEAF diagonals from the face coefficients (one load
+// fewer per cell) when every stored LEAF diagonal equals the derived one bitwise
+bool diagonal_derivable(cmg_solver* s)
+{
+    DevGrid& d = s->g->dev;
+    set_scalars(s, 0.0, 0);
+    const int n = launch_diag_check(d, coef(d, s->c0), s->partial, d.stream);
+    launch_finish(s->partial, nullptr, n, s->S, FIN_SUM, d.stream);
+    double h[S_COUNT];
+    read_scalars(s, h);
+    return h[S_TMP] == 0.0;
+}
+
+struct DeriveScope {
+    DevGrid& d;
+    DeriveScope(DevGrid& dd, bool on) : d(dd) { d.derive = on ? 1 : 0; }
+    ~DeriveScope() { d.derive = 0; }
+};
+
 cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
 {
     if (!s || !st) return CMG_ERR_BAD_ARG;
@@ -1185,6 +1204,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         if (amg) ensure_fp64(s);
         if (amg && o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
             build_direct(s, o->is_pure_neumann);
+        DeriveScope derive(d, amg && derive_allowed() && diagonal_derivable(s));
         // keep b in FP64 for the true residual / refinement
         if (amg) launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
         HIPCHK(hipStreamSynchronize(stream));

@@ -284,11 +284,14 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * process-wide.  persistent_mask: bit 0 operator, bit 1 Jacobi, bit 2 residual +
  * restriction; a set bit selects a persistent pipelined kernel (each workgroup
  * walks several tiles, prefetching the next tile's loads), a clear bit one tile
- * per workgroup.  stencil_threads: 128, 256 or 512 threads per tile.  Results
- * are bit-identical across forms.  Default: mask 0, 256 threads.  No reference counterpart (the reference
+ * per workgroup; bit 3 set keeps the stored diagonal of LEAF cells (by default a
+ * solve derives it from the face coefficients, one load fewer per cell, after
+ * checking that every stored LEAF diagonal equals the derived one bit for bit).
+ * stencil_threads: 128, 256 or 512 threads per tile.  Results are bit-identical
+ * across forms.  Default: mask 0, 256 threads.  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
-cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);
+cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..15 */
 cmg_status cmg_get_kernel_forms(int* persistent_mask, int* stencil_threads);
 
 /* hipDeviceSynchronize on the current device (timing brackets) */

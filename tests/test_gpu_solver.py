@@ -319,3 +319,25 @@ def test_flexible_cg_matches_oracle(case):
     assert rel < 1e-6 and abs(it - oit) <= 1, (it, oit)
     x = gp.channel(0).ravel()[A.cell]
     assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)
+
+
+@pytest.mark.parametrize("case", ["sphere_empty24", "c2_small", "c3_small"])
+def test_derived_diagonals_change_nothing(case):
+    """Solves that derive LEAF diagonals from the face coefficients give the bits
+    of solves that load the stored diagonals."""
+    go, gp = H.build_pair(case)
+    b = _rhs(go, gp, 23)
+    opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1)
+    saved = P.get_kernel_forms()
+    out = []
+    try:
+        solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+        for mask in (0, 8):
+            P.set_kernel_forms(mask, 256)
+            gp.set_channel(1, b)
+            it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
+            out.append((it, rel, gp.channel(0).copy()))
+    finally:
+        P.set_kernel_forms(*saved)
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
