@@ -192,6 +192,13 @@ og_solve_out og_geo_pcg(og_grid* g, int mode, int level_iters, int coarsest_iter
                         int max_iters, double tol, int pure_neumann);                       /* :472-607 / :428-601 */
 og_solve_out og_gmg_damped_jacobi(og_grid* g, int max_iters, double tol, float omega);     /* GMGSolver.cu:390-470 */
 
+/* ---- learned tile smoother (og_learn.c; no reference counterpart) ------------ */
+enum { OG_NET_W = 7776 };   /* 16*27 + 16*16*27 + 16*27 weights, no biases */
+void og_net_forward(const double* w, const double* x, double* y);          /* one 10^3 tile */
+double og_net_loss_grad(const double* w, int n, const float* x, const float* e, double* grad);
+void og_adam_step(double* w, double* m, double* v, const double* g, int n, int t, double lr, double b1, double b2,
+                  double eps);
+
 /* ---- FP64 assembled oracle of the composite LEAF operator ----------------- */
 typedef struct og_csr {
     int n;                 /* number of unknowns = LEAF INTERIOR cells            */

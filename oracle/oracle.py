@@ -82,6 +82,8 @@ def lib():
             "og_pcg_ex": (SolveOut, [vp, vp, i, i, d, i, i]),
             "og_fas_mu_solve": (SolveOut, [vp, vp, i, i, d]),
             "og_norm": (d, [vp, i, i, i, i]),
+            "og_net_forward": (None, [vp, vp, vp]), "og_net_loss_grad": (d, [vp, i, vp, vp, vp]),
+            "og_adam_step": (None, [vp, vp, vp, vp, i, i, d, d, d, d]),
             "og_divergence": (None, [vp, i, i, i]), "og_add_gradient": (None, [vp, i, i, i, i]),
             "og_mix_velocity": (None, [vp, i, i, i, i]),
             "og_geo_neg_lap_tiles": (None, [vp, i, i, i, i, i, i]), "og_geo_full_neg_lap": (None, [vp, i, i, i, i]),
@@ -315,3 +317,33 @@ class CSRMatrix:
             self.close()
         except Exception:
             pass
+
+
+NET_W = 7776
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def net_forward(w, x):
+    """FP64 forward pass of the learned tile smoother on one 10^3 tile."""
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(1000)
+    y = np.zeros(1000, np.float64)
+    lib().og_net_forward(_dptr(w), _dptr(x), _dptr(y))
+    return y
+
+
+def net_loss_grad(w, x, e, with_grad=True):
+    """mean loss over tiles x, e (n x 1000 float32) and its FP64 gradient"""
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float32)
+    e = np.ascontiguousarray(e, np.float32)
+    g = np.zeros(NET_W, np.float64) if with_grad else None
+    loss = lib().og_net_loss_grad(_dptr(w), x.shape[0], _dptr(x), _dptr(e), _dptr(g) if with_grad else None)
+    return loss, g
+
+
+def adam_step(w, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    lib().og_adam_step(_dptr(w), _dptr(m), _dptr(v), _dptr(np.ascontiguousarray(g, np.float64)), w.size, t, lr, b1, b2, eps)

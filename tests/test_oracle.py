@@ -560,3 +560,43 @@ def test_mix_velocity_keeps_a_common_velocity():
     for a in range(3):
         m = go.channel(2 + a)[ty == 1]
         assert np.abs(m - 1.5).max() <= 1e-6
+
+
+# ---- learned tile smoother oracle ---------------------------------------------------
+
+def test_net_gradient_matches_finite_differences():
+    rng = np.random.default_rng(11)
+    w = rng.normal(0, 0.2, O.NET_W)
+    x = rng.normal(0, 1, (2, 1000)).astype(np.float32)
+    e = rng.normal(0, 1, (2, 1000)).astype(np.float32)
+    _, g = O.net_loss_grad(w, x, e)
+    for i in (3, 431, 432 + 17, 432 + 6911, 7344, 7775):   # each layer, first and last weights
+        eps = 1e-6 * max(1.0, abs(w[i]))
+        wp, wm = w.copy(), w.copy()
+        wp[i] += eps
+        wm[i] -= eps
+        fd = (O.net_loss_grad(wp, x, e, False)[0] - O.net_loss_grad(wm, x, e, False)[0]) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_net_is_positively_homogeneous_and_local_loss_is_a_laplacian():
+    """No biases and ReLU: net(c x) = c net(x) for c > 0.  A perfect prediction
+    (y = e on the tile) has zero loss; y - e constant on the volume leaves only
+    the boundary layer of the interior Laplacian."""
+    rng = np.random.default_rng(12)
+    w = rng.normal(0, 0.3, O.NET_W)
+    x = rng.normal(0, 1, 1000)
+    y1, y2 = O.net_forward(w, x), O.net_forward(w, 2.5 * x)
+    assert np.allclose(y2, 2.5 * y1, rtol=1e-12, atol=1e-12)
+    e = y1.astype(np.float32)[None, :]
+    loss, _ = O.net_loss_grad(w, x[None, :].astype(np.float32), e, False)
+    assert loss < 1e-6 * np.sum(y1 ** 2)
+
+
+def test_adam_first_step_is_a_signed_learning_rate():
+    """At t = 1 the bias-corrected moments are g and g^2: w -= lr g / (|g| + eps)."""
+    w = np.array([1.0, -2.0, 0.5])
+    g = np.array([3.0, -0.25, 0.0])
+    m, v = np.zeros(3), np.zeros(3)
+    O.adam_step(w, m, v, g, 1, lr=1e-3)
+    assert np.allclose(w, [1.0 - 1e-3 * 3 / (3 + 1e-8), -2.0 + 1e-3 * 0.25 / (0.25 + 1e-8), 0.5], rtol=0, atol=1e-15)
