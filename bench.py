@@ -160,6 +160,7 @@ def main():
     ap.add_argument("--profile-kernel", type=int, default=None, choices=sorted(KERNELS),
                     help="only time one kernel (for rocprofv3 counter runs) and exit")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--learned", type=int, default=1, help="also time learned-FCG when trained weights exist")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs the single-GPU solve only: this build has no multi-GPU path "
@@ -266,6 +267,35 @@ def main():
         result["fp64_refinement"] = {"ms": t64, "outer_iters": int(st.outer_iters), "inner_iters_total": int(it64),
                                      "true_rel_res_fp64": float(st.true_rel_res), "tolerance": 1e-10}
         result["rel_l2_err_fp32_solve_vs_fp64"] = err
+    weights = os.path.join(ROOT, "cirrus-amg_amd", "data", "smoother_c4.f32")
+    if args.learned and os.path.exists(weights):
+        # learned-FCG: flexible CG whose V-cycle takes the learned tile smoother
+        # (trained on the moving-sphere scene) as the finest pre-smoothing step
+        solver.set_net(np.fromfile(weights, "<f4"))
+        ol = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                            max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, flexible=1, learned=1)
+        of = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                            max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, flexible=1)
+        legs = {}
+        for name, o in (("learned_fcg", ol), ("plain_fcg", of)):
+            ms, its = [], []
+            for rep in range(1 + args.steps):
+                g.copy_channel(5, 1)
+                P.device_synchronize()
+                t0 = time.perf_counter()
+                it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=o)
+                P.device_synchronize()
+                if rep:
+                    ms.append((time.perf_counter() - t0) * 1e3)
+                    its.append(it)
+            leg = {"ms": float(np.mean(ms)), "iterations": its, "rel_res": rel,
+                   "true_rel_res_fp64": float(solver.last_stats.true_rel_res)}
+            if x32 is not None and "fp64_refinement" in result:
+                xl = g.channel(0).astype(np.float64)
+                leg["rel_l2_err_vs_fp64"] = float(np.linalg.norm((xl - x64)[m]) / np.linalg.norm(x64[m]))
+            legs[name] = leg
+        legs["speedup_learned_vs_plain_mgpcg"] = ms_per_step / legs["learned_fcg"]["ms"]
+        result["learned"] = legs
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(cfg, 200)
     print(json.dumps(result))

@@ -56,7 +56,7 @@ class SolveOpts(C.Structure):
                 ("is_pure_neumann", C.c_int), ("cycle", C.c_int), ("smoother", C.c_int), ("omega", C.c_double),
                 ("coarsest_direct", C.c_int), ("precision", C.c_int), ("outer_tolerance", C.c_double),
                 ("inner_tolerance", C.c_double), ("warm_start", C.c_int), ("use_graph", C.c_int), ("mu", C.c_int),
-                ("flexible", C.c_int)]
+                ("flexible", C.c_int), ("learned", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -77,7 +77,9 @@ EXPORTS = [
     "cmg_status_string", "cmg_last_error", "cmg_grid_copy_channel", "cmg_bench_kernel",
     "cmg_rhs_uniform", "cmg_rhs_divergence", "cmg_device_synchronize", "cmg_set_kernel_forms",
     "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
-    "cmg_solver_apply", "cmg_op_norm", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
+    "cmg_solver_apply", "cmg_op_norm", "cmg_solver_set_net", "cmg_solver_net_infer", "cmg_trainer_create",
+    "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
+    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
 ]
 
 _lib = None
@@ -124,6 +126,13 @@ def lib():
             "cmg_gmg_damped_jacobi_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_solver_apply": (st, [vp, i, i]),
             "cmg_op_norm": (st, [vp, i, i, i, i, C.POINTER(d)]),
+            "cmg_solver_set_net": (st, [vp, vp]), "cmg_solver_net_infer": (st, [vp, i, i, i]),
+            "cmg_trainer_create": (st, [i, C.c_uint64, C.POINTER(vp)]), "cmg_trainer_destroy": (None, [vp]),
+            "cmg_trainer_collect": (st, [vp, vp, i, i, i, C.POINTER(i)]), "cmg_trainer_add_tiles": (st, [vp, vp, vp, i]),
+            "cmg_trainer_size": (i, [vp]), "cmg_trainer_get_weights": (st, [vp, vp]),
+            "cmg_trainer_set_weights": (st, [vp, vp]),
+            "cmg_trainer_loss_grad": (st, [vp, i, i, i, C.POINTER(d), vp]),
+            "cmg_trainer_step": (st, [vp, i, C.c_float, i, C.POINTER(d)]), "cmg_mfma_probe": (st, [vp, vp, vp]),
             "cmg_divergence": (st, [vp, i, i, i]), "cmg_add_gradient": (st, [vp, i, i, i, i]),
             "cmg_mix_velocity": (st, [vp, i, i, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
@@ -418,6 +427,15 @@ class AMGSolver:
         """y = A x with this solver's operator on LEAF tiles."""
         _check(lib().cmg_solver_apply(self.h, x, y), "cmg_solver_apply")
 
+    def set_net(self, weights):
+        """weights of the learned tile smoother (7776 floats)"""
+        w = np.ascontiguousarray(weights, np.float32)
+        assert w.size == NET_WEIGHTS
+        _check(lib().cmg_solver_set_net(self.h, _ptr(w)), "cmg_solver_set_net")
+
+    def net_infer(self, r, diag, y):
+        _check(lib().cmg_solver_net_infer(self.h, r, diag, y), "cmg_solver_net_infer")
+
     def bench_kernel(self, kernel, reps):
         ms, tiles = C.c_double(), C.c_int()
         _check(lib().cmg_bench_kernel(self.h, kernel, reps, C.byref(ms), C.byref(tiles)), "bench_kernel")
@@ -456,3 +474,67 @@ class GMGSolver(_GeoSolver):
 class CMGSolver(_GeoSolver):
     """CMGSolver(one_over_alpha, prolong_coeff) (src/CMGSolver.h:12-47)."""
     FAMILY = SOLVER_CMG
+
+
+NET_WEIGHTS, NET_VOLUME = 7776, 1000
+
+
+class Trainer:
+    """Training state of the learned tile smoother (device-resident tiles, Adam)."""
+
+    def __init__(self, capacity, seed=3):
+        self.h = C.c_void_p()
+        _check(lib().cmg_trainer_create(capacity, seed, C.byref(self.h)), "cmg_trainer_create")
+
+    def close(self):
+        if self.h:
+            lib().cmg_trainer_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size(self):
+        return lib().cmg_trainer_size(self.h)
+
+    def collect(self, grid, r, diag, e):
+        n = C.c_int()
+        _check(lib().cmg_trainer_collect(self.h, grid.h, r, diag, e, C.byref(n)), "cmg_trainer_collect")
+        return n.value
+
+    def add_tiles(self, x, e):
+        x = np.ascontiguousarray(x, np.float32)
+        e = np.ascontiguousarray(e, np.float32)
+        _check(lib().cmg_trainer_add_tiles(self.h, _ptr(x), _ptr(e), x.shape[0]), "cmg_trainer_add_tiles")
+
+    def weights(self):
+        w = np.zeros(NET_WEIGHTS, np.float32)
+        _check(lib().cmg_trainer_get_weights(self.h, _ptr(w)), "cmg_trainer_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(lib().cmg_trainer_set_weights(self.h, _ptr(w)), "cmg_trainer_set_weights")
+
+    def loss_grad(self, first, n, bf16=False):
+        loss = C.c_double()
+        g = np.zeros(NET_WEIGHTS, np.float32)
+        _check(lib().cmg_trainer_loss_grad(self.h, first, n, int(bf16), C.byref(loss), _ptr(g)), "cmg_trainer_loss_grad")
+        return loss.value, g
+
+    def step(self, batch, lr=1e-3, bf16=True):
+        loss = C.c_double()
+        _check(lib().cmg_trainer_step(self.h, batch, lr, int(bf16), C.byref(loss)), "cmg_trainer_step")
+        return loss.value
+
+
+def mfma_probe(A_bits, B_bits):
+    A = np.ascontiguousarray(A_bits, np.uint16)
+    B = np.ascontiguousarray(B_bits, np.uint16)
+    Cm = np.zeros((16, 16), np.float32)
+    _check(lib().cmg_mfma_probe(_ptr(A), _ptr(B), _ptr(Cm)), "cmg_mfma_probe")
+    return Cm

@@ -406,6 +406,31 @@ cmg_status cmg_gmg_damped_jacobi_solve(cmg_solver* s, const cmg_solve_opts* opts
 
 cmg_status cmg_solver_apply(cmg_solver* s, int x, int y) { return solver_apply(s, x, y); }
 
+cmg_status cmg_solver_set_net(cmg_solver* s, const float* weights) { return solver_set_net(s, weights); }
+cmg_status cmg_solver_net_infer(cmg_solver* s, int r, int diag, int y) { return solver_net_infer(s, r, diag, y); }
+cmg_status cmg_trainer_create(int capacity_tiles, uint64_t seed, cmg_trainer** out)
+{
+    return trainer_create(capacity_tiles, seed, out);
+}
+void cmg_trainer_destroy(cmg_trainer* t) { trainer_destroy(t); }
+cmg_status cmg_trainer_collect(cmg_trainer* t, cmg_grid* g, int r, int diag, int e, int* appended)
+{
+    return trainer_collect(t, g, r, diag, e, appended);
+}
+cmg_status cmg_trainer_add_tiles(cmg_trainer* t, const float* x, const float* e, int n) { return trainer_add_tiles(t, x, e, n); }
+int cmg_trainer_size(const cmg_trainer* t) { return trainer_size(t); }
+cmg_status cmg_trainer_get_weights(cmg_trainer* t, float* w) { return trainer_weights(t, w, nullptr); }
+cmg_status cmg_trainer_set_weights(cmg_trainer* t, const float* w) { return trainer_weights(t, nullptr, w); }
+cmg_status cmg_trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad)
+{
+    return trainer_loss_grad(t, first, n, bf16, loss, grad);
+}
+cmg_status cmg_trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss)
+{
+    return trainer_step(t, batch, lr, bf16, loss);
+}
+cmg_status cmg_mfma_probe(const uint16_t* A, const uint16_t* B, float* C) { return mfma_probe(A, B, C); }
+
 cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
 {
     return solver_solve(s, opts, stats);

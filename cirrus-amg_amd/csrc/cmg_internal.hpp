@@ -79,6 +79,7 @@ struct cmg_grid {
 };
 
 struct cmg_solver;
+struct cmg_trainer;
 
 namespace cmg {
 void set_error(const std::string& msg);
@@ -106,6 +107,19 @@ cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_sta
 cmg_status solver_create_geo(cmg_grid* g, int family, float one_over_alpha, float P, cmg_solver** out);
 cmg_status solver_damped_jacobi(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st);
 cmg_status solver_apply(cmg_solver* s, int x, int y);
+// learned tile smoother
+constexpr int kNetWeights = 7776, kNetVolume = 1000;
+cmg_status solver_set_net(cmg_solver* s, const float* w);
+cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y);
+cmg_status mfma_probe(const uint16_t* A, const uint16_t* B, float* C);
+cmg_status trainer_create(int capacity, uint64_t seed, cmg_trainer** out);
+void trainer_destroy(cmg_trainer* t);
+int trainer_size(const cmg_trainer* t);
+cmg_status trainer_collect(cmg_trainer* t, cmg_grid* g, int r_ch, int d_ch, int e_ch, int* appended);
+cmg_status trainer_add_tiles(cmg_trainer* t, const float* x, const float* e, int n);
+cmg_status trainer_weights(cmg_trainer* t, float* get, const float* set);
+cmg_status trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad);
+cmg_status trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss);
 cmg_status solver_get_x64(cmg_solver* s, double* host);
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 cmg_status op_copy(cmg_grid* g, int src, int dst);

@@ -736,6 +736,50 @@ __global__ __launch_bounds__(NT) void diag_check(TopoView tp, const int32_t* __r
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
+// Learned pre-smoothing step: the net's correction Y of a LEAF tile is kept when
+// |b - A Y| < |b| on the tile's INTERIOR cells (x = 0 before the step, the same-
+// level neighbours' Y in the halo, ghosts 0); otherwise the tile falls back to the
+// plain first damped-Jacobi sweep x = w b / D.  NONLEAF tiles of the level take
+// the Jacobi sweep.
+template <int NT>
+__global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Y,
+                                                     float* __restrict__ X, const float* __restrict__ Gz, Coef C,
+                                                     const float* __restrict__ B, float w, const double* __restrict__ S)
+{
+    __shared__ Lds<float> s;
+    __shared__ double red[8];
+    if (stopped(S)) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    Own<float, NT> o;
+    float b[512 / NT];
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
+    load_tile<false>(tp, t, tile_h(tp, t), Y, Gz, C.c, s, o);
+    double rr = 0, bb = 0;
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        if (!(o.ct[c] & INTERIOR)) continue;
+        const float r = b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        rr += double(r) * double(r);
+        bb += double(b[c]) * double(b[c]);
+    }
+    const double rs = block_sum(rr, red);
+    __syncthreads();
+    const double bs = block_sum(bb, red);
+    __shared__ int keep;
+    if (threadIdx.x == 0) keep = (tp.ttype[t] & LEAF) && rs < bs;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 512 / NT; ++c) {
+        const int v = threadIdx.x + c * NT;
+        const size_t idx = size_t(t) * 512 + v;
+        const float jac = 0.0f + w * ((o.ct[c] & INTERIOR) ? (b[c] - 0.0f) / o.d[c] : 0.0f);
+        X[idx] = keep ? ((o.ct[c] & INTERIOR) ? Y[idx] : 0.0f) : jac;
+    }
+}
+
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
 template <int NT>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
@@ -1642,5 +1686,14 @@ int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st)
 }
 
 bool derive_allowed() { return !(persistent_mask() & 8); }
+
+void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
+                           const double* S, hipStream_t st)
+{
+    const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return;
+    // ghosts of the level are 0 at the start of a cycle: X is zero-filled there
+    k::learned_accept<256><<<l.n, 256, 0, st>>>(view(d), l.d, Y, X, X, C, B, w, S);
+}
 
 }  // namespace cmg

@@ -103,6 +103,21 @@ void launch_mix_velocity(const DevGrid& d, Coef C, const float* const F[3], cons
 // + neighbour, 0 off INTERIOR); partials reduced by the caller
 int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st);
 
+// learned tile smoother (cmg_learn.hip)
+void launch_gather_tiles(const DevGrid& d, const int32_t* tiles, int n, const float* V, const float* D, int mode, float* out,
+                         hipStream_t st);
+int launch_net_train(const float* X, const float* E, const int32_t* idx, int n, const float* W, float* gpart, double* lpart,
+                     int max_wg, bool bf16, hipStream_t st);
+void launch_net_reduce(const float* gpart, const double* lpart, int nwg, int n, float* grad, double* loss, hipStream_t st);
+void launch_adam(float* W, float* M, float* V, const float* G, int t, float lr, float b1, float b2, float eps, hipStream_t st);
+void launch_net_infer(const DevGrid& d, const LaunchList& l, const float* R, const float* D, const float* W, float* Y,
+                      const double* S, hipStream_t st);
+void launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* C, hipStream_t st);
+// learned pre-smoothing acceptance on the finest level: keep the net's correction
+// (in Y) on a tile when it lowers the tile's residual, else the first Jacobi sweep
+void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
+                           const double* S, hipStream_t st);
+
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
 bool set_kernel_forms(int mask, int threads);
 void get_kernel_forms(int* mask, int* threads);

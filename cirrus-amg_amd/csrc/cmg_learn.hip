@@ -1,0 +1,510 @@
+// cmg_learn.hip -- the learned tile smoother on gfx950: data gathering from a
+// grid, FP32 training kernels (forward, loss, backward, weight gradients), the
+// Adam update, and BF16 MFMA inference.
+//
+// Network (no reference counterpart): a 3-layer 3D conv net on one halo'd 10^3
+// tile, 1 -> 16 -> 16 -> 1 channels, 3^3 kernels, zero "same" padding, ReLU
+// after layers 1 and 2, no biases (so net(c x) = c net(x) for c > 0).  Weights
+// are one flat array of 7776 floats: W1[16][27], W2[16 out][16 in][27],
+// W3[16 in][27]; tap (di+1)*9 + (dj+1)*3 + (dk+1).  Volume index (i*10+j)*10+k.
+// Input feature of a cell: r / D on INTERIOR cells of LEAF tiles of the tile's
+// level (0 elsewhere); the output on the tile's own 8^3 cells is a correction
+// of x.  Loss of a tile: sum over the interior 8^3 of (7-point Laplacian of
+// (y - e))^2, mean over the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "cmg_device.hpp"
+#include "cmg_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace cmg {
+namespace k {
+
+constexpr int NV = 1000, NC = 16, NW = 7776, OW2 = 432, OW3 = 7344;
+
+__device__ __forceinline__ int vid(int i, int j, int k) { return (i * 10 + j) * 10 + k; }
+
+// round to the nearest bfloat16 (ties to even) and back; NaN is kept quiet
+__device__ __forceinline__ float bf16_round(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return f;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+__device__ __forceinline__ uint16_t bf16_bits(float f) { return uint16_t(__float_as_uint(bf16_round(f)) >> 16); }
+
+// ---------------------------------------------------------------- gather ----
+// one halo'd 10^3 volume per tile of `list`: mode 0 the input feature r / D,
+// mode 1 the raw value; cells of the tile and of its same-level LEAF
+// neighbours (INTERIOR only), 0 elsewhere (corners of the volume included)
+__global__ __launch_bounds__(256) void gather_tiles(TopoView tp, const int32_t* __restrict__ tiles, int n,
+                                                    const float* __restrict__ V, const float* __restrict__ D, int mode,
+                                                    float* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const int t = tiles[b];
+    for (int p = threadIdx.x; p < NV; p += blockDim.x) {
+        const int i = p / 100 - 1, j = (p / 10) % 10 - 1, kk = p % 10 - 1;
+        const int out_axes = (i < 0 || i > 7) + (j < 0 || j > 7) + (kk < 0 || kk > 7);
+        float val = 0.0f;
+        if (out_axes <= 1) {
+            int tt = t, l[3] = {i, j, kk};
+            if (out_axes == 1) {
+                const int a = (i < 0 || i > 7) ? 0 : (j < 0 || j > 7) ? 1 : 2;
+                const bool neg = l[a] < 0;
+                tt = tp.nbr[size_t(t) * 6 + (neg ? a : a + 3)];
+                l[a] = neg ? 7 : 0;
+                if (tt >= 0 && !(tp.ttype[tt] & LEAF)) tt = -1;
+            }
+            if (tt >= 0) {
+                const size_t idx = size_t(tt) * 512 + cidx(l[0], l[1], l[2]);
+                if (tp.ctype[idx] & INTERIOR) {
+                    const float v = V[idx];
+                    if (mode == 0) {
+                        const float d = D[idx];
+                        val = d != 0 ? v / d : 0.0f;
+                    } else {
+                        val = v;
+                    }
+                }
+            }
+        }
+        out[size_t(b) * NV + p] = val;
+    }
+}
+
+// ---------------------------------------------------------------- training ----
+// One workgroup (1024 threads) per chunk of tiles; x, a1, a2 and the loss
+// buffers live in LDS.  BF16: weights and activations are rounded to bfloat16
+// in the forward pass (FP32 accumulation); the backward pass is FP32.
+struct TrainLds {
+    float x[NV];
+    float a1[NC * NV];
+    float a2[NC * NV];
+    float g[NV];    // y, then d = y - e, then dL/dy
+    float s[NV];    // interior Laplacian of d
+    double red[16];
+};
+
+template <bool BF16>
+__global__ __launch_bounds__(1024) void net_train(const float* __restrict__ X, const float* __restrict__ E,
+                                                  const int32_t* __restrict__ idx, int n, int per_wg,
+                                                  const float* __restrict__ W, float* __restrict__ gpart,
+                                                  double* __restrict__ lpart)
+{
+    extern __shared__ unsigned char smem[];
+    TrainLds& s = *reinterpret_cast<TrainLds*>(smem);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    // this thread's weight-gradient accumulators: weights tid, tid + nth, ...
+    constexpr int WPT = (NW + 1023) / 1024;
+    float gacc[WPT];
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) gacc[q] = 0.0f;
+    double lacc = 0;
+    const int b0 = blockIdx.x * per_wg, b1 = min(n, b0 + per_wg);
+    for (int bi = b0; bi < b1; ++bi) {
+        const size_t b = idx ? size_t(idx[bi]) : size_t(bi);
+        for (int p = tid; p < NV; p += nth) s.x[p] = X[b * NV + p];
+        __syncthreads();
+        // layer 1
+        for (int o = tid; o < NC * NV; o += nth) {
+            const int co = o / NV, p = o % NV, i = p / 100, j = (p / 10) % 10, kk = p % 10;
+            float acc = 0.0f;
+            for (int t = 0; t < 27; ++t) {
+                const int a = i + t / 9 - 1, bb = j + (t / 3) % 3 - 1, c = kk + t % 3 - 1;
+                if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
+                float w = W[co * 27 + t], xv = s.x[vid(a, bb, c)];
+                if (BF16) { w = bf16_round(w); xv = bf16_round(xv); }
+                acc = fmaf(w, xv, acc);
+            }
+            acc = acc > 0 ? acc : 0.0f;
+            s.a1[o] = BF16 ? bf16_round(acc) : acc;
+        }
+        __syncthreads();
+        // layer 2
+        for (int o = tid; o < NC * NV; o += nth) {
+            const int co = o / NV, p = o % NV, i = p / 100, j = (p / 10) % 10, kk = p % 10;
+            float acc = 0.0f;
+            for (int t = 0; t < 27; ++t) {
+                const int a = i + t / 9 - 1, bb = j + (t / 3) % 3 - 1, c = kk + t % 3 - 1;
+                if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
+                const int q = vid(a, bb, c);
+                for (int ci = 0; ci < NC; ++ci) {
+                    float w = W[OW2 + (co * NC + ci) * 27 + t];
+                    if (BF16) w = bf16_round(w);
+                    acc = fmaf(w, s.a1[ci * NV + q], acc);
+                }
+            }
+            acc = acc > 0 ? acc : 0.0f;
+            s.a2[o] = BF16 ? bf16_round(acc) : acc;
+        }
+        __syncthreads();
+        // layer 3 and d = y - e
+        for (int p = tid; p < NV; p += nth) {
+            const int i = p / 100, j = (p / 10) % 10, kk = p % 10;
+            float acc = 0.0f;
+            for (int t = 0; t < 27; ++t) {
+                const int a = i + t / 9 - 1, bb = j + (t / 3) % 3 - 1, c = kk + t % 3 - 1;
+                if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
+                const int q = vid(a, bb, c);
+                for (int ci = 0; ci < NC; ++ci) {
+                    float w = W[OW3 + ci * 27 + t];
+                    if (BF16) w = bf16_round(w);
+                    acc = fmaf(w, s.a2[ci * NV + q], acc);
+                }
+            }
+            s.g[p] = acc - E[b * NV + p];
+        }
+        __syncthreads();
+        // s = interior Laplacian of d, loss
+        double lt = 0;
+        for (int p = tid; p < NV; p += nth) {
+            const int i = p / 100, j = (p / 10) % 10, kk = p % 10;
+            float v = 0.0f;
+            if (i >= 1 && i <= 8 && j >= 1 && j <= 8 && kk >= 1 && kk <= 8) {
+                v = 6.0f * s.g[p] - s.g[p - 100] - s.g[p + 100] - s.g[p - 10] - s.g[p + 10] - s.g[p - 1] - s.g[p + 1];
+                lt += double(v) * double(v);
+            }
+            s.s[p] = v;
+        }
+        lacc += lt;
+        __syncthreads();
+        // dL/dy = 2 Lap^T s / n
+        const float sc = 2.0f / float(n);
+        for (int p = tid; p < NV; p += nth) {
+            const int i = p / 100, j = (p / 10) % 10, kk = p % 10;
+            float v = 6.0f * s.s[p];
+            if (i > 0) v -= s.s[p - 100];
+            if (i < 9) v -= s.s[p + 100];
+            if (j > 0) v -= s.s[p - 10];
+            if (j < 9) v -= s.s[p + 10];
+            if (kk > 0) v -= s.s[p - 1];
+            if (kk < 9) v -= s.s[p + 1];
+            s.g[p] = v * sc;
+        }
+        __syncthreads();
+        // dL/dW3 (reads a2 before it is overwritten)
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+            const int wi = tid + q * nth;
+            if (wi < OW3 || wi >= NW) continue;
+            const int ci = (wi - OW3) / 27, t = (wi - OW3) % 27;
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            float acc = 0.0f;
+            for (int p = 0; p < NV; ++p) {
+                const int i = p / 100 + di, j = (p / 10) % 10 + dj, kk = p % 10 + dk;
+                if (i < 0 || i > 9 || j < 0 || j > 9 || kk < 0 || kk > 9) continue;
+                acc = fmaf(s.g[p], s.a2[ci * NV + vid(i, j, kk)], acc);
+            }
+            gacc[q] += acc;
+        }
+        __syncthreads();
+        // dL/da2 masked by ReLU', in place of a2
+        for (int o = tid; o < NC * NV; o += nth) {
+            const int ci = o / NV, qv = o % NV, i = qv / 100, j = (qv / 10) % 10, kk = qv % 10;
+            float acc = 0.0f;
+            if (s.a2[o] > 0) {
+                for (int t = 0; t < 27; ++t) {
+                    const int a = i - (t / 9 - 1), bb = j - ((t / 3) % 3 - 1), c = kk - (t % 3 - 1);
+                    if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
+                    float w = W[OW3 + ci * 27 + t];
+                    if (BF16) w = bf16_round(w);
+                    acc = fmaf(w, s.g[vid(a, bb, c)], acc);
+                }
+            }
+            s.a2[o] = acc;
+        }
+        __syncthreads();
+        // dL/dW2
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+            const int wi = tid + q * nth;
+            if (wi < OW2 || wi >= OW3) continue;
+            const int co = (wi - OW2) / (NC * 27), ci = ((wi - OW2) / 27) % NC, t = (wi - OW2) % 27;
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            float acc = 0.0f;
+            for (int p = 0; p < NV; ++p) {
+                const int i = p / 100 + di, j = (p / 10) % 10 + dj, kk = p % 10 + dk;
+                if (i < 0 || i > 9 || j < 0 || j > 9 || kk < 0 || kk > 9) continue;
+                acc = fmaf(s.a2[co * NV + p], s.a1[ci * NV + vid(i, j, kk)], acc);
+            }
+            gacc[q] += acc;
+        }
+        __syncthreads();
+        // dL/da1 masked by ReLU', in place of a1
+        for (int o = tid; o < NC * NV; o += nth) {
+            const int ci = o / NV, qv = o % NV, i = qv / 100, j = (qv / 10) % 10, kk = qv % 10;
+            float acc = 0.0f;
+            if (s.a1[o] > 0) {
+                for (int t = 0; t < 27; ++t) {
+                    const int a = i - (t / 9 - 1), bb = j - ((t / 3) % 3 - 1), c = kk - (t % 3 - 1);
+                    if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
+                    const int p = vid(a, bb, c);
+                    for (int co = 0; co < NC; ++co) {
+                        float w = W[OW2 + (co * NC + ci) * 27 + t];
+                        if (BF16) w = bf16_round(w);
+                        acc = fmaf(w, s.a2[co * NV + p], acc);
+                    }
+                }
+            }
+            s.a1[o] = acc;
+        }
+        __syncthreads();
+        // dL/dW1
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+            const int wi = tid + q * nth;
+            if (wi >= OW2) continue;
+            const int co = wi / 27, t = wi % 27;
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            float acc = 0.0f;
+            for (int p = 0; p < NV; ++p) {
+                const int i = p / 100 + di, j = (p / 10) % 10 + dj, kk = p % 10 + dk;
+                if (i < 0 || i > 9 || j < 0 || j > 9 || kk < 0 || kk > 9) continue;
+                float xv = s.x[vid(i, j, kk)];
+                if (BF16) xv = bf16_round(xv);
+                acc = fmaf(s.a1[co * NV + p], xv, acc);
+            }
+            gacc[q] += acc;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+        const int wi = tid + q * nth;
+        if (wi < NW) gpart[size_t(blockIdx.x) * NW + wi] = gacc[q];
+    }
+    const double lb = block_sum(lacc, s.red);
+    if (tid == 0) lpart[blockIdx.x] = lb;
+}
+
+// grad[w] = sum over workgroups of the partials (fixed order, FP64); loss likewise
+__global__ __launch_bounds__(256) void net_reduce(const float* __restrict__ gpart, const double* __restrict__ lpart, int nwg,
+                                                  int n, float* __restrict__ grad, double* __restrict__ loss)
+{
+    const int wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi < NW) {
+        double acc = 0;
+        for (int g = 0; g < nwg; ++g) acc += double(gpart[size_t(g) * NW + wi]);
+        grad[wi] = float(acc);
+    }
+    if (wi == 0) {
+        double l = 0;
+        for (int g = 0; g < nwg; ++g) l += lpart[g];
+        *loss = l / double(n);
+    }
+}
+
+// Adam (Kingma and Ba, 2015), step t >= 1, FP32 state
+__global__ __launch_bounds__(256) void adam(float* __restrict__ W, float* __restrict__ M, float* __restrict__ Vs,
+                                            const float* __restrict__ G, int t, float lr, float b1, float b2, float eps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NW) return;
+    const float c1 = 1.0f - powf(b1, float(t)), c2 = 1.0f - powf(b2, float(t));
+    const float g = G[i];
+    const float m = b1 * M[i] + (1.0f - b1) * g;
+    const float v = b2 * Vs[i] + (1.0f - b2) * g * g;
+    M[i] = m;
+    Vs[i] = v;
+    W[i] = W[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+// -------------------------------------------------------------- inference ----
+// BF16 MFMA forward pass, one 256-thread workgroup (4 wave64s) per tile of a
+// list: y = net(x) on the tile's own 8^3 cells, x gathered from channel R / D
+// as for training; layer 2 as an implicit GEMM [1000 positions x 432] x
+// [432 x 16] on v_mfma_f32_16x16x16bf16 (K = 16 input channels per tap, 27 taps),
+// layers 1 and 3 on the vector units.  out: the correction on INTERIOR cells.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct InferLds {
+    float x[NV];
+    uint16_t a1[NV * NC];   // [position][channel], bf16
+    uint16_t a2[NV * NC];
+    uint16_t w2[27 * NC * NC];   // [tap][ci][co], bf16
+    float w1[NC * 27];
+    float w3[NC * 27];
+};
+
+__global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __restrict__ tiles, int n,
+                                                 const float* __restrict__ R, const float* __restrict__ D,
+                                                 const float* __restrict__ W, float* __restrict__ Y,
+                                                 const double* __restrict__ S)
+{
+    __shared__ InferLds s;
+    if (stopped(S)) return;
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const int t = tiles[b];
+    if (t < 0) return;
+    const int tid = threadIdx.x;
+    // weights
+    for (int i = tid; i < NC * 27; i += 256) {
+        s.w1[i] = bf16_round(W[i]);
+        s.w3[i] = bf16_round(W[OW3 + i]);
+    }
+    for (int i = tid; i < 27 * NC * NC; i += 256) {
+        const int tap = i / (NC * NC), ci = (i / NC) % NC, co = i % NC;
+        s.w2[i] = bf16_bits(W[OW2 + (co * NC + ci) * 27 + tap]);
+    }
+    // input volume (as gather_tiles, mode 0)
+    for (int p = tid; p < NV; p += 256) {
+        const int i = p / 100 - 1, j = (p / 10) % 10 - 1, kk = p % 10 - 1;
+        const int out_axes = (i < 0 || i > 7) + (j < 0 || j > 7) + (kk < 0 || kk > 7);
+        float val = 0.0f;
+        if (out_axes <= 1) {
+            int tt = t, l[3] = {i, j, kk};
+            if (out_axes == 1) {
+                const int a = (i < 0 || i > 7) ? 0 : (j < 0 || j > 7) ? 1 : 2;
+                const bool neg = l[a] < 0;
+                tt = tp.nbr[size_t(t) * 6 + (neg ? a : a + 3)];
+                l[a] = neg ? 7 : 0;
+                if (tt >= 0 && !(tp.ttype[tt] & LEAF)) tt = -1;
+            }
+            if (tt >= 0) {
+                const size_t idx = size_t(tt) * 512 + cidx(l[0], l[1], l[2]);
+                if (tp.ctype[idx] & INTERIOR) {
+                    const float d = D[idx];
+                    val = d != 0 ? R[idx] / d : 0.0f;
+                }
+            }
+        }
+        s.x[p] = bf16_round(val);
+    }
+    __syncthreads();
+    // layer 1 (vector units): a1[p][co]
+    for (int o = tid; o < NC * NV; o += 256) {
+        const int p = o / NC, co = o % NC, i = p / 100, j = (p / 10) % 10, kk = p % 10;
+        float acc = 0.0f;
+        for (int tap = 0; tap < 27; ++tap) {
+            const int a = i + tap / 9 - 1, bb = j + (tap / 3) % 3 - 1, c = kk + tap % 3 - 1;
+            if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
+            acc = fmaf(s.w1[co * 27 + tap], s.x[vid(a, bb, c)], acc);
+        }
+        s.a1[o] = bf16_bits(acc > 0 ? acc : 0.0f);
+    }
+    __syncthreads();
+    // layer 2 on MFMA: each wave64 takes 16-position blocks m = wave, wave + 4, ...
+    const int wave = tid >> 6, lane = tid & 63;
+    const int row = lane & 15, kq = (lane >> 4) * 4;   // A: row, k = kq..kq+3 ; B: k = kq..kq+3, col = row
+    for (int mb = wave; mb < (NV + 15) / 16; mb += 4) {
+        const int p = mb * 16 + row;   // this lane's A row (output position)
+        const int pi = p / 100, pj = (p / 10) % 10, pk = p % 10;
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int tap = 0; tap < 27; ++tap) {
+            const int a = pi + tap / 9 - 1, bb = pj + (tap / 3) % 3 - 1, c = pk + tap % 3 - 1;
+            const bool ok = p < NV && a >= 0 && a <= 9 && bb >= 0 && bb <= 9 && c >= 0 && c <= 9;
+            bf16x4 av = {0, 0, 0, 0};
+            if (ok) {
+                const uint16_t* src = &s.a1[vid(a, bb, c) * NC + kq];
+                av[0] = short(src[0]); av[1] = short(src[1]); av[2] = short(src[2]); av[3] = short(src[3]);
+            }
+            bf16x4 bv;
+            const uint16_t* wsrc = &s.w2[(tap * NC + kq) * NC + row];
+            bv[0] = short(wsrc[0]); bv[1] = short(wsrc[NC]); bv[2] = short(wsrc[2 * NC]); bv[3] = short(wsrc[3 * NC]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        }
+        // D[m = kq + e][n = row]: position mb*16 + kq + e, channel row
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int pp = mb * 16 + kq + e;
+            if (pp < NV) {
+                const float v = acc[e];
+                s.a2[pp * NC + row] = bf16_bits(v > 0 ? v : 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+    // layer 3 (vector units) on the tile's own cells; one cell pair per thread
+    for (int v = tid; v < 512; v += 256) {
+        const int i = (v >> 6) + 1, j = ((v >> 3) & 7) + 1, kk = (v & 7) + 1;
+        float acc = 0.0f;
+        for (int tap = 0; tap < 27; ++tap) {
+            const int q = vid(i + tap / 9 - 1, j + (tap / 3) % 3 - 1, kk + tap % 3 - 1);
+            for (int ci = 0; ci < NC; ++ci)
+                acc = fmaf(s.w3[ci * 27 + tap], __uint_as_float(uint32_t(s.a2[q * NC + ci]) << 16), acc);
+        }
+        const size_t idx = size_t(t) * 512 + v;
+        Y[idx] = (tp.ctype[idx] & INTERIOR) ? acc : 0.0f;
+    }
+}
+
+// in-bounds check of the MFMA operand layout: one wave64, C = A B for 16 x 16 bf16 A, B
+__global__ __launch_bounds__(64) void mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+                                                 float* __restrict__ C)
+{
+    const int lane = threadIdx.x, row = lane & 15, kq = (lane >> 4) * 4;
+    bf16x4 av, bv;
+    for (int e = 0; e < 4; ++e) {
+        av[e] = short(A[row * 16 + kq + e]);
+        bv[e] = short(B[(kq + e) * 16 + row]);
+    }
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) C[(kq + e) * 16 + row] = acc[e];
+}
+
+}  // namespace k
+
+// ---------------------------------------------------------------- launchers ---
+
+void launch_gather_tiles(const DevGrid& d, const int32_t* tiles, int n, const float* V, const float* D, int mode, float* out,
+                         hipStream_t st)
+{
+    if (n <= 0) return;
+    k::gather_tiles<<<n, 256, 0, st>>>(view(d), tiles, n, V, D, mode, out);
+}
+
+int launch_net_train(const float* X, const float* E, const int32_t* idx, int n, const float* W, float* gpart, double* lpart,
+                     int max_wg, bool bf16, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    const int nwg = std::max(1, std::min(max_wg, n));
+    const int per = (n + nwg - 1) / nwg;
+    const int used = (n + per - 1) / per;
+    const size_t lds = sizeof(k::TrainLds);
+    static bool attr = [] {
+        (void)hipFuncSetAttribute((const void*)k::net_train<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(sizeof(k::TrainLds)));
+        (void)hipFuncSetAttribute((const void*)k::net_train<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(sizeof(k::TrainLds)));
+        return true;
+    }();
+    (void)attr;
+    if (bf16) k::net_train<true><<<used, 1024, lds, st>>>(X, E, idx, n, per, W, gpart, lpart);
+    else k::net_train<false><<<used, 1024, lds, st>>>(X, E, idx, n, per, W, gpart, lpart);
+    return used;
+}
+
+void launch_net_reduce(const float* gpart, const double* lpart, int nwg, int n, float* grad, double* loss, hipStream_t st)
+{
+    k::net_reduce<<<(k::NW + 255) / 256, 256, 0, st>>>(gpart, lpart, nwg, n, grad, loss);
+}
+
+void launch_adam(float* W, float* M, float* V, const float* G, int t, float lr, float b1, float b2, float eps, hipStream_t st)
+{
+    k::adam<<<(k::NW + 255) / 256, 256, 0, st>>>(W, M, V, G, t, lr, b1, b2, eps);
+}
+
+void launch_net_infer(const DevGrid& d, const LaunchList& l, const float* R, const float* D, const float* W, float* Y,
+                      const double* S, hipStream_t st)
+{
+    if (l.n <= 0) return;
+    k::net_infer<<<l.n, 256, 0, st>>>(view(d), l.d, l.n, R, D, W, Y, S);
+}
+
+void launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* C, hipStream_t st)
+{
+    k::mfma_probe<<<1, 64, 0, st>>>(A, B, C);
+}
+
+}  // namespace cmg

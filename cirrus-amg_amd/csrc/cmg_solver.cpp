@@ -31,6 +31,7 @@ struct cmg_solver {
     double* partial2 = nullptr;
     float* tmp = nullptr;          // Jacobi ping-pong buffer
     float* scratch = nullptr;      // cycle scratch of flexible CG (keeps Ap)
+    float* net_w = nullptr;        // learned tile smoother weights (7776 floats)
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -637,7 +638,14 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero)
         coarsest(c);
         return;
     }
-    smooth(c, level, c.o->level_iters, 0, x_zero);
+    if (x_zero && level == d.max_level && c.o->learned && c.s->net_w) {
+        // the learned tile smoother replaces the pre-smoothing of the finest level;
+        // a tile whose residual it does not lower takes the first Jacobi sweep
+        launch_net_infer(d, d.subtree[level], c.B, c.C.c[3], c.s->net_w, c.s->tmp, c.S, st);
+        launch_learned_accept(d, level, c.s->tmp, c.X, c.C, c.B, float(c.o->omega), c.S, st);
+    } else {
+        smooth(c, level, c.o->level_iters, 0, x_zero);
+    }
     launch_residual_restrict(d, level, c.X, c.C, c.B, c.s->Rr, c.S, st);
     launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
     const int mu = cycle_mu(c.o);
@@ -850,7 +858,7 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
 std::string graph_key(const cmg_solve_opts* o, int derive)
 {
     std::ostringstream k;
-    k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
+    k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->learned << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
@@ -1135,7 +1143,8 @@ void solver_destroy(cmg_solver* s)
 {
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->x64, (void*)s->b64,
+    for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->net_w,
+                    (void*)s->x64, (void*)s->b64,
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
     if (s->hS) (void)hipHostFree(s->hS);
@@ -1151,6 +1160,7 @@ static cmg_status check_opts(const cmg_solve_opts* o)
     if (o->cycle < CMG_CYCLE_FAS_W || o->cycle > CMG_CYCLE_MU || o->mu < 0) return CMG_ERR_BAD_ARG;
     if (o->smoother != CMG_SMOOTH_RBGS && o->smoother != CMG_SMOOTH_JACOBI) return CMG_ERR_BAD_ARG;
     if (o->precision != CMG_PREC_FP32 && o->precision != CMG_PREC_FP64_REFINE) return CMG_ERR_BAD_ARG;
+    if (o->learned && (o->cycle == CMG_CYCLE_VCYCLE || o->cycle == CMG_CYCLE_MU)) return CMG_ERR_BAD_ARG;
     return CMG_OK;
 }
 
@@ -1194,6 +1204,10 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
     if (!s || !st) return CMG_ERR_BAD_ARG;
     if (auto e = check_opts(o)) return e;
     if (s->family != CMG_SOLVER_AMG && o->precision != CMG_PREC_FP32) return CMG_ERR_BAD_ARG;
+    if (o->learned && (!s->net_w || s->family != CMG_SOLVER_AMG)) {
+        set_error("learned smoothing needs the net's weights (cmg_solver_set_net) on an AMG solver");
+        return CMG_ERR_STATE;
+    }
     *st = cmg_solve_stats{};
     int done = 1;
     CMG_TRY({
@@ -1331,6 +1345,227 @@ cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_sta
         if (!std::isfinite(res)) return CMG_ERR_NONFINITE;
     });
     return converged ? CMG_OK : CMG_ERR_NOT_CONVERGED;
+}
+
+// ---- learned tile smoother -----------------------------------------------------
+
+cmg_status solver_set_net(cmg_solver* s, const float* w)
+{
+    if (!s || !w) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        if (!s->net_w) s->net_w = dmalloc<float>(kNetWeights);
+        HIPCHK(hipMemcpy(s->net_w, w, kNetWeights * sizeof(float), hipMemcpyHostToDevice));
+        if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;   // the graph holds the old pointer
+    });
+    return CMG_OK;
+}
+
+// y = net(r / D) on the finest-level LEAF tiles (BF16 MFMA), channels r, D, y
+cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y)
+{
+    if (!s || !s->net_w || !valid_channel(r) || !valid_channel(dch) || !valid_channel(y) || y == r || y == dch)
+        return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        DevGrid& d = s->g->dev;
+        launch_net_infer(d, d.subtree[d.max_level], dev_channel(d, r), dev_channel(d, dch), s->net_w, dev_channel(d, y),
+                         nullptr, d.stream);
+        HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status mfma_probe(const uint16_t* A, const uint16_t* B, float* C)
+{
+    if (!A || !B || !C) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        uint16_t *dA = dmalloc<uint16_t>(256), *dB = dmalloc<uint16_t>(256);
+        float* dC = dmalloc<float>(256);
+        HIPCHK(hipMemcpy(dA, A, 512, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dB, B, 512, hipMemcpyHostToDevice));
+        launch_mfma_probe(dA, dB, dC, nullptr);
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(C, dC, 1024, hipMemcpyDeviceToHost));
+        (void)hipFree(dA);
+        (void)hipFree(dB);
+        (void)hipFree(dC);
+    });
+    return CMG_OK;
+}
+
+}  // namespace cmg
+
+// training state: tiles (input volume, target volume) resident in HBM, the FP32
+// master weights, Adam moments, a seeded per-epoch permutation of the tiles
+struct cmg_trainer {
+    int capacity = 0, size = 0;
+    float *X = nullptr, *E = nullptr;
+    float *W = nullptr, *M = nullptr, *V = nullptr, *G = nullptr;
+    float* gpart = nullptr;
+    double *lpart = nullptr, *loss = nullptr;
+    int32_t* idx = nullptr;
+    int max_wg = 0;
+    int step = 0;
+    uint64_t rng = 0;
+    std::vector<int32_t> perm;
+    size_t perm_pos = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace cmg {
+
+namespace {
+
+uint64_t splitmix(uint64_t& x)
+{
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+cmg_status trainer_create(int capacity, uint64_t seed, cmg_trainer** out)
+{
+    if (capacity <= 0 || !out) return CMG_ERR_BAD_ARG;
+    auto* t = new cmg_trainer;
+    try {
+        t->capacity = capacity;
+        t->X = dmalloc<float>(size_t(capacity) * kNetVolume);
+        t->E = dmalloc<float>(size_t(capacity) * kNetVolume);
+        t->W = dmalloc<float>(kNetWeights);
+        t->M = dmalloc<float>(kNetWeights);
+        t->V = dmalloc<float>(kNetWeights);
+        t->G = dmalloc<float>(kNetWeights);
+        int dev = 0, cus = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        t->max_wg = std::max(1, cus);
+        t->gpart = dmalloc<float>(size_t(t->max_wg) * kNetWeights);
+        t->lpart = dmalloc<double>(size_t(t->max_wg));
+        t->loss = dmalloc<double>(1);
+        t->idx = dmalloc<int32_t>(size_t(capacity));
+        HIPCHK(hipMemset(t->M, 0, kNetWeights * sizeof(float)));
+        HIPCHK(hipMemset(t->V, 0, kNetWeights * sizeof(float)));
+        HIPCHK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+        // He-style initial weights from the seed: N(0, 2 / fan_in) by Box-Muller
+        std::vector<float> w(kNetWeights);
+        t->rng = seed;
+        for (int i = 0; i < kNetWeights; ++i) {
+            const int fan_in = i < 432 ? 27 : 432;
+            const double u1 = (double(splitmix(t->rng) >> 11) + 1.0) * 0x1.0p-53, u2 = double(splitmix(t->rng) >> 11) * 0x1.0p-53;
+            const double nrm = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+            w[i] = float(nrm * std::sqrt(2.0 / fan_in) * (i >= 7344 ? 0.1 : 1.0));
+        }
+        HIPCHK(hipMemcpy(t->W, w.data(), kNetWeights * sizeof(float), hipMemcpyHostToDevice));
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        trainer_destroy(t);
+        return CMG_ERR_HIP;
+    }
+    *out = t;
+    return CMG_OK;
+}
+
+int trainer_size(const cmg_trainer* t) { return t ? t->size : 0; }
+
+void trainer_destroy(cmg_trainer* t)
+{
+    if (!t) return;
+    for (void* p : {(void*)t->X, (void*)t->E, (void*)t->W, (void*)t->M, (void*)t->V, (void*)t->G, (void*)t->gpart,
+                    (void*)t->lpart, (void*)t->loss, (void*)t->idx})
+        if (p) (void)hipFree(p);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+}
+
+// the finest-level LEAF tiles of a grid: input r / D (channels r_ch, d_ch), target e_ch
+cmg_status trainer_collect(cmg_trainer* t, cmg_grid* g, int r_ch, int d_ch, int e_ch, int* appended)
+{
+    if (!t) return CMG_ERR_BAD_ARG;
+    if (auto st = check_grid(g)) return st;
+    if (!valid_channel(r_ch) || !valid_channel(d_ch) || !valid_channel(e_ch)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        DevGrid& d = g->dev;
+        const Topology& topo = *g->topo;
+        std::vector<int32_t> tiles;
+        for (int i = topo.level_start[d.max_level]; i < topo.level_start[d.max_level + 1]; ++i)
+            if (topo.t_type[i] & LEAF) tiles.push_back(i);
+        const int n = std::min(int(tiles.size()), t->capacity - t->size);
+        if (n > 0) {
+            int32_t* dt = dmalloc<int32_t>(size_t(n));
+            HIPCHK(hipMemcpy(dt, tiles.data(), size_t(n) * sizeof(int32_t), hipMemcpyHostToDevice));
+            launch_gather_tiles(d, dt, n, dev_channel(d, r_ch), dev_channel(d, d_ch), 0, t->X + size_t(t->size) * kNetVolume,
+                                d.stream);
+            launch_gather_tiles(d, dt, n, dev_channel(d, e_ch), nullptr, 1, t->E + size_t(t->size) * kNetVolume, d.stream);
+            HIPCHK(hipStreamSynchronize(d.stream));
+            (void)hipFree(dt);
+            t->size += n;
+        }
+        if (appended) *appended = std::max(n, 0);
+    });
+    return CMG_OK;
+}
+
+cmg_status trainer_add_tiles(cmg_trainer* t, const float* x, const float* e, int n)
+{
+    if (!t || !x || !e || n < 0 || t->size + n > t->capacity) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        HIPCHK(hipMemcpy(t->X + size_t(t->size) * kNetVolume, x, size_t(n) * kNetVolume * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t->E + size_t(t->size) * kNetVolume, e, size_t(n) * kNetVolume * sizeof(float), hipMemcpyHostToDevice));
+        t->size += n;
+    });
+    return CMG_OK;
+}
+
+cmg_status trainer_weights(cmg_trainer* t, float* get, const float* set)
+{
+    if (!t || (!get && !set)) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        if (set) HIPCHK(hipMemcpy(t->W, set, kNetWeights * sizeof(float), hipMemcpyHostToDevice));
+        if (get) HIPCHK(hipMemcpy(get, t->W, kNetWeights * sizeof(float), hipMemcpyDeviceToHost));
+    });
+    return CMG_OK;
+}
+
+// mean loss and gradient over tiles [first, first + n)
+cmg_status trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad)
+{
+    if (!t || first < 0 || n <= 0 || first + n > t->size || !loss) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        const int used = launch_net_train(t->X + size_t(first) * kNetVolume, t->E + size_t(first) * kNetVolume, nullptr, n,
+                                          t->W, t->gpart, t->lpart, t->max_wg, bf16 != 0, t->stream);
+        launch_net_reduce(t->gpart, t->lpart, used, n, t->G, t->loss, t->stream);
+        HIPCHK(hipMemcpyAsync(loss, t->loss, sizeof(double), hipMemcpyDeviceToHost, t->stream));
+        if (grad) HIPCHK(hipMemcpyAsync(grad, t->G, kNetWeights * sizeof(float), hipMemcpyDeviceToHost, t->stream));
+        HIPCHK(hipStreamSynchronize(t->stream));
+    });
+    return CMG_OK;
+}
+
+// one Adam step on the next `batch` tiles of a seeded per-epoch permutation;
+// *loss = the batch loss before the step
+cmg_status trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss)
+{
+    if (!t || batch <= 0 || t->size <= 0) return CMG_ERR_BAD_ARG;
+    CMG_TRY({
+        batch = std::min(batch, t->size);
+        if (t->perm.size() != size_t(t->size) || t->perm_pos + batch > t->perm.size()) {
+            t->perm.resize(size_t(t->size));
+            for (int i = 0; i < t->size; ++i) t->perm[i] = i;
+            for (int i = t->size - 1; i > 0; --i) std::swap(t->perm[i], t->perm[splitmix(t->rng) % uint64_t(i + 1)]);
+            t->perm_pos = 0;
+        }
+        HIPCHK(hipMemcpyAsync(t->idx, t->perm.data() + t->perm_pos, size_t(batch) * sizeof(int32_t), hipMemcpyHostToDevice,
+                              t->stream));
+        t->perm_pos += batch;
+        const int used = launch_net_train(t->X, t->E, t->idx, batch, t->W, t->gpart, t->lpart, t->max_wg, bf16 != 0, t->stream);
+        launch_net_reduce(t->gpart, t->lpart, used, batch, t->G, t->loss, t->stream);
+        launch_adam(t->W, t->M, t->V, t->G, ++t->step, lr, 0.9f, 0.999f, 1e-8f, t->stream);
+        if (loss) HIPCHK(hipMemcpyAsync(loss, t->loss, sizeof(double), hipMemcpyDeviceToHost, t->stream));
+        HIPCHK(hipStreamSynchronize(t->stream));
+    });
+    return CMG_OK;
 }
 
 cmg_status op_copy(cmg_grid* g, int src, int dst)

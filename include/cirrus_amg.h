@@ -45,6 +45,7 @@ enum { CMG_DIRICHLET = 1, CMG_INTERIOR = 2, CMG_NEUMANN = 4 };    /* src/Poisson
 
 typedef struct cmg_grid cmg_grid;
 typedef struct cmg_solver cmg_solver;
+typedef struct cmg_trainer cmg_trainer;
 
 /* ---------------------------------------------------------------- grid ---- */
 
@@ -183,6 +184,8 @@ typedef struct cmg_solve_opts {
     int use_graph;            /* 1: capture one PCG iteration in a hipGraph */
     int mu;                   /* coarse-visit repetitions (mu_cycle_repeat_times); 0: 1 for FAS_V, else 2 */
     int flexible;             /* 1: flexible CG (Polak-Ribiere beta) for a varying preconditioner */
+    int learned;              /* 1: the learned tile smoother replaces the finest pre-smoothing
+                               * (FAS cycles; weights from cmg_solver_set_net; use with flexible) */
 } cmg_solve_opts;
 
 typedef struct cmg_solve_stats {
@@ -235,6 +238,40 @@ cmg_status cmg_gmg_damped_jacobi_solve(cmg_solver* s, const cmg_solve_opts* opts
 cmg_status cmg_solver_apply(cmg_solver* s, int x, int y);
 /* the FP64 solution of the last FP64-refinement solve (N*512 doubles) */
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host);
+
+/* ------------------------------------------------------- learned smoother -- */
+/* No reference counterpart.  A 3-layer 3D conv net on one halo'd 10^3 tile
+ * (1 -> 16 -> 16 -> 1 channels, 3^3 kernels, zero padding, ReLU after layers 1
+ * and 2, no biases); 7776 weights W1[16][27], W2[16 out][16 in][27], W3[16][27],
+ * tap (di+1)*9 + (dj+1)*3 + (dk+1), volume index (i*10+j)*10+k.  Input: r / D on
+ * the INTERIOR cells of the tile and its same-level LEAF neighbours; output: a
+ * correction of x on the tile's own cells.  Inference runs on BF16 MFMA with FP32
+ * accumulation.  Loss of a tile: the interior 7-point Laplacian of (y - e),
+ * squared and summed; a batch takes the mean. */
+enum { CMG_NET_WEIGHTS = 7776, CMG_NET_VOLUME = 1000 };
+cmg_status cmg_solver_set_net(cmg_solver* s, const float* weights);
+/* y = net(r / D) on the finest-level LEAF tiles (channels r, diag, y) */
+cmg_status cmg_solver_net_infer(cmg_solver* s, int r, int diag, int y);
+/* training: tiles resident on the device, FP32 master weights (seeded initial
+ * values), Adam (beta 0.9 / 0.999, eps 1e-8), batches drawn from a seeded
+ * per-epoch permutation; bf16 = 1 rounds weights and activations to bfloat16 in
+ * the forward pass (FP32 accumulation) */
+cmg_status cmg_trainer_create(int capacity_tiles, uint64_t seed, cmg_trainer** out);
+void cmg_trainer_destroy(cmg_trainer* t);
+/* append the finest-level LEAF tiles of g: input r / D from channels r, diag;
+ * target from channel e; *appended = tiles added (capacity permitting) */
+cmg_status cmg_trainer_collect(cmg_trainer* t, cmg_grid* g, int r, int diag, int e, int* appended);
+cmg_status cmg_trainer_add_tiles(cmg_trainer* t, const float* x, const float* e, int n);
+int cmg_trainer_size(const cmg_trainer* t);
+cmg_status cmg_trainer_get_weights(cmg_trainer* t, float* w);
+cmg_status cmg_trainer_set_weights(cmg_trainer* t, const float* w);
+/* mean loss and its gradient over tiles [first, first + n), no update */
+cmg_status cmg_trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad);
+/* one Adam step on the next batch; *loss = the batch loss before the step */
+cmg_status cmg_trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss);
+/* C = A B for 16 x 16 bfloat16 matrices (row-major bit patterns) on one MFMA
+ * instruction: checks the operand layout the inference kernel uses */
+cmg_status cmg_mfma_probe(const uint16_t* A, const uint16_t* B, float* C);
 
 /* ------------------------------------------------------- kernels ----------- */
 /* Individual device operations on channels; each restates one reference function. */
