@@ -136,17 +136,30 @@ KERNELS = {0: ("apply (y = A p on LEAF tiles + p.y partials)", 25.0),
            5: ("p = z + b p", 12.0),
            6: ("r.z partials", 8.0),
            7: ("first Jacobi sweep from x = 0, finest level", 13.0),
-           8: ("plain copy", 8.0)}
+           8: ("plain copy", 8.0),
+           9: ("learned smoother inference, finest level (BF16 MFMA)", 8.0)}
+# learned smoother: 2 * 27 taps * (1*16 + 16*16 + 16*1) channel pairs * 1000 volume
+# positions per tile ("same" padding on the halo'd 10^3 volume)
+NET_FLOP_PER_TILE = 2.0 * 27 * (16 + 256 + 16) * 1000
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+WEIGHTS = os.path.join(ROOT, "cirrus-amg_amd", "data", "smoother_c4.f32")
 
 
 def profile_kernel(cfg, kernel, reps):
+    import numpy as np
     import cirrus_amg as P
     g = build(P, cfg)
     solver = P.AMGSolver(g, 11, 0.5, 1.0, 1.0)
+    if kernel == 9:
+        solver.set_net(np.fromfile(WEIGHTS, "<f4"))
     ms, tiles = solver.bench_kernel(kernel, reps)
     name, bpc = KERNELS[kernel]
-    print(json.dumps({"kernel": kernel, "name": name, "reps": reps, "ms": ms, "tiles": tiles,
-                      "bytes_per_launch": bpc * 512 * tiles, "GBps": bpc * 512 * tiles / (ms * 1e-3) / 1e9}))
+    out = {"kernel": kernel, "name": name, "reps": reps, "ms": ms, "tiles": tiles,
+           "bytes_per_launch": bpc * 512 * tiles, "GBps": bpc * 512 * tiles / (ms * 1e-3) / 1e9}
+    if kernel == 9:
+        out["TFLOPs"] = NET_FLOP_PER_TILE * tiles / (ms * 1e-3) / 1e12
+        out["mfma_util"] = out["TFLOPs"] / MFMA_BF16_PEAK_TFLOPS
+    print(json.dumps(out))
 
 
 def main():
@@ -267,7 +280,7 @@ def main():
         result["fp64_refinement"] = {"ms": t64, "outer_iters": int(st.outer_iters), "inner_iters_total": int(it64),
                                      "true_rel_res_fp64": float(st.true_rel_res), "tolerance": 1e-10}
         result["rel_l2_err_fp32_solve_vs_fp64"] = err
-    weights = os.path.join(ROOT, "cirrus-amg_amd", "data", "smoother_c4.f32")
+    weights = WEIGHTS
     if args.learned and os.path.exists(weights):
         # learned-FCG: flexible CG whose V-cycle takes the learned tile smoother
         # (trained on the moving-sphere scene) as the finest pre-smoothing step
@@ -295,6 +308,12 @@ def main():
                 leg["rel_l2_err_vs_fp64"] = float(np.linalg.norm((xl - x64)[m]) / np.linalg.norm(x64[m]))
             legs[name] = leg
         legs["speedup_learned_vs_plain_mgpcg"] = ms_per_step / legs["learned_fcg"]["ms"]
+        # the inference kernel alone, back to back (overwrites the work channels)
+        ims, itiles = solver.bench_kernel(9, 20)
+        tf = NET_FLOP_PER_TILE * itiles / (ims * 1e-3) / 1e12
+        legs["inference_kernel"] = {"ms": ims, "tiles": itiles, "TFLOPs": tf, "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS,
+                                    "mfma_util": tf / MFMA_BF16_PEAK_TFLOPS,
+                                    "flop_per_tile": NET_FLOP_PER_TILE}
         result["learned"] = legs
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(cfg, 200)

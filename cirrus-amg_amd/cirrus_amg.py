@@ -132,7 +132,7 @@ def lib():
             "cmg_trainer_size": (i, [vp]), "cmg_trainer_get_weights": (st, [vp, vp]),
             "cmg_trainer_set_weights": (st, [vp, vp]),
             "cmg_trainer_loss_grad": (st, [vp, i, i, i, C.POINTER(d), vp]),
-            "cmg_trainer_step": (st, [vp, i, C.c_float, i, C.POINTER(d)]), "cmg_mfma_probe": (st, [vp, vp, vp]),
+            "cmg_trainer_step": (st, [vp, i, C.c_float, i, C.POINTER(d)]), "cmg_mfma_probe": (st, [i, vp, vp, vp]),
             "cmg_divergence": (st, [vp, i, i, i]), "cmg_add_gradient": (st, [vp, i, i, i, i]),
             "cmg_mix_velocity": (st, [vp, i, i, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
@@ -533,8 +533,13 @@ class Trainer:
 
 
 def mfma_probe(A_bits, B_bits):
+    """C = A B on one MFMA instruction: A is 16 x K, B is K x 16 bfloat16 bit
+    patterns, K = 16 or 32."""
     A = np.ascontiguousarray(A_bits, np.uint16)
     B = np.ascontiguousarray(B_bits, np.uint16)
+    K = A.shape[1]
+    if A.shape != (16, K) or B.shape != (K, 16) or K not in (16, 32):
+        raise ValueError("mfma_probe: A must be 16 x K and B K x 16 with K = 16 or 32")
     Cm = np.zeros((16, 16), np.float32)
-    _check(lib().cmg_mfma_probe(_ptr(A), _ptr(B), _ptr(Cm)), "cmg_mfma_probe")
+    _check(lib().cmg_mfma_probe(K, _ptr(A), _ptr(B), _ptr(Cm)), "cmg_mfma_probe")
     return Cm

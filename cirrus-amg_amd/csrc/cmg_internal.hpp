@@ -111,7 +111,7 @@ cmg_status solver_apply(cmg_solver* s, int x, int y);
 constexpr int kNetWeights = 7776, kNetVolume = 1000;
 cmg_status solver_set_net(cmg_solver* s, const float* w);
 cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y);
-cmg_status mfma_probe(const uint16_t* A, const uint16_t* B, float* C);
+cmg_status mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C);
 cmg_status trainer_create(int capacity, uint64_t seed, cmg_trainer** out);
 void trainer_destroy(cmg_trainer* t);
 int trainer_size(const cmg_trainer* t);

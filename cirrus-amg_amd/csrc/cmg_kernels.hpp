@@ -112,7 +112,7 @@ void launch_net_reduce(const float* gpart, const double* lpart, int nwg, int n, 
 void launch_adam(float* W, float* M, float* V, const float* G, int t, float lr, float b1, float b2, float eps, hipStream_t st);
 void launch_net_infer(const DevGrid& d, const LaunchList& l, const float* R, const float* D, const float* W, float* Y,
                       const double* S, hipStream_t st);
-void launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* C, hipStream_t st);
+void launch_mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C, hipStream_t st);
 // learned pre-smoothing acceptance on the finest level: keep the net's correction
 // (in Y) on a tile when it lowers the tile's residual, else the first Jacobi sweep
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,

@@ -318,141 +318,218 @@ __global__ __launch_bounds__(256) void adam(float* __restrict__ W, float* __rest
 }
 
 // -------------------------------------------------------------- inference ----
-// BF16 MFMA forward pass, one 256-thread workgroup (4 wave64s) per tile of a
-// list: y = net(x) on the tile's own 8^3 cells, x gathered from channel R / D
-// as for training; layer 2 as an implicit GEMM [1000 positions x 432] x
-// [432 x 16] on v_mfma_f32_16x16x16bf16 (K = 16 input channels per tap, 27 taps),
-// layers 1 and 3 on the vector units.  out: the correction on INTERIOR cells.
+// BF16 MFMA forward pass on the tile's own 8^3 cells: y = net(x), x gathered
+// from channel R / D as for training.  Persistent: each 256-thread workgroup
+// (4 wave64s) holds the weights in registers and walks the tiles of a list.
+// All three layers run on v_mfma_f32_16x16x{32,16}_bf16 with output channels
+// (or taps) as the M side and 16 positions of the 10^3 volume as the N side:
+//   layer 1: [16 co x 32 (27 taps + 5 zero)] x [32 x 16 positions]
+//   layer 2: per tap pair, [16 co x 32 (2 taps x 16 ci)] x [32 x 16 positions]
+//            (14 pairs, the 28th tap zero)
+//   layer 3: U[tap][q] = sum_ci W3[ci][tap] a2[q][ci] for 9 taps at a time
+//            ([16 (9 used) x 16 ci] x [16 x 16 positions]), then
+//            y[p] = sum over taps of U[tap][p + off(tap)] in a fixed order.
+// Activations are rounded to bfloat16 after each ReLU, products accumulate in
+// FP32 (the same rounding points as the training forward pass in BF16 mode).
+// LDS (68,032 B, two workgroups per CU):
+//   [0, 32000)        a1[1000 positions][16 ch] bf16   | U[9][1000] fp32
+//   [32000, 35456)    x on the zero-padded 12^3 volume  | (aliases these two
+//   [36000, 36032)    32 zero bytes (out-of-volume taps of layer 2)
+//   [36032, 68032)    a2[1000 positions][16 ch] bf16
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-struct InferLds {
-    float x[NV];
-    uint16_t a1[NV * NC];   // [position][channel], bf16
-    uint16_t a2[NV * NC];
-    uint16_t w2[27 * NC * NC];   // [tap][ci][co], bf16
-    float w1[NC * 27];
-    float w3[NC * 27];
-};
+constexpr int IL_A1 = 0, IL_X = 32000, IL_ZERO = 36000, IL_A2 = 36032, IL_BYTES = 68032;
+constexpr int NBLK = (NV + 15) / 16;   // 63 blocks of 16 positions
+
+__device__ __forceinline__ uint32_t pack_relu_bf16(float lo, float hi)
+{
+    return uint32_t(bf16_bits(lo > 0 ? lo : 0.0f)) | (uint32_t(bf16_bits(hi > 0 ? hi : 0.0f)) << 16);
+}
+
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// offset of tap t (0..26) in a volume of edge E
+__device__ __forceinline__ int tap_off(int t, int E) { return (t / 9 - 1) * E * E + ((t / 3) % 3 - 1) * E + (t % 3 - 1); }
 
 __global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __restrict__ tiles, int n,
                                                  const float* __restrict__ R, const float* __restrict__ D,
                                                  const float* __restrict__ W, float* __restrict__ Y,
                                                  const double* __restrict__ S)
 {
-    __shared__ InferLds s;
+    extern __shared__ __align__(16) unsigned char lds[];
     if (stopped(S)) return;
-    const int b = blockIdx.x;
-    if (b >= n) return;
-    const int t = tiles[b];
-    if (t < 0) return;
-    const int tid = threadIdx.x;
-    // weights
-    for (int i = tid; i < NC * 27; i += 256) {
-        s.w1[i] = bf16_round(W[i]);
-        s.w3[i] = bf16_round(W[OW3 + i]);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, grp = lane >> 4;
+    // weights as MFMA A operands (row = col): layer 1 W1[co][tap 8 grp + j]
+    s16x8 w1v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int tap = 8 * grp + j;
+        w1v[j] = tap < 27 ? short(bf16_bits(W[col * 27 + tap])) : short(0);
     }
-    for (int i = tid; i < 27 * NC * NC; i += 256) {
-        const int tap = i / (NC * NC), ci = (i / NC) % NC, co = i % NC;
-        s.w2[i] = bf16_bits(W[OW2 + (co * NC + ci) * 27 + tap]);
+    // layer 2, pair m: tap 2m + (grp >> 1), input channel 8 (grp & 1) + j
+    s16x8 w2v[14];
+#pragma unroll
+    for (int m = 0; m < 14; ++m) {
+        const int tap = 2 * m + (grp >> 1);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int ci = 8 * (grp & 1) + j;
+            w2v[m][j] = tap < 27 ? short(bf16_bits(W[OW2 + (col * NC + ci) * 27 + tap])) : short(0);
+        }
     }
-    // input volume (as gather_tiles, mode 0)
-    for (int p = tid; p < NV; p += 256) {
-        const int i = p / 100 - 1, j = (p / 10) % 10 - 1, kk = p % 10 - 1;
-        const int out_axes = (i < 0 || i > 7) + (j < 0 || j > 7) + (kk < 0 || kk > 7);
-        float val = 0.0f;
-        if (out_axes <= 1) {
-            int tt = t, l[3] = {i, j, kk};
-            if (out_axes == 1) {
-                const int a = (i < 0 || i > 7) ? 0 : (j < 0 || j > 7) ? 1 : 2;
-                const bool neg = l[a] < 0;
-                tt = tp.nbr[size_t(t) * 6 + (neg ? a : a + 3)];
-                l[a] = neg ? 7 : 0;
-                if (tt >= 0 && !(tp.ttype[tt] & LEAF)) tt = -1;
-            }
-            if (tt >= 0) {
-                const size_t idx = size_t(tt) * 512 + cidx(l[0], l[1], l[2]);
-                if (tp.ctype[idx] & INTERIOR) {
-                    const float d = D[idx];
-                    val = d != 0 ? R[idx] / d : 0.0f;
+    // layer 3, group g: row = tap g*9 + col (col < 9), k = input channel 4 grp + j
+    bf16x4 w3v[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            w3v[g][j] = col < 9 ? short(bf16_bits(W[OW3 + (4 * grp + j) * 27 + g * 9 + col])) : short(0);
+    // layer-1 tap offsets on the padded 12^3 volume (-1: zero tap)
+    int o1[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o1[j] = 8 * grp + j < 27 ? tap_off(8 * grp + j, 12) : -1;
+    if (tid < 8) reinterpret_cast<uint32_t*>(lds + IL_ZERO)[tid] = 0u;
+    const uint16_t* xs = reinterpret_cast<const uint16_t*>(lds + IL_X);
+    float* U = reinterpret_cast<float*>(lds);
+
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const int t = tiles[b];
+        if (t < 0) continue;
+        // input on the zero-padded 12^3 volume: r / D on INTERIOR cells of the
+        // tile and of its same-level LEAF face neighbours, 0 elsewhere
+        for (int e = tid; e < 1728; e += 256) {
+            const int i = e / 144 - 2, j = (e / 12) % 12 - 2, kk = e % 12 - 2;
+            const bool in_vol = i >= -1 && i <= 8 && j >= -1 && j <= 8 && kk >= -1 && kk <= 8;
+            const int out_axes = (i < 0 || i > 7) + (j < 0 || j > 7) + (kk < 0 || kk > 7);
+            float val = 0.0f;
+            if (in_vol && out_axes <= 1) {
+                int tt = t, l[3] = {i, j, kk};
+                if (out_axes == 1) {
+                    const int a = (i < 0 || i > 7) ? 0 : (j < 0 || j > 7) ? 1 : 2;
+                    const bool neg = l[a] < 0;
+                    tt = tp.nbr[size_t(t) * 6 + (neg ? a : a + 3)];
+                    l[a] = neg ? 7 : 0;
+                    if (tt >= 0 && !(tp.ttype[tt] & LEAF)) tt = -1;
+                }
+                if (tt >= 0) {
+                    const size_t idx = size_t(tt) * 512 + cidx(l[0], l[1], l[2]);
+                    if (tp.ctype[idx] & INTERIOR) {
+                        const float d = D[idx];
+                        val = d != 0 ? R[idx] / d : 0.0f;
+                    }
                 }
             }
+            reinterpret_cast<uint16_t*>(lds + IL_X)[e] = bf16_bits(val);
         }
-        s.x[p] = bf16_round(val);
-    }
-    __syncthreads();
-    // layer 1 (vector units): a1[p][co]
-    for (int o = tid; o < NC * NV; o += 256) {
-        const int p = o / NC, co = o % NC, i = p / 100, j = (p / 10) % 10, kk = p % 10;
-        float acc = 0.0f;
-        for (int tap = 0; tap < 27; ++tap) {
-            const int a = i + tap / 9 - 1, bb = j + (tap / 3) % 3 - 1, c = kk + tap % 3 - 1;
-            if (a < 0 || a > 9 || bb < 0 || bb > 9 || c < 0 || c > 9) continue;
-            acc = fmaf(s.w1[co * 27 + tap], s.x[vid(a, bb, c)], acc);
-        }
-        s.a1[o] = bf16_bits(acc > 0 ? acc : 0.0f);
-    }
-    __syncthreads();
-    // layer 2 on MFMA: each wave64 takes 16-position blocks m = wave, wave + 4, ...
-    const int wave = tid >> 6, lane = tid & 63;
-    const int row = lane & 15, kq = (lane >> 4) * 4;   // A: row, k = kq..kq+3 ; B: k = kq..kq+3, col = row
-    for (int mb = wave; mb < (NV + 15) / 16; mb += 4) {
-        const int p = mb * 16 + row;   // this lane's A row (output position)
-        const int pi = p / 100, pj = (p / 10) % 10, pk = p % 10;
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int tap = 0; tap < 27; ++tap) {
-            const int a = pi + tap / 9 - 1, bb = pj + (tap / 3) % 3 - 1, c = pk + tap % 3 - 1;
-            const bool ok = p < NV && a >= 0 && a <= 9 && bb >= 0 && bb <= 9 && c >= 0 && c <= 9;
-            bf16x4 av = {0, 0, 0, 0};
-            if (ok) {
-                const uint16_t* src = &s.a1[vid(a, bb, c) * NC + kq];
-                av[0] = short(src[0]); av[1] = short(src[1]); av[2] = short(src[2]); av[3] = short(src[3]);
-            }
-            bf16x4 bv;
-            const uint16_t* wsrc = &s.w2[(tap * NC + kq) * NC + row];
-            bv[0] = short(wsrc[0]); bv[1] = short(wsrc[NC]); bv[2] = short(wsrc[2 * NC]); bv[3] = short(wsrc[3 * NC]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-        }
-        // D[m = kq + e][n = row]: position mb*16 + kq + e, channel row
+        __syncthreads();
+        // layer 1
+        for (int mb = wave; mb < NBLK; mb += 4) {
+            const int p = mb * 16 + col;
+            const bool ok = p < NV;
+            const int pc = ok ? p : 0;
+            const int c12 = ((pc / 100 + 1) * 12 + (pc / 10) % 10 + 1) * 12 + pc % 10 + 1;
+            s16x8 xv;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int pp = mb * 16 + kq + e;
-            if (pp < NV) {
-                const float v = acc[e];
-                s.a2[pp * NC + row] = bf16_bits(v > 0 ? v : 0.0f);
+            for (int j = 0; j < 8; ++j) xv[j] = (ok && o1[j] >= 0) ? short(xs[c12 + o1[j]]) : short(0);
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma32(w1v, xv, acc);
+            if (ok) {
+                uint2 v;
+                v.x = pack_relu_bf16(acc[0], acc[1]);
+                v.y = pack_relu_bf16(acc[2], acc[3]);
+                *reinterpret_cast<uint2*>(lds + IL_A1 + p * 32 + grp * 8) = v;
             }
         }
-    }
-    __syncthreads();
-    // layer 3 (vector units) on the tile's own cells; one cell pair per thread
-    for (int v = tid; v < 512; v += 256) {
-        const int i = (v >> 6) + 1, j = ((v >> 3) & 7) + 1, kk = (v & 7) + 1;
-        float acc = 0.0f;
-        for (int tap = 0; tap < 27; ++tap) {
-            const int q = vid(i + tap / 9 - 1, j + (tap / 3) % 3 - 1, kk + tap % 3 - 1);
-            for (int ci = 0; ci < NC; ++ci)
-                acc = fmaf(s.w3[ci * 27 + tap], __uint_as_float(uint32_t(s.a2[q * NC + ci]) << 16), acc);
+        __syncthreads();
+        // layer 2
+        for (int mb = wave; mb < NBLK; mb += 4) {
+            const int p = mb * 16 + col;
+            const bool ok = p < NV;
+            const int pi = p / 100, pj = (p / 10) % 10, pk = p % 10;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int m = 0; m < 14; ++m) {
+                const int tap = 2 * m + (grp >> 1);
+                const int di = tap / 9 - 1, dj = (tap / 3) % 3 - 1, dk = tap % 3 - 1;
+                const bool in = ok && tap < 27 && unsigned(pi + di) < 10u && unsigned(pj + dj) < 10u && unsigned(pk + dk) < 10u;
+                const int addr = in ? IL_A1 + (p + di * 100 + dj * 10 + dk) * 32 + (grp & 1) * 16 : IL_ZERO + (grp & 1) * 16;
+                const s16x8 av = *reinterpret_cast<const s16x8*>(lds + addr);
+                acc = mfma32(w2v[m], av, acc);
+            }
+            if (ok) {
+                uint2 v;
+                v.x = pack_relu_bf16(acc[0], acc[1]);
+                v.y = pack_relu_bf16(acc[2], acc[3]);
+                *reinterpret_cast<uint2*>(lds + IL_A2 + p * 32 + grp * 8) = v;
+            }
         }
-        const size_t idx = size_t(t) * 512 + v;
-        Y[idx] = (tp.ctype[idx] & INTERIOR) ? acc : 0.0f;
+        __syncthreads();
+        // layer 3 in three groups of 9 taps (di = g - 1)
+        float y0 = 0.0f, y1 = 0.0f;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            for (int mb = wave; mb < NBLK; mb += 4) {
+                const int p = mb * 16 + col;
+                const bool ok = p < NV;
+                bf16x4 av = {0, 0, 0, 0};
+                if (ok) av = *reinterpret_cast<const bf16x4*>(lds + IL_A2 + p * 32 + grp * 8);
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3v[g], av, acc, 0, 0, 0);
+                if (ok) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (4 * grp + e < 9) U[(4 * grp + e) * NV + p] = acc[e];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int v = tid + 256 * h;
+                const int q = ((v >> 6) + 1 + (g - 1)) * 100 + (((v >> 3) & 7) + 1) * 10 + (v & 7) + 1;
+                float acc = h ? y1 : y0;
+#pragma unroll
+                for (int s9 = 0; s9 < 9; ++s9) acc += U[s9 * NV + q + (s9 / 3 - 1) * 10 + (s9 % 3 - 1)];
+                if (h) y1 = acc; else y0 = acc;
+            }
+            __syncthreads();
+        }
+        const size_t base = size_t(t) * 512;
+        Y[base + tid] = (tp.ctype[base + tid] & INTERIOR) ? y0 : 0.0f;
+        Y[base + tid + 256] = (tp.ctype[base + tid + 256] & INTERIOR) ? y1 : 0.0f;
     }
 }
 
-// in-bounds check of the MFMA operand layout: one wave64, C = A B for 16 x 16 bf16 A, B
-__global__ __launch_bounds__(64) void mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+// in-bounds check of the MFMA operand layouts: one wave64, C = A B for a
+// 16 x K bf16 A and a K x 16 bf16 B (row-major bit patterns), K = 16 or 32
+__global__ __launch_bounds__(64) void mfma_probe(int K, const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
                                                  float* __restrict__ C)
 {
-    const int lane = threadIdx.x, row = lane & 15, kq = (lane >> 4) * 4;
-    bf16x4 av, bv;
-    for (int e = 0; e < 4; ++e) {
-        av[e] = short(A[row * 16 + kq + e]);
-        bv[e] = short(B[(kq + e) * 16 + row]);
-    }
+    const int lane = threadIdx.x, row = lane & 15, grp = lane >> 4;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    for (int e = 0; e < 4; ++e) C[(kq + e) * 16 + row] = acc[e];
+    if (K == 32) {
+        s16x8 av, bv;
+        for (int e = 0; e < 8; ++e) {
+            av[e] = short(A[row * 32 + 8 * grp + e]);
+            bv[e] = short(B[(8 * grp + e) * 16 + row]);
+        }
+        acc = mfma32(av, bv, acc);
+    } else {
+        bf16x4 av, bv;
+        for (int e = 0; e < 4; ++e) {
+            av[e] = short(A[row * 16 + 4 * grp + e]);
+            bv[e] = short(B[(4 * grp + e) * 16 + row]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    for (int e = 0; e < 4; ++e) C[(4 * grp + e) * 16 + row] = acc[e];
 }
-
 }  // namespace k
 
 // ---------------------------------------------------------------- launchers ---
@@ -499,12 +576,19 @@ void launch_net_infer(const DevGrid& d, const LaunchList& l, const float* R, con
                       const double* S, hipStream_t st)
 {
     if (l.n <= 0) return;
-    k::net_infer<<<l.n, 256, 0, st>>>(view(d), l.d, l.n, R, D, W, Y, S);
+    static bool attr = [] {
+        (void)hipFuncSetAttribute((const void*)k::net_infer, hipFuncAttributeMaxDynamicSharedMemorySize, k::IL_BYTES);
+        return true;
+    }();
+    (void)attr;
+    // persistent: two workgroups per CU (68 KB of LDS each)
+    const int grid = std::max(1, std::min(l.n, d.wg_cap / 4));
+    k::net_infer<<<grid, 256, k::IL_BYTES, st>>>(view(d), l.d, l.n, R, D, W, Y, S);
 }
 
-void launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* C, hipStream_t st)
+void launch_mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C, hipStream_t st)
 {
-    k::mfma_probe<<<1, 64, 0, st>>>(A, B, C);
+    k::mfma_probe<<<1, 64, 0, st>>>(K, A, B, C);
 }
 
 }  // namespace cmg

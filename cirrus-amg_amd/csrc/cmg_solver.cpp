@@ -1374,15 +1374,15 @@ cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y)
     return CMG_OK;
 }
 
-cmg_status mfma_probe(const uint16_t* A, const uint16_t* B, float* C)
+cmg_status mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C)
 {
-    if (!A || !B || !C) return CMG_ERR_BAD_ARG;
+    if (!A || !B || !C || (K != 16 && K != 32)) return CMG_ERR_BAD_ARG;
     CMG_TRY({
-        uint16_t *dA = dmalloc<uint16_t>(256), *dB = dmalloc<uint16_t>(256);
+        uint16_t *dA = dmalloc<uint16_t>(16 * K), *dB = dmalloc<uint16_t>(16 * K);
         float* dC = dmalloc<float>(256);
-        HIPCHK(hipMemcpy(dA, A, 512, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dB, B, 512, hipMemcpyHostToDevice));
-        launch_mfma_probe(dA, dB, dC, nullptr);
+        HIPCHK(hipMemcpy(dA, A, 32 * K, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dB, B, 32 * K, hipMemcpyHostToDevice));
+        launch_mfma_probe(K, dA, dB, dC, nullptr);
         HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMemcpy(C, dC, 1024, hipMemcpyDeviceToHost));
         (void)hipFree(dA);
@@ -1584,7 +1584,8 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
-    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 8 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 9 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
+    if (kernel == 9 && !s->net_w) return CMG_ERR_STATE;
     CMG_TRY({
         ensure_buffers(s);
         DevGrid& d = s->g->dev;
@@ -1611,6 +1612,7 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
             case 6: launch_dot(d, d.leafs, R, Z, s->partial, S, st); nt = d.leafs.tiles; break;
             case 7: launch_jacobi_zero(d, L, s->tmp, C.c[3], R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
             case 8: launch_copy_list(d, d.leafs, Z, s->tmp, S, st); nt = d.leafs.tiles; break;
+            case 9: launch_net_infer(d, d.subtree[L], R, C.c[3], s->net_w, s->tmp, S, st); nt = d.subtree[L].tiles; break;
             }
         };
         one();

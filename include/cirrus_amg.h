@@ -269,9 +269,10 @@ cmg_status cmg_trainer_set_weights(cmg_trainer* t, const float* w);
 cmg_status cmg_trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad);
 /* one Adam step on the next batch; *loss = the batch loss before the step */
 cmg_status cmg_trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss);
-/* C = A B for 16 x 16 bfloat16 matrices (row-major bit patterns) on one MFMA
- * instruction: checks the operand layout the inference kernel uses */
-cmg_status cmg_mfma_probe(const uint16_t* A, const uint16_t* B, float* C);
+/* C = A B for a 16 x k bfloat16 A and a k x 16 bfloat16 B (row-major bit
+ * patterns, k = 16 or 32) on one MFMA instruction: checks the operand layouts
+ * the inference kernel uses */
+cmg_status cmg_mfma_probe(int k, const uint16_t* A, const uint16_t* B, float* C);
 
 /* ------------------------------------------------------- kernels ----------- */
 /* Individual device operations on channels; each restates one reference function. */
@@ -312,8 +313,9 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * 1 = damped-Jacobi sweep on the finest level, 2 = residual + restriction on the
  * finest level, 3 = PCG x/r update (+ r.r partials), 4 = FAS prolongation on the
  * finest level, 5 = PCG p update, 6 = r.z partials, 7 = first Jacobi sweep from
- * zero on the finest level, 8 = copy of one vector on LEAF tiles.  Overwrites the solver's work channels
- * (0-4) and scalars.  No reference counterpart (the reference times whole solves
+ * zero on the finest level, 8 = copy of one vector on LEAF tiles, 9 = learned
+ * smoother inference on the finest level (needs cmg_solver_set_net; CMG_ERR_STATE
+ * otherwise).  Overwrites the solver's work channels (0-4) and scalars.  No reference counterpart (the reference times whole solves
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 

@@ -18,10 +18,11 @@ def _bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-def test_mfma_operand_layout():
+@pytest.mark.parametrize("K", [16, 32])
+def test_mfma_operand_layout(K):
     rng = np.random.default_rng(1)
-    A = rng.integers(-8, 9, (16, 16)).astype(np.float32)
-    B = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    A = rng.integers(-8, 9, (16, K)).astype(np.float32)
+    B = rng.integers(-8, 9, (K, 16)).astype(np.float32)
     bits = lambda m: (m.view(np.uint32) >> 16).astype(np.uint16)
     C = P.mfma_probe(bits(A), bits(B))
     assert np.array_equal(C, A @ B)
