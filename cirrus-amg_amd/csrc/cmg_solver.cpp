@@ -639,10 +639,12 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero)
         return;
     }
     if (x_zero && level == d.max_level && c.o->learned && c.s->net_w) {
-        // the learned tile smoother replaces the pre-smoothing of the finest level;
-        // a tile whose residual it does not lower takes the first Jacobi sweep
+        // the learned tile smoother replaces the pre-smoothing of the finest level
+        // (mode 2: runs before it); a tile whose residual it does not lower takes
+        // the first Jacobi sweep instead
         launch_net_infer(d, d.subtree[level], c.B, c.C.c[3], c.s->net_w, c.s->tmp, c.S, st);
         launch_learned_accept(d, level, c.s->tmp, c.X, c.C, c.B, float(c.o->omega), c.S, st);
+        if (c.o->learned == 2) smooth(c, level, c.o->level_iters, 0, false);
     } else {
         smooth(c, level, c.o->level_iters, 0, x_zero);
     }
@@ -1160,6 +1162,7 @@ static cmg_status check_opts(const cmg_solve_opts* o)
     if (o->cycle < CMG_CYCLE_FAS_W || o->cycle > CMG_CYCLE_MU || o->mu < 0) return CMG_ERR_BAD_ARG;
     if (o->smoother != CMG_SMOOTH_RBGS && o->smoother != CMG_SMOOTH_JACOBI) return CMG_ERR_BAD_ARG;
     if (o->precision != CMG_PREC_FP32 && o->precision != CMG_PREC_FP64_REFINE) return CMG_ERR_BAD_ARG;
+    if (o->learned < 0 || o->learned > 2) return CMG_ERR_BAD_ARG;
     if (o->learned && (o->cycle == CMG_CYCLE_VCYCLE || o->cycle == CMG_CYCLE_MU)) return CMG_ERR_BAD_ARG;
     return CMG_OK;
 }

@@ -184,8 +184,9 @@ typedef struct cmg_solve_opts {
     int use_graph;            /* 1: capture one PCG iteration in a hipGraph */
     int mu;                   /* coarse-visit repetitions (mu_cycle_repeat_times); 0: 1 for FAS_V, else 2 */
     int flexible;             /* 1: flexible CG (Polak-Ribiere beta) for a varying preconditioner */
-    int learned;              /* 1: the learned tile smoother replaces the finest pre-smoothing
-                               * (FAS cycles; weights from cmg_solver_set_net; use with flexible) */
+    int learned;              /* 1: the learned tile smoother replaces the finest pre-smoothing;
+                               * 2: it runs before the usual finest pre-sweeps (FAS cycles;
+                               * weights from cmg_solver_set_net; use with flexible) */
 } cmg_solve_opts;
 
 typedef struct cmg_solve_stats {

@@ -114,8 +114,9 @@ def test_mfma_inference_matches_oracle_forward(case):
     assert worst <= 3e-2, worst
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("case", ["sphere_empty24", "c3_small"])
-def test_learned_fcg_solve_converges(case):
+def test_learned_fcg_solve_converges(case, mode):
     """With any weights (here untrained) the learned flexible-CG solve reaches the
     tolerance: tiles whose residual the net does not lower fall back to Jacobi."""
     go, gp = H.build_pair(case)
@@ -127,7 +128,7 @@ def test_learned_fcg_solve_converges(case):
     solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
     solver.set_net(np.random.default_rng(8).normal(0, 0.1, P.NET_WEIGHTS).astype(np.float32))
     opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                          flexible=1, learned=1)
+                          flexible=1, learned=mode)
     it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
     assert rel < 1e-6, (it, rel)
     x = gp.channel(0).ravel()[A.cell]
