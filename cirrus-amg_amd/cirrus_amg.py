@@ -354,7 +354,8 @@ def device_synchronize():
 
 
 def set_kernel_forms(persistent_mask, stencil_threads):
-    """Stencil kernel forms (bit 0 operator, bit 1 Jacobi, bit 2 restrict persistent; threads per tile)."""
+    """Stencil kernel forms (bit 0 operator, bit 1 Jacobi, bit 2 restrict persistent, bit 3 stored
+    diagonals, bit 4 unfused first sweep; threads per tile)."""
     _check(lib().cmg_set_kernel_forms(persistent_mask, stencil_threads), "cmg_set_kernel_forms")
 
 

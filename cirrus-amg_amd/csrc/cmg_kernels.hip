@@ -505,10 +505,15 @@ __global__ __launch_bounds__(NT) void restrict_p(TopoView tp, const int32_t* __r
 // ------------------------------------------------------ vector (float4) ----
 // (CMG_TILE_LOOP walks the tiles of a list, 128 threads x float4 per tile)
 
+// Z0: with the first damped-Jacobi sweep from x = 0 of level `zlevel` fused in,
+// z0 = 0 + w (r / D) on INTERIOR cells (0 elsewhere) of the level's tiles, from
+// the updated r (bit-identical to jacobi_zero_v on r)
+template <bool Z0>
 __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
                                                    float* __restrict__ R, const float* __restrict__ P,
                                                    const float* __restrict__ AP, double* __restrict__ partial,
-                                                   const double* __restrict__ S)
+                                                   const double* __restrict__ S, int zlevel, float* __restrict__ Zo,
+                                                   const float* __restrict__ D, float w)
 {
     __shared__ double red[8];
     if (stopped(S)) return;
@@ -529,6 +534,15 @@ __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* _
         reinterpret_cast<float4*>(X)[q] = x;
         reinterpret_cast<float4*>(R)[q] = r;
         acc += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z) + double(r.w) * double(r.w);
+        if (Z0 && tp.tlevel[t_] == zlevel) {
+            const float4 d = reinterpret_cast<const float4*>(D)[q];
+            float4 z;
+            z.x = 0.0f + w * ((ct.x & INTERIOR) ? (r.x - 0.0f) / d.x : 0.0f);
+            z.y = 0.0f + w * ((ct.y & INTERIOR) ? (r.y - 0.0f) / d.y : 0.0f);
+            z.z = 0.0f + w * ((ct.z & INTERIOR) ? (r.z - 0.0f) / d.z : 0.0f);
+            z.w = 0.0f + w * ((ct.w & INTERIOR) ? (r.w - 0.0f) / d.w : 0.0f);
+            reinterpret_cast<float4*>(Zo)[q] = z;
+        }
     }
     double b = block_sum(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
@@ -1333,7 +1347,9 @@ static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 // (1 / 2 / 4 cells per thread).  Persistence mask: bit 0 apply, bit 1 Jacobi,
 // bit 2 residual + restrict; a set bit selects the persistent pipelined kernel,
 // a clear bit the one-tile-per-workgroup kernel.  Bit 3 keeps the stored LEAF
-// diagonals even where they could be derived from the face coefficients.  Defaults may be overridden by
+// diagonals even where they could be derived from the face coefficients.  Bit 4
+// keeps the first finest-level Jacobi sweep of the preconditioner a launch of
+// its own instead of fusing it into the PCG x/r update.  Defaults may be overridden by
 // the environment (CMG_STENCIL_THREADS, CMG_PERSISTENT) or set_kernel_forms.
 enum { PERSIST_APPLY = 1, PERSIST_JACOBI = 2, PERSIST_RESTRICT = 4 };
 static std::atomic<int> g_threads{0}, g_mask{-1};
@@ -1356,14 +1372,14 @@ static int persistent_mask()
     const char* e = getenv("CMG_PERSISTENT");
     // one tile per workgroup everywhere: faster than the persistent forms on MI355X at
     // every thread count (the persistent forms hold two tiles' loads in registers)
-    m = e ? (atoi(e) & 15) : 0;
+    m = e ? (atoi(e) & 31) : 0;
     g_mask.store(m, std::memory_order_relaxed);
     return m;
 }
 
 bool set_kernel_forms(int mask, int threads)
 {
-    if (mask < 0 || mask > 15 || (threads != 128 && threads != 256 && threads != 512)) return false;
+    if (mask < 0 || mask > 31 || (threads != 128 && threads != 256 && threads != 512)) return false;
     g_mask.store(mask, std::memory_order_relaxed);
     g_threads.store(threads, std::memory_order_relaxed);
     return true;
@@ -1562,12 +1578,14 @@ int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const flo
 }
 
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
-                     hipStream_t st)
+                     hipStream_t st, float* Z0, const float* D, float w)
 {
     if (!S) throw std::invalid_argument("update_xr needs the solver scalars (alpha)");
+    if (Z0 && !D) throw std::invalid_argument("update_xr: the fused first sweep needs the diagonal");
     if (!nonempty(d.leafs)) return 0;
     const int g = vector_grid(d, d.leafs.n);
-    k::update_xr_v<<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S);
+    if (Z0) k::update_xr_v<true><<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S, d.max_level, Z0, D, w);
+    else k::update_xr_v<false><<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S, -1, nullptr, nullptr, 0.0f);
     return g;
 }
 
@@ -1686,6 +1704,7 @@ int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st)
 }
 
 bool derive_allowed() { return !(persistent_mask() & 8); }
+bool fuse_first_sweep_allowed() { return !(persistent_mask() & 16); }
 
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
                            const double* S, hipStream_t st)

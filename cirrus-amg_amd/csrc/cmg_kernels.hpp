@@ -55,7 +55,7 @@ int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const floa
 int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const float* B, const float* C, double* partial,
                 double* partial2, const double* S, hipStream_t st);
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
-                     hipStream_t st);
+                     hipStream_t st, float* Z0 = nullptr, const float* D = nullptr, float w = 0.0f);
 void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st);
 void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st);
@@ -118,10 +118,12 @@ void launch_mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C, hi
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
                            const double* S, hipStream_t st);
 
-// stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict; threads per tile)
+// stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict, bit 3 stored
+// diagonals, bit 4 unfused first sweep; threads per tile)
 bool set_kernel_forms(int mask, int threads);
 void get_kernel_forms(int* mask, int* threads);
-bool derive_allowed();   // forms bit 3 clear
+bool derive_allowed();             // forms bit 3 clear
+bool fuse_first_sweep_allowed();   // forms bit 4 clear
 
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st);

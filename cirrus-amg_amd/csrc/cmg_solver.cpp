@@ -585,6 +585,7 @@ struct CycleCtx {
     float *X, *B, *X0;
     Coef C;
     const double* S;
+    bool zero_done = false;   // the finest level's first sweep from x = 0 is already in s->tmp
 };
 
 // `x_zero`: every x of the cycle is still zero (first sweep at the finest level)
@@ -602,7 +603,10 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
     float* cur = c.X;
     float* other = c.s->tmp;
     for (int it = 0; it < iters; ++it) {
-        if (it == 0 && x_zero) launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st);
+        if (it == 0 && x_zero) {
+            if (!(c.zero_done && level == d.max_level))
+                launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st);
+        }
         else launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st);
         std::swap(cur, other);
     }
@@ -769,7 +773,8 @@ int enqueue_apply(cmg_solver* s, float* X, float* Y, double* partial, const doub
 // the preconditioner z ~ A^-1 r of the options' cycle kind.  FAS kinds: x, rhs
 // = r (its NONLEAF tiles become coarse right-hand sides), x0 = FAS scratch.
 // Non-FAS kinds: x, f = r (kept), x0 = working right-hand side.
-void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float* RHSp, float* X0p, const double* S)
+void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float* RHSp, float* X0p, const double* S,
+                       bool zero_done = false)
 {
     DevGrid& d = s->g->dev;
     if (s->family != CMG_SOLVER_AMG) {
@@ -796,7 +801,7 @@ void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float*
     }
     default: {
         // FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
-        const CycleCtx c{s, o, Xp, RHSp, X0p, C, S};
+        const CycleCtx c{s, o, Xp, RHSp, X0p, C, S, zero_done};
         launch_fill_all(d, c.X, 0.0f, S, d.stream);
         cycle_step(c, d.max_level, true);
     }
@@ -811,18 +816,27 @@ void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x
 
 // the preconditioner of one PCG step, z ~ A^-1 r; flexible CG keeps Ap, so
 // the cycle's scratch is then a solver-private buffer
-void enqueue_precondition(cmg_solver* s, const cmg_solve_opts* o, const double* S)
+void enqueue_precondition(cmg_solver* s, const cmg_solve_opts* o, const double* S, bool zero_done = false)
 {
     DevGrid& d = s->g->dev;
     if (!o->flexible) {
-        enqueue_cycle(s, o, CH_Z, CH_R, CH_AP, S);
+        enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), dev_channel(d, CH_AP), S, zero_done);
         return;
     }
     if (!s->scratch) {
         s->scratch = dmalloc<float>(size_t(d.N) * kCells);
         HIPCHK(hipMemset(s->scratch, 0, size_t(d.N) * kCells * sizeof(float)));
     }
-    enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), s->scratch, S);
+    enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), s->scratch, S, zero_done);
+}
+
+// the preconditioner's first finest-level sweep from x = 0 can ride on the PCG
+// x/r update: FAS mu-cycles with damped Jacobi whose input r is final there
+bool first_sweep_fusable(const cmg_solver* s, const cmg_solve_opts* o)
+{
+    return s->family == CMG_SOLVER_AMG && (o->cycle == CMG_CYCLE_FAS_W || o->cycle == CMG_CYCLE_FAS_V) &&
+           o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && !o->learned && !o->is_pure_neumann &&
+           s->g->dev.max_level >= 1 && fuse_first_sweep_allowed();
 }
 
 void enqueue_recenter(cmg_solver* s, float* R)
@@ -842,10 +856,12 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
           *Z = dev_channel(d, CH_Z);
     const int n1 = enqueue_apply(s, P, AP, s->partial, s->S);
     launch_finish(s->partial, nullptr, n1, s->S, FIN_ALPHA, st);
-    const int n2 = launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
+    const bool fuse = first_sweep_fusable(s, o);
+    const int n2 = fuse ? launch_update_xr(d, X, R, P, AP, s->partial, s->S, st, s->tmp, coef(d, s->c0).c[3], float(o->omega))
+                        : launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
-    enqueue_precondition(s, o, s->S);
+    enqueue_precondition(s, o, s->S, fuse);
     if (o->flexible) {
         // flexible CG: beta = z.(r - r_old) / gamma_old (Polak-Ribiere; AMGSolver.cu:1520 uses Fletcher-Reeves)
         const int n3 = launch_dot2(d, d.leafs, Z, R, AP, s->partial, s->partial2, s->S, st);

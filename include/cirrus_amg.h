@@ -326,12 +326,15 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * walks several tiles, prefetching the next tile's loads), a clear bit one tile
  * per workgroup; bit 3 set keeps the stored diagonal of LEAF cells (by default a
  * solve derives it from the face coefficients, one load fewer per cell, after
- * checking that every stored LEAF diagonal equals the derived one bit for bit).
+ * checking that every stored LEAF diagonal equals the derived one bit for bit);
+ * bit 4 set keeps the preconditioner's first finest-level Jacobi sweep from
+ * x = 0 a launch of its own (by default PCG iterations with FAS mu-cycles and
+ * damped Jacobi fuse it into the x/r update).
  * stencil_threads: 128, 256 or 512 threads per tile.  Results are bit-identical
  * across forms.  Default: mask 0, 256 threads.  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
-cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..15 */
+cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..31 */
 cmg_status cmg_get_kernel_forms(int* persistent_mask, int* stencil_threads);
 
 /* hipDeviceSynchronize on the current device (timing brackets) */

@@ -341,3 +341,28 @@ def test_derived_diagonals_change_nothing(case):
         P.set_kernel_forms(*saved)
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
     assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
+
+
+@pytest.mark.parametrize("cycle,flexible", [(P.CYCLE_FAS_V, 0), (P.CYCLE_FAS_W, 0), (P.CYCLE_FAS_V, 1)],
+                         ids=["fas_v", "fas_w", "fas_v_flexible"])
+@pytest.mark.parametrize("case", ["sphere_empty24", "c3_small"])
+def test_fused_first_sweep_changes_nothing(case, cycle, flexible):
+    """Fusing the preconditioner's first finest-level Jacobi sweep into the PCG
+    x/r update gives the bits of the separate launch."""
+    go, gp = H.build_pair(case)
+    b = _rhs(go, gp, 29)
+    opts = P.default_opts(cycle=cycle, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1, flexible=flexible)
+    saved = P.get_kernel_forms()
+    out = []
+    try:
+        solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+        for mask in (0, 16):
+            P.set_kernel_forms(mask, 256)
+            gp.set_channel(1, b)
+            it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
+            out.append((it, rel, gp.channel(0).copy()))
+    finally:
+        P.set_kernel_forms(*saved)
+    assert out[0][1] < 1e-6
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
