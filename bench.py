@@ -145,6 +145,29 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 WEIGHTS = os.path.join(ROOT, "cirrus-amg_amd", "data", "smoother_c4.f32")
 
 
+def stream_copy_ceiling(nbytes=1 << 30, reps=10):
+    """On-box HBM copy ceiling: torch's device copy of a 1 GiB buffer (four times
+    the 256 MB last-level cache) into another, timed with events on torch's
+    current stream, where the copies run."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GBps": 2.0 * nbytes / (ms * 1e-3) / 1e9, "bytes_moved": 2 * nbytes, "ms": ms,
+            "kernel": "torch copy_ of 1 GiB FP32"}
+
+
 def profile_kernel(cfg, kernel, reps):
     import numpy as np
     import cirrus_amg as P
@@ -262,6 +285,13 @@ def main():
         "apply_kernel": {"GBps": apply_gbs, "ms": apply_ms, "tiles": apply_tiles},
         "build_s": t_build,
     }
+    try:
+        ceil = stream_copy_ceiling()
+        result["hbm_copy_ceiling"] = ceil
+        result["roofline"]["frac_of_copy_ceiling"] = k_gbs / ceil["GBps"]
+        result["solve_frac_of_copy_ceiling"] = solve_gbs / ceil["GBps"]
+    except Exception as e:   # torch without a device: the ceiling is simply not reported
+        result["hbm_copy_ceiling"] = {"error": str(e)[:200]}
     if not args.no_fp64:
         # mixed-precision form: FP32 MGPCG inner solves inside FP64 iterative
         # refinement to 1e-10; its solution is the FP64 solution of the same

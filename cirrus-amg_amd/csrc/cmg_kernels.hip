@@ -119,7 +119,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (nt < 0) continue;
         const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
         const uint8_t nty = tp.ttype[nt];
-        hcp[m] = C[axis][ni];
+        // only a + face needs the neighbour's coefficient (a - face's is the cell's own)
+        if (sgn > 0) hcp[m] = C[axis][ni];
         if (nty & GHOST) {
             hghost[m] = true;
             if (FRESH) {
@@ -286,7 +287,7 @@ __device__ __forceinline__ void issue(Stage<T, NT, WB>& st, const TopoView& tp, 
         if (nt < 0) continue;
         const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
         const uint8_t nty = tp.ttype[nt];
-        st.hcp[m] = C[axis][ni];
+        if (sgn > 0) st.hcp[m] = C[axis][ni];
         if (nty & GHOST) {
             st.hghost[m] = true;
             if (FRESH) {
