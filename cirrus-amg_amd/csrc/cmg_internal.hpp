@@ -55,6 +55,7 @@ struct DevGrid {
     LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
     LaunchList ghosts;                  // all GHOST tiles
     LaunchList leafghost;               // all LEAF and GHOST tiles
+    LaunchList leafcoarse;              // LEAF tiles below the finest level
     int max_list = 0;
     int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU
     int derive = 0;                     // set by a solve whose operator passed the diagonal check

@@ -795,27 +795,40 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
     }
 }
 
-// damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level
-template <int NT>
+// damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level;
+// DOT: also the tile's sum of b x_out (FP64, all cells) into partial[blockIdx.x]
+template <int NT, bool DOT = false>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
-                                             const float* __restrict__ B, float w, const double* __restrict__ S)
+                                             const float* __restrict__ B, float w, const double* __restrict__ S,
+                                             double* __restrict__ partial)
 {
     __shared__ Lds<float> s;
+    __shared__ double red[DOT ? 8 : 1];
     if (stopped(S)) return;
     const int t = list[blockIdx.x];
-    if (t < 0) return;
+    if (t < 0) {
+        if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
+        return;
+    }
     Own<float, NT> o;
     float b[512 / NT];
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
     load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o);
+    double acc = 0;
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
         const int v = threadIdx.x + c * NT;
         const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
         const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
-        Xout[size_t(t) * 512 + v] = o.x[c] + w * delta;
+        const float xo = o.x[c] + w * delta;
+        Xout[size_t(t) * 512 + v] = xo;
+        if (DOT) acc += double(b[c]) * double(xo);
+    }
+    if (DOT) {
+        const double sum = block_sum(acc, red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = sum;
     }
 }
 
@@ -1349,8 +1362,10 @@ static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 // bit 2 residual + restrict; a set bit selects the persistent pipelined kernel,
 // a clear bit the one-tile-per-workgroup kernel.  Bit 3 keeps the stored LEAF
 // diagonals even where they could be derived from the face coefficients.  Bit 4
-// keeps the first finest-level Jacobi sweep of the preconditioner a launch of
-// its own instead of fusing it into the PCG x/r update.  Defaults may be overridden by
+// keeps the preconditioner's boundary work in launches of their own: the first
+// finest-level Jacobi sweep (otherwise fused into the PCG x/r update) and r.z
+// (otherwise summed by the last finest-level sweep plus a dot over the coarser
+// LEAF tiles).  Defaults may be overridden by
 // the environment (CMG_STENCIL_THREADS, CMG_PERSISTENT) or set_kernel_forms.
 enum { PERSIST_APPLY = 1, PERSIST_JACOBI = 2, PERSIST_RESTRICT = 4 };
 static std::atomic<int> g_threads{0}, g_mask{-1};
@@ -1489,13 +1504,27 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
     CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
 }
 
-void launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                   const double* S, hipStream_t st)
+int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
+                  const double* S, hipStream_t st, double* partial)
 {
     const LaunchList& l = d.subtree[level];
-    if (!nonempty(l)) return;
-    if (!(persistent_mask() & PERSIST_JACOBI)) CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S);
+    if (partial) {
+        // one partial per list entry (the one-tile form only)
+        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: b.x partials need the one-tile form");
+        if (!nonempty(l)) return 0;
+        const TopoView tp = view(d);
+        switch (stencil_threads()) {
+        case 128: k::jacobi<128, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
+        case 512: k::jacobi<512, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
+        default: k::jacobi<256, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
+        }
+        return l.n;
+    }
+    if (!nonempty(l)) return 0;
+    if (!(persistent_mask() & PERSIST_JACOBI))
+        CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S, (double*)nullptr);
     else CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, view(d), l.d, l.n, Xin, Xout, G, C, B, w, S);
+    return 0;
 }
 
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
@@ -1706,6 +1735,7 @@ int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st)
 
 bool derive_allowed() { return !(persistent_mask() & 8); }
 bool fuse_first_sweep_allowed() { return !(persistent_mask() & 16); }
+bool fuse_rz_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
                            const double* S, hipStream_t st)

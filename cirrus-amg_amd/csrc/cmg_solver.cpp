@@ -180,7 +180,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     up(d.child, topo.child);
     d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
     HIPCHK(hipMemset(d.ctype, 0, size_t(N) * kCells));
-    std::vector<int32_t> leafs, all, ghosts, leafghost;
+    std::vector<int32_t> leafs, all, ghosts, leafghost, leafcoarse;
     std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels], la[kMaxLevels];
     for (int t = 0; t < N; ++t) {
         const uint8_t ty = topo.t_type[t];
@@ -192,12 +192,14 @@ void dev_upload(DevGrid& d, const Topology& topo)
         if (ty & (LEAF | NONLEAF)) sub[l].push_back(t);
         if (ty & NONLEAF) nl[l].push_back(t);
         if (ty & LEAF) la[l].push_back(t);
+        if ((ty & LEAF) && l < topo.num_levels() - 1) leafcoarse.push_back(t);
         pr[l].push_back(t);
     }
     d.leafs = make_list(topo, leafs);
     d.all = make_list(topo, all);
     d.ghosts = make_list(topo, ghosts);
     d.leafghost = make_list(topo, leafghost);
+    d.leafcoarse = make_list(topo, leafcoarse);
     d.max_list = std::max(d.leafs.n, d.all.n);
     for (int l = 0; l < topo.num_levels(); ++l) {
         d.subtree[l] = make_list(topo, sub[l]);
@@ -586,6 +588,7 @@ struct CycleCtx {
     Coef C;
     const double* S;
     bool zero_done = false;   // the finest level's first sweep from x = 0 is already in s->tmp
+    double* rz = nullptr;     // the last finest-level post-sweep writes its b.x partials here
 };
 
 // `x_zero`: every x of the cycle is still zero (first sweep at the finest level)
@@ -607,7 +610,10 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
             if (!(c.zero_done && level == d.max_level))
                 launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st);
         }
-        else launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st);
+        else {
+            const bool last_fine = c.rz && order == 1 && level == d.max_level && it == iters - 1;
+            launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st, last_fine ? c.rz : nullptr);
+        }
         std::swap(cur, other);
     }
     if (cur != c.X) launch_copy_list(d, d.subtree[level], cur, c.X, c.S, st);
@@ -774,7 +780,7 @@ int enqueue_apply(cmg_solver* s, float* X, float* Y, double* partial, const doub
 // = r (its NONLEAF tiles become coarse right-hand sides), x0 = FAS scratch.
 // Non-FAS kinds: x, f = r (kept), x0 = working right-hand side.
 void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float* RHSp, float* X0p, const double* S,
-                       bool zero_done = false)
+                       bool zero_done = false, double* rz = nullptr)
 {
     DevGrid& d = s->g->dev;
     if (s->family != CMG_SOLVER_AMG) {
@@ -801,7 +807,7 @@ void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float*
     }
     default: {
         // FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
-        const CycleCtx c{s, o, Xp, RHSp, X0p, C, S, zero_done};
+        const CycleCtx c{s, o, Xp, RHSp, X0p, C, S, zero_done, rz};
         launch_fill_all(d, c.X, 0.0f, S, d.stream);
         cycle_step(c, d.max_level, true);
     }
@@ -816,11 +822,12 @@ void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x
 
 // the preconditioner of one PCG step, z ~ A^-1 r; flexible CG keeps Ap, so
 // the cycle's scratch is then a solver-private buffer
-void enqueue_precondition(cmg_solver* s, const cmg_solve_opts* o, const double* S, bool zero_done = false)
+void enqueue_precondition(cmg_solver* s, const cmg_solve_opts* o, const double* S, bool zero_done = false,
+                          double* rz = nullptr)
 {
     DevGrid& d = s->g->dev;
     if (!o->flexible) {
-        enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), dev_channel(d, CH_AP), S, zero_done);
+        enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), dev_channel(d, CH_AP), S, zero_done, rz);
         return;
     }
     if (!s->scratch) {
@@ -837,6 +844,16 @@ bool first_sweep_fusable(const cmg_solver* s, const cmg_solve_opts* o)
     return s->family == CMG_SOLVER_AMG && (o->cycle == CMG_CYCLE_FAS_W || o->cycle == CMG_CYCLE_FAS_V) &&
            o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && !o->learned && !o->is_pure_neumann &&
            s->g->dev.max_level >= 1 && fuse_first_sweep_allowed();
+}
+
+// r.z can be summed by the last finest-level post-sweep (whose b is r and whose
+// output is z there) plus a dot over the coarser LEAF tiles, whose z is final
+// after their own level's post-smoothing
+bool rz_fusable(const cmg_solver* s, const cmg_solve_opts* o)
+{
+    return s->family == CMG_SOLVER_AMG && (o->cycle == CMG_CYCLE_FAS_W || o->cycle == CMG_CYCLE_FAS_V) &&
+           o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && !o->flexible && !o->learned &&
+           s->g->dev.max_level >= 1 && fuse_rz_allowed();
 }
 
 void enqueue_recenter(cmg_solver* s, float* R)
@@ -861,8 +878,13 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
                         : launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
-    enqueue_precondition(s, o, s->S, fuse);
-    if (o->flexible) {
+    const bool frz = rz_fusable(s, o);
+    enqueue_precondition(s, o, s->S, fuse, frz ? s->partial : nullptr);
+    if (frz) {
+        const int nf = d.subtree[d.max_level].n;
+        const int nc = launch_dot(d, d.leafcoarse, R, Z, s->partial + nf, s->S, st);
+        launch_finish(s->partial, nullptr, nf + nc, s->S, FIN_BETA, st);
+    } else if (o->flexible) {
         // flexible CG: beta = z.(r - r_old) / gamma_old (Polak-Ribiere; AMGSolver.cu:1520 uses Fletcher-Reeves)
         const int n3 = launch_dot2(d, d.leafs, Z, R, AP, s->partial, s->partial2, s->S, st);
         launch_finish(s->partial, s->partial2, n3, s->S, FIN_BETA_FLEX, st);
@@ -1074,8 +1096,10 @@ cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_s
         s->S = dmalloc<double>(S_COUNT);
         HIPCHK(hipMemset(s->S, 0, S_COUNT * sizeof(double)));
         HIPCHK(hipHostMalloc((void**)&s->hS, S_COUNT * sizeof(double), hipHostMallocDefault));
-        s->partial = dmalloc<double>(size_t(g->dev.max_list) + 1);
-        s->partial2 = dmalloc<double>(size_t(g->dev.max_list) + 1);
+        // room for a fused r.z: one partial per finest-level list entry plus a vector grid
+        const size_t np = size_t(g->dev.max_list) + size_t(g->dev.wg_cap) + 64;
+        s->partial = dmalloc<double>(np);
+        s->partial2 = dmalloc<double>(np);
         HIPCHK(hipEventCreate(&s->e0));
         HIPCHK(hipEventCreate(&s->e1));
     } catch (const std::exception& ex) {

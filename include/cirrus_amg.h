@@ -327,11 +327,14 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * per workgroup; bit 3 set keeps the stored diagonal of LEAF cells (by default a
  * solve derives it from the face coefficients, one load fewer per cell, after
  * checking that every stored LEAF diagonal equals the derived one bit for bit);
- * bit 4 set keeps the preconditioner's first finest-level Jacobi sweep from
- * x = 0 a launch of its own (by default PCG iterations with FAS mu-cycles and
- * damped Jacobi fuse it into the x/r update).
- * stencil_threads: 128, 256 or 512 threads per tile.  Results are bit-identical
- * across forms.  Default: mask 0, 256 threads.  No reference counterpart (the reference
+ * bit 4 set keeps the preconditioner's boundary work in launches of its own (by
+ * default PCG iterations with FAS mu-cycles and damped Jacobi fuse the first
+ * finest-level sweep from x = 0 into the x/r update, and sum r.z in the last
+ * finest-level sweep plus a dot over the coarser LEAF tiles, which regroups the
+ * FP64 partial sums of r.z).
+ * stencil_threads: 128, 256 or 512 threads per tile.  Stencil results are
+ * bit-identical across forms; PCG scalars differ only by the grouping of FP64
+ * partial sums.  Default: mask 0, 256 threads.  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
 cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..31 */

@@ -346,9 +346,10 @@ def test_derived_diagonals_change_nothing(case):
 @pytest.mark.parametrize("cycle,flexible", [(P.CYCLE_FAS_V, 0), (P.CYCLE_FAS_W, 0), (P.CYCLE_FAS_V, 1)],
                          ids=["fas_v", "fas_w", "fas_v_flexible"])
 @pytest.mark.parametrize("case", ["sphere_empty24", "c3_small"])
-def test_fused_first_sweep_changes_nothing(case, cycle, flexible):
+def test_fused_boundary_kernels(case, cycle, flexible):
     """Fusing the preconditioner's first finest-level Jacobi sweep into the PCG
-    x/r update gives the bits of the separate launch."""
+    x/r update gives the bits of the separate launch; summing r.z in the last
+    finest-level sweep (non-flexible PCG) regroups only FP64 partial sums."""
     go, gp = H.build_pair(case)
     b = _rhs(go, gp, 29)
     opts = P.default_opts(cycle=cycle, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1, flexible=flexible)
@@ -364,5 +365,11 @@ def test_fused_first_sweep_changes_nothing(case, cycle, flexible):
     finally:
         P.set_kernel_forms(*saved)
     assert out[0][1] < 1e-6
-    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
-    assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
+    assert out[0][0] == out[1][0]
+    if flexible:
+        assert out[0][1] == out[1][1]
+        assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
+    else:
+        assert abs(out[0][1] - out[1][1]) <= 1e-3 * out[1][1]
+        x0, x1 = out[0][2].astype(np.float64), out[1][2].astype(np.float64)
+        assert np.linalg.norm(x0 - x1) <= 1e-5 * np.linalg.norm(x1)
