@@ -391,10 +391,10 @@ __global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __r
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             w3v[g][j] = col < 9 ? short(bf16_bits(W[OW3 + (4 * grp + j) * 27 + g * 9 + col])) : short(0);
-    // layer-1 tap offsets on the padded 12^3 volume (-1: zero tap)
+    // layer-1 tap offsets on the padded 12^3 volume (taps 27..31 are zero taps)
     int o1[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) o1[j] = 8 * grp + j < 27 ? tap_off(8 * grp + j, 12) : -1;
+    for (int j = 0; j < 8; ++j) o1[j] = 8 * grp + j < 27 ? tap_off(8 * grp + j, 12) : 0;
     if (tid < 8) reinterpret_cast<uint32_t*>(lds + IL_ZERO)[tid] = 0u;
     const uint16_t* xs = reinterpret_cast<const uint16_t*>(lds + IL_X);
     float* U = reinterpret_cast<float*>(lds);
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __r
             const int c12 = ((pc / 100 + 1) * 12 + (pc / 10) % 10 + 1) * 12 + pc % 10 + 1;
             s16x8 xv;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) xv[j] = (ok && o1[j] >= 0) ? short(xs[c12 + o1[j]]) : short(0);
+            for (int j = 0; j < 8; ++j) xv[j] = (ok && 8 * grp + j < 27) ? short(xs[c12 + o1[j]]) : short(0);
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
             acc = mfma32(w1v, xv, acc);
             if (ok) {
