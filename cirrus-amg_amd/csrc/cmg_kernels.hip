@@ -71,6 +71,28 @@ struct Own {
     uint8_t ct[CPT];   // cell type
 };
 
+// a pointer argument as a workgroup-uniform value (kept in SGPRs, selectable
+// without a memory access)
+__device__ __forceinline__ const float* uniform_ptr(const float* p)
+{
+    const uint64_t u = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(u));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(u >> 32));
+    return reinterpret_cast<const float*>((uint64_t(hi) << 32) | lo);
+}
+
+// a[s] for a thread-varying s in 0..5 without indexing a private array
+__device__ __forceinline__ int pick6(const int (&a)[6], int s)
+{
+    int r = a[0];
+    r = s == 1 ? a[1] : r;
+    r = s == 2 ? a[2] : r;
+    r = s == 3 ? a[3] : r;
+    r = s == 4 ? a[4] : r;
+    r = s == 5 ? a[5] : r;
+    return r;
+}
+
 // LoadAMGLaplacianTileData (src/AMGSolver.cu:395-543).
 // FRESH: the halo is evaluated the way AMGFullNegativeLaplacianOnLeafs sees it
 //   after PropagateToChildren + AccumulateToParentsOneStep: a GHOST cell is its
@@ -86,7 +108,25 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     constexpr int HPT = (384 + NT - 1) / NT;
     const int tid = threadIdx.x;
     const size_t base = size_t(t) * 512;
-    const bool dv = tp.derive && (tp.ttype[t] & LEAF);
+    // the tile's halo record (uniform): face neighbours, their tile types, the
+    // tile's own type and (FRESH) the parents and octants of GHOST neighbours
+    const int4* hr = reinterpret_cast<const int4*>(tp.halo + size_t(t) * 8);
+    const int4 ha = hr[0], hb = hr[1];
+    const int nb[6] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y};
+    const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
+    const bool dv = tp.derive && ((ty45 >> 24) & LEAF);
+    // the coefficient planes as values (a runtime-indexed C[axis] would be a load)
+    const float* const cp0 = uniform_ptr(C[0]);
+    const float* const cp1 = uniform_ptr(C[1]);
+    const float* const cp2 = uniform_ptr(C[2]);
+    int gp[6] = {-1, -1, -1, -1, -1, -1};
+    uint32_t goct = 0;
+    if (FRESH) {
+        const int4* gr = reinterpret_cast<const int4*>(tp.ghalo + size_t(t) * 8);
+        const int4 ga = gr[0], gb = gr[1];
+        gp[0] = ga.x; gp[1] = ga.y; gp[2] = ga.z; gp[3] = ga.w; gp[4] = gb.x; gp[5] = gb.y;
+        goct = uint32_t(gb.z);
+    }
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         const int v = tid + c * NT;
@@ -99,13 +139,16 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     }
     T hraw[HPT];
     float hcp[HPT];
-    bool hghost[HPT];
+    bool hghost[HPT], hmask[HPT];
+    uint8_t hct[HPT];
 #pragma unroll
     for (int m = 0; m < HPT; ++m) {
         const int k = tid + m * NT;
         hraw[m] = T(0);
         hcp[m] = h;
         hghost[m] = false;
+        hmask[m] = false;
+        hct[m] = 0;
         if (k >= 384) continue;
         const int f = k >> 6, fi = k & 63;
         const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
@@ -114,18 +157,20 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (axis == 0) { nl[0] = sgn < 0 ? 7 : 0; nl[1] = p; nl[2] = q; }
         else if (axis == 1) { nl[0] = p; nl[1] = sgn < 0 ? 7 : 0; nl[2] = q; }
         else { nl[0] = p; nl[1] = q; nl[2] = sgn < 0 ? 7 : 0; }
-        const int nt = tp.nbr[size_t(t) * 6 + (sgn < 0 ? axis : axis + 3)];
+        const int slot = sgn < 0 ? axis : axis + 3;
+        const int nt = pick6(nb, slot);
         if (fi == 0) s.empty[f] = nt < 0;
         if (nt < 0) continue;
         const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
-        const uint8_t nty = tp.ttype[nt];
+        const uint32_t nty = (slot < 4 ? ty03 >> (8 * slot) : ty45 >> (8 * (slot - 4))) & 0xffu;
         // only a + face needs the neighbour's coefficient (a - face's is the cell's own)
-        if (sgn > 0) hcp[m] = C[axis][ni];
+        const float* cax = axis == 0 ? cp0 : axis == 1 ? cp1 : cp2;
+        if (sgn > 0) hcp[m] = cax[ni];
         if (nty & GHOST) {
             hghost[m] = true;
             if (FRESH) {
-                const int pt = tp.parent[nt];
-                const int oc = tp.toct[nt];
+                const int pt = pick6(gp, slot);
+                const int oc = int((goct >> (3 * slot)) & 7u);
                 hraw[m] = pt < 0 ? T(0)
                                  : X[size_t(pt) * 512 + cidx(4 * (oc & 1) + (nl[0] >> 1), 4 * ((oc >> 1) & 1) + (nl[1] >> 1),
                                                               4 * ((oc >> 2) & 1) + (nl[2] >> 1))];
@@ -152,8 +197,10 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
                 }
             }
         } else {
-            const T xv = X[ni];
-            hraw[m] = tp.ctype[ni] == INTERIOR ? xv : T(0);
+            // masked to INTERIOR after the barrier, so that no load waits here
+            hraw[m] = X[ni];
+            hct[m] = tp.ctype[ni];
+            hmask[m] = true;
         }
     }
 #pragma unroll
@@ -176,7 +223,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (axis == 0) { fl[0] = axk; fl[1] = p; fl[2] = q; }
         else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
         else { fl[0] = p; fl[1] = q; fl[2] = axk; }
-        T hv = hraw[m];
+        T hv = (hmask[m] && hct[m] != INTERIOR) ? T(0) : hraw[m];
         if (hghost[m]) {
             int cl[3] = {fl[0], fl[1], fl[2]};
             cl[axis] -= sgn;
@@ -636,6 +683,12 @@ __global__ __launch_bounds__(256) void jacobi_zero_v(TopoView tp, const int32_t*
     }
 }
 
+// flat float4 copy (the HBM copy ceiling of the measurement tools)
+__global__ __launch_bounds__(256) void copy_flat(const float4* __restrict__ A, float4* __restrict__ B, size_t n4)
+{
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n4; i += size_t(gridDim.x) * 256) B[i] = A[i];
+}
+
 __global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
                                               float* __restrict__ B, const double* __restrict__ S)
 {
@@ -700,8 +753,8 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __r
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) {
         if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
@@ -763,8 +816,8 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     Own<float, NT> o;
     float b[512 / NT];
@@ -805,8 +858,8 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
 {
     __shared__ Lds<float> s;
     __shared__ double red[DOT ? 8 : 1];
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) {
         if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
@@ -840,8 +893,8 @@ __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32
 {
     __shared__ Lds<float> s;
     __shared__ float res[512];
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     Own<float, NT> o;
     float b[512 / NT];
@@ -914,8 +967,8 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __res
                                                const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     Own<float, NT> o;
     load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
@@ -939,8 +992,8 @@ __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __res
                                                const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     Own<float, NT> o;
     load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
@@ -1000,8 +1053,8 @@ __global__ __launch_bounds__(512) void sum_count(TopoView tp, const int32_t* __r
                                                  const double* __restrict__ S)
 {
     __shared__ double red[8];
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     double v = 0, c = 0;
     if (t >= 0) {
         const size_t idx = size_t(t) * 512 + threadIdx.x;
@@ -1016,8 +1069,8 @@ __global__ __launch_bounds__(512) void sum_count(TopoView tp, const int32_t* __r
 __global__ __launch_bounds__(512) void sub_mean(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
                                                 const double* __restrict__ S)
 {
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     const size_t idx = size_t(t) * 512 + threadIdx.x;
     if (tp.ctype[idx] & INTERIOR) X[idx] = float(double(X[idx]) - S[S_MEAN]);
@@ -1217,8 +1270,8 @@ __global__ __launch_bounds__(128) void power_sum(TopoView tp, const int32_t* __r
 __global__ __launch_bounds__(64) void restrict_source(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
                                                       float coeff, const double* __restrict__ S)
 {
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     const int pt = tp.parent[t];
     if (pt < 0) return;
@@ -1281,8 +1334,8 @@ __global__ __launch_bounds__(512) void deduct_flux(TopoView tp, const int32_t* _
                                                    Coef C, const float* __restrict__ F, float* __restrict__ RHS,
                                                    const double* __restrict__ S)
 {
+    const int t = list[blockIdx.x];   // issued together with the stop-flag load
     if (stopped(S)) return;
-    const int t = list[blockIdx.x];
     if (t < 0) return;
     const int v = threadIdx.x;
     const size_t idx = size_t(t) * 512 + v;
@@ -1581,6 +1634,12 @@ void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float val
 {
     if (!nonempty(l)) return;
     k::fill_v<<<vector_grid(d, l.n), 256, 0, st>>>(l.d, l.n, X, value, S);
+}
+
+void launch_copy_flat(const DevGrid& d, const float* A, float* B, size_t n, hipStream_t st)
+{
+    if (n % 4) throw std::invalid_argument("copy_flat: length must be a multiple of 4");
+    k::copy_flat<<<d.wg_cap, 256, 0, st>>>(reinterpret_cast<const float4*>(A), reinterpret_cast<float4*>(B), n / 4);
 }
 
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st)

@@ -14,6 +14,8 @@ struct TopoView {
     const int32_t* parent;
     const int32_t* child;
     const uint8_t* ctype;
+    const int32_t* halo;    // 8 per tile: face neighbours, packed neighbour tile types
+    const int32_t* ghalo;   // 8 per tile: parents of GHOST face neighbours, packed octants
     float h0;
     int derive;   // LEAF-tile diagonals are derived from the face coefficients (verified equal)
 };
@@ -25,7 +27,7 @@ struct Coef {
 
 inline TopoView view(const DevGrid& d)
 {
-    return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.h0, d.derive};
+    return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.halo, d.ghalo, d.h0, d.derive};
 }
 
 enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5, FIN_BETA_FLEX = 6 };
@@ -49,6 +51,7 @@ void launch_accumulate_leafs(const DevGrid& d, float* X, hipStream_t st);
 void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, hipStream_t st);
 void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st,
                       const double* S = nullptr);
+void launch_copy_flat(const DevGrid& d, const float* A, float* B, size_t n, hipStream_t st);
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
 int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const float* B, double* partial, const double* S,
                hipStream_t st);

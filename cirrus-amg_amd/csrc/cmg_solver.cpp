@@ -136,6 +136,7 @@ float* dev_channel(DevGrid& d, int c)
 void dev_free(DevGrid& d)
 {
     for (void* p : {(void*)d.ttype, (void*)d.toct, (void*)d.tlevel, (void*)d.nbr, (void*)d.parent, (void*)d.child,
+                    (void*)d.halo, (void*)d.ghalo,
                     (void*)d.ctype, (void*)d.partial, (void*)d.scal})
         if (p) (void)hipFree(p);
     for (auto& c : d.ch)
@@ -176,6 +177,29 @@ void dev_upload(DevGrid& d, const Topology& topo)
     up(d.toct, oct);
     up(d.tlevel, topo.t_level);
     up(d.nbr, topo.nbr);
+    {
+        // halo records: one 32-byte read gives a stencil workgroup everything it
+        // needs to address its halo (no dependent per-face lookups)
+        std::vector<int32_t> halo(size_t(N) * 8, 0), ghalo(size_t(N) * 8, 0);
+        for (int t = 0; t < N; ++t) {
+            uint32_t ty03 = 0, ty45 = 0, goct = 0;
+            for (int f = 0; f < 6; ++f) {
+                const int nb = topo.nbr[size_t(t) * 6 + f];
+                const uint32_t ty = nb >= 0 ? topo.t_type[nb] : 0u;
+                halo[size_t(t) * 8 + f] = nb;
+                if (f < 4) ty03 |= ty << (8 * f);
+                else ty45 |= ty << (8 * (f - 4));
+                const bool gh = nb >= 0 && (ty & GHOST);
+                ghalo[size_t(t) * 8 + f] = gh ? topo.parent[nb] : -1;
+                if (gh) goct |= uint32_t(oct[nb] & 7) << (3 * f);
+            }
+            halo[size_t(t) * 8 + 6] = int32_t(ty03);
+            halo[size_t(t) * 8 + 7] = int32_t(ty45 | (uint32_t(topo.t_type[t]) << 24));
+            ghalo[size_t(t) * 8 + 6] = int32_t(goct);
+        }
+        up(d.halo, halo);
+        up(d.ghalo, ghalo);
+    }
     up(d.parent, topo.parent);
     up(d.child, topo.child);
     d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
@@ -1627,7 +1651,7 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
-    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 9 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 10 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     if (kernel == 9 && !s->net_w) return CMG_ERR_STATE;
     CMG_TRY({
         ensure_buffers(s);
@@ -1644,6 +1668,11 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
         HIPCHK(hipMemcpy(s->S, h, sizeof(h), hipMemcpyHostToDevice));
         const double* S = s->S;
         int nt = 0;
+        // kernel 10: flat copy of 1 GiB into another 1 GiB (4x the 256 MB last-level cache)
+        const size_t nflat = kernel == 10 ? (size_t(1) << 28) : 0;
+        float* fa = nflat ? dmalloc<float>(nflat) : nullptr;
+        float* fb = nflat ? dmalloc<float>(nflat) : nullptr;
+        if (nflat) HIPCHK(hipMemset(fa, 0, nflat * sizeof(float)));
         auto one = [&]() {
             switch (kernel) {
             case 0: launch_apply(d, Pp, C, AP, s->partial, S, st); nt = d.leafs.tiles; break;
@@ -1656,6 +1685,7 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
             case 7: launch_jacobi_zero(d, L, s->tmp, C.c[3], R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
             case 8: launch_copy_list(d, d.leafs, Z, s->tmp, S, st); nt = d.leafs.tiles; break;
             case 9: launch_net_infer(d, d.subtree[L], R, C.c[3], s->net_w, s->tmp, S, st); nt = d.subtree[L].tiles; break;
+            case 10: launch_copy_flat(d, fa, fb, nflat, st); nt = int(nflat / 512); break;
             }
         };
         one();
@@ -1667,6 +1697,8 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
         HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
         *avg_ms = double(ms) / reps;
         if (tiles) *tiles = nt;
+        if (fa) (void)hipFree(fa);
+        if (fb) (void)hipFree(fb);
     });
     return CMG_OK;
 }

@@ -316,7 +316,9 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * finest level, 5 = PCG p update, 6 = r.z partials, 7 = first Jacobi sweep from
  * zero on the finest level, 8 = copy of one vector on LEAF tiles, 9 = learned
  * smoother inference on the finest level (needs cmg_solver_set_net; CMG_ERR_STATE
- * otherwise).  Overwrites the solver's work channels (0-4) and scalars.  No reference counterpart (the reference times whole solves
+ * otherwise), 10 = flat copy of 1 GiB into a second 1 GiB buffer (the on-box HBM
+ * copy ceiling; allocates both buffers for the call; tiles = 2^28 / 512).
+ * Overwrites the solver's work channels (0-4) and scalars.  No reference counterpart (the reference times whole solves
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 
