@@ -331,17 +331,81 @@ __global__ __launch_bounds__(256) void adam(float* __restrict__ W, float* __rest
 //            y[p] = sum over taps of U[tap][p + off(tap)] in a fixed order.
 // Activations are rounded to bfloat16 after each ReLU, products accumulate in
 // FP32 (the same rounding points as the training forward pass in BF16 mode).
-// LDS (68,032 B, two workgroups per CU):
+// The input of the next tile (r / D on 512 own and 384 face-halo cells, 3.5
+// cells per thread) is loaded into registers while the current tile is
+// computed; the neighbours come from the tile's 32-byte halo record.
+// LDS (71,488 B, two workgroups per CU):
 //   [0, 32000)        a1[1000 positions][16 ch] bf16   | U[9][1000] fp32
-//   [32000, 35456)    x on the zero-padded 12^3 volume  | (aliases these two
+//   [0, 36000)                                          | (aliases a1)
 //   [36000, 36032)    32 zero bytes (out-of-volume taps of layer 2)
 //   [36032, 68032)    a2[1000 positions][16 ch] bf16
+//   [68032, 71488)    x on the zero-padded 12^3 volume, bf16 (edges and
+//                     corners of the 10^3 volume and the padding stay 0)
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int IL_A1 = 0, IL_X = 32000, IL_ZERO = 36000, IL_A2 = 36032, IL_BYTES = 68032;
+constexpr int IL_A1 = 0, IL_ZERO = 36000, IL_A2 = 36032, IL_X = 68032, IL_BYTES = 71488;
+constexpr int IN_PT = 4;   // input cells per thread (896 = 512 own + 384 halo, 256 threads)
+
+// registers of one tile's input: padded-volume position (-1: none), r, D, cell type
+struct InferIn {
+    int pos[IN_PT];
+    float r[IN_PT], d[IN_PT];
+    uint8_t ct[IN_PT];
+};
+
+__device__ __forceinline__ void infer_issue(InferIn& in, const TopoView& tp, int t, const float* __restrict__ R,
+                                            const float* __restrict__ D)
+{
+#pragma unroll
+    for (int m = 0; m < IN_PT; ++m) {
+        in.pos[m] = -1;
+        in.r[m] = 0.0f;
+        in.d[m] = 0.0f;
+        in.ct[m] = 0;
+    }
+    if (t < 0) return;
+    const int4* hr = reinterpret_cast<const int4*>(tp.halo + size_t(t) * 8);
+    const int4 ha = hr[0], hb = hr[1];
+    const int nb[6] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y};
+    const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
+#pragma unroll
+    for (int m = 0; m < IN_PT; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        if (e >= 896) continue;
+        int tt, l0, l1, l2, p0, p1, p2;   // source tile, its cell, padded position
+        if (e < 512) {
+            tt = t;
+            l0 = e >> 6; l1 = (e >> 3) & 7; l2 = e & 7;
+            p0 = l0 + 2; p1 = l1 + 2; p2 = l2 + 2;
+        } else {
+            const int k = e - 512, f = k >> 6, fi = k & 63, axis = f >> 1;
+            const bool neg = !(f & 1);
+            const int slot = neg ? axis : axis + 3;
+            int nt = nb[0];
+            nt = slot == 1 ? nb[1] : nt;
+            nt = slot == 2 ? nb[2] : nt;
+            nt = slot == 3 ? nb[3] : nt;
+            nt = slot == 4 ? nb[4] : nt;
+            nt = slot == 5 ? nb[5] : nt;
+            const uint32_t nty = (slot < 4 ? ty03 >> (8 * slot) : ty45 >> (8 * (slot - 4))) & 0xffu;
+            tt = (nt >= 0 && (nty & LEAF)) ? nt : -1;
+            const int a = fi >> 3, bq = fi & 7, lo = neg ? 7 : 0, po = neg ? 1 : 10;
+            if (axis == 0) { l0 = lo; l1 = a; l2 = bq; p0 = po; p1 = a + 2; p2 = bq + 2; }
+            else if (axis == 1) { l0 = a; l1 = lo; l2 = bq; p0 = a + 2; p1 = po; p2 = bq + 2; }
+            else { l0 = a; l1 = bq; l2 = lo; p0 = a + 2; p1 = bq + 2; p2 = po; }
+        }
+        in.pos[m] = (p0 * 12 + p1) * 12 + p2;
+        if (tt >= 0) {
+            const size_t idx = size_t(tt) * 512 + cidx(l0, l1, l2);
+            in.ct[m] = tp.ctype[idx];
+            in.d[m] = D[idx];
+            in.r[m] = R[idx];
+        }
+    }
+}
 constexpr int NBLK = (NV + 15) / 16;   // 63 blocks of 16 positions
 
 __device__ __forceinline__ uint32_t pack_relu_bf16(float lo, float hi)
@@ -396,39 +460,28 @@ __global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __r
 #pragma unroll
     for (int j = 0; j < 8; ++j) o1[j] = 8 * grp + j < 27 ? tap_off(8 * grp + j, 12) : 0;
     if (tid < 8) reinterpret_cast<uint32_t*>(lds + IL_ZERO)[tid] = 0u;
+    for (int e = tid; e < 1728 / 2; e += 256) reinterpret_cast<uint32_t*>(lds + IL_X)[e] = 0u;
     const uint16_t* xs = reinterpret_cast<const uint16_t*>(lds + IL_X);
     float* U = reinterpret_cast<float*>(lds);
 
+    InferIn cur;
+    infer_issue(cur, tp, blockIdx.x < n ? tiles[blockIdx.x] : -1, R, D);
+    __syncthreads();
     for (int b = blockIdx.x; b < n; b += gridDim.x) {
         const int t = tiles[b];
-        if (t < 0) continue;
         // input on the zero-padded 12^3 volume: r / D on INTERIOR cells of the
         // tile and of its same-level LEAF face neighbours, 0 elsewhere
-        for (int e = tid; e < 1728; e += 256) {
-            const int i = e / 144 - 2, j = (e / 12) % 12 - 2, kk = e % 12 - 2;
-            const bool in_vol = i >= -1 && i <= 8 && j >= -1 && j <= 8 && kk >= -1 && kk <= 8;
-            const int out_axes = (i < 0 || i > 7) + (j < 0 || j > 7) + (kk < 0 || kk > 7);
-            float val = 0.0f;
-            if (in_vol && out_axes <= 1) {
-                int tt = t, l[3] = {i, j, kk};
-                if (out_axes == 1) {
-                    const int a = (i < 0 || i > 7) ? 0 : (j < 0 || j > 7) ? 1 : 2;
-                    const bool neg = l[a] < 0;
-                    tt = tp.nbr[size_t(t) * 6 + (neg ? a : a + 3)];
-                    l[a] = neg ? 7 : 0;
-                    if (tt >= 0 && !(tp.ttype[tt] & LEAF)) tt = -1;
-                }
-                if (tt >= 0) {
-                    const size_t idx = size_t(tt) * 512 + cidx(l[0], l[1], l[2]);
-                    if (tp.ctype[idx] & INTERIOR) {
-                        const float d = D[idx];
-                        val = d != 0 ? R[idx] / d : 0.0f;
-                    }
-                }
+#pragma unroll
+        for (int m = 0; m < IN_PT; ++m)
+            if (cur.pos[m] >= 0) {
+                const float val = (cur.ct[m] & INTERIOR) && cur.d[m] != 0 ? cur.r[m] / cur.d[m] : 0.0f;
+                reinterpret_cast<uint16_t*>(lds + IL_X)[cur.pos[m]] = bf16_bits(val);
             }
-            reinterpret_cast<uint16_t*>(lds + IL_X)[e] = bf16_bits(val);
-        }
         __syncthreads();
+        // the next tile's input loads fly while this tile is computed
+        const int bn = b + int(gridDim.x);
+        infer_issue(cur, tp, bn < n ? tiles[bn] : -1, R, D);
+        if (t < 0) continue;
         // layer 1
         for (int mb = wave; mb < NBLK; mb += 4) {
             const int p = mb * 16 + col;
