@@ -216,7 +216,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
         if (ty & (LEAF | NONLEAF)) sub[l].push_back(t);
         if (ty & NONLEAF) nl[l].push_back(t);
         if (ty & LEAF) la[l].push_back(t);
-        if ((ty & LEAF) && l < topo.num_levels() - 1) leafcoarse.push_back(t);
+        if ((ty & LEAF) && l < topo.max_level()) leafcoarse.push_back(t);   // (num_levels counts hash levels)
         pr[l].push_back(t);
     }
     d.leafs = make_list(topo, leafs);
