@@ -81,15 +81,16 @@ __device__ __forceinline__ const float* uniform_ptr(const float* p)
     return reinterpret_cast<const float*>((uint64_t(hi) << 32) | lo);
 }
 
-// a[s] for a thread-varying s in 0..5 without indexing a private array
-__device__ __forceinline__ int pick6(const int (&a)[6], int s)
+// element s (0..5, thread-varying) of (a.x, a.y, a.z, a.w, b.x, b.y) by selects,
+// with no private array (which the compiler would place in LDS)
+__device__ __forceinline__ int pick6(int4 a, int4 b, int s)
 {
-    int r = a[0];
-    r = s == 1 ? a[1] : r;
-    r = s == 2 ? a[2] : r;
-    r = s == 3 ? a[3] : r;
-    r = s == 4 ? a[4] : r;
-    r = s == 5 ? a[5] : r;
+    int r = a.x;
+    r = s == 1 ? a.y : r;
+    r = s == 2 ? a.z : r;
+    r = s == 3 ? a.w : r;
+    r = s == 4 ? b.x : r;
+    r = s == 5 ? b.y : r;
     return r;
 }
 
@@ -112,21 +113,19 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     // tile's own type and (FRESH) the parents and octants of GHOST neighbours
     const int4* hr = reinterpret_cast<const int4*>(tp.halo + size_t(t) * 8);
     const int4 ha = hr[0], hb = hr[1];
-    const int nb[6] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y};
     const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
     const bool dv = tp.derive && ((ty45 >> 24) & LEAF);
     // the coefficient planes as values (a runtime-indexed C[axis] would be a load)
     const float* const cp0 = uniform_ptr(C[0]);
     const float* const cp1 = uniform_ptr(C[1]);
     const float* const cp2 = uniform_ptr(C[2]);
-    int gp[6] = {-1, -1, -1, -1, -1, -1};
-    uint32_t goct = 0;
+    int4 ga = make_int4(-1, -1, -1, -1), gb = make_int4(-1, -1, 0, 0);
     if (FRESH) {
         const int4* gr = reinterpret_cast<const int4*>(tp.ghalo + size_t(t) * 8);
-        const int4 ga = gr[0], gb = gr[1];
-        gp[0] = ga.x; gp[1] = ga.y; gp[2] = ga.z; gp[3] = ga.w; gp[4] = gb.x; gp[5] = gb.y;
-        goct = uint32_t(gb.z);
+        ga = gr[0];
+        gb = gr[1];
     }
+    const uint32_t goct = uint32_t(gb.z);
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         const int v = tid + c * NT;
@@ -139,16 +138,15 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     }
     T hraw[HPT];
     float hcp[HPT];
-    bool hghost[HPT], hmask[HPT];
-    uint8_t hct[HPT];
+    bool hghost[HPT];
+    uint32_t hct[HPT];   // cell type of a same-level neighbour cell; 0xff: no INTERIOR mask
 #pragma unroll
     for (int m = 0; m < HPT; ++m) {
         const int k = tid + m * NT;
         hraw[m] = T(0);
         hcp[m] = h;
         hghost[m] = false;
-        hmask[m] = false;
-        hct[m] = 0;
+        hct[m] = 0xffu;
         if (k >= 384) continue;
         const int f = k >> 6, fi = k & 63;
         const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
@@ -158,7 +156,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         else if (axis == 1) { nl[0] = p; nl[1] = sgn < 0 ? 7 : 0; nl[2] = q; }
         else { nl[0] = p; nl[1] = q; nl[2] = sgn < 0 ? 7 : 0; }
         const int slot = sgn < 0 ? axis : axis + 3;
-        const int nt = pick6(nb, slot);
+        const int nt = pick6(ha, hb, slot);
         if (fi == 0) s.empty[f] = nt < 0;
         if (nt < 0) continue;
         const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
@@ -169,7 +167,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (nty & GHOST) {
             hghost[m] = true;
             if (FRESH) {
-                const int pt = pick6(gp, slot);
+                const int pt = pick6(ga, gb, slot);
                 const int oc = int((goct >> (3 * slot)) & 7u);
                 hraw[m] = pt < 0 ? T(0)
                                  : X[size_t(pt) * 512 + cidx(4 * (oc & 1) + (nl[0] >> 1), 4 * ((oc >> 1) & 1) + (nl[1] >> 1),
@@ -200,7 +198,6 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
             // masked to INTERIOR after the barrier, so that no load waits here
             hraw[m] = X[ni];
             hct[m] = tp.ctype[ni];
-            hmask[m] = true;
         }
     }
 #pragma unroll
@@ -223,7 +220,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (axis == 0) { fl[0] = axk; fl[1] = p; fl[2] = q; }
         else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
         else { fl[0] = p; fl[1] = q; fl[2] = axk; }
-        T hv = (hmask[m] && hct[m] != INTERIOR) ? T(0) : hraw[m];
+        T hv = (hct[m] != 0xffu && hct[m] != INTERIOR) ? T(0) : hraw[m];
         if (hghost[m]) {
             int cl[3] = {fl[0], fl[1], fl[2]};
             cl[axis] -= sgn;
