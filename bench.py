@@ -318,6 +318,9 @@ def main():
                     its.append(it)
             leg = {"ms": float(np.mean(ms)), "iterations": its, "rel_res": rel,
                    "true_rel_res_fp64": float(solver.last_stats.true_rel_res)}
+            if o.learned:
+                judged, kept = solver.learned_counts()
+                leg["tiles_kept_net_correction"] = kept / max(judged, 1)
             if x32 is not None and "fp64_refinement" in result:
                 xl = g.channel(0).astype(np.float64)
                 leg["rel_l2_err_vs_fp64"] = float(np.linalg.norm((xl - x64)[m]) / np.linalg.norm(x64[m]))

@@ -116,8 +116,12 @@ def run_solve(solver, g, x0, opts, ref, mask):
     wall = (time.perf_counter() - t0) * 1e3
     x = g.channel(0)
     err = float(np.linalg.norm((x.astype(np.float64) - ref)[mask]) / max(np.linalg.norm(ref[mask]), 1e-300))
-    return {"ms": wall, "device_ms": float(solver.last_stats.ms), "iters": int(it), "rel_res": float(rel),
-            "true_rel_res_fp64": float(solver.last_stats.true_rel_res), "rel_l2_err_vs_fp64": err}, x
+    out = {"ms": wall, "device_ms": float(solver.last_stats.ms), "iters": int(it), "rel_res": float(rel),
+           "true_rel_res_fp64": float(solver.last_stats.true_rel_res), "rel_l2_err_vs_fp64": err}
+    if opts.learned:
+        judged, kept = solver.learned_counts()
+        out["tiles_kept_net_correction"] = kept / max(judged, 1)
+    return out, x
 
 
 def main():

@@ -79,7 +79,7 @@ EXPORTS = [
     "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
     "cmg_solver_apply", "cmg_op_norm", "cmg_solver_set_net", "cmg_solver_net_infer", "cmg_trainer_create",
     "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
-    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
+    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
 ]
 
 _lib = None
@@ -133,6 +133,7 @@ def lib():
             "cmg_trainer_set_weights": (st, [vp, vp]),
             "cmg_trainer_loss_grad": (st, [vp, i, i, i, C.POINTER(d), vp]),
             "cmg_trainer_step": (st, [vp, i, C.c_float, i, C.POINTER(d)]), "cmg_mfma_probe": (st, [i, vp, vp, vp]),
+            "cmg_solver_learned_counts": (st, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
             "cmg_divergence": (st, [vp, i, i, i]), "cmg_add_gradient": (st, [vp, i, i, i, i]),
             "cmg_mix_velocity": (st, [vp, i, i, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
@@ -436,6 +437,12 @@ class AMGSolver:
 
     def net_infer(self, r, diag, y):
         _check(lib().cmg_solver_net_infer(self.h, r, diag, y), "cmg_solver_net_infer")
+
+    def learned_counts(self):
+        """(tiles judged, tiles that kept the net's correction) over the last learned solve"""
+        j, k = C.c_uint64(), C.c_uint64()
+        _check(lib().cmg_solver_learned_counts(self.h, C.byref(j), C.byref(k)), "cmg_solver_learned_counts")
+        return j.value, k.value
 
     def bench_kernel(self, kernel, reps):
         ms, tiles = C.c_double(), C.c_int()

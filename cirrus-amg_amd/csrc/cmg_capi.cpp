@@ -429,6 +429,10 @@ cmg_status cmg_trainer_step(cmg_trainer* t, int batch, float lr, int bf16, doubl
 {
     return trainer_step(t, batch, lr, bf16, loss);
 }
+cmg_status cmg_solver_learned_counts(cmg_solver* s, uint64_t* judged, uint64_t* kept)
+{
+    return solver_learned_counts(s, judged, kept);
+}
 cmg_status cmg_mfma_probe(int k, const uint16_t* A, const uint16_t* B, float* C) { return mfma_probe(k, A, B, C); }
 
 cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)

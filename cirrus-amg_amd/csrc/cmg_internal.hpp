@@ -114,6 +114,7 @@ cmg_status solver_apply(cmg_solver* s, int x, int y);
 constexpr int kNetWeights = 7776, kNetVolume = 1000;
 cmg_status solver_set_net(cmg_solver* s, const float* w);
 cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y);
+cmg_status solver_learned_counts(cmg_solver* s, uint64_t* judged, uint64_t* kept);
 cmg_status mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C);
 cmg_status trainer_create(int capacity, uint64_t seed, cmg_trainer** out);
 void trainer_destroy(cmg_trainer* t);

@@ -809,7 +809,8 @@ __global__ __launch_bounds__(NT) void diag_check(TopoView tp, const int32_t* __r
 template <int NT>
 __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Y,
                                                      float* __restrict__ X, const float* __restrict__ Gz, Coef C,
-                                                     const float* __restrict__ B, float w, const double* __restrict__ S)
+                                                     const float* __restrict__ B, float w, const double* __restrict__ S,
+                                                     unsigned int* __restrict__ count)
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
@@ -834,7 +835,15 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
     __syncthreads();
     const double bs = block_sum(bb, red);
     __shared__ int keep;
-    if (threadIdx.x == 0) keep = (tp.ttype[t] & LEAF) && rs < bs;
+    if (threadIdx.x == 0) {
+        const bool leaf = tp.ttype[t] & LEAF;
+        keep = leaf && rs < bs;
+        // tiles judged and tiles that kept the net's correction
+        if (count && leaf) {
+            atomicAdd(&count[0], 1u);
+            if (keep) atomicAdd(&count[1], 1u);
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
@@ -1794,12 +1803,12 @@ bool fuse_first_sweep_allowed() { return !(persistent_mask() & 16); }
 bool fuse_rz_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
-                           const double* S, hipStream_t st)
+                           const double* S, hipStream_t st, unsigned int* count)
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
     // ghosts of the level are 0 at the start of a cycle: X is zero-filled there
-    k::learned_accept<256><<<l.n, 256, 0, st>>>(view(d), l.d, Y, X, X, C, B, w, S);
+    k::learned_accept<256><<<l.n, 256, 0, st>>>(view(d), l.d, Y, X, X, C, B, w, S, count);
 }
 
 }  // namespace cmg

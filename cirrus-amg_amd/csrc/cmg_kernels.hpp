@@ -119,8 +119,9 @@ void launch_net_infer(const DevGrid& d, const LaunchList& l, const float* R, con
 void launch_mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C, hipStream_t st);
 // learned pre-smoothing acceptance on the finest level: keep the net's correction
 // (in Y) on a tile when it lowers the tile's residual, else the first Jacobi sweep
+// count (optional): [0] += tiles judged, [1] += tiles that kept the net's correction
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
-                           const double* S, hipStream_t st);
+                           const double* S, hipStream_t st, unsigned int* count = nullptr);
 
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict, bit 3 stored
 // diagonals, bit 4 unfused first sweep; threads per tile)

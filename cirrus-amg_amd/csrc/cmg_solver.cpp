@@ -32,6 +32,7 @@ struct cmg_solver {
     float* tmp = nullptr;          // Jacobi ping-pong buffer
     float* scratch = nullptr;      // cycle scratch of flexible CG (keeps Ap)
     float* net_w = nullptr;        // learned tile smoother weights (7776 floats)
+    unsigned int* lcount = nullptr;   // learned steps of the last solve: tiles judged, tiles kept
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -677,7 +678,7 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero)
         // (mode 2: runs before it); a tile whose residual it does not lower takes
         // the first Jacobi sweep instead
         launch_net_infer(d, d.subtree[level], c.B, c.C.c[3], c.s->net_w, c.s->tmp, c.S, st);
-        launch_learned_accept(d, level, c.s->tmp, c.X, c.C, c.B, float(c.o->omega), c.S, st);
+        launch_learned_accept(d, level, c.s->tmp, c.X, c.C, c.B, float(c.o->omega), c.S, st, c.s->lcount);
         if (c.o->learned == 2) smooth(c, level, c.o->level_iters, 0, false);
     } else {
         smooth(c, level, c.o->level_iters, 0, x_zero);
@@ -1210,6 +1211,7 @@ void solver_destroy(cmg_solver* s)
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->net_w,
+                    (void*)s->lcount,
                     (void*)s->x64, (void*)s->b64,
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
@@ -1283,6 +1285,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         ensure_buffers(s);
         const bool amg = s->family == CMG_SOLVER_AMG;
         if (amg) ensure_fp64(s);
+        if (o->learned) HIPCHK(hipMemset(s->lcount, 0, 2 * sizeof(unsigned int)));   // counts of this solve
         if (amg && o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
             build_direct(s, o->is_pure_neumann);
         DeriveScope derive(d, amg && derive_allowed() && diagonal_derivable(s));
@@ -1421,6 +1424,10 @@ cmg_status solver_set_net(cmg_solver* s, const float* w)
     if (!s || !w) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         if (!s->net_w) s->net_w = dmalloc<float>(kNetWeights);
+        if (!s->lcount) {
+            s->lcount = dmalloc<unsigned int>(2);
+            HIPCHK(hipMemset(s->lcount, 0, 2 * sizeof(unsigned int)));
+        }
         HIPCHK(hipMemcpy(s->net_w, w, kNetWeights * sizeof(float), hipMemcpyHostToDevice));
         if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;   // the graph holds the old pointer
     });
@@ -1437,6 +1444,20 @@ cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y)
         launch_net_infer(d, d.subtree[d.max_level], dev_channel(d, r), dev_channel(d, dch), s->net_w, dev_channel(d, y),
                          nullptr, d.stream);
         HIPCHK(hipStreamSynchronize(d.stream));
+    });
+    return CMG_OK;
+}
+
+cmg_status solver_learned_counts(cmg_solver* s, uint64_t* judged, uint64_t* kept)
+{
+    if (!s || !judged || !kept) return CMG_ERR_BAD_ARG;
+    if (!s->lcount) return CMG_ERR_STATE;
+    CMG_TRY({
+        unsigned int h[2] = {0, 0};
+        HIPCHK(hipStreamSynchronize(s->g->dev.stream));
+        HIPCHK(hipMemcpy(h, s->lcount, sizeof(h), hipMemcpyDeviceToHost));
+        *judged = h[0];
+        *kept = h[1];
     });
     return CMG_OK;
 }

@@ -253,6 +253,9 @@ enum { CMG_NET_WEIGHTS = 7776, CMG_NET_VOLUME = 1000 };
 cmg_status cmg_solver_set_net(cmg_solver* s, const float* weights);
 /* y = net(r / D) on the finest-level LEAF tiles (channels r, diag, y) */
 cmg_status cmg_solver_net_infer(cmg_solver* s, int r, int diag, int y);
+/* over the last solve with opts.learned: finest LEAF tiles judged by the learned
+ * step and tiles that kept the net's correction (the rest took the Jacobi sweep) */
+cmg_status cmg_solver_learned_counts(cmg_solver* s, uint64_t* judged, uint64_t* kept);
 /* training: tiles resident on the device, FP32 master weights (seeded initial
  * values), Adam (beta 0.9 / 0.999, eps 1e-8), batches drawn from a seeded
  * per-epoch permutation; bf16 = 1 rounds weights and activations to bfloat16 in

@@ -131,5 +131,7 @@ def test_learned_fcg_solve_converges(case, mode):
                           flexible=1, learned=mode)
     it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
     assert rel < 1e-6, (it, rel)
+    judged, kept = solver.learned_counts()
+    assert judged > 0 and 0 <= kept <= judged
     x = gp.channel(0).ravel()[A.cell]
     assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)
