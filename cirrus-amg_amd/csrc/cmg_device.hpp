@@ -14,6 +14,14 @@ __device__ __forceinline__ int cidx(int i, int j, int k) { return (i << 6) | (j 
 
 __device__ __forceinline__ bool stopped(const double* S) { return S && S[S_DONE] != 0.0; }
 
+// the list entry of this workgroup, or -2 when the solve has stopped; the entry's
+// load is issued before the stop test so that the two round trips overlap
+__device__ __forceinline__ int list_tile(const int32_t* list, const double* S)
+{
+    const int t = list[blockIdx.x];
+    return stopped(S) ? -2 : t;
+}
+
 // wave64 + workgroup reduction of one double per thread; result valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* red)
 {

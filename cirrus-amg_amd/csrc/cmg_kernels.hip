@@ -750,10 +750,9 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __r
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // -1: no tile, -2: the solve has stopped
     if (t < 0) {
-        if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
+        if (DOT && t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
     Own<float, NT> o;
@@ -814,8 +813,7 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
     float b[512 / NT];
@@ -864,10 +862,9 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
 {
     __shared__ Lds<float> s;
     __shared__ double red[DOT ? 8 : 1];
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // -1: no tile, -2: the solve has stopped
     if (t < 0) {
-        if (DOT && threadIdx.x == 0) partial[blockIdx.x] = 0;
+        if (DOT && t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
     Own<float, NT> o;
@@ -899,8 +896,7 @@ __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32
 {
     __shared__ Lds<float> s;
     __shared__ float res[512];
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
     float b[512 / NT];
@@ -973,8 +969,7 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __res
                                                const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
     load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
@@ -998,8 +993,7 @@ __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __res
                                                const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
     load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
@@ -1059,7 +1053,7 @@ __global__ __launch_bounds__(512) void sum_count(TopoView tp, const int32_t* __r
                                                  const double* __restrict__ S)
 {
     __shared__ double red[8];
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
+    const int t = list[blockIdx.x];
     if (stopped(S)) return;
     double v = 0, c = 0;
     if (t >= 0) {
@@ -1075,8 +1069,7 @@ __global__ __launch_bounds__(512) void sum_count(TopoView tp, const int32_t* __r
 __global__ __launch_bounds__(512) void sub_mean(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
                                                 const double* __restrict__ S)
 {
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     const size_t idx = size_t(t) * 512 + threadIdx.x;
     if (tp.ctype[idx] & INTERIOR) X[idx] = float(double(X[idx]) - S[S_MEAN]);
@@ -1276,8 +1269,7 @@ __global__ __launch_bounds__(128) void power_sum(TopoView tp, const int32_t* __r
 __global__ __launch_bounds__(64) void restrict_source(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
                                                       float coeff, const double* __restrict__ S)
 {
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     const int pt = tp.parent[t];
     if (pt < 0) return;
@@ -1340,8 +1332,7 @@ __global__ __launch_bounds__(512) void deduct_flux(TopoView tp, const int32_t* _
                                                    Coef C, const float* __restrict__ F, float* __restrict__ RHS,
                                                    const double* __restrict__ S)
 {
-    const int t = list[blockIdx.x];   // issued together with the stop-flag load
-    if (stopped(S)) return;
+    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     const int v = threadIdx.x;
     const size_t idx = size_t(t) * 512 + v;
