@@ -332,8 +332,8 @@ __global__ __launch_bounds__(256) void adam(float* __restrict__ W, float* __rest
 // Activations are rounded to bfloat16 after each ReLU, products accumulate in
 // FP32 (the same rounding points as the training forward pass in BF16 mode).
 // The input of the next tile (r / D on 512 own and 384 face-halo cells, 3.5
-// cells per thread) is loaded into registers while the current tile is
-// computed; the neighbours come from the tile's 32-byte halo record.
+// cells per thread) is loaded into registers while the current tile's layer 3
+// is computed; the neighbours come from the tile's 32-byte halo record.
 // LDS (71,488 B, two workgroups per CU):
 //   [0, 32000)        a1[1000 positions][16 ch] bf16   | U[9][1000] fp32
 //   [0, 36000)                                          | (aliases a1)
@@ -421,7 +421,7 @@ __device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
 // offset of tap t (0..26) in a volume of edge E
 __device__ __forceinline__ int tap_off(int t, int E) { return (t / 9 - 1) * E * E + ((t / 3) % 3 - 1) * E + (t % 3 - 1); }
 
-__global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __restrict__ tiles, int n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void net_infer(TopoView tp, const int32_t* __restrict__ tiles, int n,
                                                  const float* __restrict__ R, const float* __restrict__ D,
                                                  const float* __restrict__ W, float* __restrict__ Y,
                                                  const double* __restrict__ S)
@@ -478,10 +478,11 @@ __global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __r
                 reinterpret_cast<uint16_t*>(lds + IL_X)[cur.pos[m]] = bf16_bits(val);
             }
         __syncthreads();
-        // the next tile's input loads fly while this tile is computed
         const int bn = b + int(gridDim.x);
-        infer_issue(cur, tp, bn < n ? tiles[bn] : -1, R, D);
-        if (t < 0) continue;
+        if (t < 0) {
+            infer_issue(cur, tp, bn < n ? tiles[bn] : -1, R, D);
+            continue;
+        }
         // layer 1
         for (int mb = wave; mb < NBLK; mb += 4) {
             const int p = mb * 16 + col;
@@ -501,29 +502,51 @@ __global__ __launch_bounds__(256) void net_infer(TopoView tp, const int32_t* __r
             }
         }
         __syncthreads();
-        // layer 2
-        for (int mb = wave; mb < NBLK; mb += 4) {
-            const int p = mb * 16 + col;
-            const bool ok = p < NV;
-            const int pi = p / 100, pj = (p / 10) % 10, pk = p % 10;
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int m = 0; m < 14; ++m) {
+        // layer 2: two position blocks per step (independent accumulators), the
+        // next tap pair's operands read from LDS while this pair's MFMAs run
+        for (int mb = wave; mb < NBLK; mb += 8) {
+            const int p0 = mb * 16 + col, p1 = p0 + 64;
+            const bool ok0 = p0 < NV, ok1 = p1 < NV;
+            const int pi0 = p0 / 100, pj0 = (p0 / 10) % 10, pk0 = p0 % 10;
+            const int pi1 = p1 / 100, pj1 = (p1 / 10) % 10, pk1 = p1 % 10;
+            auto operand = [&](int m, int p, bool ok, int pi, int pj, int pk) {
                 const int tap = 2 * m + (grp >> 1);
                 const int di = tap / 9 - 1, dj = (tap / 3) % 3 - 1, dk = tap % 3 - 1;
                 const bool in = ok && tap < 27 && unsigned(pi + di) < 10u && unsigned(pj + dj) < 10u && unsigned(pk + dk) < 10u;
                 const int addr = in ? IL_A1 + (p + di * 100 + dj * 10 + dk) * 32 + (grp & 1) * 16 : IL_ZERO + (grp & 1) * 16;
-                const s16x8 av = *reinterpret_cast<const s16x8*>(lds + addr);
-                acc = mfma32(w2v[m], av, acc);
+                return *reinterpret_cast<const s16x8*>(lds + addr);
+            };
+            f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+            s16x8 b0 = operand(0, p0, ok0, pi0, pj0, pk0), b1 = operand(0, p1, ok1, pi1, pj1, pk1);
+#pragma unroll
+            for (int m = 0; m < 14; ++m) {
+                s16x8 n0 = b0, n1 = b1;
+                if (m + 1 < 14) {
+                    n0 = operand(m + 1, p0, ok0, pi0, pj0, pk0);
+                    n1 = operand(m + 1, p1, ok1, pi1, pj1, pk1);
+                }
+                acc0 = mfma32(w2v[m], b0, acc0);
+                acc1 = mfma32(w2v[m], b1, acc1);
+                b0 = n0;
+                b1 = n1;
             }
-            if (ok) {
+            if (ok0) {
                 uint2 v;
-                v.x = pack_relu_bf16(acc[0], acc[1]);
-                v.y = pack_relu_bf16(acc[2], acc[3]);
-                *reinterpret_cast<uint2*>(lds + IL_A2 + p * 32 + grp * 8) = v;
+                v.x = pack_relu_bf16(acc0[0], acc0[1]);
+                v.y = pack_relu_bf16(acc0[2], acc0[3]);
+                *reinterpret_cast<uint2*>(lds + IL_A2 + p0 * 32 + grp * 8) = v;
+            }
+            if (ok1) {
+                uint2 v;
+                v.x = pack_relu_bf16(acc1[0], acc1[1]);
+                v.y = pack_relu_bf16(acc1[2], acc1[3]);
+                *reinterpret_cast<uint2*>(lds + IL_A2 + p1 * 32 + grp * 8) = v;
             }
         }
         __syncthreads();
+        // the next tile's input loads fly while layer 3 runs (issued here, after
+        // layer 2, whose register needs are the kernel's peak)
+        infer_issue(cur, tp, bn < n ? tiles[bn] : -1, R, D);
         // layer 3 in three groups of 9 taps (di = g - 1)
         float y0 = 0.0f, y1 = 0.0f;
 #pragma unroll
