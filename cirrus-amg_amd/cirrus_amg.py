@@ -79,7 +79,7 @@ EXPORTS = [
     "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
     "cmg_solver_apply", "cmg_op_norm", "cmg_solver_set_net", "cmg_solver_net_infer", "cmg_trainer_create",
     "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
-    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
+    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
 ]
 
 _lib = None
@@ -134,6 +134,8 @@ def lib():
             "cmg_trainer_loss_grad": (st, [vp, i, i, i, C.POINTER(d), vp]),
             "cmg_trainer_step": (st, [vp, i, C.c_float, i, C.POINTER(d)]), "cmg_mfma_probe": (st, [i, vp, vp, vp]),
             "cmg_solver_learned_counts": (st, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+            "cmg_grid_dump_blob": (st, [vp, C.c_uint8, i, vp, C.POINTER(C.c_size_t)]),
+            "cmg_grid_load_blob": (st, [vp, C.c_size_t, C.POINTER(vp)]),
             "cmg_divergence": (st, [vp, i, i, i]), "cmg_add_gradient": (st, [vp, i, i, i, i]),
             "cmg_mix_velocity": (st, [vp, i, i, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
@@ -219,6 +221,25 @@ class HADeviceGrid:
         return g
 
     # topology views
+    def dump_blob(self, tile_types=7, max_level=-1):
+        """HADeviceGrid::dumpBinaryBlob (src/HAGrid.h:687-758): the grid as 'HAGR' bytes"""
+        n = C.c_size_t(0)
+        _check(lib().cmg_grid_dump_blob(self.h, tile_types, max_level, None, C.byref(n)), "dump_blob")
+        buf = (C.c_uint8 * n.value)()
+        _check(lib().cmg_grid_dump_blob(self.h, tile_types, max_level, buf, C.byref(n)), "dump_blob")
+        return bytes(buf)
+
+    @classmethod
+    def load_blob(cls, blob):
+        """HADeviceGrid::loadBinaryBlob (src/HAGrid.h:760-853)"""
+        g = cls.__new__(cls)
+        g.h = C.c_void_p()
+        g.num_levels = int(np.frombuffer(blob, np.uint32, 1, 8)[0]) if len(blob) >= 40 else 0
+        g.h0 = float(np.frombuffer(blob, np.float32, 1, 24)[0]) if len(blob) >= 40 else 0.0
+        data = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+        _check(lib().cmg_grid_load_blob(data, len(blob), C.byref(g.h)), "load_blob")
+        return g
+
     @property
     def N(self):
         return lib().cmg_grid_num_tiles(self.h)

@@ -1,5 +1,8 @@
 // cmg_capi.cpp -- extern "C" boundary (include/cirrus_amg.h).
+#include <algorithm>
+#include <array>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -531,3 +534,160 @@ const char* cmg_status_string(cmg_status st)
 const char* cmg_last_error(void) { return g_last_error.c_str(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- HAGR blobs ----
+// HADeviceGrid::dumpBinaryBlob / loadBinaryBlob (src/HAGrid.h:687-853) for
+// PoissonTile<float, 15>: a 40-byte header {u32 magic 'HAGR', u32 version 1,
+// u32 num_levels, i32 max_level, u32 tile_size 44880, u32 coord_size 12, f32 h0,
+// (4 bytes padding), u64 tile_count}, num_levels u32 log2 hash sizes, then per
+// tile (all-tiles order) u32 level, u8 type, 3 x i32 tile coordinate and the raw
+// tile: float data[15][736] (cell v at [c][v]; entries 512..735 are node values,
+// 0 here), 6 neighbour records of 32 bytes (coordinate, pointer, level, type; the
+// reader ignores them), u8 cell types[512], two flags, i32 status, i32 serial.
+namespace {
+constexpr uint32_t kHagrMagic = 0x52474148u;
+constexpr size_t kHagrHeader = 40, kTileBytes = 44880, kChanStride = 736, kNbrOff = 15 * 736 * 4, kTypeOff = kNbrOff + 6 * 32;
+constexpr size_t kRecord = 4 + 1 + 12 + kTileBytes;
+static_assert(kTypeOff + 512 + 16 == kTileBytes, "PoissonTile<float, 15> layout");
+
+template <typename V>
+void put(uint8_t*& p, V v)
+{
+    std::memcpy(p, &v, sizeof(V));
+    p += sizeof(V);
+}
+template <typename V>
+V get(const uint8_t* p)
+{
+    V v;
+    std::memcpy(&v, p, sizeof(V));
+    return v;
+}
+}  // namespace
+
+extern "C" {
+
+cmg_status cmg_grid_dump_blob(cmg_grid* g, uint8_t tile_types, int max_level, uint8_t* out, size_t* size)
+{
+    if (auto st = need_final(g)) return st;
+    if (!size || !(tile_types & (CMG_LEAF | CMG_NONLEAF | CMG_GHOST))) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        const Topology& t = *g->topo;
+        const int ml = max_level < 0 ? t.max_level() : std::min(max_level, t.max_level());
+        const int N = t.num_tiles();
+        std::vector<int> sel;
+        for (int i = 0; i < N; ++i)
+            if ((t.t_type[i] & tile_types) && t.t_level[i] <= ml) sel.push_back(i);
+        const size_t total = kHagrHeader + 4 * size_t(t.num_levels()) + sel.size() * kRecord;
+        if (!out) {
+            *size = total;
+            return CMG_OK;
+        }
+        if (*size < total) throw std::invalid_argument("dump_blob: buffer smaller than the blob");
+        // channel values: the device copy where it exists, else the host assembly data
+        ensure_types(g);
+        std::array<std::vector<float>, kChannels> ch;
+        for (int c = 0; c < kChannels; ++c) {
+            if (g->uploaded && g->dev.ch[c]) {
+                ch[c].resize(size_t(N) * kCells);
+                if (hipMemcpy(ch[c].data(), g->dev.ch[c], ch[c].size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+                    throw std::runtime_error("hipMemcpy dump channel failed");
+            } else if (!g->cells.ch[c].empty()) {
+                ch[c] = g->cells.ch[c];
+            }
+        }
+        uint8_t* p = out;
+        put<uint32_t>(p, kHagrMagic);
+        put<uint32_t>(p, 1u);
+        put<uint32_t>(p, uint32_t(t.num_levels()));
+        put<int32_t>(p, ml);
+        put<uint32_t>(p, uint32_t(kTileBytes));
+        put<uint32_t>(p, 12u);
+        put<float>(p, t.h0());
+        put<uint32_t>(p, 0u);   // padding before the 8-byte count
+        put<uint64_t>(p, uint64_t(sel.size()));
+        for (int l = 0; l < t.num_levels(); ++l) put<uint32_t>(p, t.log2_hash(l));
+        for (int i : sel) {
+            put<uint32_t>(p, uint32_t(t.t_level[i]));
+            put<uint8_t>(p, t.t_type[i]);
+            for (int a = 0; a < 3; ++a) put<int32_t>(p, t.t_coord[i][a]);
+            uint8_t* tile = p;
+            std::memset(tile, 0, kTileBytes);
+            for (int c = 0; c < kChannels; ++c)
+                if (!ch[c].empty())
+                    std::memcpy(tile + size_t(c) * kChanStride * 4, ch[c].data() + size_t(i) * kCells, kCells * sizeof(float));
+            for (int f = 0; f < 6; ++f) {
+                const int nb = t.nbr[size_t(i) * 6 + f];
+                if (nb < 0) continue;
+                uint8_t* r = tile + kNbrOff + size_t(f) * 32;
+                for (int a = 0; a < 3; ++a) std::memcpy(r + 4 * a, &t.t_coord[nb][a], 4);
+                const int32_t lv = t.t_level[nb];
+                std::memcpy(r + 24, &lv, 4);
+                r[28] = t.t_type[nb];
+            }
+            std::memcpy(tile + kTypeOff, g->cells.type.data() + size_t(i) * kCells, kCells);
+            p += kTileBytes;
+        }
+        *size = total;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_load_blob(const uint8_t* data, size_t size, cmg_grid** out)
+{
+    if (!data || !out) return CMG_ERR_BAD_ARG;
+    *out = nullptr;
+    CAPI_GUARD({
+        std::unique_ptr<cmg_grid> g;
+        if (size < kHagrHeader) throw std::invalid_argument("load_blob: buffer too small");
+        if (get<uint32_t>(data) != kHagrMagic) throw std::invalid_argument("load_blob: bad magic");
+        if (get<uint32_t>(data + 4) != 1u) throw std::invalid_argument("load_blob: unsupported version");
+        const uint32_t nl = get<uint32_t>(data + 8);
+        if (get<uint32_t>(data + 16) != kTileBytes) throw std::invalid_argument("load_blob: tile size is not PoissonTile<float, 15>");
+        if (get<uint32_t>(data + 20) != 12u) throw std::invalid_argument("load_blob: coordinate size mismatch");
+        if (nl == 0 || nl > uint32_t(kMaxLevels)) throw std::invalid_argument("load_blob: num_levels invalid");
+        const float h0 = get<float>(data + 24);
+        const uint64_t count = get<uint64_t>(data + 32);
+        if (!(h0 > 0)) throw std::invalid_argument("load_blob: h0 invalid");
+        const size_t need = kHagrHeader + 4 * size_t(nl);
+        if (size < need || (size - need) / kRecord < count) throw std::invalid_argument("load_blob: buffer truncated");
+        std::vector<uint32_t> log2(nl);
+        for (uint32_t l = 0; l < nl; ++l) log2[l] = get<uint32_t>(data + kHagrHeader + 4 * l);
+        g.reset(new cmg_grid);
+        g->topo = std::make_unique<Topology>(h0, int(nl), log2.data());
+        const uint8_t* rec = data + need;
+        for (uint64_t r = 0; r < count; ++r, rec += kRecord) {
+            const uint32_t lv = get<uint32_t>(rec);
+            const uint8_t ty = rec[4];
+            if (lv >= nl || (ty != CMG_LEAF && ty != CMG_NONLEAF && ty != CMG_GHOST))
+                throw std::invalid_argument("load_blob: bad tile record");
+            g->topo->set_tile(int(lv), Coord{get<int32_t>(rec + 5), get<int32_t>(rec + 9), get<int32_t>(rec + 13)}, ty);
+        }
+        g->topo->compress();   // the blob's ghosts are tiles of their own: none is spawned
+        g->topo->finalize();
+        ensure_types(g.get());
+        const int N = g->topo->num_tiles();
+        rec = data + need;
+        std::array<bool, kChannels> used{};
+        std::array<std::vector<float>, kChannels> ch;
+        for (int c = 0; c < kChannels; ++c) ch[c].assign(size_t(N) * kCells, 0.0f);
+        for (uint64_t r = 0; r < count; ++r, rec += kRecord) {
+            const int i = g->topo->tile_at(int(get<uint32_t>(rec)), Coord{get<int32_t>(rec + 5), get<int32_t>(rec + 9), get<int32_t>(rec + 13)});
+            if (i < 0) throw std::invalid_argument("load_blob: a record's tile is missing after compression");
+            const uint8_t* tile = rec + 17;
+            std::memcpy(g->cells.type.data() + size_t(i) * kCells, tile + kTypeOff, kCells);
+            for (int c = 0; c < kChannels; ++c) {
+                float* dst = ch[c].data() + size_t(i) * kCells;
+                std::memcpy(dst, tile + size_t(c) * kChanStride * 4, kCells * sizeof(float));
+                for (int v = 0; v < kCells && !used[c]; ++v) used[c] = dst[v] != 0.0f;
+            }
+        }
+        for (int c = 0; c < kChannels; ++c)
+            if (used[c]) g->cells.ch[c] = std::move(ch[c]);
+        *out = g.release();
+    });
+    return CMG_OK;
+}
+
+}  // extern "C"
+

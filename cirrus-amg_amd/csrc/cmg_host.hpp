@@ -44,6 +44,7 @@ public:
     int num_levels() const { return num_levels_; }
     int max_level() const { return max_level_; }
     float h(int level) const { return h0_ / float(1 << level); }
+    uint32_t log2_hash(int level) const { return log2_[level]; }
 
     // HATileAccessor::hash / tileIdx (src/HAAccessor.h:217-246)
     static uint32_t hash(const Coord& c, uint32_t log2);

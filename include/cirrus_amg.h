@@ -23,6 +23,7 @@
 #ifndef CIRRUS_AMG_H
 #define CIRRUS_AMG_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -135,6 +136,21 @@ cmg_status cmg_assemble_cutcell(cmg_grid* g, int sdf_kind, const float* sdf, int
  * No reference counterpart for the generators themselves. */
 cmg_status cmg_rhs_uniform(cmg_grid* g, int channel, uint64_t seed, int zero_mean);
 cmg_status cmg_rhs_divergence(cmg_grid* g, int channel, int u0, uint64_t seed);
+
+/* ------------------------------------------------------- HAGR blobs -------- */
+/* HADeviceGrid::dumpBinaryBlob (src/HAGrid.h:687-758) for PoissonTile<float, 15>:
+ * the tiles whose type is in tile_types (CMG_LEAF | CMG_NONLEAF | CMG_GHOST) up to
+ * max_level (-1: all), in the all-tiles order, as a 'HAGR' version-1 blob; a tile
+ * record carries channels 0..14 (cells 0..511; the node entries 512..735 are 0),
+ * the cell types and the face neighbours' coordinates, levels and types.  With
+ * out == NULL, *size receives the blob's size; otherwise *size is the buffer's
+ * capacity on entry and the bytes written on return. */
+cmg_status cmg_grid_dump_blob(cmg_grid* g, uint8_t tile_types, int max_level, uint8_t* out, size_t* size);
+/* HADeviceGrid::loadBinaryBlob (src/HAGrid.h:760-853): a new finalized grid with the
+ * blob's tiles (its ghosts are tiles of the blob; none is spawned), cell types and
+ * every channel that holds a nonzero value.  CMG_ERR_BAD_ARG for a bad magic,
+ * version, tile or coordinate size, or a truncated buffer. */
+cmg_status cmg_grid_load_blob(const uint8_t* data, size_t size, cmg_grid** out);
 
 /* ------------------------------------------------------- projection -------- */
 /* Face velocities: channel u + a holds the velocity through each cell's negative
