@@ -79,7 +79,7 @@ EXPORTS = [
     "cmg_get_kernel_forms", "cmg_mg_solve", "cmg_geo_solver_create", "cmg_gmg_damped_jacobi_solve",
     "cmg_solver_apply", "cmg_op_norm", "cmg_solver_set_net", "cmg_solver_net_infer", "cmg_trainer_create",
     "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
-    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
+    "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_assemble_cutcell_device", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
 ]
 
 _lib = None
@@ -136,6 +136,7 @@ def lib():
             "cmg_solver_learned_counts": (st, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
             "cmg_grid_dump_blob": (st, [vp, C.c_uint8, i, vp, C.POINTER(C.c_size_t)]),
             "cmg_grid_load_blob": (st, [vp, C.c_size_t, C.POINTER(vp)]),
+            "cmg_assemble_cutcell_device": (st, [vp, i, vp, i, i, f, i, i]),
             "cmg_divergence": (st, [vp, i, i, i]), "cmg_add_gradient": (st, [vp, i, i, i, i]),
             "cmg_mix_velocity": (st, [vp, i, i, i, i]),
             "cmg_get_kernel_forms": (st, [C.POINTER(i), C.POINTER(i)]),
@@ -326,6 +327,13 @@ class HADeviceGrid:
                "assemble_cutcell")
 
     # kernels
+    def assemble_cutcell_device(self, sdf_kind, sdf, face_rule, coeff_channel, R_matrix, wall_chk_level=-1,
+                                dirichlet_ytop=False):
+        """the host assembly (optional wall typing, solid cells, operator) run on the device"""
+        arr = (C.c_float * len(sdf))(*sdf) if sdf is not None else None
+        _check(lib().cmg_assemble_cutcell_device(self.h, sdf_kind, arr, face_rule, coeff_channel, R_matrix,
+                                                 wall_chk_level, int(dirichlet_ytop)), "cmg_assemble_cutcell_device")
+
     def full_neg_lap(self, x, c0, y):
         _check(lib().cmg_op_full_neg_lap(self.h, x, c0, y), "full_neg_lap")
 

@@ -341,6 +341,31 @@ cmg_status cmg_assemble_cutcell(cmg_grid* g, int sdf_kind, const float* sdf, int
     return CMG_OK;
 }
 
+cmg_status cmg_assemble_cutcell_device(cmg_grid* g, int sdf_kind, const float* sdf, int face_rule, int c0, float R,
+                                      int wall_chk_level, int dirichlet_ytop)
+{
+    if (auto st = need_final(g)) return st;
+    if (c0 < 0 || c0 + 3 >= kChannels || face_rule < -1 || face_rule > 1) return CMG_ERR_BAD_ARG;
+    if (sdf && sdf_kind != CMG_SDF_SPHERE && sdf_kind != CMG_SDF_TORUS) return CMG_ERR_BAD_ARG;
+    if (!sdf && face_rule >= 0) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_types(g);
+        ensure_device(g);
+        DevGrid& d = g->dev;
+        if (!g->types_on_device) upload_types(g);
+        if (wall_chk_level >= 0) launch_assemble_walls(d, wall_chk_level, dirichlet_ytop != 0, d.stream);
+        if (sdf) launch_assemble_solid(d, sdf_kind, sdf, d.stream);
+        launch_assemble_operator(d, c0, R, face_rule, sdf_kind, sdf, d.acount, d.stream);
+        // the host copy of the types follows the device; the host coefficient data is stale
+        if (hipMemcpy(g->cells.type.data(), d.ctype, g->cells.type.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("hipMemcpy cell types failed");
+        for (int a = 0; a < 4; ++a) g->cells.ch[c0 + a].clear();
+        g->types_on_device = true;
+        g->assembled_c0 = c0;
+    });
+    return CMG_OK;
+}
+
 cmg_status cmg_rhs_uniform(cmg_grid* g, int channel, uint64_t seed, int zero_mean)
 {
     if (auto st = need_final(g)) return st;

@@ -123,6 +123,13 @@ void launch_mfma_probe(int K, const uint16_t* A, const uint16_t* B, float* C, hi
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
                            const double* S, hipStream_t st, unsigned int* count = nullptr);
 
+// device cut-cell assembly (same operations as the host assembly, bit for bit)
+void launch_assemble_solid(const DevGrid& d, int kind, const float* sdf, hipStream_t st);
+void launch_assemble_walls(const DevGrid& d, int chk, int ytop, hipStream_t st);
+// returns the number of closed cells retyped NEUMANN (synchronises the stream)
+int launch_assemble_operator(const DevGrid& d, int c0, float R, int face_rule, int kind, const float* sdf, unsigned int* count,
+                             hipStream_t st);
+
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict, bit 3 stored
 // diagonals, bit 4 unfused first sweep; threads per tile)
 bool set_kernel_forms(int mask, int threads);

@@ -137,7 +137,7 @@ float* dev_channel(DevGrid& d, int c)
 void dev_free(DevGrid& d)
 {
     for (void* p : {(void*)d.ttype, (void*)d.toct, (void*)d.tlevel, (void*)d.nbr, (void*)d.parent, (void*)d.child,
-                    (void*)d.halo, (void*)d.ghalo,
+                    (void*)d.halo, (void*)d.ghalo, (void*)d.tcoord, (void*)d.acount,
                     (void*)d.ctype, (void*)d.partial, (void*)d.scal})
         if (p) (void)hipFree(p);
     for (auto& c : d.ch)
@@ -178,6 +178,13 @@ void dev_upload(DevGrid& d, const Topology& topo)
     up(d.toct, oct);
     up(d.tlevel, topo.t_level);
     up(d.nbr, topo.nbr);
+    {
+        std::vector<int32_t> tc(size_t(N) * 3);
+        for (int t = 0; t < N; ++t)
+            for (int a = 0; a < 3; ++a) tc[size_t(t) * 3 + a] = topo.t_coord[t][a];
+        up(d.tcoord, tc);
+        d.acount = dmalloc<unsigned int>(1);
+    }
     {
         // halo records: one 32-byte read gives a stencil workgroup everything it
         // needs to address its halo (no dependent per-face lookups)

@@ -137,6 +137,18 @@ cmg_status cmg_assemble_cutcell(cmg_grid* g, int sdf_kind, const float* sdf, int
 cmg_status cmg_rhs_uniform(cmg_grid* g, int channel, uint64_t seed, int zero_mean);
 cmg_status cmg_rhs_divergence(cmg_grid* g, int channel, int u0, uint64_t seed);
 
+/* The same assembly on the device, starting from the grid's current cell types:
+ * (wall_chk_level >= 0) the wall typing of cmg_grid_set_wall_types, (sdf != NULL)
+ * solid cells as cmg_assemble_cutcell, then ghost types, faces (face_rule as
+ * cmg_assemble_cutcell; -1 also without an SDF, = cmg_prepare_types_and_coeffs),
+ * coarsening, diagonals, NONLEAF types and the closed-cell fix; bit-identical to
+ * the host assembly.  Sphere and torus SDFs.  The cell types are copied back to
+ * the host; the coefficients stay on the device.  No reference counterpart as an
+ * entry point (the reference assembles on the device in
+ * CreateAMGLaplacianSystemWithSolidCutOnNodeSDF, cirrus_cutcell/MeshCutCell.cu:416-543). */
+cmg_status cmg_assemble_cutcell_device(cmg_grid* g, int sdf_kind, const float* sdf, int face_rule, int coeff_channel,
+                                      float R_matrix, int wall_chk_level, int dirichlet_ytop);
+
 /* ------------------------------------------------------- HAGR blobs -------- */
 /* HADeviceGrid::dumpBinaryBlob (src/HAGrid.h:687-758) for PoissonTile<float, 15>:
  * the tiles whose type is in tile_types (CMG_LEAF | CMG_NONLEAF | CMG_GHOST) up to
