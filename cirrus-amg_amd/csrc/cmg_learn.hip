@@ -334,11 +334,14 @@ __global__ __launch_bounds__(256) void adam(float* __restrict__ W, float* __rest
 // The input of the next tile (r / D on 512 own and 384 face-halo cells, 3.5
 // cells per thread) is loaded into registers while the current tile's layer 3
 // is computed; the neighbours come from the tile's 32-byte halo record.
-// LDS (71,488 B, two workgroups per CU):
-//   [0, 32000)        a1[1000 positions][16 ch] bf16   | U[9][1000] fp32
+// LDS (71,488 B, two workgroups per CU).  Activations are stored by channel
+// group so that the 16 positions one MFMA operand reads are contiguous (no bank
+// conflicts): a1 as [2 halves of 8 ch][1000 positions][8 ch], a2 as
+// [4 quarters of 4 ch][1000 positions][4 ch].
+//   [0, 32000)        a1, bf16                         | U[9][1000] fp32
 //   [0, 36000)                                          | (aliases a1)
-//   [36000, 36032)    32 zero bytes (out-of-volume taps of layer 2)
-//   [36032, 68032)    a2[1000 positions][16 ch] bf16
+//   [36000, 36032)    zero bytes (out-of-volume taps of layer 2)
+//   [36032, 68032)    a2, bf16
 //   [68032, 71488)    x on the zero-padded 12^3 volume, bf16 (edges and
 //                     corners of the 10^3 volume and the padding stay 0)
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
                 uint2 v;
                 v.x = pack_relu_bf16(acc[0], acc[1]);
                 v.y = pack_relu_bf16(acc[2], acc[3]);
-                *reinterpret_cast<uint2*>(lds + IL_A1 + p * 32 + grp * 8) = v;
+                *reinterpret_cast<uint2*>(lds + IL_A1 + (grp >> 1) * 16000 + p * 16 + (grp & 1) * 8) = v;
             }
         }
         __syncthreads();
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
                 const int tap = 2 * m + (grp >> 1);
                 const int di = tap / 9 - 1, dj = (tap / 3) % 3 - 1, dk = tap % 3 - 1;
                 const bool in = ok && tap < 27 && unsigned(pi + di) < 10u && unsigned(pj + dj) < 10u && unsigned(pk + dk) < 10u;
-                const int addr = in ? IL_A1 + (p + di * 100 + dj * 10 + dk) * 32 + (grp & 1) * 16 : IL_ZERO + (grp & 1) * 16;
+                const int addr = in ? IL_A1 + (grp & 1) * 16000 + (p + di * 100 + dj * 10 + dk) * 16 : IL_ZERO;
                 return *reinterpret_cast<const s16x8*>(lds + addr);
             };
             f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -534,13 +537,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
                 uint2 v;
                 v.x = pack_relu_bf16(acc0[0], acc0[1]);
                 v.y = pack_relu_bf16(acc0[2], acc0[3]);
-                *reinterpret_cast<uint2*>(lds + IL_A2 + p0 * 32 + grp * 8) = v;
+                *reinterpret_cast<uint2*>(lds + IL_A2 + grp * 8000 + p0 * 8) = v;
             }
             if (ok1) {
                 uint2 v;
                 v.x = pack_relu_bf16(acc1[0], acc1[1]);
                 v.y = pack_relu_bf16(acc1[2], acc1[3]);
-                *reinterpret_cast<uint2*>(lds + IL_A2 + p1 * 32 + grp * 8) = v;
+                *reinterpret_cast<uint2*>(lds + IL_A2 + grp * 8000 + p1 * 8) = v;
             }
         }
         __syncthreads();
@@ -555,7 +558,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
                 const int p = mb * 16 + col;
                 const bool ok = p < NV;
                 bf16x4 av = {0, 0, 0, 0};
-                if (ok) av = *reinterpret_cast<const bf16x4*>(lds + IL_A2 + p * 32 + grp * 8);
+                if (ok) av = *reinterpret_cast<const bf16x4*>(lds + IL_A2 + grp * 8000 + p * 8);
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3v[g], av, acc, 0, 0, 0);
                 if (ok) {
