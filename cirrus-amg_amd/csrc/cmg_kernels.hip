@@ -680,10 +680,21 @@ __global__ __launch_bounds__(256) void jacobi_zero_v(TopoView tp, const int32_t*
     }
 }
 
-// flat float4 copy (the HBM copy ceiling of the measurement tools)
-__global__ __launch_bounds__(256) void copy_flat(const float4* __restrict__ A, float4* __restrict__ B, size_t n4)
+// flat 16-byte copy (the HBM copy ceiling of the measurement tools): four
+// independent loads in flight per thread, streaming (non-temporal) stores
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void copy_flat(const f32x4v* __restrict__ A, f32x4v* __restrict__ B, size_t n4)
 {
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n4; i += size_t(gridDim.x) * 256) B[i] = A[i];
+    const size_t stride = size_t(gridDim.x) * 256;
+    size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const f32x4v a0 = A[i], a1 = A[i + stride], a2 = A[i + 2 * stride], a3 = A[i + 3 * stride];
+        __builtin_nontemporal_store(a0, &B[i]);
+        __builtin_nontemporal_store(a1, &B[i + stride]);
+        __builtin_nontemporal_store(a2, &B[i + 2 * stride]);
+        __builtin_nontemporal_store(a3, &B[i + 3 * stride]);
+    }
+    for (; i < n4; i += stride) __builtin_nontemporal_store(A[i], &B[i]);
 }
 
 __global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
@@ -1636,7 +1647,7 @@ void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float val
 void launch_copy_flat(const DevGrid& d, const float* A, float* B, size_t n, hipStream_t st)
 {
     if (n % 4) throw std::invalid_argument("copy_flat: length must be a multiple of 4");
-    k::copy_flat<<<d.wg_cap, 256, 0, st>>>(reinterpret_cast<const float4*>(A), reinterpret_cast<float4*>(B), n / 4);
+    k::copy_flat<<<d.wg_cap, 256, 0, st>>>(reinterpret_cast<const k::f32x4v*>(A), reinterpret_cast<k::f32x4v*>(B), n / 4);
 }
 
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st)
