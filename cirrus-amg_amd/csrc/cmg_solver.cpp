@@ -146,6 +146,8 @@ void dev_free(DevGrid& d)
     free_list(d.all);
     free_list(d.ghosts);
     free_list(d.leafghost);
+    free_list(d.leafcoarse);
+    free_list(d.cyclezero);
     for (int l = 0; l < kMaxLevels; ++l) {
         free_list(d.subtree[l]);
         free_list(d.nonleaf[l]);
@@ -212,7 +214,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     up(d.child, topo.child);
     d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
     HIPCHK(hipMemset(d.ctype, 0, size_t(N) * kCells));
-    std::vector<int32_t> leafs, all, ghosts, leafghost, leafcoarse;
+    std::vector<int32_t> leafs, all, ghosts, leafghost, leafcoarse, cyclezero;
     std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels], la[kMaxLevels];
     for (int t = 0; t < N; ++t) {
         const uint8_t ty = topo.t_type[t];
@@ -225,6 +227,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
         if (ty & NONLEAF) nl[l].push_back(t);
         if (ty & LEAF) la[l].push_back(t);
         if ((ty & LEAF) && l < topo.max_level()) leafcoarse.push_back(t);   // (num_levels counts hash levels)
+        if ((ty & GHOST) || ((ty & LEAF) && l < topo.max_level())) cyclezero.push_back(t);
         pr[l].push_back(t);
     }
     d.leafs = make_list(topo, leafs);
@@ -232,6 +235,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.ghosts = make_list(topo, ghosts);
     d.leafghost = make_list(topo, leafghost);
     d.leafcoarse = make_list(topo, leafcoarse);
+    d.cyclezero = make_list(topo, cyclezero);
     d.max_list = std::max(d.leafs.n, d.all.n);
     for (int l = 0; l < topo.num_levels(); ++l) {
         d.subtree[l] = make_list(topo, sub[l]);
@@ -838,9 +842,16 @@ void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float*
         break;
     }
     default: {
-        // FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level
+        // FASMuCycle (src/AMGSolver.cu:1279-1292): x = 0 on every tile, then the step at the finest level.
+        // With damped Jacobi the finest LEAF tiles are written before they are read
+        // (the first sweep goes to the ping-pong buffer) and every NONLEAF tile
+        // takes its x from the restriction, so only GHOST tiles and the coarser LEAF
+        // tiles need the zero.
         const CycleCtx c{s, o, Xp, RHSp, X0p, C, S, zero_done, rz};
-        launch_fill_all(d, c.X, 0.0f, S, d.stream);
+        if (o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && d.max_level >= 1)
+            launch_fill_list(d, d.cyclezero, c.X, 0.0f, d.stream, S);
+        else
+            launch_fill_all(d, c.X, 0.0f, S, d.stream);
         cycle_step(c, d.max_level, true);
     }
     }
