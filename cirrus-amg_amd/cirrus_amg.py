@@ -80,6 +80,7 @@ EXPORTS = [
     "cmg_solver_apply", "cmg_op_norm", "cmg_solver_set_net", "cmg_solver_net_infer", "cmg_trainer_create",
     "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
     "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_assemble_cutcell_device", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
+    "cmg_grid_rebuild_device", "cmg_grid_iterative_refine_device", "cmg_grid_finalize_device",
 ]
 
 _lib = None
@@ -97,6 +98,8 @@ def lib():
             "cmg_grid_create": (st, [f, i, vp, C.POINTER(vp)]), "cmg_grid_destroy": (None, [vp]),
             "cmg_grid_set_tile": (st, [vp, i, i, i, i, C.c_uint8]), "cmg_grid_rebuild": (st, [vp]),
             "cmg_grid_iterative_refine": (st, [vp, C.POINTER(Target)]), "cmg_grid_finalize": (st, [vp]),
+            "cmg_grid_rebuild_device": (st, [vp]), "cmg_grid_finalize_device": (st, [vp]),
+            "cmg_grid_iterative_refine_device": (st, [vp, C.POINTER(Target)]),
             "cmg_grid_num_tiles": (i, [vp]), "cmg_grid_max_level": (i, [vp]),
             "cmg_grid_level_starts": (st, [vp, vp]), "cmg_grid_get_tiles": (st, [vp, vp, vp, vp]),
             "cmg_grid_get_slots": (st, [vp, vp]), "cmg_grid_get_neighbors": (st, [vp, vp]),
@@ -201,23 +204,34 @@ class HADeviceGrid:
     def setTileHost(self, level, coord, tile_type):
         _check(lib().cmg_grid_set_tile(self.h, level, *coord, tile_type), "cmg_grid_set_tile")
 
-    def rebuild(self):
-        _check(lib().cmg_grid_rebuild(self.h), "cmg_grid_rebuild")
+    # device=True runs the step's flag passes, hash insertions and table builds as kernels; the
+    # resulting grid is the host construction's, slot for slot
+    def rebuild(self, device=False):
+        if device:
+            _check(lib().cmg_grid_rebuild_device(self.h), "cmg_grid_rebuild_device")
+        else:
+            _check(lib().cmg_grid_rebuild(self.h), "cmg_grid_rebuild")
 
-    def iterativeRefine(self, target):
-        _check(lib().cmg_grid_iterative_refine(self.h, C.byref(target)), "cmg_grid_iterative_refine")
+    def iterativeRefine(self, target, device=False):
+        if device:
+            _check(lib().cmg_grid_iterative_refine_device(self.h, C.byref(target)), "cmg_grid_iterative_refine_device")
+        else:
+            _check(lib().cmg_grid_iterative_refine(self.h, C.byref(target)), "cmg_grid_iterative_refine")
 
-    def finalize(self):
-        _check(lib().cmg_grid_finalize(self.h), "cmg_grid_finalize")
+    def finalize(self, device=False):
+        if device:
+            _check(lib().cmg_grid_finalize_device(self.h), "cmg_grid_finalize_device")
+        else:
+            _check(lib().cmg_grid_finalize(self.h), "cmg_grid_finalize")
 
     @classmethod
-    def test_grid(cls, target, h0=0.125, log2_hash=DEFAULT_LOG2_HASH):
+    def test_grid(cls, target, h0=0.125, log2_hash=DEFAULT_LOG2_HASH, device=False):
         """CreateTestGrid (tests/TestGrids.cuh:144-169): one LEAF root, rebuild, refine, LEAF cell types."""
         g = cls(h0, log2_hash)
         g.setTileHost(0, (0, 0, 0), LEAF)
-        g.rebuild()
-        g.iterativeRefine(target)
-        g.finalize()
+        g.rebuild(device)
+        g.iterativeRefine(target, device)
+        g.finalize(device)
         g.set_leaf_types_preset(target)
         return g
 

@@ -157,6 +157,41 @@ cmg_status cmg_grid_finalize(cmg_grid* g)
     return CMG_OK;
 }
 
+cmg_status cmg_grid_rebuild_device(cmg_grid* g)
+{
+    if (auto st = need_topology(g)) return st;
+    CAPI_GUARD({
+        device_rebuild(*g->topo);
+        g->uploaded = false;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_iterative_refine_device(cmg_grid* g, const cmg_target* target)
+{
+    if (auto st = need_topology(g)) return st;
+    if (!target || (target->kind == CMG_TARGET_CALLBACK && !target->fn)) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        device_iterative_refine(*g->topo, *target);
+        g->uploaded = false;
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_finalize_device(cmg_grid* g)
+{
+    if (auto st = need_topology(g)) return st;
+    CAPI_GUARD({
+        device_finalize(*g->topo);
+        dev_free(g->dev);
+        g->uploaded = false;
+        g->cells = HostCells{};
+        ensure_types(g);
+        g->assembled_c0 = -1;
+    });
+    return CMG_OK;
+}
+
 int cmg_grid_num_tiles(const cmg_grid* g) { return (g && g->topo && g->topo->finalized()) ? g->topo->num_tiles() : 0; }
 int cmg_grid_max_level(const cmg_grid* g) { return (g && g->topo) ? g->topo->max_level() : -1; }
 

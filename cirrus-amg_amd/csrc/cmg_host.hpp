@@ -73,6 +73,7 @@ public:
     std::vector<int32_t> level_start;
 
 private:
+    friend class DeviceBuilder;   // device construction reads and writes the tables (cmg_build.hip)
     float h0_;
     int num_levels_;
     int max_level_ = -1;

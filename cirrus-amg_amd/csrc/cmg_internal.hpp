@@ -73,6 +73,12 @@ void dev_upload(DevGrid& d, const Topology& topo);
 void dev_free(DevGrid& d);
 float* dev_channel(DevGrid& d, int c);   // allocates (zeroed) on first use
 
+// octree construction on the device (cmg_build.hip); each leaves the host topology exactly as
+// Topology::compress + spawn_ghosts, iterative_refine and finalize would
+void device_rebuild(Topology& topo);
+void device_iterative_refine(Topology& topo, const cmg_target& t);
+void device_finalize(Topology& topo);
+
 }  // namespace cmg
 
 struct cmg_grid {

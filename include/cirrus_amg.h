@@ -84,6 +84,17 @@ cmg_status cmg_grid_iterative_refine(cmg_grid* g, const cmg_target* target);
  * (src/PoissonGrid.cu:139-152), parent/child tables, device upload */
 cmg_status cmg_grid_finalize(cmg_grid* g);
 
+/* The same three construction steps on the device, leaving the grid exactly as
+ * the host forms above do (same hash slots, tile order and tables).  Refine
+ * flags, 2:1 grading through ghost children, child and ghost insertion into the
+ * level hash tables and slot-order compression run as kernels; the reference
+ * drives these steps from the host with a device flag pass per level
+ * (refineStep, src/HAGrid.h:1088-1171; spawnGhostTiles, :856-889).  Star and
+ * callback level targets are evaluated on the host, one level at a time. */
+cmg_status cmg_grid_rebuild_device(cmg_grid* g);
+cmg_status cmg_grid_iterative_refine_device(cmg_grid* g, const cmg_target* target);
+cmg_status cmg_grid_finalize_device(cmg_grid* g);
+
 int cmg_grid_num_tiles(const cmg_grid* g);
 int cmg_grid_max_level(const cmg_grid* g);
 /* hNumTiles prefix sums: starts[l] .. starts[l+1] are the tiles of level l */
