@@ -19,7 +19,7 @@ TORUS_BANDS_C3 = {7: 0.01, 6: 0.02, 5: 0.04, 4: 0.08}
 
 def _topology(g):
     return [*g.tiles(), g.slots(), g.neighbors(), g.parents(), g.children(), g.level_starts(),
-            np.array([g.max_level()])]
+            np.array([g.max_level])]
 
 
 def _assert_same(a, b, what):
