@@ -1140,8 +1140,26 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
     __shared__ double red[16];
     __shared__ double red2[16];
     if (kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0) return;
+    // each thread adds its strided partials in index order; the loads of 8
+    // consecutive terms are issued together so that they overlap
+    constexpr int U = 8;
+    const int step = blockDim.x;
     double v = 0, v2 = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int i = threadIdx.x;
+    for (; i + (U - 1) * step < n; i += U * step) {
+        double a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a[u] = partial[i + u * step];
+            b[u] = partial2 ? partial2[i + u * step] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v += a[u];
+            if (partial2) v2 += b[u];
+        }
+    }
+    for (; i < n; i += step) {
         v += partial[i];
         if (partial2) v2 += partial2[i];
     }
@@ -1484,7 +1502,9 @@ static inline int persistent_grid(const DevGrid& d, int n, const void* kern, int
     g = std::max(8, g - g % 8);
     return std::max(1, std::min(n, g));
 }
-static inline int vector_grid(const DevGrid& d, int n) { return std::max(1, std::min((n + 1) / 2, d.wg_cap)); }
+// one two-tile step per workgroup over a full grid: on MI355X a full grid streams at
+// 6.2-6.5 TB/s where a grid-stride loop over 8 workgroups per CU reaches 4.5-4.9
+static inline int vector_grid(const DevGrid&, int n) { return std::max(1, (n + 1) / 2); }
 
 #define CMG_NT_PERSISTENT(KERNEL, N, STREAM, ...)                                                                   \
     do {                                                                                                          \
