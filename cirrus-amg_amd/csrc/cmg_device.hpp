@@ -38,10 +38,14 @@ __device__ __forceinline__ double block_sum(double v, double* red)
 __device__ __forceinline__ float tile_h(const TopoView& tp, int t) { return tp.h0 / float(1 << tp.tlevel[t]); }
 
 // Grid-stride loops over the tiles of a list, 128 threads x float4 per tile,
-// two tiles per 256-thread workgroup step.
-#define CMG_TILE_LOOP(list, n)                                                               \
-    for (int pos = blockIdx.x * 2 + (threadIdx.x >> 7); pos < (n); pos += gridDim.x * 2)   \
-        if (const int t_ = (list)[pos]; t_ >= 0)
+// two tiles per 256-thread workgroup step.  Launch lists hold Morton-consecutive
+// tiles 8 entries apart (entry b is meant for a one-tile workgroup on XCD b % 8),
+// so step w takes entries 16 (w / 8) + w % 8 and the one 8 after it: two Morton
+// neighbours of the XCD that a workgroup w runs on.
+#define CMG_TILE_LOOP(list, n)                                                                      \
+    for (int w_ = blockIdx.x; 16 * (w_ >> 3) < (n); w_ += gridDim.x)                               \
+        if (const int pos = 16 * (w_ >> 3) + (w_ & 7) + 8 * int(threadIdx.x >> 7); pos < (n))       \
+            if (const int t_ = (list)[pos]; t_ >= 0)
 
 __device__ __forceinline__ size_t quad_index(int t) { return size_t(t) * 128 + (threadIdx.x & 127); }
 

@@ -1504,7 +1504,7 @@ static inline int persistent_grid(const DevGrid& d, int n, const void* kern, int
 }
 // one two-tile step per workgroup over a full grid: on MI355X a full grid streams at
 // 6.2-6.5 TB/s where a grid-stride loop over 8 workgroups per CU reaches 4.5-4.9
-static inline int vector_grid(const DevGrid&, int n) { return std::max(1, (n + 1) / 2); }
+static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((n + 15) / 16)); }
 
 #define CMG_NT_PERSISTENT(KERNEL, N, STREAM, ...)                                                                   \
     do {                                                                                                          \
