@@ -295,6 +295,28 @@ def main():
         result["fp64_refinement"] = {"ms": t64, "outer_iters": int(st.outer_iters), "inner_iters_total": int(it64),
                                      "true_rel_res_fp64": float(st.true_rel_res), "tolerance": 1e-10}
         result["rel_l2_err_fp32_solve_vs_fp64"] = err
+    # the same solve with the cycle reading FP16 copies of the coefficients (c / h);
+    # the PCG operator stays FP32, so the tolerance and the solution are the same
+    o16 = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                         max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=1)
+    ms16, its16, dev16 = [], [], []
+    for rep in range(args.warmup + args.steps):
+        g.copy_channel(5, 1)
+        P.device_synchronize()
+        t0 = time.perf_counter()
+        it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=o16)
+        P.device_synchronize()
+        if rep >= args.warmup:
+            ms16.append((time.perf_counter() - t0) * 1e3)
+            its16.append(it)
+            dev16.append(solver.last_stats.ms)
+    leg16 = {"ms": float(np.mean(ms16)), "device_ms": float(np.mean(dev16)), "iterations": its16, "rel_res": rel,
+             "true_rel_res_fp64": float(solver.last_stats.true_rel_res),
+             "speedup_vs_fp32_cycle": ms_per_step / float(np.mean(ms16))}
+    if x32 is not None and "fp64_refinement" in result:
+        x16 = g.channel(0).astype(np.float64)
+        leg16["rel_l2_err_vs_fp64"] = float(np.linalg.norm((x16 - x64)[m]) / np.linalg.norm(x64[m]))
+    result["fp16_cycle_coefficients"] = leg16
     weights = WEIGHTS
     if args.learned and os.path.exists(weights):
         # learned-FCG: flexible CG whose V-cycle takes the learned tile smoother

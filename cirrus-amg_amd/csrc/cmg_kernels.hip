@@ -73,12 +73,28 @@ struct Own {
 
 // a pointer argument as a workgroup-uniform value (kept in SGPRs, selectable
 // without a memory access)
-__device__ __forceinline__ const float* uniform_ptr(const float* p)
+template <typename P>
+__device__ __forceinline__ const P* uniform_ptr(const P* p)
 {
     const uint64_t u = reinterpret_cast<uint64_t>(p);
     const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(u));
     const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(u >> 32));
-    return reinterpret_cast<const float*>((uint64_t(hi) << 32) | lo);
+    return reinterpret_cast<const P*>((uint64_t(hi) << 32) | lo);
+}
+
+// four consecutive FP16 coefficients (quad q of the tile-major layout), back to FP32
+__device__ __forceinline__ float4 coef_q4(const uint16_t* q, size_t quad, float h)
+{
+    const ushort4 u = reinterpret_cast<const ushort4*>(q)[quad];
+    return make_float4(float(__builtin_bit_cast(_Float16, u.x)) * h, float(__builtin_bit_cast(_Float16, u.y)) * h,
+                       float(__builtin_bit_cast(_Float16, u.z)) * h, float(__builtin_bit_cast(_Float16, u.w)) * h);
+}
+
+// a coefficient stored as FP16 bits of c / h, back to FP32 (h is a power of two
+// times h0, so the product restores the scale exactly)
+__device__ __forceinline__ float coef_q(const uint16_t* q, size_t i, float h)
+{
+    return float(__builtin_bit_cast(_Float16, q[i])) * h;
 }
 
 // element s (0..5, thread-varying) of (a.x, a.y, a.z, a.w, b.x, b.y) by selects,
@@ -99,11 +115,13 @@ __device__ __forceinline__ int pick6(int4 a, int4 b, int s)
 //   after PropagateToChildren + AccumulateToParentsOneStep: a GHOST cell is its
 //   parent cell and a NONLEAF cell is the mean of its 8 children.
 // !FRESH: ghost values come from G, NONLEAF values from X (cycle state).
+// Q: the face coefficients and diagonals come from the FP16 copies Qc (c / h).
 // All global loads (own cells and the 384 halo cells) are issued before the
 // first barrier; only the ghost extrapolation waits for the tile's own values.
-template <bool FRESH, typename T, int NT>
+template <bool FRESH, typename T, int NT, bool Q = false>
 __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, const T* __restrict__ X,
-                                          const T* __restrict__ G, const float* const* C, Lds<T>& s, Own<T, NT>& o)
+                                          const T* __restrict__ G, const float* const* C, Lds<T>& s, Own<T, NT>& o,
+                                          const uint16_t* const* Qc = nullptr)
 {
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
@@ -114,11 +132,14 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     const int4* hr = reinterpret_cast<const int4*>(tp.halo + size_t(t) * 8);
     const int4 ha = hr[0], hb = hr[1];
     const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
-    const bool dv = tp.derive && ((ty45 >> 24) & LEAF);
+    const bool dv = !Q && tp.derive && ((ty45 >> 24) & LEAF);   // the Q form reads every diagonal
     // the coefficient planes as values (a runtime-indexed C[axis] would be a load)
     const float* const cp0 = uniform_ptr(C[0]);
     const float* const cp1 = uniform_ptr(C[1]);
     const float* const cp2 = uniform_ptr(C[2]);
+    const uint16_t* const qp0 = Q ? uniform_ptr(Qc[0]) : nullptr;
+    const uint16_t* const qp1 = Q ? uniform_ptr(Qc[1]) : nullptr;
+    const uint16_t* const qp2 = Q ? uniform_ptr(Qc[2]) : nullptr;
     int4 ga = make_int4(-1, -1, -1, -1), gb = make_int4(-1, -1, 0, 0);
     if (FRESH) {
         const int4* gr = reinterpret_cast<const int4*>(tp.ghalo + size_t(t) * 8);
@@ -132,9 +153,16 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         o.ct[c] = tp.ctype[base + v];
         const T xr = X[base + v];
         o.x[c] = o.ct[c] == INTERIOR ? xr : T(0);
+        if constexpr (Q) {
+            o.c[c][0] = coef_q(qp0, base + v, h);
+            o.c[c][1] = coef_q(qp1, base + v, h);
+            o.c[c][2] = coef_q(qp2, base + v, h);
+            o.d[c] = dv ? 0.0f : coef_q(Qc[3], base + v, h);
+        } else {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) o.c[c][a] = C[a][base + v];
-        o.d[c] = dv ? 0.0f : C[3][base + v];
+            for (int a = 0; a < 3; ++a) o.c[c][a] = C[a][base + v];
+            o.d[c] = dv ? 0.0f : C[3][base + v];
+        }
     }
     T hraw[HPT];
     float hcp[HPT];
@@ -162,8 +190,13 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
         const uint32_t nty = (slot < 4 ? ty03 >> (8 * slot) : ty45 >> (8 * (slot - 4))) & 0xffu;
         // only a + face needs the neighbour's coefficient (a - face's is the cell's own)
-        const float* cax = axis == 0 ? cp0 : axis == 1 ? cp1 : cp2;
-        if (sgn > 0) hcp[m] = cax[ni];
+        if constexpr (Q) {
+            const uint16_t* qax = axis == 0 ? qp0 : axis == 1 ? qp1 : qp2;
+            if (sgn > 0) hcp[m] = coef_q(qax, ni, h);
+        } else {
+            const float* cax = axis == 0 ? cp0 : axis == 1 ? cp1 : cp2;
+            if (sgn > 0) hcp[m] = cax[ni];
+        }
         if (nty & GHOST) {
             hghost[m] = true;
             if (FRESH) {
@@ -558,7 +591,7 @@ __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* _
                                                    float* __restrict__ R, const float* __restrict__ P,
                                                    const float* __restrict__ AP, double* __restrict__ partial,
                                                    const double* __restrict__ S, int zlevel, float* __restrict__ Zo,
-                                                   const float* __restrict__ D, float w)
+                                                   const float* __restrict__ D, float w, const uint16_t* __restrict__ Dq)
 {
     __shared__ double red[8];
     if (stopped(S)) return;
@@ -580,7 +613,7 @@ __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* _
         reinterpret_cast<float4*>(R)[q] = r;
         acc += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z) + double(r.w) * double(r.w);
         if (Z0 && tp.tlevel[t_] == zlevel) {
-            const float4 d = reinterpret_cast<const float4*>(D)[q];
+            const float4 d = Dq ? coef_q4(Dq, q, tile_h(tp, t_)) : reinterpret_cast<const float4*>(D)[q];
             float4 z;
             z.x = 0.0f + w * ((ct.x & INTERIOR) ? (r.x - 0.0f) / d.x : 0.0f);
             z.y = 0.0f + w * ((ct.y & INTERIOR) ? (r.y - 0.0f) / d.y : 0.0f);
@@ -662,7 +695,7 @@ __global__ __launch_bounds__(256) void dot2_v(const int32_t* __restrict__ list, 
 // the general sweep; it reads b, D and the cell type only
 __global__ __launch_bounds__(256) void jacobi_zero_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ Xout,
                                                      const float* __restrict__ D, const float* __restrict__ B, float w,
-                                                     const double* __restrict__ S)
+                                                     const double* __restrict__ S, const uint16_t* __restrict__ Dq)
 {
     if (stopped(S)) return;
     CMG_TILE_LOOP(list, n)
@@ -670,7 +703,7 @@ __global__ __launch_bounds__(256) void jacobi_zero_v(TopoView tp, const int32_t*
         const size_t q = quad_index(t_);
         const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
         const float4 b = reinterpret_cast<const float4*>(B)[q];
-        const float4 d = reinterpret_cast<const float4*>(D)[q];
+        const float4 d = Dq ? coef_q4(Dq, q, tile_h(tp, t_)) : reinterpret_cast<const float4*>(D)[q];
         float4 x;
         x.x = 0.0f + w * ((ct.x & INTERIOR) ? (b.x - 0.0f) / d.x : 0.0f);
         x.y = 0.0f + w * ((ct.y & INTERIOR) ? (b.y - 0.0f) / d.y : 0.0f);
@@ -688,6 +721,31 @@ __global__ __launch_bounds__(256) void copy_flat(const f32x4v* __restrict__ A, f
 {
     const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
     if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(&A[i]), &B[i]);
+}
+
+// q[a] = FP16 bits of c[a] / h on every tile of the list; *bad counts workgroups
+// that produced a non-finite value
+__global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* __restrict__ list, Coef C,
+                                                    uint16_t* __restrict__ q0, uint16_t* __restrict__ q1,
+                                                    uint16_t* __restrict__ q2, uint16_t* __restrict__ q3,
+                                                    unsigned int* __restrict__ bad)
+{
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const float h = tile_h(tp, t);
+    bool nf = false;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const size_t i = size_t(t) * 512 + threadIdx.x + c * 256;
+        const _Float16 v0 = _Float16(C.c[0][i] / h), v1 = _Float16(C.c[1][i] / h);
+        const _Float16 v2 = _Float16(C.c[2][i] / h), v3 = _Float16(C.c[3][i] / h);
+        nf = nf || !isfinite(float(v0)) || !isfinite(float(v1)) || !isfinite(float(v2)) || !isfinite(float(v3));
+        q0[i] = __builtin_bit_cast(uint16_t, v0);
+        q1[i] = __builtin_bit_cast(uint16_t, v1);
+        q2[i] = __builtin_bit_cast(uint16_t, v2);
+        q3[i] = __builtin_bit_cast(uint16_t, v3);
+    }
+    if (__syncthreads_or(nf) && threadIdx.x == 0) atomicAdd(bad, 1u);
 }
 
 __global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
@@ -858,7 +916,7 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
 
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level;
 // DOT: also the tile's sum of b x_out (FP64, all cells) into partial[blockIdx.x]
-template <int NT, bool DOT = false>
+template <int NT, bool DOT = false, bool Q = false>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
                                              const float* __restrict__ B, float w, const double* __restrict__ S,
@@ -875,7 +933,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
     float b[512 / NT];
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o);
+    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o, C.q);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
@@ -893,7 +951,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
 }
 
 // ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
-template <int NT>
+template <int NT, bool Q = false>
 __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
                                                         Coef C, float* __restrict__ B, float Rr,
                                                         const double* __restrict__ S)
@@ -906,7 +964,7 @@ __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32
     float b[512 / NT];
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), X, X, C.c, s, o, C.q);
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
         const int v = threadIdx.x + c * NT;
@@ -967,7 +1025,7 @@ __global__ __launch_bounds__(NT) void residual64(TopoView tp, const int32_t* __r
 // ------------------------------------------------------------- smoothers ----
 // one colour of red-black SOR on subtree-LEAF tiles of a level, in place
 // (NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel, src/AMGSolver.cu:838-877)
-template <int NT>
+template <int NT, bool Q = false>
 __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X, Coef C,
                                                const float* __restrict__ B, int color, double omega,
                                                const double* __restrict__ S)
@@ -976,7 +1034,7 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __res
     const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), X, X, C.c, s, o, C.q);
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
         const int v = threadIdx.x + c * NT;
@@ -991,7 +1049,7 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __res
 
 // ------------------------------------------------------------- transfers ----
 // FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
-template <int NT>
+template <int NT, bool Q = false>
 __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
                                                Coef C, float* __restrict__ B, float* __restrict__ X0,
                                                const double* __restrict__ S)
@@ -1000,7 +1058,7 @@ __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __res
     const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
-    load_tile<false>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), X, X, C.c, s, o, C.q);
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
         const int v = threadIdx.x + c * NT;
@@ -1527,6 +1585,16 @@ static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((
         }                                                                                     \
     } while (0)
 
+// the same with the FP16-coefficient form (second template argument true)
+#define CMG_NT_DISPATCH_Q(KERNEL, GRID, STREAM, ...)                                          \
+    do {                                                                                      \
+        switch (stencil_threads()) {                                                          \
+        case 128: KERNEL<128, true><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;       \
+        case 512: KERNEL<512, true><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;       \
+        default: KERNEL<256, true><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;       \
+        }                                                                                     \
+    } while (0)
+
 template <bool DOT, int NT>
 static int apply_launch(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
 {
@@ -1575,7 +1643,8 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
+    if (C.q[0]) CMG_NT_DISPATCH_Q(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
+    else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
 }
 
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
@@ -1587,6 +1656,14 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: b.x partials need the one-tile form");
         if (!nonempty(l)) return 0;
         const TopoView tp = view(d);
+        if (C.q[0]) {
+            switch (stencil_threads()) {
+            case 128: k::jacobi<128, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
+            case 512: k::jacobi<512, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
+            default: k::jacobi<256, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
+            }
+            return l.n;
+        }
         switch (stencil_threads()) {
         case 128: k::jacobi<128, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
         case 512: k::jacobi<512, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
@@ -1595,6 +1672,16 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         return l.n;
     }
     if (!nonempty(l)) return 0;
+    if (C.q[0]) {
+        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: FP16 coefficients need the one-tile form");
+        const TopoView tp = view(d);
+        switch (stencil_threads()) {
+        case 128: k::jacobi<128, false, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
+        case 512: k::jacobi<512, false, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
+        default: k::jacobi<256, false, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
+        }
+        return 0;
+    }
     if (!(persistent_mask() & PERSIST_JACOBI))
         CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S, (double*)nullptr);
     else CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, view(d), l.d, l.n, Xin, Xout, G, C, B, w, S);
@@ -1602,17 +1689,23 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
 }
 
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
-                        hipStream_t st)
+                        hipStream_t st, const uint16_t* Dq)
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    k::jacobi_zero_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, Xout, D, B, w, S);
+    k::jacobi_zero_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, Xout, D, B, w, S, Dq);
 }
 
 void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st)
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
+    if (C.q[0]) {
+        if (persistent_mask() & PERSIST_RESTRICT)
+            throw std::invalid_argument("residual_restrict: FP16 coefficients need the one-tile form");
+        CMG_NT_DISPATCH_Q(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
+        return;
+    }
     if (!(persistent_mask() & PERSIST_RESTRICT)) CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
     else CMG_NT_PERSISTENT(k::restrict_p, l.n, st, view(d), l.d, l.n, X, C, B, Rr, S);
 }
@@ -1621,7 +1714,14 @@ void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float*
 {
     const LaunchList& l = d.nonleaf[level];
     if (!nonempty(l)) return;
-    CMG_NT_DISPATCH(k::fas_save, l.n, st, view(d), l.d, X, C, B, X0, S);
+    if (C.q[0]) CMG_NT_DISPATCH_Q(k::fas_save, l.n, st, view(d), l.d, X, C, B, X0, S);
+    else CMG_NT_DISPATCH(k::fas_save, l.n, st, view(d), l.d, X, C, B, X0, S);
+}
+
+void launch_coef_to_half(const DevGrid& d, Coef C, uint16_t* const q[4], unsigned int* bad, hipStream_t st)
+{
+    if (!nonempty(d.all)) return;
+    k::coef_to_half<<<d.all.n, 256, 0, st>>>(view(d), d.all.d, C, q[0], q[1], q[2], q[3], bad);
 }
 
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)
@@ -1691,14 +1791,17 @@ int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const flo
 }
 
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
-                     hipStream_t st, float* Z0, const float* D, float w)
+                     hipStream_t st, float* Z0, const float* D, float w, const uint16_t* Dq)
 {
     if (!S) throw std::invalid_argument("update_xr needs the solver scalars (alpha)");
     if (Z0 && !D) throw std::invalid_argument("update_xr: the fused first sweep needs the diagonal");
     if (!nonempty(d.leafs)) return 0;
     const int g = vector_grid(d, d.leafs.n);
-    if (Z0) k::update_xr_v<true><<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S, d.max_level, Z0, D, w);
-    else k::update_xr_v<false><<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S, -1, nullptr, nullptr, 0.0f);
+    if (Z0)
+        k::update_xr_v<true><<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S, d.max_level, Z0, D, w, Dq);
+    else
+        k::update_xr_v<false><<<g, 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, R, P, AP, partial, S, -1, nullptr, nullptr, 0.0f,
+                                                 nullptr);
     return g;
 }
 

@@ -20,9 +20,12 @@ struct TopoView {
     int derive;   // LEAF-tile diagonals are derived from the face coefficients (verified equal)
 };
 
-// the four operator channels c0..c0+3
+// the four operator channels c0..c0+3; q (optional): FP16 copies of c[a] / h
+// (h = the tile's spacing) that the cycle kernels (damped Jacobi, residual and
+// restriction, FAS save) read in place of c when set
 struct Coef {
     const float* c[4];
+    const uint16_t* q[4];
 };
 
 inline TopoView view(const DevGrid& d)
@@ -41,8 +44,9 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 // partial: one b.x_out sum per list entry (one-tile form); returns their number
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
                   const double* S, hipStream_t st, double* partial = nullptr);
+// Dq (optional): the diagonal as FP16 bits of D / h instead of D
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
-                        hipStream_t st);
+                        hipStream_t st, const uint16_t* Dq = nullptr);
 void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, float* B, float Rr, const double* S, hipStream_t st);
 void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float* B, float* X0, const double* S, hipStream_t st);
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st);
@@ -52,6 +56,8 @@ void launch_fill_all(const DevGrid& d, float* X, float value, const double* S, h
 void launch_fill_list(const DevGrid& d, const LaunchList& l, float* X, float value, hipStream_t st,
                       const double* S = nullptr);
 void launch_copy_flat(const DevGrid& d, const float* A, float* B, size_t n, hipStream_t st);
+// q[a] = fp16(c[a] / h) on every tile; counts non-finite results in *bad (set to 0 first)
+void launch_coef_to_half(const DevGrid& d, Coef C, uint16_t* const q[4], unsigned int* bad, hipStream_t st);
 void launch_copy_list(const DevGrid& d, const LaunchList& l, const float* A, float* B, const double* S, hipStream_t st);
 int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const float* B, double* partial, const double* S,
                hipStream_t st);
@@ -59,7 +65,8 @@ int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const floa
 int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const float* B, const float* C, double* partial,
                 double* partial2, const double* S, hipStream_t st);
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
-                     hipStream_t st, float* Z0 = nullptr, const float* D = nullptr, float w = 0.0f);
+                     hipStream_t st, float* Z0 = nullptr, const float* D = nullptr, float w = 0.0f,
+                     const uint16_t* Dq = nullptr);
 void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st);
 void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st);

@@ -33,6 +33,8 @@ struct cmg_solver {
     float* scratch = nullptr;      // cycle scratch of flexible CG (keeps Ap)
     float* net_w = nullptr;        // learned tile smoother weights (7776 floats)
     unsigned int* lcount = nullptr;   // learned steps of the last solve: tiles judged, tiles kept
+    uint16_t* q16 = nullptr;          // FP16 copies of the four coefficient channels / h (cycle_coef_fp16)
+    unsigned int* qbad = nullptr;     // workgroups whose FP16 copy has a non-finite value
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -115,7 +117,7 @@ void free_list(LaunchList& l)
 
 Coef coef(DevGrid& d, int c0)
 {
-    Coef c;
+    Coef c{};
     for (int a = 0; a < 4; ++a) c.c[a] = dev_channel(d, c0 + a);
     return c;
 }
@@ -644,7 +646,7 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
     for (int it = 0; it < iters; ++it) {
         if (it == 0 && x_zero) {
             if (!(c.zero_done && level == d.max_level))
-                launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st);
+                launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st, c.C.q[3]);
         }
         else {
             const bool last_fine = c.rz && order == 1 && level == d.max_level && it == iters - 1;
@@ -823,7 +825,12 @@ void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float*
         geo_vcycle(s, o, Xp, RHSp, X0p, S);
         return;
     }
-    const Coef C = coef(d, s->c0);
+    Coef C = coef(d, s->c0);
+    if (o->cycle_coef_fp16) {
+        if (!s->q16) throw std::logic_error("cycle_coef_fp16: coefficient copies not prepared");
+        const size_t n = size_t(d.N) * kCells;
+        for (int a = 0; a < 4; ++a) C.q[a] = s->q16 + size_t(a) * n;
+    }
     switch (o->cycle) {
     case CMG_CYCLE_VCYCLE:
     case CMG_CYCLE_MU: {
@@ -917,7 +924,8 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     const int n1 = enqueue_apply(s, P, AP, s->partial, s->S);
     launch_finish(s->partial, nullptr, n1, s->S, FIN_ALPHA, st);
     const bool fuse = first_sweep_fusable(s, o);
-    const int n2 = fuse ? launch_update_xr(d, X, R, P, AP, s->partial, s->S, st, s->tmp, coef(d, s->c0).c[3], float(o->omega))
+    const uint16_t* dq = o->cycle_coef_fp16 ? s->q16 + size_t(3) * size_t(d.N) * kCells : nullptr;
+    const int n2 = fuse ? launch_update_xr(d, X, R, P, AP, s->partial, s->S, st, s->tmp, coef(d, s->c0).c[3], float(o->omega), dq)
                         : launch_update_xr(d, X, R, P, AP, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
@@ -942,7 +950,7 @@ std::string graph_key(const cmg_solve_opts* o, int derive)
 {
     std::ostringstream k;
     k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->learned << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
-      << o->coarsest_direct << ',' << o->is_pure_neumann;
+      << o->coarsest_direct << ',' << o->is_pure_neumann << ',' << o->cycle_coef_fp16;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
     k << ',' << mask << ',' << threads << ',' << derive;
@@ -1177,7 +1185,7 @@ cmg_status solver_apply(cmg_solver* s, int x, int y)
 // GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:390-470)
 cmg_status solver_damped_jacobi(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
 {
-    if (!s || !o || !st || s->family != CMG_SOLVER_GMG || o->max_iters < 0) return CMG_ERR_BAD_ARG;
+    if (!s || !o || !st || s->family != CMG_SOLVER_GMG || o->max_iters < 0 || o->cycle_coef_fp16) return CMG_ERR_BAD_ARG;
     *st = cmg_solve_stats{};
     bool converged = false;
     CMG_TRY({
@@ -1229,7 +1237,7 @@ void solver_destroy(cmg_solver* s)
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->net_w,
-                    (void*)s->lcount,
+                    (void*)s->lcount, (void*)s->q16, (void*)s->qbad,
                     (void*)s->x64, (void*)s->b64,
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
@@ -1248,23 +1256,47 @@ static cmg_status check_opts(const cmg_solve_opts* o)
     if (o->precision != CMG_PREC_FP32 && o->precision != CMG_PREC_FP64_REFINE) return CMG_ERR_BAD_ARG;
     if (o->learned < 0 || o->learned > 2) return CMG_ERR_BAD_ARG;
     if (o->learned && (o->cycle == CMG_CYCLE_VCYCLE || o->cycle == CMG_CYCLE_MU)) return CMG_ERR_BAD_ARG;
+    if (o->cycle_coef_fp16 < 0 || o->cycle_coef_fp16 > 1) return CMG_ERR_BAD_ARG;
     return CMG_OK;
 }
+
+void prepare_coef_fp16(cmg_solver* s);
 
 cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0)
 {
     if (!s) return CMG_ERR_BAD_ARG;
     if (auto st = check_opts(o)) return st;
+    if (o->cycle_coef_fp16 && s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     if (!valid_channel(x) || !valid_channel(rhs) || !valid_channel(x0)) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         ensure_buffers(s);
         if (s->family == CMG_SOLVER_AMG && o->coarsest_direct &&
             (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
             build_direct(s, o->is_pure_neumann);
+        if (s->family == CMG_SOLVER_AMG && o->cycle_coef_fp16) prepare_coef_fp16(s);
         enqueue_cycle(s, o, x, rhs, x0, nullptr);
         HIPCHK(hipStreamSynchronize(s->g->dev.stream));
     });
     return CMG_OK;
+}
+
+// FP16 copies of the coefficient channels divided by the tile spacing, for the
+// cycle kernels; rebuilt at every solve (the channels may change between solves)
+void prepare_coef_fp16(cmg_solver* s)
+{
+    DevGrid& d = s->g->dev;
+    const size_t n = size_t(d.N) * kCells;
+    if (!s->q16) {
+        s->q16 = dmalloc<uint16_t>(4 * n);
+        s->qbad = dmalloc<unsigned int>(1);
+    }
+    uint16_t* const q[4] = {s->q16, s->q16 + n, s->q16 + 2 * n, s->q16 + 3 * n};
+    HIPCHK(hipMemsetAsync(s->qbad, 0, sizeof(unsigned int), d.stream));
+    launch_coef_to_half(d, coef(d, s->c0), q, s->qbad, d.stream);
+    unsigned int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, s->qbad, sizeof(bad), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    if (bad) throw std::invalid_argument("cycle_coef_fp16: a coefficient divided by the tile spacing is outside the FP16 range");
 }
 
 // the stencils may derive LEAF diagonals from the face coefficients (one load
@@ -1291,6 +1323,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
     if (!s || !st) return CMG_ERR_BAD_ARG;
     if (auto e = check_opts(o)) return e;
     if (s->family != CMG_SOLVER_AMG && o->precision != CMG_PREC_FP32) return CMG_ERR_BAD_ARG;
+    if (s->family != CMG_SOLVER_AMG && o->cycle_coef_fp16) return CMG_ERR_BAD_ARG;
     if (o->learned && (!s->net_w || s->family != CMG_SOLVER_AMG)) {
         set_error("learned smoothing needs the net's weights (cmg_solver_set_net) on an AMG solver");
         return CMG_ERR_STATE;
@@ -1306,6 +1339,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         if (o->learned) HIPCHK(hipMemset(s->lcount, 0, 2 * sizeof(unsigned int)));   // counts of this solve
         if (amg && o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
             build_direct(s, o->is_pure_neumann);
+        if (amg && o->cycle_coef_fp16) prepare_coef_fp16(s);
         DeriveScope derive(d, amg && derive_allowed() && diagonal_derivable(s));
         // keep b in FP64 for the true residual / refinement
         if (amg) launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
@@ -1381,6 +1415,7 @@ cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_sta
     if (!s || !st) return CMG_ERR_BAD_ARG;
     if (auto e = check_opts(o)) return e;
     if (o->cycle != CMG_CYCLE_FAS_W && o->cycle != CMG_CYCLE_FAS_V) return CMG_ERR_BAD_ARG;
+    if (o->cycle_coef_fp16) return CMG_ERR_BAD_ARG;   // a stand-alone cycle solve would converge to the FP16 operator
     if (s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     *st = cmg_solve_stats{};
     bool converged = false;

@@ -226,6 +226,12 @@ typedef struct cmg_solve_opts {
     int learned;              /* 1: the learned tile smoother replaces the finest pre-smoothing;
                                * 2: it runs before the usual finest pre-sweeps (FAS cycles;
                                * weights from cmg_solver_set_net; use with flexible) */
+    int cycle_coef_fp16;      /* 1: the cycle's smoothing sweeps, residual restrictions and FAS
+                               * saves read FP16 copies of the coefficients divided by the tile
+                               * spacing (made at each solve; LEAF diagonals are then read, not
+                               * derived); the PCG operator and the FP64 level-0 solve keep FP32
+                               * coefficients, so the solution and its tolerance are those of
+                               * the FP32 operator */
 } cmg_solve_opts;
 
 typedef struct cmg_solve_stats {
