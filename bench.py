@@ -138,7 +138,8 @@ KERNELS = {0: ("apply (y = A p on LEAF tiles + p.y partials)", 25.0),
            7: ("first Jacobi sweep from x = 0, finest level", 13.0),
            8: ("plain copy", 8.0),
            9: ("learned smoother inference, finest level (BF16 MFMA)", 8.0),
-           10: ("flat copy 1 GiB -> 1 GiB", 8.0)}
+           10: ("flat copy 1 GiB -> 1 GiB", 8.0),
+           11: ("damped-Jacobi sweep, finest level, FP16 coefficient copies", 21.0)}
 # learned smoother: 2 * 27 taps * (1*16 + 16*16 + 16*1) channel pairs * 1000 volume
 # positions per tile ("same" padding on the halo'd 10^3 volume)
 NET_FLOP_PER_TILE = 2.0 * 27 * (16 + 256 + 16) * 1000
@@ -185,6 +186,9 @@ def main():
                     help="only time one kernel (for rocprofv3 counter runs) and exit")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--learned", type=int, default=1, help="also time learned-FCG when trained weights exist")
+    ap.add_argument("--cycle-coef", choices=("fp16", "fp32"), default="fp16",
+                    help="coefficients the preconditioning cycle reads: FP16 copies of c / h, or the FP32 channels "
+                         "(the PCG operator is FP32 either way); the other form is timed as a secondary leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs the single-GPU solve only: this build has no multi-GPU path "
@@ -206,8 +210,9 @@ def main():
     dof = int((types[ty == 1] == 2).sum())
     g.copy_channel(1, 5)   # keep b resident
 
+    q16 = 1 if args.cycle_coef == "fp16" else 0
     opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                          max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1)
+                          max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=q16)
     solver = P.AMGSolver(g, 11, 0.5, 1.0, 1.0)
 
     def step():
@@ -230,11 +235,18 @@ def main():
     true_rel = solver.last_stats.true_rel_res
     x32 = g.channel(0).astype(np.float64) if not args.no_fp64 else None
 
-    # dominant kernel: damped-Jacobi sweep on the finest level (4 per PCG iteration)
-    k_ms, k_tiles = solver.bench_kernel(1, 50)
-    alg_bytes = 29.0 * 512 * k_tiles          # x_in, b, 3 off-diagonals, diagonal (4 B each), type (1 B), x_out
+    # dominant kernel: damped-Jacobi sweep on the finest level (3 per PCG iteration plus the
+    # first sweep from zero); FP32 form: x_in, b, 3 face coefficients, diagonal (4 B each),
+    # type (1 B), x_out = 29 B per cell; FP16-coefficient form: coefficients and diagonal 2 B = 21 B
+    kid = 11 if q16 else 1
+    k_ms, k_tiles = solver.bench_kernel(kid, 50)
+    alg_bytes = KERNELS[kid][1] * 512 * k_tiles
     k_gbs = alg_bytes / (k_ms * 1e-3) / 1e9
-    traffic = load_traffic("jacobi_finest_%s_pmc.txt" % args.config)
+    traffic = load_traffic(("jacobi_fp16coef_finest_%s_pmc.txt" if q16 else "jacobi_finest_%s_pmc.txt") % args.config)
+    o_ms, o_tiles = solver.bench_kernel(1 if q16 else 11, 50)
+    o_bytes = KERNELS[1 if q16 else 11][1] * 512 * o_tiles
+    other_form = {"kernel": KERNELS[1 if q16 else 11][0], "ms": o_ms, "GBps": o_bytes / (o_ms * 1e-3) / 1e9,
+                  "bytes_per_launch": o_bytes}
     apply_ms, apply_tiles = solver.bench_kernel(0, 50)
     apply_gbs = 25.0 * 512 * apply_tiles / (apply_ms * 1e-3) / 1e9
 
@@ -254,10 +266,13 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32 (f64 dots, f64 level-0 direct solve)",
+        "dtype": ("f32 (f16 cycle coefficients, f64 dots, f64 level-0 direct solve)" if q16
+                  else "f32 (f64 dots, f64 level-0 direct solve)"),
         "data": "synthetic (seeded)",
         "config": {"workload": cfg["name"], "tiles": int(g.N), "leaf_tiles": int((ty == 1).sum()), "dof": dof,
-                   "levels": int(g.max_level) + 1, "solver": "PCG + FAS V(2,2) damped Jacobi w=2/3, direct level 0",
+                   "levels": int(g.max_level) + 1,
+                   "solver": "PCG (FP32 operator) + FAS V(2,2) damped Jacobi w=2/3, direct level 0"
+                             + (", cycle on FP16 coefficient copies c/h" if q16 else ""),
                    "parallelism": "single GPU"},
         "iterations": iters,
         "rel_res": rels,
@@ -270,6 +285,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k_gbs / HBM_PEAK_GBS,
                      "traffic": traffic, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
                      "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles},
+        "jacobi_other_form": other_form,
         "apply_kernel": {"GBps": apply_gbs, "ms": apply_ms, "tiles": apply_tiles},
         "build_s": t_build,
     }
@@ -283,7 +299,8 @@ def main():
         # discrete operator against which the FP32 1e-6 solve's error is stated
         g.copy_channel(5, 1)
         o64 = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                             precision=P.PREC_FP64_REFINE, outer_tolerance=1e-10, inner_tolerance=1e-4)
+                             precision=P.PREC_FP64_REFINE, outer_tolerance=1e-10, inner_tolerance=1e-4,
+                             cycle_coef_fp16=q16)
         t64 = time.perf_counter()
         it64, rel64 = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=o64)
         P.device_synchronize()
@@ -295,10 +312,10 @@ def main():
         result["fp64_refinement"] = {"ms": t64, "outer_iters": int(st.outer_iters), "inner_iters_total": int(it64),
                                      "true_rel_res_fp64": float(st.true_rel_res), "tolerance": 1e-10}
         result["rel_l2_err_fp32_solve_vs_fp64"] = err
-    # the same solve with the cycle reading FP16 copies of the coefficients (c / h);
-    # the PCG operator stays FP32, so the tolerance and the solution are the same
+    # the same solve with the other cycle-coefficient form (the PCG operator is FP32 in both,
+    # so the tolerance and the solution are the same)
     o16 = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                         max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=1)
+                         max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=1 - q16)
     ms16, its16, dev16 = [], [], []
     for rep in range(args.warmup + args.steps):
         g.copy_channel(5, 1)
@@ -312,20 +329,22 @@ def main():
             dev16.append(solver.last_stats.ms)
     leg16 = {"ms": float(np.mean(ms16)), "device_ms": float(np.mean(dev16)), "iterations": its16, "rel_res": rel,
              "true_rel_res_fp64": float(solver.last_stats.true_rel_res),
-             "speedup_vs_fp32_cycle": ms_per_step / float(np.mean(ms16))}
+             "ms_ratio_to_headline": float(np.mean(ms16)) / ms_per_step}
     if x32 is not None and "fp64_refinement" in result:
         x16 = g.channel(0).astype(np.float64)
         leg16["rel_l2_err_vs_fp64"] = float(np.linalg.norm((x16 - x64)[m]) / np.linalg.norm(x64[m]))
-    result["fp16_cycle_coefficients"] = leg16
+    result["fp32_cycle_coefficients" if q16 else "fp16_cycle_coefficients"] = leg16
     weights = WEIGHTS
     if args.learned and os.path.exists(weights):
         # learned-FCG: flexible CG whose V-cycle takes the learned tile smoother
         # (trained on the moving-sphere scene) as the finest pre-smoothing step
         solver.set_net(np.fromfile(weights, "<f4"))
         ol = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                            max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, flexible=1, learned=1)
+                            max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, flexible=1, learned=1,
+                            cycle_coef_fp16=q16)
         of = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                            max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, flexible=1)
+                            max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, flexible=1,
+                            cycle_coef_fp16=q16)
         legs = {}
         for name, o in (("learned_fcg", ol), ("plain_fcg", of)):
             ms, its = [], []

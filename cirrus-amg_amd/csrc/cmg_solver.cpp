@@ -1725,7 +1725,7 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
-    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 10 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 11 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     if (kernel == 9 && !s->net_w) return CMG_ERR_STATE;
     CMG_TRY({
         ensure_buffers(s);
@@ -1735,6 +1735,12 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
         float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *Pp = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
               *Z = dev_channel(d, CH_Z);
         Coef C = coef(d, s->c0);
+        Coef Cq = C;   // kernel 11: the cycle's FP16 coefficient copies
+        if (kernel == 11) {
+            prepare_coef_fp16(s);
+            const size_t n = size_t(d.N) * kCells;
+            for (int a = 0; a < 4; ++a) Cq.q[a] = s->q16 + size_t(a) * n;
+        }
         // live scalars for the kernels that read alpha / beta: not stopped, alpha = beta = 1/2
         double h[S_COUNT] = {};
         h[S_ALPHA] = 0.5;
@@ -1760,6 +1766,7 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
             case 8: launch_copy_list(d, d.leafs, Z, s->tmp, S, st); nt = d.leafs.tiles; break;
             case 9: launch_net_infer(d, d.subtree[L], R, C.c[3], s->net_w, s->tmp, S, st); nt = d.subtree[L].tiles; break;
             case 10: launch_copy_flat(d, fa, fb, nflat, st); nt = int(nflat / 512); break;
+            case 11: launch_jacobi(d, L, Z, s->tmp, Z, Cq, R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
             }
         };
         one();

@@ -365,7 +365,9 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * zero on the finest level, 8 = copy of one vector on LEAF tiles, 9 = learned
  * smoother inference on the finest level (needs cmg_solver_set_net; CMG_ERR_STATE
  * otherwise), 10 = flat copy of 1 GiB into a second 1 GiB buffer (the on-box HBM
- * copy ceiling; allocates both buffers for the call; tiles = 2^28 / 512).
+ * copy ceiling; allocates both buffers for the call; tiles = 2^28 / 512),
+ * 11 = the damped-Jacobi sweep of kernel 1 reading the FP16 coefficient copies
+ * of cycle_coef_fp16 (made by the call).
  * Overwrites the solver's work channels (0-4) and scalars.  No reference counterpart (the reference times whole solves
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
