@@ -70,24 +70,36 @@ public:
     {
         check(cmg_grid_set_tile(h_.get(), level, b_ijk[0], b_ijk[1], b_ijk[2], type), "setTileHost");
     }
-    // compressHost + spawnGhostTiles (src/HAGrid.h:634-653,856-897)
-    void rebuild() { check(cmg_grid_rebuild(h_.get()), "rebuild"); }
+    // compressHost + spawnGhostTiles (src/HAGrid.h:634-653,856-897); on_device: the same
+    // steps as kernels (cmg_grid_rebuild_device), with the same result
+    void rebuild(bool on_device = false)
+    {
+        check(on_device ? cmg_grid_rebuild_device(h_.get()) : cmg_grid_rebuild(h_.get()), "rebuild");
+    }
 
     // iterativeRefine(levelTarget) with a host callback evaluated per LEAF tile
     // (src/HAGrid.h:1173-1182); finalize() freezes the topology afterwards
-    void iterativeRefine(const std::function<int(int, const std::array<int32_t, 3>&)>& level_target)
+    void iterativeRefine(const std::function<int(int, const std::array<int32_t, 3>&)>& level_target,
+                         bool on_device = false)
     {
         cmg_target t{};
         t.kind = CMG_TARGET_CALLBACK;
         t.fn = &HADeviceGrid::trampoline;
         auto fn = level_target;
         t.user = &fn;
-        check(cmg_grid_iterative_refine(h_.get(), &t), "iterativeRefine");
+        iterativeRefine(t, on_device);
     }
-    void iterativeRefine(const cmg_target& preset) { check(cmg_grid_iterative_refine(h_.get(), &preset), "iterativeRefine"); }
+    void iterativeRefine(const cmg_target& preset, bool on_device = false)
+    {
+        check(on_device ? cmg_grid_iterative_refine_device(h_.get(), &preset) : cmg_grid_iterative_refine(h_.get(), &preset),
+              "iterativeRefine");
+    }
 
     // all-tiles arrays, CalculateNeighborTiles (src/PoissonGrid.cu:139-152), parent/child links
-    void finalize() { check(cmg_grid_finalize(h_.get()), "finalize"); }
+    void finalize(bool on_device = false)
+    {
+        check(on_device ? cmg_grid_finalize_device(h_.get()) : cmg_grid_finalize(h_.get()), "finalize");
+    }
 
     int numTotalTiles() const { return cmg_grid_num_tiles(h_.get()); }
     int maxLevel() const { return cmg_grid_max_level(h_.get()); }

@@ -21,13 +21,14 @@ using namespace cirrus_amg;
         }                                                                         \
     } while (0)
 
-// 64^3 uniform grid: 8^3 tiles at level 3 under one root tile (h0 = 1/8)
-static void build_uniform(HADeviceGrid& grid)
+// 64^3 uniform grid: 8^3 tiles at level 3 under one root tile (h0 = 1/8); on_device:
+// the construction steps run as kernels
+static void build_uniform(HADeviceGrid& grid, bool on_device = false)
 {
     grid.setTileHost(0, {0, 0, 0}, LEAF);
-    grid.rebuild();
-    grid.iterativeRefine([](int level, const std::array<int32_t, 3>&) { return level < 3 ? level + 1 : 3; });
-    grid.finalize();
+    grid.rebuild(on_device);
+    grid.iterativeRefine([](int level, const std::array<int32_t, 3>&) { return level < 3 ? level + 1 : 3; }, on_device);
+    grid.finalize(on_device);
     // every cell of every tile INTERIOR; the box faces are Dirichlet through the
     // empty neighbours (src/AMGSolver.cu:301-304)
     grid.setCellTypes(std::vector<uint8_t>(size_t(grid.numTotalTiles()) * Tile::SIZE, INTERIOR));
@@ -70,6 +71,17 @@ static int gpu_part()
 {
     HADeviceGrid grid(0.125f, {16, 16, 16, 16});
     build_uniform(grid);
+    {
+        // the device construction gives the same tiles in the same order
+        HADeviceGrid dgrid(0.125f, {16, 16, 16, 16});
+        build_uniform(dgrid, true);
+        EXPECT(dgrid.numTotalTiles() == grid.numTotalTiles());
+        std::vector<int32_t> sa(size_t(grid.numTotalTiles())), sb(sa.size());
+        check(cmg_grid_get_slots(grid.handle(), sa.data()), "slots");
+        check(cmg_grid_get_slots(dgrid.handle(), sb.data()), "slots");
+        EXPECT(sa == sb);
+        EXPECT(dgrid.tileTypes() == grid.tileTypes());
+    }
     AMGSolver solver(11, 0.5, 1, 1);
     solver.prepareTypesAndCoeffs(grid);
     check(cmg_rhs_uniform(grid.handle(), Tile::b_channel, 0, 1), "rhs");
