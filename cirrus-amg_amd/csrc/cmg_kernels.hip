@@ -916,11 +916,15 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
 
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level;
 // DOT: also the tile's sum of b x_out (FP64, all cells) into partial[blockIdx.x]
-template <int NT, bool DOT = false, bool Q = false>
+// SAVE: the first sweep of a coarse level right after the restriction also does
+// that level's FAS save on its NONLEAF tiles (B += A x on INTERIOR cells, 0
+// elsewhere; X0 = x), with the A x the sweep computes anyway; the sweep then
+// uses the updated B.  Same values as fas_save followed by the plain sweep.
+template <int NT, bool DOT = false, bool Q = false, bool SAVE = false>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
                                              const float* __restrict__ B, float w, const double* __restrict__ S,
-                                             double* __restrict__ partial)
+                                             double* __restrict__ partial, float* __restrict__ X0 = nullptr)
 {
     __shared__ Lds<float> s;
     __shared__ double red[DOT ? 8 : 1];
@@ -934,11 +938,22 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
     load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o, C.q);
+    const bool save = SAVE && (tp.ttype[t] & NONLEAF);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
         const int v = threadIdx.x + c * NT;
         const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        if (save) {
+            const size_t idx = size_t(t) * 512 + v;
+            if (o.ct[c] & INTERIOR) {
+                b[c] = b[c] + ax;
+                X0[idx] = o.x[c];
+            } else {
+                b[c] = 0.0f;
+            }
+            const_cast<float*>(B)[idx] = b[c];   // B is the level's right-hand side, updated in place
+        }
         const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
         const float xo = o.x[c] + w * delta;
         Xout[size_t(t) * 512 + v] = xo;
@@ -1648,8 +1663,9 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 }
 
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial)
+                  const double* S, hipStream_t st, double* partial, float* X0)
 {
+    if (X0 && partial) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
     const LaunchList& l = d.subtree[level];
     if (partial) {
         // one partial per list entry (the one-tile form only)
@@ -1672,6 +1688,25 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         return l.n;
     }
     if (!nonempty(l)) return 0;
+    if (X0) {
+        // the first coarse-level sweep with the level's FAS save (one-tile form)
+        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: the FAS save needs the one-tile form");
+        const TopoView tp = view(d);
+        if (C.q[0]) {
+            switch (stencil_threads()) {
+            case 128: k::jacobi<128, false, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            case 512: k::jacobi<512, false, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            default: k::jacobi<256, false, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            }
+        } else {
+            switch (stencil_threads()) {
+            case 128: k::jacobi<128, false, false, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            case 512: k::jacobi<512, false, false, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            default: k::jacobi<256, false, false, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            }
+        }
+        return 0;
+    }
     if (C.q[0]) {
         if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: FP16 coefficients need the one-tile form");
         const TopoView tp = view(d);
@@ -1921,6 +1956,7 @@ int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st)
 
 bool derive_allowed() { return !(persistent_mask() & 8); }
 bool fuse_first_sweep_allowed() { return !(persistent_mask() & 16); }
+bool fuse_fas_save_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 bool fuse_rz_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,

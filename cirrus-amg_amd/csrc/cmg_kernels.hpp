@@ -42,8 +42,9 @@ void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* 
 void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
                      hipStream_t st);
 // partial: one b.x_out sum per list entry (one-tile form); returns their number
+// X0: the level's FAS save on its NONLEAF tiles in the same pass (B updated in place, x -> X0)
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial = nullptr);
+                  const double* S, hipStream_t st, double* partial = nullptr, float* X0 = nullptr);
 // Dq (optional): the diagonal as FP16 bits of D / h instead of D
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
                         hipStream_t st, const uint16_t* Dq = nullptr);
@@ -144,6 +145,7 @@ void get_kernel_forms(int* mask, int* threads);
 bool derive_allowed();             // forms bit 3 clear
 bool fuse_first_sweep_allowed();   // forms bit 4 clear
 bool fuse_rz_allowed();            // forms bit 4 clear and the one-tile Jacobi form
+bool fuse_fas_save_allowed();      // forms bit 4 clear and the one-tile Jacobi form
 
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st);

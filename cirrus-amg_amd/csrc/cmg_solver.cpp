@@ -629,8 +629,9 @@ struct CycleCtx {
     double* rz = nullptr;     // the last finest-level post-sweep writes its b.x partials here
 };
 
-// `x_zero`: every x of the cycle is still zero (first sweep at the finest level)
-void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false)
+// `x_zero`: every x of the cycle is still zero (first sweep at the finest level);
+// `fas_save`: the level's FAS save is still due and rides on the first sweep
+void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
@@ -650,7 +651,8 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
         }
         else {
             const bool last_fine = c.rz && order == 1 && level == d.max_level && it == iters - 1;
-            launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st, last_fine ? c.rz : nullptr);
+            launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st, last_fine ? c.rz : nullptr,
+                          fas_save && it == 0 ? c.X0 : nullptr);
         }
         std::swap(cur, other);
     }
@@ -678,11 +680,14 @@ void coarsest(const CycleCtx& c)
 }
 
 // FASMuCycleStep (src/AMGSolver.cu:1257-1277); x_zero: x is still 0 everywhere
-void cycle_step(const CycleCtx& c, int level, bool x_zero)
+// `fas_save`: the FAS save of this level (after the restriction from level + 1)
+// has not run yet; the first pre-sweep does it
+void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
     if (level == 0) {
+        if (fas_save) launch_fas_save(d, 0, c.X, c.C, c.B, c.X0, c.S, st);
         coarsest(c);
         return;
     }
@@ -694,12 +699,14 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero)
         launch_learned_accept(d, level, c.s->tmp, c.X, c.C, c.B, float(c.o->omega), c.S, st, c.s->lcount);
         if (c.o->learned == 2) smooth(c, level, c.o->level_iters, 0, false);
     } else {
-        smooth(c, level, c.o->level_iters, 0, x_zero);
+        smooth(c, level, c.o->level_iters, 0, x_zero, fas_save);
     }
     launch_residual_restrict(d, level, c.X, c.C, c.B, c.s->Rr, c.S, st);
-    launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
+    // damped Jacobi: the coarser level's FAS save rides on its first pre-sweep
+    const bool fused = c.o->smoother == CMG_SMOOTH_JACOBI && c.o->level_iters >= 1 && fuse_fas_save_allowed();
+    if (!fused) launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
     const int mu = cycle_mu(c.o);
-    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false);
+    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false, fused && i == 0);
     launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
     smooth(c, level, c.o->level_iters, 1);
 }
