@@ -710,10 +710,8 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false
     const bool fused = c.o->smoother == CMG_SMOOTH_JACOBI && c.o->level_iters >= 1 && fuse_fas_save_allowed();
     if (!fused) launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
     const int mu = cycle_mu(c.o);
-    // ... and the prolongation into this level rides on the coarser level's last post-sweep,
-    // except into the finest level, whose wide vector kernel is faster than a tail of 5+
-    // children per workgroup of the coarser sweep
-    const bool up = fused && level - 1 >= 1 && level < d.max_level;
+    // ... and the prolongation into this level rides on the coarser level's last post-sweep
+    const bool up = fused && level - 1 >= 1;
     for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false, fused && i == 0, up && i == mu - 1);
     if (!up) launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
     smooth(c, level, c.o->level_iters, 1, false, false, prolong_up);
