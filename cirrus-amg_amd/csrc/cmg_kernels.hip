@@ -916,21 +916,15 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
 
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level;
 // DOT: also the tile's sum of b x_out (FP64, all cells) into partial[blockIdx.x]
-// MODE 1 (JAC_SAVE): the first sweep of a coarse level right after the
-// restriction also does that level's FAS save on its NONLEAF tiles (B += A x on
-// INTERIOR cells, 0 elsewhere; X0 = x), with the A x the sweep computes anyway;
-// the sweep then uses the updated B.  Same values as fas_save, then the sweep.
-// MODE 2 (JAC_PROLONG): the last post-sweep of a coarse level also does the FAS
-// prolongation into its tile's children one level up (Xc += P (x' - X0) at the
-// parent cell; a GHOST child takes x'), from the new values staged in LDS.  Same
-// values as the sweep, then fas_prolong_v on the finer level.
-enum { JAC_PLAIN = 0, JAC_SAVE = 1, JAC_PROLONG = 2 };
-template <int NT, bool DOT = false, bool Q = false, int MODE = JAC_PLAIN>
+// SAVE: the first sweep of a coarse level right after the restriction also does
+// that level's FAS save on its NONLEAF tiles (B += A x on INTERIOR cells, 0
+// elsewhere; X0 = x), with the A x the sweep computes anyway; the sweep then
+// uses the updated B.  Same values as fas_save followed by the plain sweep.
+template <int NT, bool DOT = false, bool Q = false, bool SAVE = false>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
                                              const float* __restrict__ B, float w, const double* __restrict__ S,
-                                             double* __restrict__ partial, float* __restrict__ X0 = nullptr,
-                                             float* Xc = nullptr, float P = 0.0f)
+                                             double* __restrict__ partial, float* __restrict__ X0 = nullptr)
 {
     __shared__ Lds<float> s;
     __shared__ double red[DOT ? 8 : 1];
@@ -944,8 +938,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
     load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o, C.q);
-    const bool save = MODE == JAC_SAVE && (tp.ttype[t] & NONLEAF);
-    float xnew[512 / NT];
+    const bool save = SAVE && (tp.ttype[t] & NONLEAF);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
@@ -964,38 +957,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
         const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
         const float xo = o.x[c] + w * delta;
         Xout[size_t(t) * 512 + v] = xo;
-        xnew[c] = xo;
         if (DOT) acc += double(b[c]) * double(xo);
-    }
-    if constexpr (MODE == JAC_PROLONG) {
-        __syncthreads();   // the stencil's LDS is dead: stage x' and X0 of the tile
-#pragma unroll
-        for (int c = 0; c < 512 / NT; ++c) {
-            const int v = threadIdx.x + c * NT;
-            s.x[v] = xnew[c];
-            s.off[0][v] = X0[size_t(t) * 512 + v];
-        }
-        __syncthreads();
-        const int4* cr = reinterpret_cast<const int4*>(tp.child + size_t(t) * 8);
-        const int4 ca = cr[0], cb = cr[1];
-        const int kids[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-#pragma unroll
-        for (int oc = 0; oc < 8; ++oc) {
-            const int ct = kids[oc];
-            if (ct < 0) continue;
-            const bool ghost = tp.ttype[ct] & GHOST;
-#pragma unroll
-            for (int c = 0; c < 512 / NT; ++c) {
-                const int v = threadIdx.x + c * NT;
-                const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
-                const int pc = cidx(4 * (oc & 1) + (i >> 1), 4 * ((oc >> 1) & 1) + (j >> 1), 4 * ((oc >> 2) & 1) + (k >> 1));
-                const size_t ci = size_t(ct) * 512 + v;
-                const bool in = tp.ctype[ci] & INTERIOR;
-                const float c1 = s.x[pc], c0 = s.off[0][pc];
-                const float xv = Xc[ci];
-                Xc[ci] = ghost ? (in ? c1 : 0.0f) : (in ? xv + P * (c1 - c0) : 0.0f);
-            }
-        }
     }
     if (DOT) {
         const double sum = block_sum(acc, red);
@@ -1701,10 +1663,9 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 }
 
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial, float* X0, float* Xc, float P)
+                  const double* S, hipStream_t st, double* partial, float* X0)
 {
-    if (X0 && partial) throw std::invalid_argument("jacobi: the fused forms write no partials");
-    if (Xc && (!X0 || level >= d.max_level)) throw std::invalid_argument("jacobi: the prolongation needs X0 and a finer level");
+    if (X0 && partial) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
     const LaunchList& l = d.subtree[level];
     if (partial) {
         // one partial per list entry (the one-tile form only)
@@ -1728,22 +1689,22 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
     }
     if (!nonempty(l)) return 0;
     if (X0) {
-        // the first coarse-level sweep with the level's FAS save, or the last one with the
-        // prolongation into the finer level (one-tile form)
-        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: the fused forms need the one-tile form");
+        // the first coarse-level sweep with the level's FAS save (one-tile form)
+        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: the FAS save needs the one-tile form");
         const TopoView tp = view(d);
-#define CMG_JAC_MODE(QV, MV)                                                                                             \
-    switch (stencil_threads()) {                                                                                          \
-    case 128: k::jacobi<128, false, QV, MV><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0, Xc, P); break; \
-    case 512: k::jacobi<512, false, QV, MV><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0, Xc, P); break; \
-    default: k::jacobi<256, false, QV, MV><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0, Xc, P); break;  \
-    }
-        if (Xc) {
-            if (C.q[0]) { CMG_JAC_MODE(true, k::JAC_PROLONG) } else { CMG_JAC_MODE(false, k::JAC_PROLONG) }
+        if (C.q[0]) {
+            switch (stencil_threads()) {
+            case 128: k::jacobi<128, false, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            case 512: k::jacobi<512, false, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            default: k::jacobi<256, false, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            }
         } else {
-            if (C.q[0]) { CMG_JAC_MODE(true, k::JAC_SAVE) } else { CMG_JAC_MODE(false, k::JAC_SAVE) }
+            switch (stencil_threads()) {
+            case 128: k::jacobi<128, false, false, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            case 512: k::jacobi<512, false, false, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            default: k::jacobi<256, false, false, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
+            }
         }
-#undef CMG_JAC_MODE
         return 0;
     }
     if (C.q[0]) {

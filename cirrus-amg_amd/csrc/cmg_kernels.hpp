@@ -42,12 +42,9 @@ void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* 
 void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
                      hipStream_t st);
 // partial: one b.x_out sum per list entry (one-tile form); returns their number
-// X0 alone: the level's FAS save on its NONLEAF tiles in the same pass (B updated in place,
-// x -> X0); X0 and Xc: the FAS prolongation of the new values into the children one level up
-// (Xc += P (x' - X0), GHOST children x'), as fas_prolong_v on level + 1 would do after the sweep
+// X0: the level's FAS save on its NONLEAF tiles in the same pass (B updated in place, x -> X0)
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial = nullptr, float* X0 = nullptr, float* Xc = nullptr,
-                  float P = 0.0f);
+                  const double* S, hipStream_t st, double* partial = nullptr, float* X0 = nullptr);
 // Dq (optional): the diagonal as FP16 bits of D / h instead of D
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
                         hipStream_t st, const uint16_t* Dq = nullptr);

@@ -630,10 +630,8 @@ struct CycleCtx {
 };
 
 // `x_zero`: every x of the cycle is still zero (first sweep at the finest level);
-// `fas_save`: the level's FAS save is still due and rides on the first sweep;
-// `prolong_up`: the last sweep also prolongs into the children one level up
-void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false,
-            bool prolong_up = false)
+// `fas_save`: the level's FAS save is still due and rides on the first sweep
+void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
@@ -653,9 +651,8 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
         }
         else {
             const bool last_fine = c.rz && order == 1 && level == d.max_level && it == iters - 1;
-            const bool up = prolong_up && it == iters - 1;
             launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st, last_fine ? c.rz : nullptr,
-                          (fas_save && it == 0) || up ? c.X0 : nullptr, up ? c.X : nullptr, c.s->P);
+                          fas_save && it == 0 ? c.X0 : nullptr);
         }
         std::swap(cur, other);
     }
@@ -684,9 +681,8 @@ void coarsest(const CycleCtx& c)
 
 // FASMuCycleStep (src/AMGSolver.cu:1257-1277); x_zero: x is still 0 everywhere
 // `fas_save`: the FAS save of this level (after the restriction from level + 1)
-// has not run yet; the first pre-sweep does it.  `prolong_up`: the last
-// post-sweep also does the FAS prolongation into level + 1
-void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false, bool prolong_up = false)
+// has not run yet; the first pre-sweep does it
+void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
@@ -710,11 +706,9 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false
     const bool fused = c.o->smoother == CMG_SMOOTH_JACOBI && c.o->level_iters >= 1 && fuse_fas_save_allowed();
     if (!fused) launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
     const int mu = cycle_mu(c.o);
-    // ... and the prolongation into this level rides on the coarser level's last post-sweep
-    const bool up = fused && level - 1 >= 1;
-    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false, fused && i == 0, up && i == mu - 1);
-    if (!up) launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
-    smooth(c, level, c.o->level_iters, 1, false, false, prolong_up);
+    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false, fused && i == 0);
+    launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
+    smooth(c, level, c.o->level_iters, 1);
 }
 
 // AMGSolver::FASFCycle (src/AMGSolver.cu:1358-1387)
