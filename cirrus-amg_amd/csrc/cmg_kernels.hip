@@ -723,27 +723,34 @@ __global__ __launch_bounds__(256) void copy_flat(const f32x4v* __restrict__ A, f
     if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(&A[i]), &B[i]);
 }
 
-// q[a] = FP16 bits of c[a] / h on every tile of the list; *bad counts workgroups
-// that produced a non-finite value
-__global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* __restrict__ list, Coef C,
+// q[a] = FP16 bits of c[a] / h on every tile of the list, 128 threads x 4 cells a
+// tile (float4 in, 4 halves out per channel); *bad counts workgroups that produced
+// a non-finite value
+__device__ __forceinline__ uint2 half4_of(float4 v, float h, bool& nf)
+{
+    const _Float16 a = _Float16(v.x / h), b = _Float16(v.y / h), c = _Float16(v.z / h), d = _Float16(v.w / h);
+    nf = nf || !isfinite(float(a)) || !isfinite(float(b)) || !isfinite(float(c)) || !isfinite(float(d));
+    const uint32_t lo = uint32_t(__builtin_bit_cast(uint16_t, a)) | (uint32_t(__builtin_bit_cast(uint16_t, b)) << 16);
+    const uint32_t hi = uint32_t(__builtin_bit_cast(uint16_t, c)) | (uint32_t(__builtin_bit_cast(uint16_t, d)) << 16);
+    return make_uint2(lo, hi);
+}
+
+__global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* __restrict__ list, int n, Coef C,
                                                     uint16_t* __restrict__ q0, uint16_t* __restrict__ q1,
                                                     uint16_t* __restrict__ q2, uint16_t* __restrict__ q3,
                                                     unsigned int* __restrict__ bad)
 {
-    const int t = list[blockIdx.x];
-    if (t < 0) return;
-    const float h = tile_h(tp, t);
     bool nf = false;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const size_t i = size_t(t) * 512 + threadIdx.x + c * 256;
-        const _Float16 v0 = _Float16(C.c[0][i] / h), v1 = _Float16(C.c[1][i] / h);
-        const _Float16 v2 = _Float16(C.c[2][i] / h), v3 = _Float16(C.c[3][i] / h);
-        nf = nf || !isfinite(float(v0)) || !isfinite(float(v1)) || !isfinite(float(v2)) || !isfinite(float(v3));
-        q0[i] = __builtin_bit_cast(uint16_t, v0);
-        q1[i] = __builtin_bit_cast(uint16_t, v1);
-        q2[i] = __builtin_bit_cast(uint16_t, v2);
-        q3[i] = __builtin_bit_cast(uint16_t, v3);
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const float h = tile_h(tp, t_);
+        const float4 c0 = reinterpret_cast<const float4*>(C.c[0])[q], c1 = reinterpret_cast<const float4*>(C.c[1])[q];
+        const float4 c2 = reinterpret_cast<const float4*>(C.c[2])[q], c3 = reinterpret_cast<const float4*>(C.c[3])[q];
+        reinterpret_cast<uint2*>(q0)[q] = half4_of(c0, h, nf);
+        reinterpret_cast<uint2*>(q1)[q] = half4_of(c1, h, nf);
+        reinterpret_cast<uint2*>(q2)[q] = half4_of(c2, h, nf);
+        reinterpret_cast<uint2*>(q3)[q] = half4_of(c3, h, nf);
     }
     if (__syncthreads_or(nf) && threadIdx.x == 0) atomicAdd(bad, 1u);
 }
@@ -1756,7 +1763,7 @@ void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float*
 void launch_coef_to_half(const DevGrid& d, Coef C, uint16_t* const q[4], unsigned int* bad, hipStream_t st)
 {
     if (!nonempty(d.all)) return;
-    k::coef_to_half<<<d.all.n, 256, 0, st>>>(view(d), d.all.d, C, q[0], q[1], q[2], q[3], bad);
+    k::coef_to_half<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, q[0], q[1], q[2], q[3], bad);
 }
 
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)
