@@ -498,29 +498,24 @@ class DeviceBuilder {
 public:
     explicit DeviceBuilder(Topology& topo) : topo_(topo), L_(topo.num_levels_)
     {
-        hip_ok(hipStreamCreate(&st_), "stream");
-        hip_ok(hipMalloc(&small_, 64), "counters");
-        for (int l = 0; l < L_; ++l) {
-            const size_t size = topo_.table_[l].size();
-            hip_ok(hipMalloc(&slot_[l], size * sizeof(int4)), "slots");
-            hip_ok(hipMalloc(&word_[l], size * sizeof(unsigned long long)), "insertion words");
-            hip_ok(hipMalloc(&comp_[l], size * sizeof(int32_t)), "compressed array");
-            hip_ok(hipMemsetAsync(slot_[l], 0, size * sizeof(int4), st_), "clear slots");
-            hip_ok(hipMemsetAsync(word_[l], 0xff, size * sizeof(unsigned long long), st_), "clear words");
+        try {
+            hip_ok(hipStreamCreate(&st_), "stream");
+            hip_ok(hipMalloc(&small_, 64), "counters");
+            for (int l = 0; l < L_; ++l) {
+                const size_t size = topo_.table_[l].size();
+                hip_ok(hipMalloc(&slot_[l], size * sizeof(int4)), "slots");
+                hip_ok(hipMalloc(&word_[l], size * sizeof(unsigned long long)), "insertion words");
+                hip_ok(hipMalloc(&comp_[l], size * sizeof(int32_t)), "compressed array");
+                hip_ok(hipMemsetAsync(slot_[l], 0, size * sizeof(int4), st_), "clear slots");
+                hip_ok(hipMemsetAsync(word_[l], 0xff, size * sizeof(unsigned long long), st_), "clear words");
+            }
+            hip_ok(hipMemsetAsync(small_, 0, 64, st_), "clear counters");
+        } catch (...) {
+            release();   // a throwing constructor gets no destructor call
+            throw;
         }
-        hip_ok(hipMemsetAsync(small_, 0, 64, st_), "clear counters");
     }
-    ~DeviceBuilder()
-    {
-        if (st_) (void)hipStreamSynchronize(st_);
-        for (int l = 0; l < L_; ++l) {
-            if (slot_[l]) (void)hipFree(slot_[l]);
-            if (word_[l]) (void)hipFree(word_[l]);
-            if (comp_[l]) (void)hipFree(comp_[l]);
-        }
-        if (small_) (void)hipFree(small_);
-        if (st_) (void)hipStreamDestroy(st_);
-    }
+    ~DeviceBuilder() { release(); }
 
     // the host tables as they stand (used slots, compressed arrays of the last compression, max level)
     void upload()
@@ -747,6 +742,23 @@ public:
     }
 
 private:
+    void release()
+    {
+        if (st_) (void)hipStreamSynchronize(st_);
+        for (int l = 0; l < L_; ++l) {
+            if (slot_[l]) (void)hipFree(slot_[l]);
+            if (word_[l]) (void)hipFree(word_[l]);
+            if (comp_[l]) (void)hipFree(comp_[l]);
+            slot_[l] = nullptr;
+            word_[l] = nullptr;
+            comp_[l] = nullptr;
+        }
+        if (small_) (void)hipFree(small_);
+        small_ = nullptr;
+        if (st_) (void)hipStreamDestroy(st_);
+        st_ = nullptr;
+    }
+
     Tabs tabs() const
     {
         Tabs T{};
