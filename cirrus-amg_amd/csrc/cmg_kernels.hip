@@ -1521,12 +1521,14 @@ static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 // the environment (CMG_STENCIL_THREADS, CMG_PERSISTENT) or set_kernel_forms.
 enum { PERSIST_APPLY = 1, PERSIST_JACOBI = 2, PERSIST_RESTRICT = 4 };
 static std::atomic<int> g_threads{0}, g_mask{-1};
+static std::atomic<bool> g_threads_chosen{false};   // by set_kernel_forms or CMG_STENCIL_THREADS
 
 static int stencil_threads()
 {
     int v = g_threads.load(std::memory_order_relaxed);
     if (v) return v;
     const char* e = getenv("CMG_STENCIL_THREADS");
+    if (e) g_threads_chosen.store(true, std::memory_order_relaxed);
     v = e ? atoi(e) : 256;
     v = (v == 128 || v == 256 || v == 512) ? v : 256;
     g_threads.store(v, std::memory_order_relaxed);
@@ -1550,7 +1552,18 @@ bool set_kernel_forms(int mask, int threads)
     if (mask < 0 || mask > 31 || (threads != 128 && threads != 256 && threads != 512)) return false;
     g_mask.store(mask, std::memory_order_relaxed);
     g_threads.store(threads, std::memory_order_relaxed);
+    g_threads_chosen.store(true, std::memory_order_relaxed);
     return true;
+}
+
+// the Jacobi sweep on FP16 coefficient copies reads half the coefficient bytes of the
+// FP32 form, so a tile's latency chain carries fewer bytes; twice the tiles in flight
+// per CU pays there (MI355X, finest level of the 1024^3 torus workload: 72.6 us at 128
+// threads a tile against 77.8 at 256).  It runs 128 unless a thread count was chosen.
+static int q_jacobi_threads()
+{
+    const int v = stencil_threads();
+    return g_threads_chosen.load(std::memory_order_relaxed) ? v : 128;
 }
 
 void get_kernel_forms(int* mask, int* threads)
@@ -1680,7 +1693,7 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         if (!nonempty(l)) return 0;
         const TopoView tp = view(d);
         if (C.q[0]) {
-            switch (stencil_threads()) {
+            switch (q_jacobi_threads()) {
             case 128: k::jacobi<128, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
             case 512: k::jacobi<512, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
             default: k::jacobi<256, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
@@ -1700,7 +1713,7 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: the FAS save needs the one-tile form");
         const TopoView tp = view(d);
         if (C.q[0]) {
-            switch (stencil_threads()) {
+            switch (q_jacobi_threads()) {
             case 128: k::jacobi<128, false, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
             case 512: k::jacobi<512, false, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
             default: k::jacobi<256, false, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
@@ -1717,7 +1730,7 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
     if (C.q[0]) {
         if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: FP16 coefficients need the one-tile form");
         const TopoView tp = view(d);
-        switch (stencil_threads()) {
+        switch (q_jacobi_threads()) {
         case 128: k::jacobi<128, false, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
         case 512: k::jacobi<512, false, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
         default: k::jacobi<256, false, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;

@@ -386,7 +386,8 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * FP64 partial sums of r.z).
  * stencil_threads: 128, 256 or 512 threads per tile.  Stencil results are
  * bit-identical across forms; PCG scalars differ only by the grouping of FP64
- * partial sums.  Default: mask 0, 256 threads.  No reference counterpart (the reference
+ * partial sums.  Default: mask 0, 256 threads (the Jacobi sweep on FP16 coefficient
+ * copies: 128 until a thread count is set here or by CMG_STENCIL_THREADS).  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
 cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..31 */
