@@ -299,12 +299,28 @@ __global__ void b_representative(uint8_t* valid, int n, const uint32_t* dslot, c
     atomicAdd(count, 1u);
 }
 
+// one round's room check on the device: the round's new keys (*dnew) must fit the
+// level table next to its used slots; sets *skip (and err) when they do not
+__global__ void b_capacity(const unsigned int* dnew, int32_t* dused, uint32_t size, int* skip, int* err)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned long long need = (unsigned long long)(uint32_t(*dused)) + *dnew;
+    if (need > size) {
+        *skip = 1;
+        *err = 1;
+    } else {
+        *skip = 0;
+        *dused = int32_t(need);
+    }
+}
+
 // ordered linear probing: word = 0 for a key of an earlier round, ~0 for an empty slot, else
 // (rank << 32 | candidate); a smaller word has the higher priority
-__global__ void b_insert(unsigned long long* word, uint32_t mask, const int4* cand, const uint8_t* valid, int n, int* err)
+__global__ void b_insert(unsigned long long* word, uint32_t mask, const int4* cand, const uint8_t* valid, int n,
+                         const int* skip, int* err)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || !valid[c]) return;
+    if (c >= n || *skip || !valid[c]) return;
     unsigned long long mine = ((unsigned long long)(uint32_t(c) + 1u) << 32) | uint32_t(c);
     const int4 key = cand[c];
     uint32_t k = khash(key.x, key.y, key.z) & mask;
@@ -494,13 +510,16 @@ struct Buf {
 }  // namespace
 
 // Owns the device copy of a Topology's hash tables for the duration of one call.
+// Counts stay on the device between kernels (scan totals, new-key counts, used
+// slots); the host reads them back once per compression, together with the
+// error flag and the refined-tile counts.
 class DeviceBuilder {
 public:
     explicit DeviceBuilder(Topology& topo) : topo_(topo), L_(topo.num_levels_)
     {
         try {
             hip_ok(hipStreamCreate(&st_), "stream");
-            hip_ok(hipMalloc(&small_, 64), "counters");
+            hip_ok(hipMalloc(&small_, sizeof(Small)), "counters");
             for (int l = 0; l < L_; ++l) {
                 const size_t size = topo_.table_[l].size();
                 hip_ok(hipMalloc(&slot_[l], size * sizeof(int4)), "slots");
@@ -509,7 +528,7 @@ public:
                 hip_ok(hipMemsetAsync(slot_[l], 0, size * sizeof(int4), st_), "clear slots");
                 hip_ok(hipMemsetAsync(word_[l], 0xff, size * sizeof(unsigned long long), st_), "clear words");
             }
-            hip_ok(hipMemsetAsync(small_, 0, 64, st_), "clear counters");
+            hip_ok(hipMemsetAsync(small_, 0, sizeof(Small), st_), "clear counters");
         } catch (...) {
             release();   // a throwing constructor gets no destructor call
             throw;
@@ -517,23 +536,25 @@ public:
     }
     ~DeviceBuilder() { release(); }
 
-    // the host tables as they stand (used slots, compressed arrays of the last compression, max level)
+    // the host tables as they stand: the used slots (those of the last compression plus the
+    // ones claimed since), the compressed arrays of the last compression, the max level
     void upload()
     {
+        std::vector<int32_t> rec, used(kMaxLevels, 0);
         for (int l = 0; l < L_; ++l) {
             const auto& tab = topo_.table_[l];
-            std::vector<int32_t> rec;
-            for (size_t k = 0; k < tab.size(); ++k) {
-                const Slot& s = tab[k];
-                if (!s.used) continue;
-                rec.insert(rec.end(), {int32_t(k), s.c[0], s.c[1], s.c[2], make_meta(s.type, s.refine ? 1 : 0)});
-            }
-            used_[l] = int(rec.size() / 5);
-            if (used_[l]) {
+            rec.clear();
+            for (const auto* list : {&topo_.compressed_[l], &topo_.touched_[l]})
+                for (int32_t k : *list) {
+                    const Slot& s = tab[k];
+                    rec.insert(rec.end(), {k, s.c[0], s.c[1], s.c[2], make_meta(s.type, s.refine ? 1 : 0)});
+                }
+            used[l] = int(rec.size() / 5);
+            if (used[l]) {
                 int32_t* d = recbuf_.need<int32_t>(rec.size());
                 hip_ok(hipMemcpyAsync(d, rec.data(), rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, st_), "upload");
-                k::b_upload<<<grid_of(used_[l]), kBlock, 0, st_>>>(d, used_[l], slot_[l], word_[l]);
-                hip_ok(hipStreamSynchronize(st_), "upload");
+                k::b_upload<<<grid_of(used[l]), kBlock, 0, st_>>>(d, used[l], slot_[l], word_[l]);
+                hip_ok(hipStreamSynchronize(st_), "upload");   // rec is reused for the next level
             }
             const auto& comp = topo_.compressed_[l];
             cnt_[l] = int(comp.size());
@@ -543,6 +564,9 @@ public:
                 hip_ok(hipStreamSynchronize(st_), "upload");
             }
         }
+        hip_ok(hipMemcpyAsync(small_->used, used.data(), kMaxLevels * sizeof(int32_t), hipMemcpyHostToDevice, st_),
+               "upload");
+        hip_ok(hipStreamSynchronize(st_), "upload");
         max_level_ = topo_.max_level_;
     }
 
@@ -562,7 +586,7 @@ public:
             td.h0 = topo_.h0_;
         }
         const Tabs T = tabs();
-        int total = 0;
+        hip_ok(hipMemsetAsync(small_->refined, 0, sizeof(small_->refined), st_), "refined counts");
         for (int i = max_level_; i >= 0; --i) {
             const int n = cnt_[i];
             if (!n) continue;
@@ -584,19 +608,22 @@ public:
             int32_t* flag = flag_.need<int32_t>(n);
             k::b_leaf_flags<<<grid_of(n), kBlock, 0, st_>>>(T, i, comp_[i], n, tgt, max_level_, flag, err());
             k::b_ghost_flags<<<grid_of(n), kBlock, 0, st_>>>(T, i, comp_[i], n, err());
+            if (i + 1 >= L_) continue;   // no level above: every flag is clear
             int32_t* pos = pos_.need<int32_t>(n);
-            const int nref = scan(flag, n, pos);
-            if (!nref) continue;
-            total += nref;
-            const int nc = 8 * nref;
+            scan(flag, n, pos, small_->refined + i);
+            // children of the flagged tiles: candidate c = 8 (flagged index) + octant, at most 8 n
+            const int nc = 8 * n;
             int4* cand = cand_.need<int4>(nc);
             uint8_t* valid = valid_.need<uint8_t>(nc);
-            hip_ok(hipMemsetAsync(counter(), 0, sizeof(unsigned int), st_), "counter");
-            k::b_children<<<grid_of(long(n) * 8), kBlock, 0, st_>>>(T, i, comp_[i], n, flag, pos, cand, valid, counter(),
-                                                                     err());
-            insert(i + 1, cand, valid, nc, read_counter());
+            hip_ok(hipMemsetAsync(valid, 0, nc, st_), "candidates");
+            hip_ok(hipMemsetAsync(small_->fresh + (i + 1), 0, sizeof(unsigned int), st_), "counter");
+            k::b_children<<<grid_of(long(n) * 8), kBlock, 0, st_>>>(T, i, comp_[i], n, flag, pos, cand, valid,
+                                                                     small_->fresh + (i + 1), err());
+            insert(i + 1, cand, valid, nc);
         }
         compress();
+        int total = 0;
+        for (int l = 0; l < L_; ++l) total += host_.refined[l];
         return total;
     }
 
@@ -619,24 +646,28 @@ public:
             hip_ok(hipMemsetAsync(drank, 0xff, dsize * sizeof(uint32_t), st_), "clear set");
             hip_ok(hipMemsetAsync(dslot, 0xff, size_t(nc) * sizeof(uint32_t), st_), "clear set");
             k::b_dedupe<<<grid_of(nc), kBlock, 0, st_>>>(cand, valid, nc, dword, drank, dsize - 1, dslot, err());
-            hip_ok(hipMemsetAsync(counter(), 0, sizeof(unsigned int), st_), "counter");
-            k::b_representative<<<grid_of(nc), kBlock, 0, st_>>>(valid, nc, dslot, drank, dsize - 1, counter());
-            insert(L, cand, valid, nc, read_counter());
+            hip_ok(hipMemsetAsync(small_->fresh + L, 0, sizeof(unsigned int), st_), "counter");
+            k::b_representative<<<grid_of(nc), kBlock, 0, st_>>>(valid, nc, dslot, drank, dsize - 1, small_->fresh + L);
+            insert(L, cand, valid, nc);
         }
         compress();
     }
 
+    // compressHost: per level the used slots in slot order; one read-back of every count
     void compress()
     {
-        max_level_ = -1;
         for (int l = 0; l < L_; ++l) {
             const uint32_t size = uint32_t(topo_.table_[l].size());
             int32_t* flag = flag_.need<int32_t>(size);
             int32_t* pos = pos_.need<int32_t>(size);
             k::b_used<<<grid_of(size), kBlock, 0, st_>>>(slot_[l], size, flag);
-            cnt_[l] = scan(flag, int(size), pos);
+            scan(flag, int(size), pos, small_->count + l);
             k::b_scatter<<<grid_of(size), kBlock, 0, st_>>>(flag, pos, size, comp_[l]);
-            used_[l] = cnt_[l];
+        }
+        read_back();
+        max_level_ = -1;
+        for (int l = 0; l < L_; ++l) {
+            cnt_[l] = host_.count[l];
             if (cnt_[l]) max_level_ = l;
         }
     }
@@ -648,6 +679,7 @@ public:
             const int n = cnt_[l];
             auto& comp = topo_.compressed_[l];
             comp.resize(n);
+            topo_.touched_[l].clear();
             if (!n) continue;
             std::vector<int4> rec(n);
             int4* d = gather_.need<int4>(n);
@@ -716,8 +748,7 @@ public:
             d2h(T.parent.data(), par, N * sizeof(int32_t));
             d2h(T.child.data(), chi, size_t(N) * 8 * sizeof(int32_t));
         }
-        hip_ok(hipStreamSynchronize(st_), "download tables");
-        check_err();
+        read_back();
         T.level_start = start;
         for (int l = 0; l < L_; ++l) {
             T.slot_tile_[l].assign(T.table_[l].size(), -1);
@@ -742,6 +773,17 @@ public:
     }
 
 private:
+    // device-resident counters, read back in one copy
+    struct Small {
+        int err;                           // 1: a level table is full; else an insertion did not finish
+        int pad[15];
+        int32_t count[kMaxLevels];         // compressed count per level (scan totals)
+        int32_t used[kMaxLevels];          // used slots per level
+        int32_t refined[kMaxLevels];       // flagged LEAF tiles per level in this refine pass
+        unsigned int fresh[kMaxLevels];    // new keys of the level's insertion round
+        int skip[kMaxLevels];              // the round did not fit: nothing inserted
+    };
+
     void release()
     {
         if (st_) (void)hipStreamSynchronize(st_);
@@ -769,64 +811,50 @@ private:
         T.num_levels = L_;
         return T;
     }
-    int* err() { return reinterpret_cast<int*>(small_); }
-    unsigned int* counter() { return reinterpret_cast<unsigned int*>(small_ + 16); }
-    int32_t* total() { return reinterpret_cast<int32_t*>(small_ + 32); }
+    int* err() { return &small_->err; }
 
-    void check_err()
+    // every device counter to the host (one synchronisation); throws on the error flag
+    void read_back()
     {
-        int e = 0;
-        hip_ok(hipMemcpyAsync(&e, err(), sizeof(int), hipMemcpyDeviceToHost, st_), "error flag");
-        hip_ok(hipStreamSynchronize(st_), "error flag");
-        if (e == 1) throw std::runtime_error("hash table full (device octree construction)");
-        if (e) throw std::runtime_error("device octree construction: insertion did not finish (code " + std::to_string(e) +
-                                        ")");
+        hip_ok(hipMemcpyAsync(&host_, small_, sizeof(Small), hipMemcpyDeviceToHost, st_), "counters");
+        hip_ok(hipStreamSynchronize(st_), "counters");
+        if (host_.err == 1) throw std::runtime_error("hash table full (device octree construction)");
+        if (host_.err)
+            throw std::runtime_error("device octree construction: insertion did not finish (code " +
+                                     std::to_string(host_.err) + ")");
     }
 
-    unsigned int read_counter()
-    {
-        unsigned int h = 0;
-        hip_ok(hipMemcpyAsync(&h, counter(), sizeof(h), hipMemcpyDeviceToHost, st_), "counter");
-        hip_ok(hipStreamSynchronize(st_), "counter");
-        check_err();
-        return h;
-    }
-
-    // exclusive prefix sums of n 0/1 flags into pos; returns the total
-    int scan(const int32_t* flag, int n, int32_t* pos)
+    // exclusive prefix sums of n 0/1 flags into pos; the total goes to *total (device)
+    void scan(const int32_t* flag, int n, int32_t* pos, int32_t* total)
     {
         const int nt = (n + k::kScanTile - 1) / k::kScanTile;
         int32_t* tsum = tsum_.need<int32_t>(nt);
         k::b_scan_tiles<<<nt, kBlock, 0, st_>>>(flag, n, pos, tsum);
-        k::b_scan_sums<<<1, kBlock, 0, st_>>>(tsum, nt, total());
+        k::b_scan_sums<<<1, kBlock, 0, st_>>>(tsum, nt, total);
         k::b_scan_add<<<grid_of(n), kBlock, 0, st_>>>(pos, n, tsum);
-        int32_t h = 0;
-        hip_ok(hipMemcpyAsync(&h, total(), sizeof(h), hipMemcpyDeviceToHost, st_), "scan total");
-        hip_ok(hipStreamSynchronize(st_), "scan");
-        check_err();
-        return h;
     }
 
-    // one round of ordered insertion of the valid candidates into a level table
-    void insert(int level, const int4* cand, const uint8_t* valid, int nc, unsigned int nnew)
+    // one round of ordered insertion of the valid candidates (at most nc) into a level
+    // table; the room check runs on the device (small_->fresh[level] new keys)
+    void insert(int level, const int4* cand, const uint8_t* valid, int nc)
     {
-        if (!nnew) return;
         const size_t size = topo_.table_[level].size();
-        if (size_t(used_[level]) + nnew > size) throw std::runtime_error("hash table full at level " + std::to_string(level));
-        k::b_insert<<<grid_of(nc), kBlock, 0, st_>>>(word_[level], uint32_t(size) - 1u, cand, valid, nc, err());
+        k::b_capacity<<<1, 64, 0, st_>>>(small_->fresh + level, small_->used + level, uint32_t(size),
+                                         small_->skip + level, err());
+        k::b_insert<<<grid_of(nc), kBlock, 0, st_>>>(word_[level], uint32_t(size) - 1u, cand, valid, nc,
+                                                     small_->skip + level, err());
         k::b_materialize<<<grid_of(long(size)), kBlock, 0, st_>>>(word_[level], slot_[level], uint32_t(size), cand);
-        used_[level] += int(nnew);
     }
 
     Topology& topo_;
     int L_;
     hipStream_t st_ = nullptr;
-    char* small_ = nullptr;   // error flag, counter, scan total
+    Small* small_ = nullptr;
+    Small host_{};
     int4* slot_[kMaxLevels] = {};
     unsigned long long* word_[kMaxLevels] = {};
     int32_t* comp_[kMaxLevels] = {};
     int cnt_[kMaxLevels] = {};
-    int used_[kMaxLevels] = {};
     int max_level_ = -1;
     Buf recbuf_, tgt_, gather_, flag_, pos_, tsum_, cand_, valid_, dword_, drank_, dslot_;
 };

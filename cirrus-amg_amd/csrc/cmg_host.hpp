@@ -81,6 +81,7 @@ private:
     std::array<uint32_t, kMaxLevels> log2_{};
     std::array<std::vector<Slot>, kMaxLevels> table_;
     std::array<std::vector<int32_t>, kMaxLevels> compressed_;   // slot indices in slot order
+    std::array<std::vector<int32_t>, kMaxLevels> touched_;      // slots claimed since the last compression
     std::array<std::vector<int32_t>, kMaxLevels> slot_tile_;    // slot -> tile index
 };
 

@@ -75,6 +75,7 @@ int Topology::set_tile(int level, const Coord& c, uint8_t type)
         s.used = true;
         s.c = c;
         s.refine = false;
+        touched_[level].push_back(int32_t(k));
     }
     s.type = type;
     finalized_ = false;
@@ -85,6 +86,7 @@ void Topology::compress()
 {
     max_level_ = -1;
     for (int l = 0; l < num_levels_; ++l) {
+        touched_[l].clear();
         auto& out = compressed_[l];
         out.clear();
         for (size_t k = 0; k < table_[l].size(); ++k)
