@@ -81,6 +81,7 @@ EXPORTS = [
     "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
     "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_assemble_cutcell_device", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
     "cmg_grid_rebuild_device", "cmg_grid_iterative_refine_device", "cmg_grid_finalize_device",
+    "cmg_grid_build_device",
 ]
 
 _lib = None
@@ -100,6 +101,7 @@ def lib():
             "cmg_grid_iterative_refine": (st, [vp, C.POINTER(Target)]), "cmg_grid_finalize": (st, [vp]),
             "cmg_grid_rebuild_device": (st, [vp]), "cmg_grid_finalize_device": (st, [vp]),
             "cmg_grid_iterative_refine_device": (st, [vp, C.POINTER(Target)]),
+            "cmg_grid_build_device": (st, [vp, C.POINTER(Target)]),
             "cmg_grid_num_tiles": (i, [vp]), "cmg_grid_max_level": (i, [vp]),
             "cmg_grid_level_starts": (st, [vp, vp]), "cmg_grid_get_tiles": (st, [vp, vp, vp, vp]),
             "cmg_grid_get_slots": (st, [vp, vp]), "cmg_grid_get_neighbors": (st, [vp, vp]),
@@ -224,14 +226,21 @@ class HADeviceGrid:
         else:
             _check(lib().cmg_grid_finalize(self.h), "cmg_grid_finalize")
 
+    def build_device(self, target):
+        """rebuild + iterativeRefine + finalize on the device over one copy of the tables"""
+        _check(lib().cmg_grid_build_device(self.h, C.byref(target)), "cmg_grid_build_device")
+
     @classmethod
     def test_grid(cls, target, h0=0.125, log2_hash=DEFAULT_LOG2_HASH, device=False):
         """CreateTestGrid (tests/TestGrids.cuh:144-169): one LEAF root, rebuild, refine, LEAF cell types."""
         g = cls(h0, log2_hash)
         g.setTileHost(0, (0, 0, 0), LEAF)
-        g.rebuild(device)
-        g.iterativeRefine(target, device)
-        g.finalize(device)
+        if device:
+            g.build_device(target)
+        else:
+            g.rebuild()
+            g.iterativeRefine(target)
+            g.finalize()
         g.set_leaf_types_preset(target)
         return g
 

@@ -389,7 +389,9 @@ __device__ int block_inclusive(int v, int* s)
     return r;
 }
 
-__global__ void __launch_bounds__(kBlock) b_scan_tiles(const int32_t* in, int n, int32_t* out, int32_t* tsum)
+// total (one-tile scans only): the scan's total, written here instead of by b_scan_sums
+__global__ void __launch_bounds__(kBlock) b_scan_tiles(const int32_t* in, int n, int32_t* out, int32_t* tsum,
+                                                       int32_t* total)
 {
     __shared__ int s[kBlock];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -404,7 +406,10 @@ __global__ void __launch_bounds__(kBlock) b_scan_tiles(const int32_t* in, int n,
         if (base + e < n) out[base + e] = run;
         run += v[e];
     }
-    if (threadIdx.x == kBlock - 1) tsum[blockIdx.x] = incl;
+    if (threadIdx.x == kBlock - 1) {
+        if (total) total[0] = incl;
+        else tsum[blockIdx.x] = incl;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) b_scan_sums(int32_t* tsum, int nt, int32_t* total)
@@ -568,6 +573,7 @@ public:
                "upload");
         hip_ok(hipStreamSynchronize(st_), "upload");
         max_level_ = topo_.max_level_;
+        for (int l = 0; l < L_; ++l) dirty_[l] = true;   // the host's compressed arrays may be stale
     }
 
     int refine_step(const cmg_target& t)
@@ -653,10 +659,13 @@ public:
         compress();
     }
 
-    // compressHost: per level the used slots in slot order; one read-back of every count
+    // compressHost: per level the used slots in slot order (levels with no insertion round
+    // since the last compression are already compressed); one read-back of every count
     void compress()
     {
         for (int l = 0; l < L_; ++l) {
+            if (!dirty_[l]) continue;
+            dirty_[l] = false;
             const uint32_t size = uint32_t(topo_.table_[l].size());
             int32_t* flag = flag_.need<int32_t>(size);
             int32_t* pos = pos_.need<int32_t>(size);
@@ -828,8 +837,12 @@ private:
     void scan(const int32_t* flag, int n, int32_t* pos, int32_t* total)
     {
         const int nt = (n + k::kScanTile - 1) / k::kScanTile;
+        if (nt == 1) {   // one workgroup does it all
+            k::b_scan_tiles<<<1, kBlock, 0, st_>>>(flag, n, pos, nullptr, total);
+            return;
+        }
         int32_t* tsum = tsum_.need<int32_t>(nt);
-        k::b_scan_tiles<<<nt, kBlock, 0, st_>>>(flag, n, pos, tsum);
+        k::b_scan_tiles<<<nt, kBlock, 0, st_>>>(flag, n, pos, tsum, nullptr);
         k::b_scan_sums<<<1, kBlock, 0, st_>>>(tsum, nt, total);
         k::b_scan_add<<<grid_of(n), kBlock, 0, st_>>>(pos, n, tsum);
     }
@@ -838,6 +851,7 @@ private:
     // table; the room check runs on the device (small_->fresh[level] new keys)
     void insert(int level, const int4* cand, const uint8_t* valid, int nc)
     {
+        dirty_[level] = true;
         const size_t size = topo_.table_[level].size();
         k::b_capacity<<<1, 64, 0, st_>>>(small_->fresh + level, small_->used + level, uint32_t(size),
                                          small_->skip + level, err());
@@ -855,6 +869,7 @@ private:
     unsigned long long* word_[kMaxLevels] = {};
     int32_t* comp_[kMaxLevels] = {};
     int cnt_[kMaxLevels] = {};
+    bool dirty_[kMaxLevels] = {};   // an insertion round ran since the level's last compression
     int max_level_ = -1;
     Buf recbuf_, tgt_, gather_, flag_, pos_, tsum_, cand_, valid_, dword_, drank_, dslot_;
 };
@@ -878,6 +893,20 @@ void device_iterative_refine(Topology& topo, const cmg_target& t)
         if (n == 0) break;
     }
     b.download();
+}
+
+void device_build(Topology& topo, const cmg_target& t)
+{
+    DeviceBuilder b(topo);
+    b.upload();
+    b.compress();
+    b.spawn_ghosts();
+    for (;;) {
+        const int n = b.refine_step(t);
+        b.spawn_ghosts();
+        if (n == 0) break;
+    }
+    b.finalize();
 }
 
 void device_finalize(Topology& topo)

@@ -178,6 +178,21 @@ cmg_status cmg_grid_iterative_refine_device(cmg_grid* g, const cmg_target* targe
     return CMG_OK;
 }
 
+cmg_status cmg_grid_build_device(cmg_grid* g, const cmg_target* target)
+{
+    if (auto st = need_topology(g)) return st;
+    if (!target || (target->kind == CMG_TARGET_CALLBACK && !target->fn)) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        device_build(*g->topo, *target);
+        dev_free(g->dev);
+        g->uploaded = false;
+        g->cells = HostCells{};
+        ensure_types(g);
+        g->assembled_c0 = -1;
+    });
+    return CMG_OK;
+}
+
 cmg_status cmg_grid_finalize_device(cmg_grid* g)
 {
     if (auto st = need_topology(g)) return st;

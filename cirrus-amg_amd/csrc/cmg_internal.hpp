@@ -78,6 +78,8 @@ float* dev_channel(DevGrid& d, int c);   // allocates (zeroed) on first use
 void device_rebuild(Topology& topo);
 void device_iterative_refine(Topology& topo, const cmg_target& t);
 void device_finalize(Topology& topo);
+// device_rebuild, device_iterative_refine and device_finalize over one device copy of the tables
+void device_build(Topology& topo, const cmg_target& t);
 
 }  // namespace cmg
 

@@ -94,6 +94,10 @@ cmg_status cmg_grid_finalize(cmg_grid* g);
 cmg_status cmg_grid_rebuild_device(cmg_grid* g);
 cmg_status cmg_grid_iterative_refine_device(cmg_grid* g, const cmg_target* target);
 cmg_status cmg_grid_finalize_device(cmg_grid* g);
+/* cmg_grid_rebuild_device, cmg_grid_iterative_refine_device and
+ * cmg_grid_finalize_device in one call over one device copy of the tables
+ * (CreateTestGrid's sequence after setTileHost, tests/TestGrids.cuh:144-169) */
+cmg_status cmg_grid_build_device(cmg_grid* g, const cmg_target* target);
 
 int cmg_grid_num_tiles(const cmg_grid* g);
 int cmg_grid_max_level(const cmg_grid* g);

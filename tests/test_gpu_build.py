@@ -100,8 +100,15 @@ def test_device_build_benchmark_octrees(name, target):
     t0 = time.perf_counter()
     gd = _build(target(), True)
     td = time.perf_counter() - t0
-    print(f"{name}: {gh.N} tiles, host construction {th * 1e3:.1f} ms, device construction {td * 1e3:.1f} ms")
+    t0 = time.perf_counter()
+    g1 = P.HADeviceGrid()
+    g1.setTileHost(0, (0, 0, 0), P.LEAF)
+    g1.build_device(target())
+    t1 = time.perf_counter() - t0
+    print(f"{name}: {gh.N} tiles, host construction {th * 1e3:.1f} ms, device construction {td * 1e3:.1f} ms "
+          f"(three calls), {t1 * 1e3:.1f} ms (cmg_grid_build_device)")
     _assert_same(gh, gd, name)
+    _assert_same(gh, g1, name + " one call")
 
 
 def test_mixed_host_and_device_steps():
