@@ -93,20 +93,30 @@ def test_device_build_callback_target():
 ])
 def test_device_build_benchmark_octrees(name, target):
     print("\ncase", name, flush=True)
-    t0 = time.perf_counter()
-    gh = _build(target(), False)
-    th = time.perf_counter() - t0
+
+    def one_call():
+        g = P.HADeviceGrid()
+        g.setTileHost(0, (0, 0, 0), P.LEAF)
+        g.build_device(target())
+        return g
+
+    def best_of(fn, reps=3):
+        best, out = None, None
+        for _ in range(reps):
+            if out is not None:
+                out.close()
+            t0 = time.perf_counter()
+            out = fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return out, best
+
     _build(target(), True).close()   # first call pays module load and allocator warm-up
-    t0 = time.perf_counter()
-    gd = _build(target(), True)
-    td = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    g1 = P.HADeviceGrid()
-    g1.setTileHost(0, (0, 0, 0), P.LEAF)
-    g1.build_device(target())
-    t1 = time.perf_counter() - t0
-    print(f"{name}: {gh.N} tiles, host construction {th * 1e3:.1f} ms, device construction {td * 1e3:.1f} ms "
-          f"(three calls), {t1 * 1e3:.1f} ms (cmg_grid_build_device)")
+    gh, th = best_of(lambda: _build(target(), False))
+    gd, td = best_of(lambda: _build(target(), True))
+    g1, t1 = best_of(one_call)
+    print(f"{name}: {gh.N} tiles, best of 3: host construction {th * 1e3:.1f} ms, device construction "
+          f"{td * 1e3:.1f} ms (three calls), {t1 * 1e3:.1f} ms (cmg_grid_build_device)")
     _assert_same(gh, gd, name)
     _assert_same(gh, g1, name + " one call")
 
