@@ -154,14 +154,6 @@ __device__ int eval_target(const TargetDev& t, int level, int bx, int by, int bz
 
 namespace k {
 
-__global__ void b_target(Tabs T, int level, const int32_t* comp, int n, TargetDev t, int32_t* tgt)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int4 s = T.slot[level][comp[j]];
-    tgt[j] = (m_type(s.w) & LEAF) ? eval_target(t, level, s.x, s.y, s.z) : 0;
-}
-
 __global__ void b_gather(const int4* slot, const int32_t* comp, int n, int4* out)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,8 +161,9 @@ __global__ void b_gather(const int4* slot, const int32_t* comp, int n, int4* out
 }
 
 // MarkRefineFlagOnLeafsWithLevelTargetHelperKernel (src/HAGrid.h:279-322)
-__global__ void b_leaf_flags(Tabs T, int i, const int32_t* comp, int n, const int32_t* tgt, int max_level, int32_t* flag,
-                             int* err)
+// tgt: host-evaluated targets per compressed entry, or nullptr to evaluate t here
+__global__ void b_leaf_flags(Tabs T, int i, const int32_t* comp, int n, const int32_t* tgt, TargetDev t, int max_level,
+                             int32_t* flag, int* err)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -181,7 +174,7 @@ __global__ void b_leaf_flags(Tabs T, int i, const int32_t* comp, int n, const in
         flag[j] = 0;
         return;
     }
-    bool r = tgt[j] > i;
+    bool r = (tgt ? tgt[j] : eval_target(t, i, s.x, s.y, s.z)) > i;
     if (i < max_level)
         for (int o = 0; o < 8; ++o) {
             const int q = dfind(T, i + 1, 2 * s.x + (o & 1), 2 * s.y + ((o >> 1) & 1), 2 * s.z + ((o >> 2) & 1), err);
@@ -261,6 +254,16 @@ __global__ void b_ghost_cand(Tabs T, int L, const int32_t* comp, int n, int4* ca
     valid[idx] = 1;
 }
 
+__global__ void b_set_init(uint32_t* dword, uint32_t* drank, uint32_t dsize, uint32_t* dslot, int nc)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < dsize) {
+        dword[k] = kEmpty32;
+        drank[k] = kEmpty32;
+    }
+    if (k < uint32_t(nc)) dslot[k] = kEmpty32;
+}
+
 // lock-free set of proposed keys; drank[slot] ends as the lowest rank proposing the key
 __global__ void b_dedupe(const int4* cand, const uint8_t* valid, int n, uint32_t* dword, uint32_t* drank, uint32_t dmask,
                          uint32_t* dslot, int* err)
@@ -299,28 +302,20 @@ __global__ void b_representative(uint8_t* valid, int n, const uint32_t* dslot, c
     atomicAdd(count, 1u);
 }
 
-// one round's room check on the device: the round's new keys (*dnew) must fit the
-// level table next to its used slots; sets *skip (and err) when they do not
-__global__ void b_capacity(const unsigned int* dnew, int32_t* dused, uint32_t size, int* skip, int* err)
-{
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const unsigned long long need = (unsigned long long)(uint32_t(*dused)) + *dnew;
-    if (need > size) {
-        *skip = 1;
-        *err = 1;
-    } else {
-        *skip = 0;
-        *dused = int32_t(need);
-    }
-}
-
 // ordered linear probing: word = 0 for a key of an earlier round, ~0 for an empty slot, else
 // (rank << 32 | candidate); a smaller word has the higher priority
+// The round's new keys (*fresh) must fit the table next to its used slots (*used),
+// else nothing is inserted and err = 1.  limit (optional): only candidates below
+// per * *limit belong to the round.
 __global__ void b_insert(unsigned long long* word, uint32_t mask, const int4* cand, const uint8_t* valid, int n,
-                         const int* skip, int* err)
+                         const int32_t* limit, int per, const unsigned int* fresh, const int32_t* used, int* err)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || *skip || !valid[c]) return;
+    if ((unsigned long long)(uint32_t(*used)) + *fresh > mask + 1ull) {
+        if (c == 0) *err = 1;
+        return;
+    }
+    if (c >= n || (limit && c >= per * *limit) || !valid[c]) return;
     unsigned long long mine = ((unsigned long long)(uint32_t(c) + 1u) << 32) | uint32_t(c);
     const int4 key = cand[c];
     uint32_t k = khash(key.x, key.y, key.z) & mask;
@@ -340,9 +335,16 @@ __global__ void b_insert(unsigned long long* word, uint32_t mask, const int4* ca
     *err = 2;
 }
 
-__global__ void b_materialize(unsigned long long* word, int4* slot, uint32_t size, const int4* cand)
+// also books the round: used += fresh when it fitted, fresh = 0 for the next round
+__global__ void b_materialize(unsigned long long* word, int4* slot, uint32_t size, const int4* cand, unsigned int* fresh,
+                              int32_t* used)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        const unsigned long long need = (unsigned long long)(uint32_t(*used)) + *fresh;
+        if (need <= size) *used = int32_t(need);
+        *fresh = 0;
+    }
     if (k >= size) return;
     const unsigned long long w = word[k];
     if (w == 0 || w == kEmpty) return;
@@ -359,16 +361,10 @@ __global__ void b_upload(const int32_t* rec, int n, int4* slot, unsigned long lo
     word[k] = 0;
 }
 
-__global__ void b_used(const int4* slot, uint32_t size, int32_t* flag)
+__global__ void b_scatter(const int4* slot, const int32_t* pos, uint32_t size, int32_t* comp)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < size) flag[k] = slot[k].w & 1;
-}
-
-__global__ void b_scatter(const int32_t* flag, const int32_t* pos, uint32_t size, int32_t* comp)
-{
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < size && flag[k]) comp[pos[k]] = int32_t(k);
+    if (k < size && (slot[k].w & 1)) comp[pos[k]] = int32_t(k);
 }
 
 // exclusive scan of 0/1 flags: 1024 items a workgroup, then the workgroup sums, then the offsets
@@ -390,14 +386,15 @@ __device__ int block_inclusive(int v, int* s)
 }
 
 // total (one-tile scans only): the scan's total, written here instead of by b_scan_sums
-__global__ void __launch_bounds__(kBlock) b_scan_tiles(const int32_t* in, int n, int32_t* out, int32_t* tsum,
-                                                       int32_t* total)
+// slots (optional): the flags are the slots' used bits instead of in[]
+__global__ void __launch_bounds__(kBlock) b_scan_tiles(const int32_t* in, const int4* slots, int n, int32_t* out,
+                                                       int32_t* tsum, int32_t* total)
 {
     __shared__ int s[kBlock];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems], sum = 0;
     for (int e = 0; e < kScanItems; ++e) {
-        v[e] = base + e < n ? in[base + e] : 0;
+        v[e] = base + e < n ? (slots ? (slots[base + e].w & 1) : in[base + e]) : 0;
         sum += v[e];
     }
     const int incl = block_inclusive(sum, s);
@@ -501,8 +498,10 @@ struct Buf {
         if (b > bytes) {
             if (p) hip_ok(hipFree(p), "free");
             p = nullptr;
-            hip_ok(hipMalloc(&p, b), "scratch allocation");
-            bytes = b;
+            size_t grown = 4096;   // power-of-two growth: few reallocations as a build proceeds
+            while (grown < b) grown <<= 1;
+            hip_ok(hipMalloc(&p, grown), "scratch allocation");
+            bytes = grown;
         }
         return static_cast<T*>(p);
     }
@@ -525,14 +524,25 @@ public:
         try {
             hip_ok(hipStreamCreate(&st_), "stream");
             hip_ok(hipMalloc(&small_, sizeof(Small)), "counters");
+            hip_ok(hipHostMalloc(&pinned_, sizeof(Small), hipHostMallocDefault), "pinned counters");
+            // every level's slots, then every level's insertion words, then the compressed arrays
+            size_t slots = 0;
+            for (int l = 0; l < L_; ++l) slots += topo_.table_[l].size();
+            hip_ok(hipMalloc(&block_, slots * (sizeof(int4) + sizeof(unsigned long long) + sizeof(int32_t))), "tables");
+            int4* sp = static_cast<int4*>(block_);
+            unsigned long long* wp = reinterpret_cast<unsigned long long*>(sp + slots);
+            int32_t* cp = reinterpret_cast<int32_t*>(wp + slots);
             for (int l = 0; l < L_; ++l) {
                 const size_t size = topo_.table_[l].size();
-                hip_ok(hipMalloc(&slot_[l], size * sizeof(int4)), "slots");
-                hip_ok(hipMalloc(&word_[l], size * sizeof(unsigned long long)), "insertion words");
-                hip_ok(hipMalloc(&comp_[l], size * sizeof(int32_t)), "compressed array");
-                hip_ok(hipMemsetAsync(slot_[l], 0, size * sizeof(int4), st_), "clear slots");
-                hip_ok(hipMemsetAsync(word_[l], 0xff, size * sizeof(unsigned long long), st_), "clear words");
+                slot_[l] = sp;
+                word_[l] = wp;
+                comp_[l] = cp;
+                sp += size;
+                wp += size;
+                cp += size;
             }
+            hip_ok(hipMemsetAsync(slot_[0], 0, slots * sizeof(int4), st_), "clear slots");
+            hip_ok(hipMemsetAsync(word_[0], 0xff, slots * sizeof(unsigned long long), st_), "clear words");
             hip_ok(hipMemsetAsync(small_, 0, sizeof(Small), st_), "clear counters");
         } catch (...) {
             release();   // a throwing constructor gets no destructor call
@@ -596,10 +606,8 @@ public:
         for (int i = max_level_; i >= 0; --i) {
             const int n = cnt_[i];
             if (!n) continue;
-            int32_t* tgt = tgt_.need<int32_t>(n);
-            if (on_device) {
-                k::b_target<<<grid_of(n), kBlock, 0, st_>>>(T, i, comp_[i], n, td, tgt);
-            } else {
+            int32_t* tgt = on_device ? nullptr : tgt_.need<int32_t>(n);
+            if (!on_device) {
                 std::vector<int4> rec(n);
                 int4* d = gather_.need<int4>(n);
                 k::b_gather<<<grid_of(n), kBlock, 0, st_>>>(slot_[i], comp_[i], n, d);
@@ -612,7 +620,8 @@ public:
                 hip_ok(hipStreamSynchronize(st_), "targets");
             }
             int32_t* flag = flag_.need<int32_t>(n);
-            k::b_leaf_flags<<<grid_of(n), kBlock, 0, st_>>>(T, i, comp_[i], n, tgt, max_level_, flag, err());
+            k::b_leaf_flags<<<grid_of(n), kBlock, 0, st_>>>(T, i, comp_[i], n, on_device ? nullptr : tgt, td, max_level_,
+                                                             flag, err());
             k::b_ghost_flags<<<grid_of(n), kBlock, 0, st_>>>(T, i, comp_[i], n, err());
             if (i + 1 >= L_) continue;   // no level above: every flag is clear
             int32_t* pos = pos_.need<int32_t>(n);
@@ -621,11 +630,9 @@ public:
             const int nc = 8 * n;
             int4* cand = cand_.need<int4>(nc);
             uint8_t* valid = valid_.need<uint8_t>(nc);
-            hip_ok(hipMemsetAsync(valid, 0, nc, st_), "candidates");
-            hip_ok(hipMemsetAsync(small_->fresh + (i + 1), 0, sizeof(unsigned int), st_), "counter");
             k::b_children<<<grid_of(long(n) * 8), kBlock, 0, st_>>>(T, i, comp_[i], n, flag, pos, cand, valid,
                                                                      small_->fresh + (i + 1), err());
-            insert(i + 1, cand, valid, nc);
+            insert(i + 1, cand, valid, nc, small_->refined + i, 8);   // entries below 8 x (flagged tiles)
         }
         compress();
         int total = 0;
@@ -648,13 +655,10 @@ public:
             uint32_t* dword = dword_.need<uint32_t>(dsize);
             uint32_t* drank = drank_.need<uint32_t>(dsize);
             uint32_t* dslot = dslot_.need<uint32_t>(nc);
-            hip_ok(hipMemsetAsync(dword, 0xff, dsize * sizeof(uint32_t), st_), "clear set");
-            hip_ok(hipMemsetAsync(drank, 0xff, dsize * sizeof(uint32_t), st_), "clear set");
-            hip_ok(hipMemsetAsync(dslot, 0xff, size_t(nc) * sizeof(uint32_t), st_), "clear set");
+            k::b_set_init<<<grid_of(std::max<long>(dsize, nc)), kBlock, 0, st_>>>(dword, drank, dsize, dslot, nc);
             k::b_dedupe<<<grid_of(nc), kBlock, 0, st_>>>(cand, valid, nc, dword, drank, dsize - 1, dslot, err());
-            hip_ok(hipMemsetAsync(small_->fresh + L, 0, sizeof(unsigned int), st_), "counter");
             k::b_representative<<<grid_of(nc), kBlock, 0, st_>>>(valid, nc, dslot, drank, dsize - 1, small_->fresh + L);
-            insert(L, cand, valid, nc);
+            insert(L, cand, valid, nc, nullptr, 0);
         }
         compress();
     }
@@ -667,11 +671,9 @@ public:
             if (!dirty_[l]) continue;
             dirty_[l] = false;
             const uint32_t size = uint32_t(topo_.table_[l].size());
-            int32_t* flag = flag_.need<int32_t>(size);
             int32_t* pos = pos_.need<int32_t>(size);
-            k::b_used<<<grid_of(size), kBlock, 0, st_>>>(slot_[l], size, flag);
-            scan(flag, int(size), pos, small_->count + l);
-            k::b_scatter<<<grid_of(size), kBlock, 0, st_>>>(flag, pos, size, comp_[l]);
+            scan(nullptr, int(size), pos, small_->count + l, slot_[l]);
+            k::b_scatter<<<grid_of(size), kBlock, 0, st_>>>(slot_[l], pos, size, comp_[l]);
         }
         read_back();
         max_level_ = -1;
@@ -789,23 +791,23 @@ private:
         int32_t count[kMaxLevels];         // compressed count per level (scan totals)
         int32_t used[kMaxLevels];          // used slots per level
         int32_t refined[kMaxLevels];       // flagged LEAF tiles per level in this refine pass
-        unsigned int fresh[kMaxLevels];    // new keys of the level's insertion round
-        int skip[kMaxLevels];              // the round did not fit: nothing inserted
+        unsigned int fresh[kMaxLevels];    // new keys of the level's insertion round (0 between rounds)
     };
 
     void release()
     {
         if (st_) (void)hipStreamSynchronize(st_);
+        if (block_) (void)hipFree(block_);
+        block_ = nullptr;
         for (int l = 0; l < L_; ++l) {
-            if (slot_[l]) (void)hipFree(slot_[l]);
-            if (word_[l]) (void)hipFree(word_[l]);
-            if (comp_[l]) (void)hipFree(comp_[l]);
             slot_[l] = nullptr;
             word_[l] = nullptr;
             comp_[l] = nullptr;
         }
         if (small_) (void)hipFree(small_);
         small_ = nullptr;
+        if (pinned_) (void)hipHostFree(pinned_);
+        pinned_ = nullptr;
         if (st_) (void)hipStreamDestroy(st_);
         st_ = nullptr;
     }
@@ -825,46 +827,50 @@ private:
     // every device counter to the host (one synchronisation); throws on the error flag
     void read_back()
     {
-        hip_ok(hipMemcpyAsync(&host_, small_, sizeof(Small), hipMemcpyDeviceToHost, st_), "counters");
+        hip_ok(hipMemcpyAsync(pinned_, small_, sizeof(Small), hipMemcpyDeviceToHost, st_), "counters");
         hip_ok(hipStreamSynchronize(st_), "counters");
+        host_ = *pinned_;
         if (host_.err == 1) throw std::runtime_error("hash table full (device octree construction)");
         if (host_.err)
             throw std::runtime_error("device octree construction: insertion did not finish (code " +
                                      std::to_string(host_.err) + ")");
     }
 
-    // exclusive prefix sums of n 0/1 flags into pos; the total goes to *total (device)
-    void scan(const int32_t* flag, int n, int32_t* pos, int32_t* total)
+    // exclusive prefix sums of n 0/1 flags (or of the used bits of n slots) into pos; the
+    // total goes to *total (device)
+    void scan(const int32_t* flag, int n, int32_t* pos, int32_t* total, const int4* slots = nullptr)
     {
         const int nt = (n + k::kScanTile - 1) / k::kScanTile;
         if (nt == 1) {   // one workgroup does it all
-            k::b_scan_tiles<<<1, kBlock, 0, st_>>>(flag, n, pos, nullptr, total);
+            k::b_scan_tiles<<<1, kBlock, 0, st_>>>(flag, slots, n, pos, nullptr, total);
             return;
         }
         int32_t* tsum = tsum_.need<int32_t>(nt);
-        k::b_scan_tiles<<<nt, kBlock, 0, st_>>>(flag, n, pos, tsum, nullptr);
+        k::b_scan_tiles<<<nt, kBlock, 0, st_>>>(flag, slots, n, pos, tsum, nullptr);
         k::b_scan_sums<<<1, kBlock, 0, st_>>>(tsum, nt, total);
         k::b_scan_add<<<grid_of(n), kBlock, 0, st_>>>(pos, n, tsum);
     }
 
-    // one round of ordered insertion of the valid candidates (at most nc) into a level
-    // table; the room check runs on the device (small_->fresh[level] new keys)
-    void insert(int level, const int4* cand, const uint8_t* valid, int nc)
+    // one round of ordered insertion of the valid candidates (at most nc; with limit, those
+    // below per * *limit) into a level table; the room check (small_->fresh[level] new keys
+    // next to small_->used[level]) and its bookkeeping run on the device
+    void insert(int level, const int4* cand, const uint8_t* valid, int nc, const int32_t* limit, int per)
     {
         dirty_[level] = true;
         const size_t size = topo_.table_[level].size();
-        k::b_capacity<<<1, 64, 0, st_>>>(small_->fresh + level, small_->used + level, uint32_t(size),
-                                         small_->skip + level, err());
-        k::b_insert<<<grid_of(nc), kBlock, 0, st_>>>(word_[level], uint32_t(size) - 1u, cand, valid, nc,
-                                                     small_->skip + level, err());
-        k::b_materialize<<<grid_of(long(size)), kBlock, 0, st_>>>(word_[level], slot_[level], uint32_t(size), cand);
+        k::b_insert<<<grid_of(nc), kBlock, 0, st_>>>(word_[level], uint32_t(size) - 1u, cand, valid, nc, limit, per,
+                                                     small_->fresh + level, small_->used + level, err());
+        k::b_materialize<<<grid_of(long(size)), kBlock, 0, st_>>>(word_[level], slot_[level], uint32_t(size), cand,
+                                                                  small_->fresh + level, small_->used + level);
     }
 
     Topology& topo_;
     int L_;
     hipStream_t st_ = nullptr;
     Small* small_ = nullptr;
+    Small* pinned_ = nullptr;   // host-pinned landing buffer of the counters
     Small host_{};
+    void* block_ = nullptr;     // the level tables in one allocation
     int4* slot_[kMaxLevels] = {};
     unsigned long long* word_[kMaxLevels] = {};
     int32_t* comp_[kMaxLevels] = {};
