@@ -146,6 +146,19 @@ def test_mixed_host_and_device_steps():
     _assert_same(out[0], out[1], "edit+rebuild")
 
 
+@pytest.mark.parametrize("log2_hash", [(1, 4, 7, 9, 12, 13, 16, 16, 16, 16), (1, 4, 8, 9, 12, 13, 16, 16, 16, 16)])
+def test_device_build_dense_tables(log2_hash):
+    """Tables near full (long probe clusters, many displacements in each insertion round):
+    the device slots still equal the sequential insertion's."""
+    t = lambda: P.make_target(P.TARGET_SDF_BANDS, 2, 5, P.SDF_SPHERE, H.SPHERE_C2, {5: 0.05, 4: 0.15})
+    gh = _build(t(), False, log2_hash)
+    lv, _, _ = gh.tiles()
+    load = [int((lv == l).sum()) / 2 ** log2_hash[l] for l in range(6)]
+    print("\nload factors", [round(x, 2) for x in load], flush=True)
+    assert max(load) > 0.9
+    _assert_same(gh, _build(t(), True, log2_hash), log2_hash)
+
+
 def test_device_build_hash_full():
     """A level table too small for the refinement fails with CMG_ERR_HASH_FULL on both sides."""
     small = (4, 4, 5, 16, 16, 16, 16, 16, 16, 16)   # level 2 needs 64 slots
