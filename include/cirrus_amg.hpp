@@ -95,6 +95,9 @@ public:
               "iterativeRefine");
     }
 
+    // rebuild + iterativeRefine + finalize on the device over one copy of the tables
+    void buildOnDevice(const cmg_target& preset) { check(cmg_grid_build_device(h_.get(), &preset), "buildOnDevice"); }
+
     // all-tiles arrays, CalculateNeighborTiles (src/PoissonGrid.cu:139-152), parent/child links
     void finalize(bool on_device = false)
     {
