@@ -149,6 +149,7 @@ cmg_status cmg_grid_finalize(cmg_grid* g)
     CAPI_GUARD({
         g->topo->finalize();
         dev_free(g->dev);
+        ++g->generation;
         g->uploaded = false;
         g->cells = HostCells{};
         ensure_types(g);
@@ -185,6 +186,7 @@ cmg_status cmg_grid_build_device(cmg_grid* g, const cmg_target* target)
     CAPI_GUARD({
         device_build(*g->topo, *target);
         dev_free(g->dev);
+        ++g->generation;
         g->uploaded = false;
         g->cells = HostCells{};
         ensure_types(g);
@@ -199,6 +201,7 @@ cmg_status cmg_grid_finalize_device(cmg_grid* g)
     CAPI_GUARD({
         device_finalize(*g->topo);
         dev_free(g->dev);
+        ++g->generation;
         g->uploaded = false;
         g->cells = HostCells{};
         ensure_types(g);

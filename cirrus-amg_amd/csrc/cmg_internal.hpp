@@ -90,6 +90,7 @@ struct cmg_grid {
     bool uploaded = false;
     bool types_on_device = false;
     int assembled_c0 = -1;
+    int generation = 0;   // bumped by every (re)finalization: solvers made before it are stale
 };
 
 struct cmg_solver;

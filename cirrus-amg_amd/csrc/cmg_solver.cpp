@@ -22,6 +22,7 @@
 
 struct cmg_solver {
     cmg_grid* g = nullptr;
+    int gen = 0;                   // the grid's generation this solver's buffers were sized for
     int family = CMG_SOLVER_AMG;   // AMGSolver, GMGSolver or CMGSolver
     int c0 = 11;
     float Rm = 0.5f, Rr = 1.0f, P = 1.0f;
@@ -1146,6 +1147,7 @@ cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_s
     if (!out || c0 < 0 || c0 + 3 >= kChannels) return CMG_ERR_BAD_ARG;
     auto* s = new cmg_solver;
     s->g = g;
+    s->gen = g->generation;
     s->c0 = c0;
     s->Rm = Rm;
     s->Rr = Rr;
@@ -1177,9 +1179,22 @@ cmg_status solver_create_geo(cmg_grid* g, int family, float one_over_alpha, floa
     return st;
 }
 
+// a solver's buffers are sized for the grid as it was when the solver was made; a grid
+// finalized again since (new tiles, new device arrays) needs a new solver
+static cmg_status check_solver(const cmg_solver* s)
+{
+    if (!s || !s->g) return CMG_ERR_BAD_ARG;
+    if (s->gen != s->g->generation) {
+        set_error("the grid was finalized again after this solver was created: create a new solver");
+        return CMG_ERR_STATE;
+    }
+    return CMG_OK;
+}
+
 cmg_status solver_apply(cmg_solver* s, int x, int y)
 {
     if (!s || !valid_channel(x) || !valid_channel(y) || x == y) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     CMG_TRY({
         ensure_buffers(s);
         DevGrid& d = s->g->dev;
@@ -1193,6 +1208,7 @@ cmg_status solver_apply(cmg_solver* s, int x, int y)
 cmg_status solver_damped_jacobi(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
 {
     if (!s || !o || !st || s->family != CMG_SOLVER_GMG || o->max_iters < 0 || o->cycle_coef_fp16) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     *st = cmg_solve_stats{};
     bool converged = false;
     CMG_TRY({
@@ -1272,6 +1288,7 @@ void prepare_coef_fp16(cmg_solver* s);
 cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x0)
 {
     if (!s) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     if (auto st = check_opts(o)) return st;
     if (o->cycle_coef_fp16 && s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     if (!valid_channel(x) || !valid_channel(rhs) || !valid_channel(x0)) return CMG_ERR_BAD_ARG;
@@ -1328,6 +1345,7 @@ struct DeriveScope {
 cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
 {
     if (!s || !st) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     if (auto e = check_opts(o)) return e;
     if (s->family != CMG_SOLVER_AMG && o->precision != CMG_PREC_FP32) return CMG_ERR_BAD_ARG;
     if (s->family != CMG_SOLVER_AMG && o->cycle_coef_fp16) return CMG_ERR_BAD_ARG;
@@ -1420,6 +1438,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
 cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats* st)
 {
     if (!s || !st) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     if (auto e = check_opts(o)) return e;
     if (o->cycle != CMG_CYCLE_FAS_W && o->cycle != CMG_CYCLE_FAS_V) return CMG_ERR_BAD_ARG;
     if (o->cycle_coef_fp16) return CMG_ERR_BAD_ARG;   // a stand-alone cycle solve would converge to the FP16 operator
@@ -1499,6 +1518,7 @@ cmg_status solver_net_infer(cmg_solver* s, int r, int dch, int y)
 {
     if (!s || !s->net_w || !valid_channel(r) || !valid_channel(dch) || !valid_channel(y) || y == r || y == dch)
         return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     CMG_TRY({
         DevGrid& d = s->g->dev;
         launch_net_infer(d, d.subtree[d.max_level], dev_channel(d, r), dev_channel(d, dch), s->net_w, dev_channel(d, y),
@@ -1733,6 +1753,7 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
     if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 11 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     if (kernel == 9 && !s->net_w) return CMG_ERR_STATE;
     CMG_TRY({
         ensure_buffers(s);
@@ -1794,6 +1815,7 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
 cmg_status solver_get_x64(cmg_solver* s, double* host)
 {
     if (!s || !host) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
     if (!s->x64) return CMG_ERR_STATE;
     CMG_TRY({
         HIPCHK(hipMemcpy(host, s->x64, size_t(s->g->dev.N) * kCells * sizeof(double), hipMemcpyDeviceToHost));

@@ -252,7 +252,10 @@ void cmg_solve_opts_default(cmg_solve_opts* o);   /* FluidEuler::project setting
  * CMGSolver (src/CMGSolver.h:12-47) */
 enum { CMG_SOLVER_AMG = 0, CMG_SOLVER_GMG = 1, CMG_SOLVER_CMG = 2 };
 
-/* AMGSolver(coeff_channel, R_matrix, R_restrict, P) (src/AMGSolver.h:32-42) */
+/* AMGSolver(coeff_channel, R_matrix, R_restrict, P) (src/AMGSolver.h:32-42).
+ * A solver holds device buffers sized for the grid as finalized when it was made;
+ * once the grid is finalized again (cmg_grid_finalize, _finalize_device,
+ * _build_device) its calls return CMG_ERR_STATE and a new solver is needed. */
 cmg_status cmg_solver_create(cmg_grid* g, int coeff_channel, float R_matrix, float R_restrict,
                              float P, cmg_solver** out);
 /* GMGSolver(one_over_alpha, prolong_coeff) / CMGSolver(one_over_alpha, prolong_coeff)

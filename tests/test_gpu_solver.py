@@ -373,3 +373,23 @@ def test_fused_boundary_kernels(case, cycle, flexible):
         assert abs(out[0][1] - out[1][1]) <= 1e-3 * out[1][1]
         x0, x1 = out[0][2].astype(np.float64), out[1][2].astype(np.float64)
         assert np.linalg.norm(x0 - x1) <= 1e-5 * np.linalg.norm(x1)
+
+
+def test_solver_refuses_a_refinalized_grid():
+    """A solver's buffers are sized for the grid it was made on; after the grid is
+    finalized again every solver call is refused (CMG_ERR_STATE), a new solver works."""
+    go, gp = H.build_pair("uniform3")
+    _rhs(go, gp, 3)
+    s = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    s.solve()
+    gp.finalize()
+    gp.set_leaf_types_preset(P.make_target(P.TARGET_UNIFORM, 3, 3))
+    gp.prepare_types_and_coeffs(H.C0, H.R_MATRIX)
+    gp.set_channel(1, go.channel(1))
+    with pytest.raises(P.CmgError) as e:
+        s.solve()
+    assert e.value.status == 2
+    with pytest.raises(P.CmgError):
+        s.cycle(P.default_opts(), 4, 1, 3)
+    it, rel = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0).solve()
+    assert rel < 1e-6
