@@ -1682,64 +1682,45 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
     else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
 }
 
+// one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE> with nt threads a tile
+template <bool DOT, bool Q, bool SAVE>
+static void jacobi_at(int nt, const LaunchList& l, hipStream_t st, const TopoView& tp, const float* Xin, float* Xout,
+                      const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0)
+{
+    switch (nt) {
+    case 128: k::jacobi<128, DOT, Q, SAVE><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    case 512: k::jacobi<512, DOT, Q, SAVE><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    default: k::jacobi<256, DOT, Q, SAVE><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    }
+}
+
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
                   const double* S, hipStream_t st, double* partial, float* X0)
 {
     if (X0 && partial) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
     const LaunchList& l = d.subtree[level];
+    if (!nonempty(l)) return 0;
+    const bool q = C.q[0] != nullptr;
+    const bool one_tile = !(persistent_mask() & PERSIST_JACOBI);
+    if ((partial || X0 || q) && !one_tile)
+        throw std::invalid_argument("jacobi: b.x partials, the FAS save and FP16 coefficients need the one-tile form");
+    const TopoView tp = view(d);
+    const int nt = q ? q_jacobi_threads() : stencil_threads();
     if (partial) {
-        // one partial per list entry (the one-tile form only)
-        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: b.x partials need the one-tile form");
-        if (!nonempty(l)) return 0;
-        const TopoView tp = view(d);
-        if (C.q[0]) {
-            switch (q_jacobi_threads()) {
-            case 128: k::jacobi<128, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
-            case 512: k::jacobi<512, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
-            default: k::jacobi<256, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
-            }
-            return l.n;
-        }
-        switch (stencil_threads()) {
-        case 128: k::jacobi<128, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
-        case 512: k::jacobi<512, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
-        default: k::jacobi<256, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial); break;
-        }
+        // one partial per list entry
+        if (q) jacobi_at<true, true, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
+        else jacobi_at<true, false, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
         return l.n;
     }
-    if (!nonempty(l)) return 0;
     if (X0) {
-        // the first coarse-level sweep with the level's FAS save (one-tile form)
-        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: the FAS save needs the one-tile form");
-        const TopoView tp = view(d);
-        if (C.q[0]) {
-            switch (q_jacobi_threads()) {
-            case 128: k::jacobi<128, false, true, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
-            case 512: k::jacobi<512, false, true, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
-            default: k::jacobi<256, false, true, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
-            }
-        } else {
-            switch (stencil_threads()) {
-            case 128: k::jacobi<128, false, false, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
-            case 512: k::jacobi<512, false, false, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
-            default: k::jacobi<256, false, false, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr, X0); break;
-            }
-        }
+        // the first coarse-level sweep with the level's FAS save
+        if (q) jacobi_at<false, true, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0);
+        else jacobi_at<false, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0);
         return 0;
     }
-    if (C.q[0]) {
-        if (persistent_mask() & PERSIST_JACOBI) throw std::invalid_argument("jacobi: FP16 coefficients need the one-tile form");
-        const TopoView tp = view(d);
-        switch (q_jacobi_threads()) {
-        case 128: k::jacobi<128, false, true><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
-        case 512: k::jacobi<512, false, true><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
-        default: k::jacobi<256, false, true><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, nullptr); break;
-        }
-        return 0;
-    }
-    if (!(persistent_mask() & PERSIST_JACOBI))
-        CMG_NT_DISPATCH(k::jacobi, l.n, st, view(d), l.d, Xin, Xout, G, C, B, w, S, (double*)nullptr);
-    else CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, view(d), l.d, l.n, Xin, Xout, G, C, B, w, S);
+    if (q) jacobi_at<false, true, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, nullptr);
+    else if (one_tile) jacobi_at<false, false, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, nullptr);
+    else CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, tp, l.d, l.n, Xin, Xout, G, C, B, w, S);
     return 0;
 }
 
