@@ -89,3 +89,19 @@ def test_fp16_cycle_only_as_a_preconditioner():
     assert e.value.status == 1
     with pytest.raises(P.CmgError):
         solver.solve(opts=P.default_opts(cycle_coef_fp16=2))
+
+
+def test_fp16_copies_out_of_range_are_refused():
+    """A coefficient whose c / h does not fit FP16 makes the solve fail with
+    CMG_ERR_BAD_ARG instead of running the cycle on infinities."""
+    go, gp = H.build_pair("uniform3")
+    go.rhs_uniform(1, 2, True)
+    gp.set_channel(1, go.channel(1))
+    c = gp.channel(H.C0).copy()
+    c[0, 0] = 1e6          # c / h = 6.4e7 on a level-3 tile, beyond FP16's 65504
+    gp.set_channel(H.C0, c)
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    with pytest.raises(P.CmgError) as e:
+        solver.solve(opts=P.default_opts(cycle_coef_fp16=1))
+    assert e.value.status == 1
+    assert "FP16" in str(e.value)
