@@ -260,9 +260,17 @@ void dev_upload(DevGrid& d, const Topology& topo)
 
 // ------------------------------------------------------------- single ops ---
 
+// argument errors found while enqueueing (std::invalid_argument) are CMG_ERR_BAD_ARG,
+// other logic errors CMG_ERR_STATE, everything else (HIP failures) CMG_ERR_HIP
 #define CMG_TRY(...)                                         \
     try {                                                    \
         __VA_ARGS__;                                         \
+    } catch (const std::invalid_argument& ex) {              \
+        set_error(ex.what());                                \
+        return CMG_ERR_BAD_ARG;                              \
+    } catch (const std::logic_error& ex) {                   \
+        set_error(ex.what());                                \
+        return CMG_ERR_STATE;                                \
     } catch (const std::exception& ex) {                     \
         set_error(ex.what());                                \
         return CMG_ERR_HIP;                                  \
