@@ -59,7 +59,6 @@ struct DevGrid {
     LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
     LaunchList ghosts;                  // all GHOST tiles
     LaunchList leafghost;               // all LEAF and GHOST tiles
-    LaunchList leafcoarse;              // LEAF tiles below the finest level
     LaunchList cyclezero;               // GHOST tiles and LEAF tiles below the finest level
     int max_list = 0;
     int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU

@@ -946,6 +946,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
     for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
     load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o, C.q);
     const bool save = SAVE && (tp.ttype[t] & NONLEAF);
+    const bool dot_tile = DOT && (tp.ttype[t] & LEAF);   // b.x is r.z on LEAF tiles only
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 512 / NT; ++c) {
@@ -964,7 +965,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
         const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
         const float xo = o.x[c] + w * delta;
         Xout[size_t(t) * 512 + v] = xo;
-        if (DOT) acc += double(b[c]) * double(xo);
+        if (dot_tile) acc += double(b[c]) * double(xo);
     }
     if (DOT) {
         const double sum = block_sum(acc, red);
@@ -1220,9 +1221,9 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
     __shared__ double red[16];
     __shared__ double red2[16];
     if (kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0) return;
-    // each thread adds its strided partials in index order; the loads of 8
+    // each thread adds its strided partials in index order; the loads of 16
     // consecutive terms are issued together so that they overlap
-    constexpr int U = 8;
+    constexpr int U = 16;
     const int step = blockDim.x;
     double v = 0, v2 = 0;
     int i = threadIdx.x;
@@ -1516,8 +1517,8 @@ static inline bool nonempty(const LaunchList& l) { return l.n > 0; }
 // diagonals even where they could be derived from the face coefficients.  Bit 4
 // keeps the preconditioner's boundary work in launches of their own: the first
 // finest-level Jacobi sweep (otherwise fused into the PCG x/r update) and r.z
-// (otherwise summed by the last finest-level sweep plus a dot over the coarser
-// LEAF tiles).  Defaults may be overridden by
+// (otherwise summed over each level's LEAF tiles by that level's last
+// post-sweep).  Defaults may be overridden by
 // the environment (CMG_STENCIL_THREADS, CMG_PERSISTENT) or set_kernel_forms.
 enum { PERSIST_APPLY = 1, PERSIST_JACOBI = 2, PERSIST_RESTRICT = 4 };
 static std::atomic<int> g_threads{0}, g_mask{-1};
@@ -1565,6 +1566,10 @@ static int q_jacobi_threads()
     const int v = stencil_threads();
     return g_threads_chosen.load(std::memory_order_relaxed) ? v : 128;
 }
+
+// the operator on LEAF tiles likewise runs 128 threads a tile unless a thread count
+// was chosen (106 us against 108 at 256 on the 1024^3 torus workload)
+static int apply_threads() { return q_jacobi_threads(); }
 
 void get_kernel_forms(int* mask, int* threads)
 {
@@ -1645,13 +1650,13 @@ int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* par
     if (!(persistent_mask() & PERSIST_APPLY)) {
         const TopoView tp = view(d);
         if (partial) {
-            switch (stencil_threads()) {
+            switch (apply_threads()) {
             case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
             case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
             default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
             }
         } else {
-            switch (stencil_threads()) {
+            switch (apply_threads()) {
             case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
             case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
             default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;

@@ -149,7 +149,6 @@ void dev_free(DevGrid& d)
     free_list(d.all);
     free_list(d.ghosts);
     free_list(d.leafghost);
-    free_list(d.leafcoarse);
     free_list(d.cyclezero);
     for (int l = 0; l < kMaxLevels; ++l) {
         free_list(d.subtree[l]);
@@ -217,7 +216,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     up(d.child, topo.child);
     d.ctype = dmalloc<uint8_t>(size_t(N) * kCells);
     HIPCHK(hipMemset(d.ctype, 0, size_t(N) * kCells));
-    std::vector<int32_t> leafs, all, ghosts, leafghost, leafcoarse, cyclezero;
+    std::vector<int32_t> leafs, all, ghosts, leafghost, cyclezero;
     std::vector<int32_t> sub[kMaxLevels], nl[kMaxLevels], pr[kMaxLevels], la[kMaxLevels];
     for (int t = 0; t < N; ++t) {
         const uint8_t ty = topo.t_type[t];
@@ -229,7 +228,6 @@ void dev_upload(DevGrid& d, const Topology& topo)
         if (ty & (LEAF | NONLEAF)) sub[l].push_back(t);
         if (ty & NONLEAF) nl[l].push_back(t);
         if (ty & LEAF) la[l].push_back(t);
-        if ((ty & LEAF) && l < topo.max_level()) leafcoarse.push_back(t);   // (num_levels counts hash levels)
         if ((ty & GHOST) || ((ty & LEAF) && l < topo.max_level())) cyclezero.push_back(t);
         pr[l].push_back(t);
     }
@@ -237,7 +235,6 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.all = make_list(topo, all);
     d.ghosts = make_list(topo, ghosts);
     d.leafghost = make_list(topo, leafghost);
-    d.leafcoarse = make_list(topo, leafcoarse);
     d.cyclezero = make_list(topo, cyclezero);
     d.max_list = std::max(d.leafs.n, d.all.n);
     for (int l = 0; l < topo.num_levels(); ++l) {
@@ -638,6 +635,15 @@ struct CycleCtx {
     double* rz = nullptr;     // the last finest-level post-sweep writes its b.x partials here
 };
 
+// slice of the r.z partials written by a level's last post-sweep: one per list entry,
+// the finest level first
+int rz_offset(const DevGrid& d, int level)
+{
+    int off = 0;
+    for (int l = d.max_level; l > level; --l) off += d.subtree[l].n;
+    return off;
+}
+
 // `x_zero`: every x of the cycle is still zero (first sweep at the finest level);
 // `fas_save`: the level's FAS save is still due and rides on the first sweep
 void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false)
@@ -659,8 +665,10 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
                 launch_jacobi_zero(d, level, other, c.C.c[3], c.B, float(c.o->omega), c.S, st, c.C.q[3]);
         }
         else {
-            const bool last_fine = c.rz && order == 1 && level == d.max_level && it == iters - 1;
-            launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st, last_fine ? c.rz : nullptr,
+            // the level's last post-sweep sums b.x = r.z over its LEAF tiles into its slice of c.rz
+            const bool last = c.rz && order == 1 && it == iters - 1;
+            launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st,
+                          last ? c.rz + rz_offset(d, level) : nullptr,
                           fas_save && it == 0 ? c.X0 : nullptr);
         }
         std::swap(cur, other);
@@ -912,9 +920,9 @@ bool first_sweep_fusable(const cmg_solver* s, const cmg_solve_opts* o)
            s->g->dev.max_level >= 1 && fuse_first_sweep_allowed();
 }
 
-// r.z can be summed by the last finest-level post-sweep (whose b is r and whose
-// output is z there) plus a dot over the coarser LEAF tiles, whose z is final
-// after their own level's post-smoothing
+// r.z can be summed by each level's last post-sweep over its LEAF tiles (whose b
+// is r and whose output is z, final after that level's post-smoothing), plus a
+// dot over the LEAF tiles of level 0 when there are any
 bool rz_fusable(const cmg_solver* s, const cmg_solve_opts* o)
 {
     return s->family == CMG_SOLVER_AMG && (o->cycle == CMG_CYCLE_FAS_W || o->cycle == CMG_CYCLE_FAS_V) &&
@@ -948,9 +956,11 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     const bool frz = rz_fusable(s, o);
     enqueue_precondition(s, o, s->S, fuse, frz ? s->partial : nullptr);
     if (frz) {
-        const int nf = d.subtree[d.max_level].n;
-        const int nc = launch_dot(d, d.leafcoarse, R, Z, s->partial + nf, s->S, st);
-        launch_finish(s->partial, nullptr, nf + nc, s->S, FIN_BETA, st);
+        // every level's last post-sweep wrote its LEAF tiles' r.z; level 0 (the direct solve) has
+        // LEAF tiles only on one-level grids
+        const int nl = rz_offset(d, 0);
+        const int n0 = launch_dot(d, d.leafat[0], R, Z, s->partial + nl, s->S, st);
+        launch_finish(s->partial, nullptr, nl + n0, s->S, FIN_BETA, st);
     } else if (o->flexible) {
         // flexible CG: beta = z.(r - r_old) / gamma_old (Polak-Ribiere; AMGSolver.cu:1520 uses Fletcher-Reeves)
         const int n3 = launch_dot2(d, d.leafs, Z, R, AP, s->partial, s->partial2, s->S, st);
