@@ -1277,6 +1277,15 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
         if (!isfinite(S[S_BETA])) S[S_DONE] = 3;
         break;
     case FIN_MEAN: S[S_MEAN] = s / s2; break;
+    case FIN_RHS2:
+        // |b|^2 turns the relative threshold tol^2 into tol^2 |b|^2; a zero b is solved by
+        // x = 0, a non-finite one stops the solve
+        S[S_TMP] = s;
+        S[S_RHS2] = s;
+        if (!isfinite(s)) S[S_DONE] = 3;
+        else if (s == 0.0) S[S_DONE] = 1;
+        else S[S_THR2] = fmax(S[S_THR2] * s, 2.2250738585072014e-308);
+        break;
     case FIN_BETA_FLEX:
         // flexible CG (Polak-Ribiere): beta = z.(r - r_old) / gamma_old = -alpha z.Ap / gamma_old
         S[S_GAMMA_OLD] = S[S_GAMMA];

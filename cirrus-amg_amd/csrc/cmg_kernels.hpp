@@ -33,7 +33,8 @@ inline TopoView view(const DevGrid& d)
     return TopoView{d.ttype, d.toct, d.tlevel, d.nbr, d.parent, d.child, d.ctype, d.halo, d.ghalo, d.h0, d.derive};
 }
 
-enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5, FIN_BETA_FLEX = 6 };
+enum FinishKind { FIN_SUM = 0, FIN_GAMMA0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4, FIN_MEAN = 5, FIN_BETA_FLEX = 6,
+                  FIN_RHS2 = 7 };
 
 // launchers that write per-workgroup partials return how many they wrote
 int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st);

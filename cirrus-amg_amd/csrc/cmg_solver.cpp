@@ -1009,11 +1009,12 @@ void ensure_fp64(cmg_solver* s)
     }
 }
 
-void set_scalars(cmg_solver* s, double tol2, int max_iters)
+void set_scalars(cmg_solver* s, double tol2, int max_iters, int stride = 0)
 {
     double h[S_COUNT] = {};
-    h[S_THR2] = tol2;   // relative; FIN_RHS2 below turns it absolute
+    h[S_THR2] = tol2;   // relative; the FIN_RHS2 reduction turns it absolute
     h[S_MAXIT] = max_iters;
+    h[S_STRIDE] = stride > 0 ? stride : 0;
     DevGrid& d = s->g->dev;
     HIPCHK(hipMemcpyAsync(s->S, h, sizeof(h), hipMemcpyHostToDevice, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));   // h lives on this stack frame
@@ -1037,28 +1038,29 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     hipStream_t st = d.stream;
     float *X = dev_channel(d, CH_X), *R = dev_channel(d, CH_R), *P = dev_channel(d, CH_P), *AP = dev_channel(d, CH_AP),
           *Z = dev_channel(d, CH_Z);
-    set_scalars(s, tol * tol, max_iters);
-    // |b|^2 fixes the tolerance (AMGSolver.cu:1410-1425 with x0 = 0)
-    const int nb = launch_dot(d, d.leafs, R, R, s->partial, nullptr, st);
-    launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, st);
+    set_scalars(s, tol * tol, max_iters, stride);
     double h[S_COUNT];
-    read_scalars(s, h);
-    const double rhs2 = h[S_TMP];
     *iters = 0;
     *rel_res = 0;
+    // |b|^2 fixes the tolerance (AMGSolver.cu:1410-1425 with x0 = 0)
+    const int nb = launch_dot(d, d.leafs, R, R, s->partial, nullptr, st);
     if (!warm) {
+        // on the device, with no host round trip: a zero or non-finite b stops the solve
+        // through the stop flag before any cycle runs
+        launch_finish(s->partial, nullptr, nb, s->S, FIN_RHS2, st);
         launch_fill_list(d, d.leafs, X, 0.0f, st);
-    } else if (rhs2 != 0) {
+    } else {
+        launch_finish(s->partial, nullptr, nb, s->S, FIN_SUM, st);
+        read_scalars(s, h);
+        const double rhs2 = h[S_TMP];
+        if (rhs2 == 0) return 1;
+        if (!std::isfinite(rhs2)) return 3;
         // r = b - A x (warm start; the reference always starts from 0, AMGSolver.cu:1407);
         // the stopping test stays relative to |b|
         enqueue_apply(s, X, AP, nullptr, nullptr);
         launch_axpy(d, d.leafs, R, AP, -1.0f, st);
-    }
-    if (rhs2 == 0) return 1;
-    if (!std::isfinite(rhs2)) return 3;
-    double thr2 = tol * tol * rhs2;
-    thr2 = std::max(thr2, 2.2250738585072014e-308);
-    if (warm) {
+        double thr2 = tol * tol * rhs2;
+        thr2 = std::max(thr2, 2.2250738585072014e-308);
         // a warm start may already satisfy the tolerance: no iteration then
         const int nr = launch_dot(d, d.leafs, R, R, s->partial, nullptr, st);
         launch_finish(s->partial, nullptr, nr, s->S, FIN_SUM, st);
@@ -1067,14 +1069,14 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
             *rel_res = std::sqrt(h[S_TMP] / rhs2);
             return 1;
         }
+        double init[S_COUNT] = {};
+        init[S_THR2] = thr2;
+        init[S_MAXIT] = max_iters;
+        init[S_RHS2] = rhs2;
+        init[S_STRIDE] = stride > 0 ? stride : 0;
+        HIPCHK(hipMemcpyAsync(s->S, init, sizeof(init), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
     }
-    double init[S_COUNT] = {};
-    init[S_THR2] = thr2;
-    init[S_MAXIT] = max_iters;
-    init[S_RHS2] = rhs2;
-    init[S_STRIDE] = stride > 0 ? stride : 0;
-    HIPCHK(hipMemcpyAsync(s->S, init, sizeof(init), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
     if (o->is_pure_neumann) enqueue_recenter(s, R);
     enqueue_precondition(s, o, s->S);
     launch_copy_list(d, d.leafs, Z, P, s->S, st);
@@ -1082,6 +1084,8 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     launch_finish(s->partial, nullptr, ng, s->S, FIN_GAMMA0, st);
     if (max_iters <= 0) {
         read_scalars(s, h);
+        if (h[S_DONE] == 1 && h[S_RHS2] == 0) return 1;   // zero b
+        if (h[S_DONE] == 3) return 3;
         *rel_res = 1.0;
         return 4;
     }
@@ -1112,6 +1116,8 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
         if (h[S_DONE] != 0 || launched >= max_iters) break;
     }
     *iters = int(h[S_ITER]);
+    const double rhs2 = h[S_RHS2];
+    if (!(rhs2 > 0) || !std::isfinite(rhs2)) return rhs2 == 0 ? 1 : 3;   // zero b: x = 0; non-finite b
     *rel_res = std::sqrt(h[S_RR] / rhs2);
     const int done = int(h[S_DONE]);
     if (stride <= 0) {
