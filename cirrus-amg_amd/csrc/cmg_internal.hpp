@@ -28,6 +28,7 @@ struct LaunchList {
     int32_t* d = nullptr;
     int n = 0;        // workgroups to launch
     int tiles = 0;    // real tiles in the list
+    int4* meta = nullptr;   // prolongation lists: per entry (tile, parent, octant | type << 8, 0)
 };
 
 struct DevGrid {

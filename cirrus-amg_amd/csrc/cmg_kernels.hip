@@ -777,26 +777,34 @@ __global__ __launch_bounds__(256) void fill_v(const int32_t* __restrict__ list, 
 }
 
 // FASProlongateAndUpdateAMG128Kernel (src/AMGSolver.cu:1216-1255), four cells per thread
-__global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+// meta: the prolongation list's (tile, parent, octant | type << 8) records
+__global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int4* __restrict__ meta, int n, float* __restrict__ X,
                                                      const float* __restrict__ X0, float P, const double* __restrict__ S)
 {
     if (stopped(S)) return;
-    CMG_TILE_LOOP(list, n)
+    for (int w_ = blockIdx.x; 16 * (w_ >> 3) < n; w_ += gridDim.x)   // the pairing of CMG_TILE_LOOP
     {
+        const int pos = 16 * (w_ >> 3) + (w_ & 7) + 8 * int(threadIdx.x >> 7);
+        if (pos >= n) continue;
+        const int4 m = meta[pos];
+        const int t_ = m.x;
+        if (t_ < 0) continue;
         const size_t q = quad_index(t_);
         const int v0 = int(threadIdx.x & 127) * 4;   // cells v0..v0+3 share i, j; k = 0..3 or 4..7
-        const int pt = tp.parent[t_];
-        const int oc = tp.toct[t_];
-        const bool ghost = tp.ttype[t_] & GHOST;
+        const int pt = m.y;
+        const int oc = m.z & 7;
+        const bool ghost = (m.z >> 8) & GHOST;
         const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
         float4 x = reinterpret_cast<const float4*>(X)[q];
         const int i = v0 >> 6, j = (v0 >> 3) & 7, k0 = v0 & 7;
         const int pbase = cidx(4 * (oc & 1) + (i >> 1), 4 * ((oc >> 1) & 1) + (j >> 1), 4 * ((oc >> 2) & 1) + (k0 >> 1));
         float cx[2] = {0.0f, 0.0f}, cx0[2] = {0.0f, 0.0f};
         if (pt >= 0) {
+            // pbase is even (k0 / 2 is 0 or 2), so the two parent cells are one aligned float2
             const size_t pb = size_t(pt) * 512 + pbase;
-            cx[0] = X[pb]; cx[1] = X[pb + 1];
-            cx0[0] = X0[pb]; cx0[1] = X0[pb + 1];
+            const float2 a = *reinterpret_cast<const float2*>(X + pb), b = *reinterpret_cast<const float2*>(X0 + pb);
+            cx[0] = a.x; cx[1] = a.y;
+            cx0[0] = b.x; cx0[1] = b.y;
         }
         const uint8_t cts[4] = {ct.x, ct.y, ct.z, ct.w};
         float xs[4] = {x.x, x.y, x.z, x.w};
@@ -1778,7 +1786,8 @@ void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, 
 {
     const LaunchList& l = d.prolong[level];
     if (!nonempty(l)) return;
-    k::fas_prolong_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.d, l.n, X, X0, P, S);
+    if (!l.meta) throw std::logic_error("fas_prolong: the prolongation list has no parent records");
+    k::fas_prolong_v<<<vector_grid(d, l.n), 256, 0, st>>>(view(d), l.meta, l.n, X, X0, P, S);
 }
 
 void launch_propagate_ghosts(const DevGrid& d, float* X, hipStream_t st)

@@ -76,7 +76,7 @@ uint64_t spread3(uint32_t x)
 // Morton order (coordinates scaled to the finest level), then an XCD-aware
 // permutation: workgroup b runs on XCD b % 8, so XCD x gets the x-th contiguous
 // Morton chunk and neighbouring tiles share an L2.
-LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles)
+LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, bool with_meta = false)
 {
     LaunchList l;
     l.tiles = int(tiles.size());
@@ -107,12 +107,27 @@ LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles)
     l.n = int(order.size());
     l.d = dmalloc<int32_t>(order.size());
     HIPCHK(hipMemcpy(l.d, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (with_meta) {
+        // a tile's parent, octant and type next to it, so that one load gives a
+        // prolongation workgroup all it needs to address the parent
+        std::vector<int4> meta(order.size(), make_int4(-1, -1, 0, 0));
+        for (size_t i = 0; i < order.size(); ++i) {
+            const int t = order[i];
+            if (t < 0) continue;
+            const Coord& c = topo.t_coord[t];
+            const int oct = (c[0] & 1) | ((c[1] & 1) << 1) | ((c[2] & 1) << 2);
+            meta[i] = make_int4(t, topo.parent[t], oct | (int(topo.t_type[t]) << 8), 0);
+        }
+        l.meta = dmalloc<int4>(meta.size());
+        HIPCHK(hipMemcpy(l.meta, meta.data(), meta.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
     return l;
 }
 
 void free_list(LaunchList& l)
 {
     if (l.d) (void)hipFree(l.d);
+    if (l.meta) (void)hipFree(l.meta);
     l = LaunchList{};
 }
 
@@ -240,7 +255,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     for (int l = 0; l < topo.num_levels(); ++l) {
         d.subtree[l] = make_list(topo, sub[l]);
         d.nonleaf[l] = make_list(topo, nl[l]);
-        d.prolong[l] = make_list(topo, pr[l]);
+        d.prolong[l] = make_list(topo, pr[l], true);
         d.leafat[l] = make_list(topo, la[l]);
     }
     {
