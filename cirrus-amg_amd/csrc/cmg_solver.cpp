@@ -27,7 +27,7 @@ struct cmg_solver {
     int c0 = 11;
     float Rm = 0.5f, Rr = 1.0f, P = 1.0f;
     double* S = nullptr;
-    double* hS = nullptr;          // pinned ring of scalar snapshots: two slots of S_COUNT
+    double* hS = nullptr;          // pinned ring of scalar snapshots
     double* partial = nullptr;
     double* partial2 = nullptr;
     float* tmp = nullptr;          // Jacobi ping-pong buffer
@@ -48,7 +48,6 @@ struct cmg_solver {
     hipGraphExec_t gexec = nullptr;
     std::string gkey;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEvent_t poll_ev[2] = {nullptr, nullptr};   // snapshot copies of the two ring slots done
 };
 
 namespace cmg {
@@ -1119,31 +1118,17 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
             s->gkey = key;
         }
     }
-    // The host keeps one batch of iterations queued ahead of the snapshot it tests, so the
-    // device never waits for the host between batches.  A batch queued after the stop flag
-    // was set runs as no-ops (every iteration kernel returns on the flag and finish changes
-    // nothing), so the scalars of the tested snapshot are the final ones.
     const int poll = stride > 0 ? stride : std::max(1, max_iters);
     int launched = 0;
-    auto launch_batch = [&](int slot) {
+    for (;;) {
         const int k = std::min(poll, max_iters - launched);
         for (int i = 0; i < k; ++i) {
             if (o->use_graph) HIPCHK(hipGraphLaunch(s->gexec, st));
             else enqueue_iteration(s, o);
         }
         launched += k;
-        HIPCHK(hipMemcpyAsync(s->hS + slot * S_COUNT, s->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipEventRecord(s->poll_ev[slot], st));
-    };
-    int cur = 0;
-    launch_batch(cur);
-    for (;;) {
-        const bool more = launched < max_iters;
-        if (more) launch_batch(cur ^ 1);
-        HIPCHK(hipEventSynchronize(s->poll_ev[cur]));
-        std::memcpy(h, s->hS + cur * S_COUNT, S_COUNT * sizeof(double));
-        if (h[S_DONE] != 0 || !more) break;
-        cur ^= 1;
+        read_scalars(s, h);
+        if (h[S_DONE] != 0 || launched >= max_iters) break;
     }
     *iters = int(h[S_ITER]);
     const double rhs2 = h[S_RHS2];
@@ -1209,14 +1194,13 @@ cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_s
     try {
         s->S = dmalloc<double>(S_COUNT);
         HIPCHK(hipMemset(s->S, 0, S_COUNT * sizeof(double)));
-        HIPCHK(hipHostMalloc((void**)&s->hS, 2 * S_COUNT * sizeof(double), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&s->hS, S_COUNT * sizeof(double), hipHostMallocDefault));
         // room for a fused r.z: one partial per finest-level list entry plus a vector grid
         const size_t np = size_t(g->dev.max_list) + size_t(g->dev.wg_cap) + 64;
         s->partial = dmalloc<double>(np);
         s->partial2 = dmalloc<double>(np);
         HIPCHK(hipEventCreate(&s->e0));
         HIPCHK(hipEventCreate(&s->e1));
-        for (hipEvent_t& e : s->poll_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     } catch (const std::exception& ex) {
         set_error(ex.what());
         solver_destroy(s);
@@ -1322,8 +1306,6 @@ void solver_destroy(cmg_solver* s)
     if (s->hS) (void)hipHostFree(s->hS);
     if (s->e0) (void)hipEventDestroy(s->e0);
     if (s->e1) (void)hipEventDestroy(s->e1);
-    for (hipEvent_t e : s->poll_ev)
-        if (e) (void)hipEventDestroy(e);
     delete s;
 }
 
