@@ -61,7 +61,6 @@ struct DevGrid {
     LaunchList ghosts;                  // all GHOST tiles
     LaunchList leafghost;               // all LEAF and GHOST tiles
     LaunchList cyclezero;               // GHOST tiles and LEAF tiles below the finest level
-    LaunchList pupdate;                 // LEAF tiles (as in leafs), then GHOST tiles
     int max_list = 0;
     int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU
     int derive = 0;                     // set by a solve whose operator passed the diagonal check

@@ -626,33 +626,22 @@ __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* _
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// p = z + beta p on LEAF INTERIOR cells (AMGSolver.cu:1520-1536).  CLEAR: the list is the
-// LEAF tiles followed by the GHOST tiles, and z is set to 0 on GHOST tiles and on LEAF
-// tiles below the finest level after its last read -- the tiles whose x the next FAS
-// cycle with damped Jacobi must find at 0 (the cycle's own fill of those tiles is then
-// left out)
-template <bool CLEAR>
 __global__ __launch_bounds__(256) void update_p_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ P,
-                                                  float* __restrict__ Z, const double* __restrict__ S, int max_level)
+                                                  const float* __restrict__ Z, const double* __restrict__ S)
 {
     if (stopped(S)) return;
     const float b = float(S[S_BETA]);
     CMG_TILE_LOOP(list, n)
     {
         const size_t q = quad_index(t_);
-        const int ty = CLEAR ? int(tp.ttype[t_]) : int(LEAF);
-        if (ty & LEAF) {
-            const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
-            float4 p = reinterpret_cast<const float4*>(P)[q];
-            const float4 z = reinterpret_cast<const float4*>(Z)[q];
-            if (ct.x & INTERIOR) p.x = fmaf(b, p.x, z.x);
-            if (ct.y & INTERIOR) p.y = fmaf(b, p.y, z.y);
-            if (ct.z & INTERIOR) p.z = fmaf(b, p.z, z.z);
-            if (ct.w & INTERIOR) p.w = fmaf(b, p.w, z.w);
-            reinterpret_cast<float4*>(P)[q] = p;
-        }
-        if (CLEAR && ((ty & GHOST) || tp.tlevel[t_] < max_level))
-            reinterpret_cast<float4*>(Z)[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        float4 p = reinterpret_cast<const float4*>(P)[q];
+        const float4 z = reinterpret_cast<const float4*>(Z)[q];
+        if (ct.x & INTERIOR) p.x = fmaf(b, p.x, z.x);
+        if (ct.y & INTERIOR) p.y = fmaf(b, p.y, z.y);
+        if (ct.z & INTERIOR) p.z = fmaf(b, p.z, z.z);
+        if (ct.w & INTERIOR) p.w = fmaf(b, p.w, z.w);
+        reinterpret_cast<float4*>(P)[q] = p;
     }
 }
 
@@ -1875,17 +1864,11 @@ int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const
     return g;
 }
 
-void launch_update_p(const DevGrid& d, float* P, float* Z, const double* S, hipStream_t st, bool clear_cyclezero)
+void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st)
 {
     if (!S) throw std::invalid_argument("update_p needs the solver scalars (beta)");
-    if (clear_cyclezero) {
-        if (!nonempty(d.pupdate)) return;
-        k::update_p_v<true><<<vector_grid(d, d.pupdate.n), 256, 0, st>>>(view(d), d.pupdate.d, d.pupdate.n, P, Z, S,
-                                                                           d.max_level);
-        return;
-    }
     if (!nonempty(d.leafs)) return;
-    k::update_p_v<false><<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, P, Z, S, d.max_level);
+    k::update_p_v<<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, P, Z, S);
 }
 
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st)

@@ -69,8 +69,7 @@ int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const flo
 int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
                      hipStream_t st, float* Z0 = nullptr, const float* D = nullptr, float w = 0.0f,
                      const uint16_t* Dq = nullptr);
-// clear_cyclezero: also set z to 0 on the cycle-zero tiles (GHOST, LEAF below the finest level)
-void launch_update_p(const DevGrid& d, float* P, float* Z, const double* S, hipStream_t st, bool clear_cyclezero = false);
+void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st);
 void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st);
 void launch_dot64(const LaunchList& l, const double* A, const double* B, double* partial, hipStream_t st);

@@ -165,7 +165,6 @@ void dev_free(DevGrid& d)
     free_list(d.ghosts);
     free_list(d.leafghost);
     free_list(d.cyclezero);
-    free_list(d.pupdate);
     for (int l = 0; l < kMaxLevels; ++l) {
         free_list(d.subtree[l]);
         free_list(d.nonleaf[l]);
@@ -252,9 +251,6 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.ghosts = make_list(topo, ghosts);
     d.leafghost = make_list(topo, leafghost);
     d.cyclezero = make_list(topo, cyclezero);
-    std::vector<int32_t> pupdate(leafs);
-    pupdate.insert(pupdate.end(), ghosts.begin(), ghosts.end());
-    d.pupdate = make_list(topo, pupdate);
     d.max_list = std::max(d.leafs.n, d.all.n);
     for (int l = 0; l < topo.num_levels(); ++l) {
         d.subtree[l] = make_list(topo, sub[l]);
@@ -860,10 +856,8 @@ int enqueue_apply(cmg_solver* s, float* X, float* Y, double* partial, const doub
 // the preconditioner z ~ A^-1 r of the options' cycle kind.  FAS kinds: x, rhs
 // = r (its NONLEAF tiles become coarse right-hand sides), x0 = FAS scratch.
 // Non-FAS kinds: x, f = r (kept), x0 = working right-hand side.
-// pre_zeroed: the caller guarantees x = 0 on the cycle-zero tiles (the PCG iteration's p
-// update cleared them), so the FAS cycle with damped Jacobi needs no fill
 void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float* RHSp, float* X0p, const double* S,
-                       bool zero_done = false, double* rz = nullptr, bool pre_zeroed = false)
+                       bool zero_done = false, double* rz = nullptr)
 {
     DevGrid& d = s->g->dev;
     if (s->family != CMG_SOLVER_AMG) {
@@ -900,9 +894,9 @@ void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float*
         // takes its x from the restriction, so only GHOST tiles and the coarser LEAF
         // tiles need the zero.
         const CycleCtx c{s, o, Xp, RHSp, X0p, C, S, zero_done, rz};
-        if (o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && d.max_level >= 1) {
-            if (!pre_zeroed) launch_fill_list(d, d.cyclezero, c.X, 0.0f, d.stream, S);
-        } else
+        if (o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && d.max_level >= 1)
+            launch_fill_list(d, d.cyclezero, c.X, 0.0f, d.stream, S);
+        else
             launch_fill_all(d, c.X, 0.0f, S, d.stream);
         cycle_step(c, d.max_level, true);
     }
@@ -918,12 +912,11 @@ void enqueue_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, int x
 // the preconditioner of one PCG step, z ~ A^-1 r; flexible CG keeps Ap, so
 // the cycle's scratch is then a solver-private buffer
 void enqueue_precondition(cmg_solver* s, const cmg_solve_opts* o, const double* S, bool zero_done = false,
-                          double* rz = nullptr, bool pre_zeroed = false)
+                          double* rz = nullptr)
 {
     DevGrid& d = s->g->dev;
     if (!o->flexible) {
-        enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), dev_channel(d, CH_AP), S, zero_done, rz,
-                          pre_zeroed);
+        enqueue_cycle_ptr(s, o, dev_channel(d, CH_Z), dev_channel(d, CH_R), dev_channel(d, CH_AP), S, zero_done, rz);
         return;
     }
     if (!s->scratch) {
@@ -940,16 +933,6 @@ bool first_sweep_fusable(const cmg_solver* s, const cmg_solve_opts* o)
     return s->family == CMG_SOLVER_AMG && (o->cycle == CMG_CYCLE_FAS_W || o->cycle == CMG_CYCLE_FAS_V) &&
            o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && !o->learned && !o->is_pure_neumann &&
            s->g->dev.max_level >= 1 && fuse_first_sweep_allowed();
-}
-
-// the FAS cycle with damped Jacobi needs z = 0 only on the cycle-zero tiles (GHOST, LEAF
-// below the finest level); inside the PCG loop the p update, the last reader of z, clears
-// them for the next iteration's cycle, and one fill before the loop covers the first
-bool cyclezero_in_update_p(const cmg_solver* s, const cmg_solve_opts* o)
-{
-    return s->family == CMG_SOLVER_AMG && (o->cycle == CMG_CYCLE_FAS_W || o->cycle == CMG_CYCLE_FAS_V) &&
-           o->smoother == CMG_SMOOTH_JACOBI && o->level_iters >= 1 && !o->flexible && !o->learned &&
-           s->g->dev.max_level >= 1;
 }
 
 // r.z can be summed by each level's last post-sweep over its LEAF tiles (whose b
@@ -986,8 +969,7 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
     const bool frz = rz_fusable(s, o);
-    const bool cz = cyclezero_in_update_p(s, o);
-    enqueue_precondition(s, o, s->S, fuse, frz ? s->partial : nullptr, cz);
+    enqueue_precondition(s, o, s->S, fuse, frz ? s->partial : nullptr);
     if (frz) {
         // every level's last post-sweep wrote its LEAF tiles' r.z; level 0 (the direct solve) has
         // LEAF tiles only on one-level grids
@@ -1002,7 +984,7 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
         const int n3 = launch_dot(d, d.leafs, R, Z, s->partial, s->S, st);
         launch_finish(s->partial, nullptr, n3, s->S, FIN_BETA, st);
     }
-    launch_update_p(d, P, Z, s->S, st, cz);
+    launch_update_p(d, P, Z, s->S, st);
 }
 
 std::string graph_key(const cmg_solve_opts* o, int derive)
@@ -1115,7 +1097,6 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     launch_copy_list(d, d.leafs, Z, P, s->S, st);
     const int ng = launch_dot(d, d.leafs, Z, R, s->partial, s->S, st);
     launch_finish(s->partial, nullptr, ng, s->S, FIN_GAMMA0, st);
-    if (cyclezero_in_update_p(s, o)) launch_fill_list(d, d.cyclezero, Z, 0.0f, st, s->S);
     if (max_iters <= 0) {
         read_scalars(s, h);
         if (h[S_DONE] == 1 && h[S_RHS2] == 0) return 1;   // zero b
