@@ -1223,35 +1223,44 @@ __global__ __launch_bounds__(512) void to_float(const int32_t* __restrict__ list
 // ------------------------------------------------------------ reductions ----
 // deterministic single-workgroup sum of partials, then the scalar step of the
 // PCG recurrence (alpha, convergence test, beta), replacing TernaryOnArray
+// one thread's share of a finish reduction: terms threadIdx.x, threadIdx.x + blockDim.x, ...
+// added in index order; U terms are loaded together (masked past n), so a thread's loads
+// overlap in one round trip per U terms
+template <int U, bool TWO>
+__device__ __forceinline__ void finish_terms(const double* __restrict__ p, const double* __restrict__ p2, int n, double& v,
+                                             double& v2)
+{
+    const int step = blockDim.x;
+    for (int i = threadIdx.x; i < n; i += U * step) {
+        double a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = i + u * step;
+            a[u] = j < n ? p[j] : 0.0;
+            if (TWO) b[u] = j < n ? p2[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i + u * step < n) {
+                v += a[u];
+                if (TWO) v2 += b[u];
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partial, const double* __restrict__ partial2, int n,
                                                double* __restrict__ S, int kind)
 {
     __shared__ double red[16];
     __shared__ double red2[16];
-    if (kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0) return;
-    // each thread adds its strided partials in index order; the loads of 16
-    // consecutive terms are issued together so that they overlap
-    constexpr int U = 16;
-    const int step = blockDim.x;
+    // the stop flag is read first and tested only before the writes, so that its load
+    // overlaps the partials' (a stopped solve's partials are stale but in bounds)
+    const bool skip = kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0;
+    // each thread adds its strided partials in index order
     double v = 0, v2 = 0;
-    int i = threadIdx.x;
-    for (; i + (U - 1) * step < n; i += U * step) {
-        double a[U], b[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            a[u] = partial[i + u * step];
-            b[u] = partial2 ? partial2[i + u * step] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            v += a[u];
-            if (partial2) v2 += b[u];
-        }
-    }
-    for (; i < n; i += step) {
-        v += partial[i];
-        if (partial2) v2 += partial2[i];
-    }
+    if (partial2) finish_terms<16, true>(partial, partial2, n, v, v2);
+    else finish_terms<40, false>(partial, nullptr, n, v, v2);
     for (int o = 32; o > 0; o >>= 1) {
         v += __shfl_xor(v, o, 64);
         v2 += __shfl_xor(v2, o, 64);
@@ -1259,7 +1268,7 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) { red[w] = v; red2[w] = v2; }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0 || skip) return;
     double s = 0, s2 = 0;
     for (int i = 0; i < int(blockDim.x >> 6); ++i) { s += red[i]; s2 += red2[i]; }
     switch (kind) {
