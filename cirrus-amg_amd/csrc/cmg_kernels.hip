@@ -723,9 +723,9 @@ __global__ __launch_bounds__(256) void copy_flat(const f32x4v* __restrict__ A, f
     if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(&A[i]), &B[i]);
 }
 
-// q[a] = FP16 bits of c[a] / h (h the tile's spacing), four cells at a time (float4
-// in, four halves out); nf records a non-finite result.  The kernel below counts in
-// *bad the workgroups that produced one
+// q[a] = FP16 bits of c[a] / h on every tile of the list, 128 threads x 4 cells a
+// tile (float4 in, 4 halves out per channel); *bad counts workgroups that produced
+// a non-finite value
 __device__ __forceinline__ uint2 half4_of(float4 v, float h, bool& nf)
 {
     const _Float16 a = _Float16(v.x / h), b = _Float16(v.y / h), c = _Float16(v.z / h), d = _Float16(v.w / h);
@@ -735,25 +735,22 @@ __device__ __forceinline__ uint2 half4_of(float4 v, float h, bool& nf)
     return make_uint2(lo, hi);
 }
 
-typedef unsigned int u32x2v __attribute__((ext_vector_type(2)));
-// flat over every tile: thread i converts cell quad i (tile i / 128) of the four channels;
-// streaming loads and stores, one quad per thread over a full grid (the copy kernel's form)
-__global__ __launch_bounds__(256) void coef_to_half(TopoView tp, size_t n4, Coef C, uint16_t* __restrict__ q0,
-                                                    uint16_t* __restrict__ q1, uint16_t* __restrict__ q2,
-                                                    uint16_t* __restrict__ q3, unsigned int* __restrict__ bad)
+__global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* __restrict__ list, int n, Coef C,
+                                                    uint16_t* __restrict__ q0, uint16_t* __restrict__ q1,
+                                                    uint16_t* __restrict__ q2, uint16_t* __restrict__ q3,
+                                                    unsigned int* __restrict__ bad)
 {
-    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
     bool nf = false;
-    if (i < n4) {
-        const float h = tile_h(tp, int(i >> 7));
-        uint16_t* const qs[4] = {q0, q1, q2, q3};
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const f32x4v v = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(C.c[a]) + i);
-            const uint2 r = half4_of(make_float4(v.x, v.y, v.z, v.w), h, nf);
-            const u32x2v w = {r.x, r.y};
-            __builtin_nontemporal_store(w, reinterpret_cast<u32x2v*>(qs[a]) + i);
-        }
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const float h = tile_h(tp, t_);
+        const float4 c0 = reinterpret_cast<const float4*>(C.c[0])[q], c1 = reinterpret_cast<const float4*>(C.c[1])[q];
+        const float4 c2 = reinterpret_cast<const float4*>(C.c[2])[q], c3 = reinterpret_cast<const float4*>(C.c[3])[q];
+        reinterpret_cast<uint2*>(q0)[q] = half4_of(c0, h, nf);
+        reinterpret_cast<uint2*>(q1)[q] = half4_of(c1, h, nf);
+        reinterpret_cast<uint2*>(q2)[q] = half4_of(c2, h, nf);
+        reinterpret_cast<uint2*>(q3)[q] = half4_of(c3, h, nf);
     }
     if (__syncthreads_or(nf) && threadIdx.x == 0) atomicAdd(bad, 1u);
 }
@@ -1790,10 +1787,8 @@ void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float*
 
 void launch_coef_to_half(const DevGrid& d, Coef C, uint16_t* const q[4], unsigned int* bad, hipStream_t st)
 {
-    const size_t n4 = size_t(d.N) * 128, blocks = (n4 + 255) / 256;
-    if (blocks == 0) return;
-    if (blocks > 0x7fffffffu) throw std::invalid_argument("coef_to_half: too many tiles");
-    k::coef_to_half<<<unsigned(blocks), 256, 0, st>>>(view(d), n4, C, q[0], q[1], q[2], q[3], bad);
+    if (!nonempty(d.all)) return;
+    k::coef_to_half<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, q[0], q[1], q[2], q[3], bad);
 }
 
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)
