@@ -212,13 +212,21 @@ def main():
 
     q16 = 1 if args.cycle_coef == "fp16" else 0
     opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                          max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=q16)
+                          max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=q16,
+                          no_true_residual=1)
     solver = P.AMGSolver(g, 11, 0.5, 1.0, 1.0)
 
-    def step():
+    def step(o=opts):
         g.copy_channel(5, 1)
-        it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=opts)
+        it, rel = solver.solve(max_iters=200, relative_tolerance=1e-6, level_iters=2, coarsest_iters=0, opts=o)
         return it, rel
+
+    def checked(o):
+        """the same options with the FP64 true-residual recompute on (a diagnostic of this
+        library that the reference's solve does not compute; run untimed)"""
+        c = P.SolveOpts.from_buffer_copy(o)
+        c.no_true_residual = 0
+        return c
 
     for _ in range(args.warmup):
         step()
@@ -232,6 +240,7 @@ def main():
         dev_ms.append(solver.last_stats.ms)
     P.device_synchronize()
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    step(checked(opts))   # untimed: the FP64 true residual of the same solve
     true_rel = solver.last_stats.true_rel_res
     x32 = g.channel(0).astype(np.float64) if not args.no_fp64 else None
 
@@ -315,7 +324,8 @@ def main():
     # the same solve with the other cycle-coefficient form (the PCG operator is FP32 in both,
     # so the tolerance and the solution are the same)
     o16 = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
-                         max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=1 - q16)
+                         max_iters=200, relative_tolerance=1e-6, level_iters=2, sync_stride=1, cycle_coef_fp16=1 - q16,
+                         no_true_residual=1)
     ms16, its16, dev16 = [], [], []
     for rep in range(args.warmup + args.steps):
         g.copy_channel(5, 1)
@@ -327,6 +337,7 @@ def main():
             ms16.append((time.perf_counter() - t0) * 1e3)
             its16.append(it)
             dev16.append(solver.last_stats.ms)
+    step(checked(o16))   # untimed: the FP64 true residual of the same solve
     leg16 = {"ms": float(np.mean(ms16)), "device_ms": float(np.mean(dev16)), "iterations": its16, "rel_res": rel,
              "true_rel_res_fp64": float(solver.last_stats.true_rel_res),
              "ms_ratio_to_headline": float(np.mean(ms16)) / ms_per_step}

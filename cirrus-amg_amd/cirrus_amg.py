@@ -56,7 +56,8 @@ class SolveOpts(C.Structure):
                 ("is_pure_neumann", C.c_int), ("cycle", C.c_int), ("smoother", C.c_int), ("omega", C.c_double),
                 ("coarsest_direct", C.c_int), ("precision", C.c_int), ("outer_tolerance", C.c_double),
                 ("inner_tolerance", C.c_double), ("warm_start", C.c_int), ("use_graph", C.c_int), ("mu", C.c_int),
-                ("flexible", C.c_int), ("learned", C.c_int), ("cycle_coef_fp16", C.c_int)]
+                ("flexible", C.c_int), ("learned", C.c_int), ("cycle_coef_fp16", C.c_int),
+                ("no_true_residual", C.c_int)]
 
 
 class SolveStats(C.Structure):

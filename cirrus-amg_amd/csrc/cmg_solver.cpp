@@ -1405,8 +1405,9 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
             build_direct(s, o->is_pure_neumann);
         if (amg && o->cycle_coef_fp16) prepare_coef_fp16(s);
         DeriveScope derive(d, amg && derive_allowed() && diagonal_derivable(s));
-        // keep b in FP64 for the true residual / refinement
-        if (amg) launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
+        // keep b in FP64 for the true residual / refinement (not needed when an FP32 solve skips it)
+        if (amg && !(o->precision == CMG_PREC_FP32 && o->no_true_residual))
+            launch_to_double(d, d.leafs, dev_channel(d, CH_R), s->b64, true, stream);
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipEventRecord(s->e0, stream));
         if (o->precision == CMG_PREC_FP32) {
@@ -1421,7 +1422,9 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
             HIPCHK(hipEventElapsedTime(&ms, s->e0, s->e1));
             st->ms = ms;
             // true residual from the FP32 solution (outside the timed region)
-            if (amg) {
+            if (amg && o->no_true_residual) {
+                st->true_rel_res = -1;
+            } else if (amg) {
                 const double rn2 = residual_norm2(s, false);
                 const double bn2 = b_norm2(s);
                 st->true_rel_res = bn2 > 0 ? std::sqrt(rn2 / bn2) : 0.0;

@@ -236,13 +236,18 @@ typedef struct cmg_solve_opts {
                                * derived); the PCG operator and the FP64 level-0 solve keep FP32
                                * coefficients, so the solution and its tolerance are those of
                                * the FP32 operator */
+    int no_true_residual;     /* 1: an FP32 solve skips the FP64 recompute of ||b - A x|| / ||b||
+                               * after the iterations (stats.true_rel_res = -1); a diagnostic of
+                               * this library that AMGSolver::solve does not compute.  0 (the
+                               * default): computed */
 } cmg_solve_opts;
 
 typedef struct cmg_solve_stats {
     int iters;                /* PCG iterations (summed over refinement steps) */
     int outer_iters;          /* FP64 refinement steps (0 in FP32 mode) */
     double rel_res;           /* recursive relative residual, as AMGSolver::solve returns */
-    double true_rel_res;      /* ||b - A x|| / ||b|| recomputed in FP64 on the device */
+    double true_rel_res;      /* ||b - A x|| / ||b|| recomputed in FP64 on the device (-1 when
+                               * no_true_residual skipped it) */
     double ms;                /* device time of the solve (hipEvents on the handle's stream) */
 } cmg_solve_stats;
 

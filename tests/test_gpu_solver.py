@@ -393,3 +393,24 @@ def test_solver_refuses_a_refinalized_grid():
         s.cycle(P.default_opts(), 4, 1, 3)
     it, rel = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0).solve()
     assert rel < 1e-6
+
+
+def test_true_residual_can_be_skipped():
+    """no_true_residual = 1 leaves the solve itself unchanged (the same bits of x, the same
+    iterations and recursive residual) and reports true_rel_res = -1; the default computes it."""
+    go, gp = H.build_pair("c2_small")
+    go.rhs_uniform(1, 21, True)
+    b = go.channel(1).copy()
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    out = []
+    for skip in (0, 1, 0):
+        print("\nno_true_residual", skip, flush=True)
+        gp.set_channel(1, b)
+        o = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                           cycle_coef_fp16=1, no_true_residual=skip)
+        it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=o)
+        out.append((it, rel, solver.last_stats.true_rel_res, gp.channel(0).copy()))
+    assert out[0][0] == out[1][0] == out[2][0] and out[0][1] == out[1][1] == out[2][1]
+    assert np.array_equal(H.bits(out[0][3]), H.bits(out[1][3]))
+    assert np.array_equal(H.bits(out[0][3]), H.bits(out[2][3]))
+    assert 0 < out[0][2] < 2e-6 and out[1][2] == -1 and out[2][2] == out[0][2]
