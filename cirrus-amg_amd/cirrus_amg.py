@@ -82,7 +82,7 @@ EXPORTS = [
     "cmg_trainer_destroy", "cmg_trainer_collect", "cmg_trainer_add_tiles", "cmg_trainer_size", "cmg_trainer_get_weights",
     "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_assemble_cutcell_device", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
     "cmg_grid_rebuild_device", "cmg_grid_iterative_refine_device", "cmg_grid_finalize_device",
-    "cmg_grid_build_device",
+    "cmg_grid_build_device", "cmg_op_norm_noalloc",
 ]
 
 _lib = None
@@ -132,6 +132,7 @@ def lib():
             "cmg_gmg_damped_jacobi_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_solver_apply": (st, [vp, i, i]),
             "cmg_op_norm": (st, [vp, i, i, i, i, C.POINTER(d)]),
+            "cmg_op_norm_noalloc": (st, [vp, i, i, i, i, C.POINTER(d)]),
             "cmg_solver_set_net": (st, [vp, vp]), "cmg_solver_net_infer": (st, [vp, i, i, i]),
             "cmg_trainer_create": (st, [i, C.c_uint64, C.POINTER(vp)]), "cmg_trainer_destroy": (None, [vp]),
             "cmg_trainer_collect": (st, [vp, vp, i, i, i, C.POINTER(i)]), "cmg_trainer_add_tiles": (st, [vp, vp, vp, i]),
@@ -395,6 +396,13 @@ class HADeviceGrid:
         """NormSync (src/PoissonGrid.cu:322-375) over LEAF tiles."""
         out = C.c_double()
         _check(lib().cmg_op_norm(self.h, order, channel, int(volume_weighted), cell_types, C.byref(out)), "cmg_op_norm")
+        return out.value
+
+    def norm_noalloc(self, order, channel, volume_weighted=False, cell_types=2):
+        """norm() with no allocation: the grid and the channel must already be on the device."""
+        out = C.c_double()
+        _check(lib().cmg_op_norm_noalloc(self.h, order, channel, int(volume_weighted), cell_types, C.byref(out)),
+               "cmg_op_norm_noalloc")
         return out.value
 
     def dot(self, a, b):

@@ -545,6 +545,10 @@ cmg_status cmg_op_norm(cmg_grid* g, int order, int channel, int volume_weighted,
 {
     return op_norm(g, order, channel, volume_weighted, cell_types, out);
 }
+cmg_status cmg_op_norm_noalloc(cmg_grid* g, int order, int channel, int volume_weighted, int cell_types, double* out)
+{
+    return op_norm_noalloc(g, order, channel, volume_weighted, cell_types, out);
+}
 cmg_status cmg_divergence(cmg_grid* g, int u0, int channel, int coeff_channel)
 {
     return op_divergence(g, u0, channel, coeff_channel);

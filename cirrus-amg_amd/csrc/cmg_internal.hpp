@@ -111,6 +111,8 @@ cmg_status op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0);
 cmg_status op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P);
 cmg_status op_dot(cmg_grid* g, int a, int b, double* out);
 cmg_status op_norm(cmg_grid* g, int order, int ch, int volume_weighted, int cell_types, double* out);
+// op_norm with no allocation (cmg_noalloc.hip)
+cmg_status op_norm_noalloc(cmg_grid* g, int order, int ch, int volume_weighted, int cell_types, double* out);
 cmg_status op_divergence(cmg_grid* g, int u0, int ch, int c0);
 cmg_status op_add_gradient(cmg_grid* g, int x, int c0, int u0, int weighted);
 cmg_status op_mix_velocity(cmg_grid* g, int c0, int uf0, int us0, int um0);

@@ -362,6 +362,15 @@ cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out);
  * weighted), order p >= 1 = (sum |x|^p w / sum w)^(1/p) over the cells whose type is in
  * cell_types, w = 1 or the cell volume h^3; sums in FP64 */
 cmg_status cmg_op_norm(cmg_grid* g, int order, int channel, int volume_weighted, int cell_types, double* out);
+/* cmg_op_norm with no allocation of any kind.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and the same result bits as
+ * cmg_op_norm.  Its partial sums live in a fixed device array of 1 MiB that each GPU
+ * holds until the process ends; at most 65,536 LEAF-list entries (a larger grid returns
+ * CMG_ERR_BAD_ARG).  The grid must already be on the device and `channel` allocated
+ * there (any earlier device call on the grid and channel does it; CMG_ERR_STATE
+ * otherwise).  Calls that overlap on one GPU share the array and race. */
+cmg_status cmg_op_norm_noalloc(cmg_grid* g, int order, int channel, int volume_weighted, int cell_types, double* out);
 /* the solver's preconditioner cycle on channels (x, rhs, x0): AMG FAS kinds use rhs as
  * the right-hand side and x0 as FAS scratch; the non-FAS kinds and the GMG / CMG
  * V-cycles keep rhs (f) and use x0 as the working right-hand side */
