@@ -84,8 +84,10 @@ float og_face_fluid_ratio(float phi0, float phi1, float phi2, float phi3)
 }
 
 /* 4x4 supersampled open fraction of the negative face of cell gi along axis:
- * samples at sub-face centres, open where the SDF is >= 0; same 0.1 floor */
-static float face_supersampled(int kind, const float* prm, const int* gi, int axis, float h)
+ * samples at sub-face centres, open where the SDF is >= 0; same 0.1 floor.
+ * The config's rule (no reference counterpart; the reference's corner rule is
+ * og_face_fluid_ratio). */
+float og_face_supersampled(int kind, const float* prm, const int* gi, int axis, float h)
 {
     int b = (axis + 1) % 3, c = (axis + 2) % 3;
     int open = 0;
@@ -158,7 +160,7 @@ static void face_coeffs(og_grid* g, int c0, int face_rule, int sdf_kind, const f
                         }
                         coeff = h * og_face_fluid_ratio(phi[0], phi[1], phi[2], phi[3]);
                     } else {
-                        coeff = h * face_supersampled(sdf_kind, sdf, gi, a, h);
+                        coeff = h * og_face_supersampled(sdf_kind, sdf, gi, a, h);
                     }
                 }
                 g->ch[c0 + a][(size_t)t * 512 + v] = -coeff;

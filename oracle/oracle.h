@@ -102,6 +102,8 @@ void og_cutcell_coeffs(og_grid* g, int sdf_kind, const float* sdf, int face_rule
 void og_prepare_system(og_grid* g, int c0, float R);                        /* AMGSolver.cu:89-254 */
 float og_face_fluid_ratio(float p0, float p1, float p2, float p3);          /* MeshCutCell.cu:305-408 */
 float og_sdf(int kind, const float* prm, float x, float y, float z);
+/* 4x4 supersampled open fraction of the negative face of cell gi (global cell coords) along axis */
+float og_face_supersampled(int kind, const float* prm, const int* gi, int axis, float h);
 
 /* ---- matrix-free operator and multigrid pieces: src/AMGSolver.cu ---------- */
 void og_propagate_ghosts(og_grid* g, int ch);                  /* PoissonGrid.cu:377-405 (GHOST, subtree, coarse first) */

@@ -69,6 +69,7 @@ def lib():
             "og_cutcell_coeffs": (None, [vp, i, vp, i, i, f]),
             "og_prepare_system": (None, [vp, i, f]),
             "og_face_fluid_ratio": (f, [f, f, f, f]), "og_sdf": (f, [i, vp, f, f, f]),
+            "og_face_supersampled": (f, [i, vp, vp, i, f]),
             "og_propagate_ghosts": (None, [vp, i]), "og_accumulate_leafs": (None, [vp, i]),
             "og_neg_lap_leafs": (None, [vp, i, i, i]), "og_full_neg_lap": (None, [vp, i, i, i]),
             "og_gs_color": (None, [vp, i, i, i, i, i, d]),
@@ -347,3 +348,10 @@ def net_loss_grad(w, x, e, with_grad=True):
 
 def adam_step(w, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     lib().og_adam_step(_dptr(w), _dptr(m), _dptr(v), _dptr(np.ascontiguousarray(g, np.float64)), w.size, t, lr, b1, b2, eps)
+
+
+def face_supersampled(sdf_kind, sdf, gi, axis, h):
+    """4x4 supersampled open fraction of the negative face of global cell gi along axis."""
+    prm = (C.c_float * max(1, len(sdf)))(*sdf)
+    g = (C.c_int * 3)(*gi)
+    return lib().og_face_supersampled(sdf_kind, prm, g, axis, h)

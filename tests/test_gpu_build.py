@@ -184,3 +184,16 @@ def test_device_built_grid_solves():
         s.close()
     assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
     assert np.array_equal(H.bits(res[0][2]), H.bits(res[1][2]))
+
+
+@pytest.mark.parametrize("one_call", [False, True])
+def test_device_build_hand_octree(one_call):
+    """device construction gives the slots worked out by hand from the reference's steps (hand_cases.py)"""
+    from hand_cases import HAND_BANDS, HAND_LOG2_HASH, HAND_SPHERE, check_hand_octree
+    print("\nhand octree, one call" if one_call else "\nhand octree, three calls", flush=True)
+    t = P.make_target(P.TARGET_SDF_BANDS, 1, 2, P.SDF_SPHERE, HAND_SPHERE, HAND_BANDS)
+    if one_call:
+        g = P.HADeviceGrid.test_grid(t, log2_hash=HAND_LOG2_HASH, device=True)
+    else:
+        g = _build(t, True, HAND_LOG2_HASH)
+    check_hand_octree(g)

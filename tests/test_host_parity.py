@@ -71,3 +71,10 @@ def test_callback_target_matches_preset():
     g.finalize()
     for a, b in zip(ref.tiles(), g.tiles()):
         assert np.array_equal(a, b)
+
+
+def test_hand_octree_insertion_order_host():
+    """the library's host construction gives the hand-derived slots (hand_cases.py)"""
+    from hand_cases import HAND_BANDS, HAND_LOG2_HASH, HAND_SPHERE, check_hand_octree
+    t = P.make_target(P.TARGET_SDF_BANDS, 1, 2, P.SDF_SPHERE, HAND_SPHERE, HAND_BANDS)
+    check_hand_octree(P.HADeviceGrid.test_grid(t, log2_hash=HAND_LOG2_HASH))
