@@ -52,6 +52,7 @@ cmg_status need_final(const cmg_grid* g)
 // host cell types -> device (only once the device copy exists)
 void upload_types(cmg_grid* g)
 {
+    touch_types(g);
     if (!g->uploaded) return;
     auto& d = g->dev;
     if (hipMemcpy(d.ctype, g->cells.type.data(), g->cells.type.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -61,6 +62,7 @@ void upload_types(cmg_grid* g)
 
 void upload_channel(cmg_grid* g, int c)
 {
+    touch_channel(g, c);
     if (!g->uploaded) return;
     auto& d = g->dev;
     float* dst = dev_channel(d, c);
@@ -69,10 +71,21 @@ void upload_channel(cmg_grid* g, int c)
         throw std::runtime_error("hipMemcpy channel failed");
 }
 
+// a solver's own runs write its work channels 0..4 (x, r, p, Ap, z)
+void touch_work(cmg_solver* s)
+{
+    if (s) touch_channels(solver_grid(s), 0, 5);
+}
+
 void ensure_types(cmg_grid* g)
 {
     const size_t n = size_t(g->topo->num_tiles()) * kCells;
     if (g->cells.type.size() != n) g->cells.type.assign(n, 0);
+    if (g->host_types_stale && g->uploaded) {
+        if (hipMemcpy(g->cells.type.data(), g->dev.ctype, n, hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("hipMemcpy cell types failed");
+        g->host_types_stale = false;
+    }
 }
 
 }  // namespace
@@ -83,6 +96,7 @@ void ensure_device(cmg_grid* g)
     if (g->uploaded) return;
     dev_upload(g->dev, *g->topo);
     g->uploaded = true;
+    touch_all(g);
     upload_types(g);
     for (int c = 0; c < kChannels; ++c)
         if (!g->cells.ch[c].empty()) upload_channel(g, c);
@@ -281,8 +295,10 @@ cmg_status cmg_grid_get_cell_types(cmg_grid* g, uint8_t* types)
 {
     if (auto st = need_final(g)) return st;
     if (!types) return CMG_ERR_BAD_ARG;
-    ensure_types(g);
-    std::memcpy(types, g->cells.type.data(), g->cells.type.size());
+    CAPI_GUARD({
+        ensure_types(g);
+        std::memcpy(types, g->cells.type.data(), g->cells.type.size());
+    });
     return CMG_OK;
 }
 
@@ -305,6 +321,7 @@ cmg_status cmg_grid_set_leaf_types_preset(cmg_grid* g, const cmg_target* target)
     if (!target) return CMG_ERR_BAD_ARG;
     CAPI_GUARD({
         set_leaf_types_preset(*g->topo, g->cells, *target);
+        g->host_types_stale = false;   // every type was just rewritten on the host
         upload_types(g);
     });
     return CMG_OK;
@@ -326,6 +343,7 @@ cmg_status cmg_grid_set_channel(cmg_grid* g, int channel, const float* host)
 {
     if (auto st = need_final(g)) return st;
     if (channel < 0 || channel >= kChannels || !host) return CMG_ERR_BAD_ARG;
+    touch_channel(g, channel);
     CAPI_GUARD({
         ensure_device(g);
         float* dst = dev_channel(g->dev, channel);
@@ -352,17 +370,191 @@ cmg_status cmg_grid_fill_channel(cmg_grid* g, int channel, float value)
 {
     if (auto st = need_final(g)) return st;
     if (channel < 0 || channel >= kChannels) return CMG_ERR_BAD_ARG;
+    touch_channel(g, channel);
     CAPI_GUARD({
         ensure_device(g);
-        std::vector<float> h(size_t(g->dev.N) * kCells, value);
-        float* dst = dev_channel(g->dev, channel);
-        if (hipMemcpy(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-            throw std::runtime_error("hipMemcpy fill failed");
+        auto& d = g->dev;
+        launch_fill_all(d, dev_channel(d, channel), value, nullptr, d.stream);
+        if (hipStreamSynchronize(d.stream) != hipSuccess) throw std::runtime_error("fill failed");
     });
     return CMG_OK;
 }
 
-cmg_status cmg_grid_copy_channel(cmg_grid* g, int src, int dst) { return op_copy(g, src, dst); }
+cmg_status cmg_grid_copy_channel(cmg_grid* g, int src, int dst)
+{
+    touch_channel(g, dst);
+    return op_copy(g, src, dst);
+}
+
+// ---------------------------------------------------------- device-resident exchange
+
+cmg_status cmg_grid_set_channel_device(cmg_grid* g, int channel, const float* d_src)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || !d_src) return CMG_ERR_BAD_ARG;
+    touch_channel(g, channel);
+    CAPI_GUARD({
+        ensure_device(g);
+        auto& d = g->dev;
+        float* dst = dev_channel(d, channel);
+        if (hipMemcpyAsync(dst, d_src, size_t(d.N) * kCells * sizeof(float), hipMemcpyDeviceToDevice, d.stream) != hipSuccess ||
+            hipStreamSynchronize(d.stream) != hipSuccess)
+            throw std::runtime_error("device channel copy failed");
+        g->cells.ch[channel].clear();   // a host assembly copy of this channel is stale now
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_channel_device(cmg_grid* g, int channel, float* d_dst)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || !d_dst) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_device(g);
+        auto& d = g->dev;
+        const float* src = dev_channel(d, channel);
+        if (hipMemcpyAsync(d_dst, src, size_t(d.N) * kCells * sizeof(float), hipMemcpyDeviceToDevice, d.stream) != hipSuccess ||
+            hipStreamSynchronize(d.stream) != hipSuccess)
+            throw std::runtime_error("device channel copy failed");
+    });
+    return CMG_OK;
+}
+
+namespace {
+
+// exactly one of the address array and the slab base, a usable stride for a slab
+cmg_status check_tiles_arg(const void* addrs, const void* base, size_t stride)
+{
+    if ((addrs == nullptr) == (base == nullptr)) {
+        set_error("give either the device array of tile addresses or the slab base, not both");
+        return CMG_ERR_BAD_ARG;
+    }
+    if (base && (stride < 44880 || stride % 16 != 0)) {
+        set_error("slab stride must be a multiple of 16 and at least sizeof(PoissonTile<float,15>) = 44880");
+        return CMG_ERR_BAD_ARG;
+    }
+    return CMG_OK;
+}
+
+cmg_status report_bad_tiles(unsigned int bad)
+{
+    if (!bad) return CMG_OK;
+    set_error(std::to_string(bad) + " tile address(es) null or not 16-byte aligned; those tiles were skipped");
+    return CMG_ERR_BAD_ARG;
+}
+
+}  // namespace
+
+cmg_status cmg_grid_gather_tile_channel(cmg_grid* g, const uint64_t* d_tile_addrs, const void* d_base, size_t stride,
+                                        int tile_channel, int channel)
+{
+    if (auto st = need_final(g)) return st;
+    if (auto st = check_tiles_arg(d_tile_addrs, d_base, stride)) return st;
+    if (channel < 0 || channel >= kChannels || tile_channel < 0 || tile_channel >= 15) return CMG_ERR_BAD_ARG;
+    touch_channel(g, channel);
+    unsigned int bad = 0;
+    CAPI_GUARD({
+        ensure_device(g);
+        auto& d = g->dev;
+        bad = tile_channel_io(d, true, d_tile_addrs, const_cast<void*>(d_base), stride, tile_channel, dev_channel(d, channel));
+        g->cells.ch[channel].clear();
+    });
+    return report_bad_tiles(bad);
+}
+
+cmg_status cmg_grid_scatter_tile_channel(cmg_grid* g, int channel, const uint64_t* d_tile_addrs, void* d_base, size_t stride,
+                                         int tile_channel)
+{
+    if (auto st = need_final(g)) return st;
+    if (auto st = check_tiles_arg(d_tile_addrs, d_base, stride)) return st;
+    if (channel < 0 || channel >= kChannels || tile_channel < 0 || tile_channel >= 15) return CMG_ERR_BAD_ARG;
+    unsigned int bad = 0;
+    CAPI_GUARD({
+        ensure_device(g);
+        auto& d = g->dev;
+        bad = tile_channel_io(d, false, d_tile_addrs, d_base, stride, tile_channel, dev_channel(d, channel));
+    });
+    return report_bad_tiles(bad);
+}
+
+cmg_status cmg_grid_gather_tile_types(cmg_grid* g, const uint64_t* d_tile_addrs, const void* d_base, size_t stride)
+{
+    if (auto st = need_final(g)) return st;
+    if (auto st = check_tiles_arg(d_tile_addrs, d_base, stride)) return st;
+    touch_types(g);
+    unsigned int bad = 0;
+    CAPI_GUARD({
+        ensure_types(g);
+        ensure_device(g);
+        bad = tile_types_io(g->dev, true, d_tile_addrs, const_cast<void*>(d_base), stride);
+        g->types_on_device = true;
+        g->host_types_stale = true;   // copied back only when a host path reads the types
+    });
+    return report_bad_tiles(bad);
+}
+
+cmg_status cmg_grid_scatter_tile_types(cmg_grid* g, const uint64_t* d_tile_addrs, void* d_base, size_t stride)
+{
+    if (auto st = need_final(g)) return st;
+    if (auto st = check_tiles_arg(d_tile_addrs, d_base, stride)) return st;
+    unsigned int bad = 0;
+    CAPI_GUARD({
+        ensure_device(g);
+        if (!g->types_on_device) upload_types(g);
+        bad = tile_types_io(g->dev, false, d_tile_addrs, d_base, stride);
+    });
+    return report_bad_tiles(bad);
+}
+
+// ---------------------------------------------------------- region of interest
+
+cmg_status cmg_mark_region_of_interest(cmg_grid* g, int subtree_level, uint8_t launch_types, int channel, int pred, float a,
+                                       float b, int calc_locked)
+{
+    if (auto st = need_final(g)) return st;
+    if (channel < 0 || channel >= kChannels || pred < CMG_ROI_MAX_GT || pred > CMG_ROI_SPAN_GT) return CMG_ERR_BAD_ARG;
+    CAPI_GUARD({
+        ensure_device(g);
+        auto& d = g->dev;
+        if (!g->types_on_device) upload_types(g);
+        const size_t N = size_t(std::max(d.N, 1));
+        auto zalloc = [&](auto*& p, size_t bytes) {
+            if (p) return;
+            if (hipMalloc(reinterpret_cast<void**>(&p), bytes) != hipSuccess || hipMemset(p, 0, bytes) != hipSuccess)
+                throw std::runtime_error("hipMalloc roi flags failed");
+        };
+        zalloc(d.roi, N);
+        zalloc(d.roi_locked, N);
+        zalloc(d.roi_min, N * sizeof(float));
+        zalloc(d.roi_max, N * sizeof(float));
+        launch_roi_flags(d, dev_channel(d, channel), subtree_level, launch_types, pred, a, b, calc_locked != 0, d.roi,
+                         d.roi_locked, d.roi_min, d.roi_max, d.stream);
+        if (hipStreamSynchronize(d.stream) != hipSuccess) throw std::runtime_error("roi flags failed");
+    });
+    return CMG_OK;
+}
+
+cmg_status cmg_grid_get_roi(cmg_grid* g, uint8_t* interest, uint8_t* locked, float* tile_min, float* tile_max)
+{
+    if (auto st = need_final(g)) return st;
+    auto& d = g->dev;
+    if (!g->uploaded || !d.roi) {
+        set_error("no region-of-interest marking on this grid yet");
+        return CMG_ERR_STATE;
+    }
+    CAPI_GUARD({
+        const size_t N = size_t(d.N);
+        auto get = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes && hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+                throw std::runtime_error("hipMemcpy roi flags failed");
+        };
+        get(interest, d.roi, N);
+        get(locked, d.roi_locked, N);
+        get(tile_min, d.roi_min, N * sizeof(float));
+        get(tile_max, d.roi_max, N * sizeof(float));
+    });
+    return CMG_OK;
+}
 
 cmg_status cmg_prepare_types_and_coeffs(cmg_grid* g, int c0, float R)
 {
@@ -415,6 +607,8 @@ cmg_status cmg_assemble_cutcell_device(cmg_grid* g, int sdf_kind, const float* s
         for (int a = 0; a < 4; ++a) g->cells.ch[c0 + a].clear();
         g->types_on_device = true;
         g->assembled_c0 = c0;
+        touch_types(g);
+        touch_channels(g, c0, 4);
     });
     return CMG_OK;
 }
@@ -482,13 +676,25 @@ cmg_status cmg_geo_solver_create(cmg_grid* g, int family, float one_over_alpha, 
 
 cmg_status cmg_gmg_damped_jacobi_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
 {
+    touch_work(s);
     return solver_damped_jacobi(s, opts, stats);
 }
 
-cmg_status cmg_solver_apply(cmg_solver* s, int x, int y) { return solver_apply(s, x, y); }
+cmg_status cmg_solver_apply(cmg_solver* s, int x, int y)
+{
+    if (s) {
+        touch_channel(solver_grid(s), x);
+        touch_channel(solver_grid(s), y);
+    }
+    return solver_apply(s, x, y);
+}
 
 cmg_status cmg_solver_set_net(cmg_solver* s, const float* weights) { return solver_set_net(s, weights); }
-cmg_status cmg_solver_net_infer(cmg_solver* s, int r, int diag, int y) { return solver_net_infer(s, r, diag, y); }
+cmg_status cmg_solver_net_infer(cmg_solver* s, int r, int diag, int y)
+{
+    if (s) touch_channel(solver_grid(s), y);
+    return solver_net_infer(s, r, diag, y);
+}
 cmg_status cmg_trainer_create(int capacity_tiles, uint64_t seed, cmg_trainer** out)
 {
     return trainer_create(capacity_tiles, seed, out);
@@ -518,28 +724,56 @@ cmg_status cmg_mfma_probe(int k, const uint16_t* A, const uint16_t* B, float* C)
 
 cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
 {
+    touch_work(s);
     return solver_solve(s, opts, stats);
 }
 
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host) { return solver_get_x64(s, host); }
 
-cmg_status cmg_op_full_neg_lap(cmg_grid* g, int x, int c0, int y) { return op_full_neg_lap(g, x, c0, y); }
-cmg_status cmg_op_propagate_ghosts(cmg_grid* g, int ch) { return op_propagate_ghosts(g, ch); }
-cmg_status cmg_op_accumulate_leafs(cmg_grid* g, int ch) { return op_accumulate_leafs(g, ch); }
+cmg_status cmg_op_full_neg_lap(cmg_grid* g, int x, int c0, int y)
+{
+    touch_channel(g, x);   // propagated ghosts and accumulated parents are left in x
+    touch_channel(g, y);
+    return op_full_neg_lap(g, x, c0, y);
+}
+cmg_status cmg_op_propagate_ghosts(cmg_grid* g, int ch)
+{
+    touch_channel(g, ch);
+    return op_propagate_ghosts(g, ch);
+}
+cmg_status cmg_op_accumulate_leafs(cmg_grid* g, int ch)
+{
+    touch_channel(g, ch);
+    return op_accumulate_leafs(g, ch);
+}
 cmg_status cmg_op_gs_color(cmg_grid* g, int level, int x, int c0, int rhs, int color, double omega)
 {
+    touch_channel(g, x);
     return op_gs_color(g, level, x, c0, rhs, color, omega);
 }
 cmg_status cmg_op_jacobi(cmg_grid* g, int level, int x_in, int x_out, int gch, int c0, int rhs, float omega)
 {
+    touch_channel(g, x_in);   // ghost values may be refreshed in place
+    touch_channel(g, x_out);
     return op_jacobi(g, level, x_in, x_out, gch, c0, rhs, omega);
 }
 cmg_status cmg_op_residual_restrict(cmg_grid* g, int level, int x, int c0, int rhs, float Rr)
 {
+    touch_channel(g, x);
+    touch_channel(g, rhs);
     return op_residual_restrict(g, level, x, c0, rhs, Rr);
 }
-cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0) { return op_fas_save(g, level, x, c0, rhs, x0); }
-cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P) { return op_fas_prolong(g, level, x, x0, P); }
+cmg_status cmg_op_fas_save(cmg_grid* g, int level, int x, int c0, int rhs, int x0)
+{
+    touch_channel(g, rhs);
+    touch_channel(g, x0);
+    return op_fas_save(g, level, x, c0, rhs, x0);
+}
+cmg_status cmg_op_fas_prolong(cmg_grid* g, int level, int x, int x0, float P)
+{
+    touch_channel(g, x);
+    return op_fas_prolong(g, level, x, x0, P);
+}
 cmg_status cmg_op_dot(cmg_grid* g, int a, int b, double* out) { return op_dot(g, a, b, out); }
 cmg_status cmg_op_norm(cmg_grid* g, int order, int channel, int volume_weighted, int cell_types, double* out)
 {
@@ -551,32 +785,47 @@ cmg_status cmg_op_norm_noalloc(cmg_grid* g, int order, int channel, int volume_w
 }
 cmg_status cmg_divergence(cmg_grid* g, int u0, int channel, int coeff_channel)
 {
+    touch_channels(g, u0, 3);   // faces accumulated to parents
+    touch_channel(g, channel);
     return op_divergence(g, u0, channel, coeff_channel);
 }
 cmg_status cmg_add_gradient(cmg_grid* g, int x, int coeff_channel, int u0, int weighted)
 {
+    touch_channel(g, x);
+    touch_channels(g, u0, 3);
     return op_add_gradient(g, x, coeff_channel, u0, weighted);
 }
 cmg_status cmg_mix_velocity(cmg_grid* g, int coeff_channel, int u_fluid, int u_solid, int u_mix)
 {
+    touch_channels(g, u_mix, 3);
     return op_mix_velocity(g, coeff_channel, u_fluid, u_solid, u_mix);
 }
-cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0) { return solver_cycle(s, opts, x, rhs, x0); }
+cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rhs, int x0)
+{
+    if (s) {
+        touch_channel(solver_grid(s), x);
+        touch_channel(solver_grid(s), rhs);
+        touch_channel(solver_grid(s), x0);
+    }
+    return solver_cycle(s, opts, x, rhs, x0);
+}
 
 cmg_status cmg_mg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats* stats)
 {
+    touch_work(s);
     return solver_mg_solve(s, opts, stats);
 }
 
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
+    touch_work(s);
     return bench_kernel(s, kernel, reps, avg_ms, tiles);
 }
 
 cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads)
 {
     if (!set_kernel_forms(persistent_mask, stencil_threads)) {
-        set_error("kernel forms: mask must be 0..15 and threads 128, 256 or 512");
+        set_error("kernel forms: mask must be 0..31 and threads 128, 256 or 512");
         return CMG_ERR_BAD_ARG;
     }
     return CMG_OK;

@@ -67,6 +67,11 @@ struct DevGrid {
     // reduction scratch for the op-level entry points
     double* partial = nullptr;
     double* scal = nullptr;
+    // region-of-interest flags and the tile minima / maxima they came from (N each, on first use)
+    uint8_t* roi = nullptr;
+    uint8_t* roi_locked = nullptr;
+    float* roi_min = nullptr;
+    float* roi_max = nullptr;
 };
 
 void dev_upload(DevGrid& d, const Topology& topo);
@@ -91,12 +96,37 @@ struct cmg_grid {
     bool types_on_device = false;
     int assembled_c0 = -1;
     int generation = 0;   // bumped by every (re)finalization: solvers made before it are stale
+    // bumped by every entry point that may write the channel / the cell types; a solver keeps what
+    // it derives from its operator (FP16 copies, the diagonal check, the level-0 inverse) while
+    // the generations of its coefficient channels and of the types are unchanged
+    uint64_t ch_gen[cmg::kChannels] = {};
+    uint64_t types_gen = 0;
+    bool host_types_stale = false;   // the device types were written on the device (the host copy follows on demand)
 };
 
 struct cmg_solver;
 struct cmg_trainer;
 
 namespace cmg {
+inline void touch_channel(cmg_grid* g, int c)
+{
+    if (g && c >= 0 && c < kChannels) ++g->ch_gen[c];
+}
+inline void touch_channels(cmg_grid* g, int c, int n)
+{
+    for (int i = 0; i < n; ++i) touch_channel(g, c + i);
+}
+inline void touch_types(cmg_grid* g)
+{
+    if (g) ++g->types_gen;
+}
+inline void touch_all(cmg_grid* g)
+{
+    touch_channels(g, 0, kChannels);
+    touch_types(g);
+}
+// the grid a solver runs on
+cmg_grid* solver_grid(const cmg_solver* s);
 void set_error(const std::string& msg);
 // uploads topology, cell types and host-assembled channels on first device use
 void ensure_device(cmg_grid* g);

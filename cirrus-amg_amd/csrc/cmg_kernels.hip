@@ -1597,6 +1597,8 @@ static int q_jacobi_threads()
 // was chosen (106 us against 108 at 256 on the 1024^3 torus workload)
 static int apply_threads() { return q_jacobi_threads(); }
 
+int launched_threads_key() { return apply_threads() * 1000 + q_jacobi_threads(); }
+
 void get_kernel_forms(int* mask, int* threads)
 {
     if (mask) *mask = persistent_mask();

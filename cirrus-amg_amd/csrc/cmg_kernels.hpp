@@ -142,11 +142,24 @@ int launch_assemble_operator(const DevGrid& d, int c0, float R, int face_rule, i
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict, bit 3 stored
 // diagonals, bit 4 unfused first sweep; threads per tile)
 bool set_kernel_forms(int mask, int threads);
+// the thread counts the operator and the FP16-copy Jacobi sweep launch with now (these follow
+// stencil_threads only once a count was chosen): part of a captured graph's identity
+int launched_threads_key();
 void get_kernel_forms(int* mask, int* threads);
 bool derive_allowed();             // forms bit 3 clear
 bool fuse_first_sweep_allowed();   // forms bit 4 clear
 bool fuse_rz_allowed();            // forms bit 4 clear and the one-tile Jacobi form
 bool fuse_fas_save_allowed();      // forms bit 4 clear and the one-tile Jacobi form
+
+// reference tile images (PoissonTile<float,15>) <-> this grid, device to device (cmg_tiles.hip):
+// tile t at ptrs[t] (device array of N addresses) or at base + t * stride; returns the number
+// of tiles skipped for a null or unaligned address; synchronises the grid's stream
+unsigned int tile_channel_io(DevGrid& d, bool gather, const uint64_t* ptrs, void* base, size_t stride, int tile_channel,
+                             float* X);
+unsigned int tile_types_io(DevGrid& d, bool gather, const uint64_t* ptrs, void* base, size_t stride);
+// MarkRegionOfInterestWithChannelMinAndMax128Kernel (src/PoissonGridKernels.cuh:11-54) on every tile
+void launch_roi_flags(const DevGrid& d, const float* X, int subtree_level, int launch_types, int pred, float a, float b,
+                      bool calc_locked, uint8_t* interest, uint8_t* locked, float* tmin, float* tmax, hipStream_t st);
 
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st);

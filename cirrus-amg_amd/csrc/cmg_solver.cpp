@@ -44,6 +44,17 @@ struct cmg_solver {
     int n0 = 0, nz0 = 0;
     bool direct_ready = false;
     bool direct_pure_neumann = false;
+    // generations of the cell types and of channels c0..c0+3 that each piece of derived data
+    // was computed from; it is recomputed as soon as one of them moves
+    struct Stamp {
+        uint64_t t = ~0ull, c[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+        bool operator==(const Stamp& o) const
+        {
+            return t == o.t && c[0] == o.c[0] && c[1] == o.c[1] && c[2] == o.c[2] && c[3] == o.c[3];
+        }
+    };
+    Stamp direct_stamp, q16_stamp, derive_stamp;
+    bool derive_ok = false;
     // captured iteration
     hipGraphExec_t gexec = nullptr;
     std::string gkey;
@@ -51,6 +62,8 @@ struct cmg_solver {
 };
 
 namespace cmg {
+
+cmg_grid* solver_grid(const cmg_solver* s) { return s ? s->g : nullptr; }
 
 namespace {
 
@@ -156,7 +169,8 @@ void dev_free(DevGrid& d)
 {
     for (void* p : {(void*)d.ttype, (void*)d.toct, (void*)d.tlevel, (void*)d.nbr, (void*)d.parent, (void*)d.child,
                     (void*)d.halo, (void*)d.ghalo, (void*)d.tcoord, (void*)d.acount,
-                    (void*)d.ctype, (void*)d.partial, (void*)d.scal})
+                    (void*)d.ctype, (void*)d.partial, (void*)d.scal, (void*)d.roi, (void*)d.roi_locked,
+                    (void*)d.roi_min, (void*)d.roi_max})
         if (p) (void)hipFree(p);
     for (auto& c : d.ch)
         if (c) (void)hipFree(c), c = nullptr;
@@ -528,6 +542,15 @@ namespace {
 
 enum { CH_X = 0, CH_R = 1, CH_P = 2, CH_AP = 3, CH_Z = 4 };
 
+// the generations the solver's operator data has now
+cmg_solver::Stamp stamp_now(const cmg_solver* s)
+{
+    cmg_solver::Stamp st;
+    st.t = s->g->types_gen;
+    for (int a = 0; a < 4; ++a) st.c[a] = s->g->ch_gen[s->c0 + a];
+    return st;
+}
+
 // dense FP64 inverse of the level-0 operator on its INTERIOR cells, probed
 // cell by cell with the same halo rules as the device stencil (level 0 has no
 // ghosts); Gauss-Jordan with partial pivoting.  For a pure-Neumann system the
@@ -538,12 +561,17 @@ void build_direct(cmg_solver* s, bool pure_neumann)
     const int lo = d.level_start[0], hi = d.level_start[1];
     const int nt = hi - lo;
     const Topology& topo = *s->g->topo;
+    const cmg_solver::Stamp stamp = stamp_now(s);
+    // level 0 is the first block of tiles and its face neighbours are level-0 tiles, so only
+    // those tiles' types and coefficients are read back
+    if (lo != 0) throw std::logic_error("level 0 does not start the tile order");
+    const size_t n0cells = size_t(hi) * kCells;
     std::vector<float> C[4];
-    std::vector<uint8_t> T(size_t(d.N) * kCells);
+    std::vector<uint8_t> T(n0cells);
     HIPCHK(hipMemcpy(T.data(), d.ctype, T.size(), hipMemcpyDeviceToHost));
     for (int a = 0; a < 4; ++a) {
-        C[a].resize(size_t(d.N) * kCells);
-        HIPCHK(hipMemcpy(C[a].data(), dev_channel(d, s->c0 + a), C[a].size() * sizeof(float), hipMemcpyDeviceToHost));
+        C[a].resize(n0cells);
+        HIPCHK(hipMemcpy(C[a].data(), dev_channel(d, s->c0 + a), n0cells * sizeof(float), hipMemcpyDeviceToHost));
     }
     std::vector<int> map(size_t(nt) * kCells, -1);
     std::vector<int32_t> cells, zeros;
@@ -613,6 +641,8 @@ void build_direct(cmg_solver* s, bool pure_neumann)
             A[size_t(k) * n + j] *= dk;
             inv[size_t(k) * n + j] *= dk;
         }
+        // rows are independent once row k is scaled
+#pragma omp parallel for schedule(static) if (n >= 256)
         for (int i = 0; i < n; ++i) {
             if (i == k) continue;
             const double f = A[size_t(i) * n + k];
@@ -638,6 +668,18 @@ void build_direct(cmg_solver* s, bool pure_neumann)
     if (!zeros.empty()) HIPCHK(hipMemcpy(s->zero0, zeros.data(), zeros.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     s->direct_ready = true;
     s->direct_pure_neumann = pure_neumann;
+    s->direct_stamp = stamp;
+    // a captured iteration holds the old inverse's address and the old cell counts
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;
+    s->gkey.clear();
+}
+
+// the level-0 inverse is (re)built when missing, when the null-space handling changes, or when
+// the cell types or the coefficient channels have been written since it was built
+void ensure_direct(cmg_solver* s, bool pure_neumann)
+{
+    if (!s->direct_ready || s->direct_pure_neumann != pure_neumann || !(s->direct_stamp == stamp_now(s)))
+        build_direct(s, pure_neumann);
 }
 
 struct CycleCtx {
@@ -994,7 +1036,7 @@ std::string graph_key(const cmg_solve_opts* o, int derive)
       << o->coarsest_direct << ',' << o->is_pure_neumann << ',' << o->cycle_coef_fp16;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
-    k << ',' << mask << ',' << threads << ',' << derive;
+    k << ',' << mask << ',' << threads << ',' << launched_threads_key() << ',' << derive;
     return k.str();
 }
 
@@ -1333,9 +1375,7 @@ cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, 
     if (!valid_channel(x) || !valid_channel(rhs) || !valid_channel(x0)) return CMG_ERR_BAD_ARG;
     CMG_TRY({
         ensure_buffers(s);
-        if (s->family == CMG_SOLVER_AMG && o->coarsest_direct &&
-            (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
-            build_direct(s, o->is_pure_neumann);
+        if (s->family == CMG_SOLVER_AMG && o->coarsest_direct) ensure_direct(s, o->is_pure_neumann != 0);
         if (s->family == CMG_SOLVER_AMG && o->cycle_coef_fp16) prepare_coef_fp16(s);
         enqueue_cycle(s, o, x, rhs, x0, nullptr);
         HIPCHK(hipStreamSynchronize(s->g->dev.stream));
@@ -1344,11 +1384,15 @@ cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, 
 }
 
 // FP16 copies of the coefficient channels divided by the tile spacing, for the
-// cycle kernels; rebuilt at every solve (the channels may change between solves)
+// cycle kernels; rebuilt whenever the coefficient channels or the types have been written
+// since the last copy
 void prepare_coef_fp16(cmg_solver* s)
 {
     DevGrid& d = s->g->dev;
     const size_t n = size_t(d.N) * kCells;
+    const cmg_solver::Stamp stamp = stamp_now(s);
+    if (s->q16 && s->q16_stamp == stamp) return;
+    s->q16_stamp = cmg_solver::Stamp{};
     if (!s->q16) {
         s->q16 = dmalloc<uint16_t>(4 * n);
         s->qbad = dmalloc<unsigned int>(1);
@@ -1360,6 +1404,7 @@ void prepare_coef_fp16(cmg_solver* s)
     HIPCHK(hipMemcpyAsync(&bad, s->qbad, sizeof(bad), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     if (bad) throw std::invalid_argument("cycle_coef_fp16: a coefficient divided by the tile spacing is outside the FP16 range");
+    s->q16_stamp = stamp;
 }
 
 // the stencils may derive LEAF diagonals from the face coefficients (one load
@@ -1367,12 +1412,16 @@ void prepare_coef_fp16(cmg_solver* s)
 bool diagonal_derivable(cmg_solver* s)
 {
     DevGrid& d = s->g->dev;
+    const cmg_solver::Stamp stamp = stamp_now(s);
+    if (s->derive_stamp == stamp) return s->derive_ok;
     set_scalars(s, 0.0, 0);
     const int n = launch_diag_check(d, coef(d, s->c0), s->partial, d.stream);
     launch_finish(s->partial, nullptr, n, s->S, FIN_SUM, d.stream);
     double h[S_COUNT];
     read_scalars(s, h);
-    return h[S_TMP] == 0.0;
+    s->derive_ok = h[S_TMP] == 0.0;
+    s->derive_stamp = stamp;
+    return s->derive_ok;
 }
 
 struct DeriveScope {
@@ -1401,8 +1450,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         const bool amg = s->family == CMG_SOLVER_AMG;
         if (amg) ensure_fp64(s);
         if (o->learned) HIPCHK(hipMemset(s->lcount, 0, 2 * sizeof(unsigned int)));   // counts of this solve
-        if (amg && o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
-            build_direct(s, o->is_pure_neumann);
+        if (amg && o->coarsest_direct) ensure_direct(s, o->is_pure_neumann != 0);
         if (amg && o->cycle_coef_fp16) prepare_coef_fp16(s);
         DeriveScope derive(d, amg && derive_allowed() && diagonal_derivable(s));
         // keep b in FP64 for the true residual / refinement (not needed when an FP32 solve skips it)
@@ -1491,8 +1539,7 @@ cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_sta
         DevGrid& d = s->g->dev;
         hipStream_t stream = d.stream;
         ensure_buffers(s);
-        if (o->coarsest_direct && (!s->direct_ready || s->direct_pure_neumann != bool(o->is_pure_neumann)))
-            build_direct(s, o->is_pure_neumann);
+        if (o->coarsest_direct) ensure_direct(s, o->is_pure_neumann != 0);
         float *X = dev_channel(d, CH_X), *B = dev_channel(d, CH_R), *X0 = dev_channel(d, CH_P);
         const Coef C = coef(d, s->c0);
         set_scalars(s, 0.0, o->max_iters);
