@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcirrus_amg.so")
 
 LEAF, NONLEAF, GHOST = 1, 2, 4
+ROI_MAX_GT, ROI_MIN_LT, ROI_RANGE_MEETS, ROI_SPAN_GT = 0, 1, 2, 3   # cmg_mark_region_of_interest predicates
 DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
 X_CHANNEL, B_CHANNEL, P_CHANNEL, AP_CHANNEL, Z_CHANNEL = 0, 1, 2, 3, 4   # src/PoissonTile.h:13-18
 
@@ -83,6 +84,9 @@ EXPORTS = [
     "cmg_trainer_set_weights", "cmg_trainer_loss_grad", "cmg_trainer_step", "cmg_mfma_probe", "cmg_solver_learned_counts", "cmg_grid_dump_blob", "cmg_grid_load_blob", "cmg_assemble_cutcell_device", "cmg_divergence", "cmg_add_gradient", "cmg_mix_velocity",
     "cmg_grid_rebuild_device", "cmg_grid_iterative_refine_device", "cmg_grid_finalize_device",
     "cmg_grid_build_device", "cmg_op_norm_noalloc",
+    "cmg_grid_set_channel_device", "cmg_grid_get_channel_device", "cmg_grid_gather_tile_channel",
+    "cmg_grid_scatter_tile_channel", "cmg_grid_gather_tile_types", "cmg_grid_scatter_tile_types",
+    "cmg_mark_region_of_interest", "cmg_grid_get_roi",
 ]
 
 _lib = None
@@ -133,6 +137,13 @@ def lib():
             "cmg_solver_apply": (st, [vp, i, i]),
             "cmg_op_norm": (st, [vp, i, i, i, i, C.POINTER(d)]),
             "cmg_op_norm_noalloc": (st, [vp, i, i, i, i, C.POINTER(d)]),
+            "cmg_grid_set_channel_device": (st, [vp, i, vp]), "cmg_grid_get_channel_device": (st, [vp, i, vp]),
+            "cmg_grid_gather_tile_channel": (st, [vp, vp, vp, C.c_size_t, i, i]),
+            "cmg_grid_scatter_tile_channel": (st, [vp, i, vp, vp, C.c_size_t, i]),
+            "cmg_grid_gather_tile_types": (st, [vp, vp, vp, C.c_size_t]),
+            "cmg_grid_scatter_tile_types": (st, [vp, vp, vp, C.c_size_t]),
+            "cmg_mark_region_of_interest": (st, [vp, i, C.c_uint8, i, i, f, f, i]),
+            "cmg_grid_get_roi": (st, [vp, vp, vp, vp, vp]),
             "cmg_solver_set_net": (st, [vp, vp]), "cmg_solver_net_infer": (st, [vp, i, i, i]),
             "cmg_trainer_create": (st, [i, C.c_uint64, C.POINTER(vp)]), "cmg_trainer_destroy": (None, [vp]),
             "cmg_trainer_collect": (st, [vp, vp, i, i, i, C.POINTER(i)]), "cmg_trainer_add_tiles": (st, [vp, vp, vp, i]),
@@ -409,6 +420,44 @@ class HADeviceGrid:
         out = C.c_double()
         _check(lib().cmg_op_dot(self.h, a, b, C.byref(out)), "dot")
         return out.value
+
+    # device-resident exchange: device addresses as Python ints
+    def set_channel_device(self, c, d_src):
+        _check(lib().cmg_grid_set_channel_device(self.h, c, C.c_void_p(d_src)), "cmg_grid_set_channel_device")
+
+    def get_channel_device(self, c, d_dst):
+        _check(lib().cmg_grid_get_channel_device(self.h, c, C.c_void_p(d_dst)), "cmg_grid_get_channel_device")
+
+    def gather_tile_channel(self, tile_channel, channel, d_addrs=None, d_base=None, stride=0):
+        """channel <- tile_channel of PoissonTile<float,15> device images (src/PoissonTile.h:9-175)"""
+        _check(lib().cmg_grid_gather_tile_channel(self.h, C.c_void_p(d_addrs), C.c_void_p(d_base), stride, tile_channel,
+                                                  channel), "cmg_grid_gather_tile_channel")
+
+    def scatter_tile_channel(self, channel, tile_channel, d_addrs=None, d_base=None, stride=0):
+        _check(lib().cmg_grid_scatter_tile_channel(self.h, channel, C.c_void_p(d_addrs), C.c_void_p(d_base), stride,
+                                                   tile_channel), "cmg_grid_scatter_tile_channel")
+
+    def gather_tile_types(self, d_addrs=None, d_base=None, stride=0):
+        _check(lib().cmg_grid_gather_tile_types(self.h, C.c_void_p(d_addrs), C.c_void_p(d_base), stride),
+               "cmg_grid_gather_tile_types")
+
+    def scatter_tile_types(self, d_addrs=None, d_base=None, stride=0):
+        _check(lib().cmg_grid_scatter_tile_types(self.h, C.c_void_p(d_addrs), C.c_void_p(d_base), stride),
+               "cmg_grid_scatter_tile_types")
+
+    def mark_region_of_interest(self, channel, pred=0, a=0.0, b=0.0, subtree_level=-1, launch_types=LEAF,
+                                calc_locked=False):
+        """MarkRegionOfInterestWithChannelMinAndMax128Kernel (src/PoissonGridKernels.cuh:11-54)"""
+        _check(lib().cmg_mark_region_of_interest(self.h, subtree_level, launch_types, channel, pred, a, b,
+                                                 int(calc_locked)), "cmg_mark_region_of_interest")
+
+    def roi(self):
+        """(interest, locked, tile min, tile max) of the last marking"""
+        n = self.N
+        fi, fl = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+        mn, mx = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _check(lib().cmg_grid_get_roi(self.h, _ptr(fi), _ptr(fl), _ptr(mn), _ptr(mx)), "cmg_grid_get_roi")
+        return fi, fl, mn, mx
 
 
 def device_synchronize():

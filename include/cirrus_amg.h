@@ -131,6 +131,55 @@ cmg_status cmg_grid_fill_channel(cmg_grid* g, int channel, float value);
 /* device-side copy of a whole channel (e.g. to restore a right-hand side) */
 cmg_status cmg_grid_copy_channel(cmg_grid* g, int src, int dst);
 
+/* ------------------------------------------------------- device-resident exchange */
+/* One channel of every tile (N*512 floats, tile-major, the layout of every per-cell
+ * array here) from / to device memory of the caller, device to device; no host copy.
+ * The caller's writes to d_src must be complete before the call (e.g. its stream
+ * synchronised); the copy has completed when the call returns.  No reference
+ * counterpart as an entry point: in the reference the solver reads the tiles'
+ * channels in place (PoissonTile::mData, src/PoissonTile.h:35). */
+cmg_status cmg_grid_set_channel_device(cmg_grid* g, int channel, const float* d_src);
+cmg_status cmg_grid_get_channel_device(cmg_grid* g, int channel, float* d_dst);
+/* The reference's own device tiles.  Each tile of this grid's all-tiles order is a
+ * PoissonTile<float,15> device image (src/PoissonTile.h:9-175, 44,880 bytes:
+ * mData[15][736] floats, mNeighbors[6], mCellType[512], ...) at d_tile_addrs[t] (a
+ * device array of N 64-bit addresses: HATileInfo::mTilePtr of dAllTiles,
+ * src/HAGrid.h:654-685), or, with d_tile_addrs == NULL, at d_base + t * stride (tiles in
+ * one slab; stride a multiple of 16 and at least 44,880).  Gather copies cells 0..511 of
+ * tile channel `tile_channel` (0..14) into this grid's `channel`; scatter writes this
+ * grid's `channel` into the tiles' `tile_channel` (node entries 512..735 untouched).
+ * The types calls move mCellType.  Tiles must be 16-byte aligned; a null or unaligned
+ * address returns CMG_ERR_BAD_ARG after the other tiles were copied.  This is how
+ * FluidEuler::project (cirrus_cutcell/FluidEuler.cu:688-752) hands its velocities,
+ * types and right-hand side to the solve and takes the pressure back without a host
+ * round trip. */
+cmg_status cmg_grid_gather_tile_channel(cmg_grid* g, const uint64_t* d_tile_addrs, const void* d_base, size_t stride,
+                                        int tile_channel, int channel);
+cmg_status cmg_grid_scatter_tile_channel(cmg_grid* g, int channel, const uint64_t* d_tile_addrs, void* d_base,
+                                         size_t stride, int tile_channel);
+cmg_status cmg_grid_gather_tile_types(cmg_grid* g, const uint64_t* d_tile_addrs, const void* d_base, size_t stride);
+cmg_status cmg_grid_scatter_tile_types(cmg_grid* g, const uint64_t* d_tile_addrs, void* d_base, size_t stride);
+
+/* ------------------------------------------------------- region of interest */
+/* MarkRegionOfInterestWithChannelMinAndMax128Kernel (src/PoissonGridKernels.cuh:11-54)
+ * over every tile: a tile whose subtree type (NONLEAF at subtree_level counts as LEAF,
+ * src/HATileInfo.h:79-84; -1 = none) is in launch_types gets the min and max of the
+ * 512 values of `channel` and the flag pred(min, max); other tiles get 0.  The
+ * reference passes the predicate as a device lambda; here it is one of the forms
+ * below (its callers use CMG_ROI_MAX_GT with a = 0, src/MarkerParticles.cu:49-61).
+ * calc_locked != 0 also sets the locked-refine flags (mIsLockedRefine) to the same
+ * value; otherwise they are left as they are.  Flags stay on the device (one byte a
+ * tile, all-tiles order) until read. */
+enum { CMG_ROI_MAX_GT = 0,        /* max > a */
+       CMG_ROI_MIN_LT = 1,        /* min < a */
+       CMG_ROI_RANGE_MEETS = 2,   /* [min, max] meets [a, b] */
+       CMG_ROI_SPAN_GT = 3 };     /* max - min > a */
+cmg_status cmg_mark_region_of_interest(cmg_grid* g, int subtree_level, uint8_t launch_types, int channel, int pred,
+                                       float a, float b, int calc_locked);
+/* host copies of the flags and of the tile minima / maxima of the last marking (N each;
+ * any pointer may be NULL; min / max of unselected tiles are left 0) */
+cmg_status cmg_grid_get_roi(cmg_grid* g, uint8_t* interest, uint8_t* locked, float* tile_min, float* tile_max);
+
 /* ------------------------------------------------------- assembly ---------- */
 
 /* AMGSolver::prepareTypesAndCoeffs -> CoarsenTypesAndAMGCoeffs (src/AMGSolver.cu:256-318,1394-1397) */

@@ -509,6 +509,38 @@ double og_dot(og_grid* g, int a, int b)
     return s;
 }
 
+/* MarkRegionOfInterestWithChannelMinAndMax128Kernel (src/PoissonGridKernels.cuh:11-54):
+ * for every tile whose subtree type (NONLEAF at subtree_level counts as LEAF,
+ * src/HATileInfo.h:79-84) is in launch_types, min and max of the 512 values of ch and
+ * the flag pred(min, max): 0 max > a, 1 min < a, 2 [min, max] meets [a, b], 3 max - min > a
+ * (the reference takes a device lambda; its callers use max > 0, src/MarkerParticles.cu:49-61);
+ * other tiles get flag 0 and keep tmin / tmax */
+void og_roi_flags(const og_grid* g, int ch, int subtree_level, int launch_types, int pred, float a, float b,
+                  uint8_t* interest, float* tmin, float* tmax)
+{
+    for (int t = 0; t < g->N; t++) {
+        uint8_t tt = g->t_type[t];
+        if (g->t_level[t] == subtree_level && tt == OG_NONLEAF) tt = OG_LEAF;
+        if (!(tt & launch_types)) { interest[t] = 0; continue; }
+        const float* x = g->ch[ch] + (size_t)t * 512;
+        float lo = x[0], hi = x[0];
+        for (int v = 1; v < 512; v++) {
+            if (x[v] < lo) lo = x[v];
+            if (x[v] > hi) hi = x[v];
+        }
+        int f;
+        switch (pred) {
+        case 0: f = hi > a; break;
+        case 1: f = lo < a; break;
+        case 2: f = lo <= b && hi >= a; break;
+        default: f = (hi - lo) > a; break;
+        }
+        interest[t] = (uint8_t)f;
+        tmin[t] = lo;
+        tmax[t] = hi;
+    }
+}
+
 /* NormSync (PoissonGrid.cu:322-375) with ChannelPowerSumKernel128 (:223-319):
  * over LEAF tiles, cells whose type is in cell_types, |x|; order -1 = max,
  * else (sum |x|^p w / sum w)^(1/p), w = h^3 when volume weighted; FP64 */

@@ -124,6 +124,8 @@ void og_fas_prolong(og_grid* g, int level, int x, int x0, float P);             
 double og_dot(og_grid* g, int a, int b);                       /* PoissonGrid.cu:156-214, LEAF tiles */
 void og_recenter(og_grid* g, int ch);                          /* PoissonGrid.cu:889-921 */
 double og_norm(og_grid* g, int order, int ch, int volume_weighted, int cell_types); /* PoissonGrid.cu:322-375 */
+void og_roi_flags(const og_grid* g, int ch, int subtree_level, int launch_types, int pred, float a, float b,
+                  uint8_t* interest, float* tmin, float* tmax);                  /* PoissonGridKernels.cuh:11-54 */
 /* projection wrappers: divergence of face velocities u0..u0+2 into ch (c0 < 0:
  * AMGVolumeWeightedDivergenceWithoutCoeffOnLeafs, AMGSolver.cu:750-788; c0 >= 0:
  * AMGVolumeWeightedDivergenceOnLeafs, :790-835), gradient of x added to faces

@@ -70,6 +70,7 @@ def lib():
             "og_prepare_system": (None, [vp, i, f]),
             "og_face_fluid_ratio": (f, [f, f, f, f]), "og_sdf": (f, [i, vp, f, f, f]),
             "og_face_supersampled": (f, [i, vp, vp, i, f]),
+            "og_roi_flags": (None, [vp, i, i, i, i, f, f, vp, vp, vp]),
             "og_propagate_ghosts": (None, [vp, i]), "og_accumulate_leafs": (None, [vp, i]),
             "og_neg_lap_leafs": (None, [vp, i, i, i]), "og_full_neg_lap": (None, [vp, i, i, i]),
             "og_gs_color": (None, [vp, i, i, i, i, i, d]),
@@ -224,6 +225,14 @@ class Grid:
     # --- ops
     def full_neg_lap(self, x, c0, y):
         lib().og_full_neg_lap(self.h, x, c0, y)
+
+    def roi_flags(self, ch, pred=0, a=0.0, b=0.0, subtree_level=-1, launch_types=1):
+        """MarkRegionOfInterestWithChannelMinAndMax128Kernel (src/PoissonGridKernels.cuh:11-54)"""
+        n = self.N
+        fi = np.zeros(n, np.uint8)
+        mn, mx = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        lib().og_roi_flags(self.h, ch, subtree_level, launch_types, pred, a, b, _ptr(fi), _ptr(mn), _ptr(mx))
+        return fi, mn, mx
 
     def dot(self, a, b):
         return lib().og_dot(self.h, a, b)

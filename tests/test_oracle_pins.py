@@ -157,3 +157,26 @@ def test_net_forward_matches_torch_conv3d():
         a2 = F.relu(F.conv3d(a1, w2, padding=1))
         yt = F.conv3d(a2, w3, padding=1).view(10, 10, 10).numpy()
         assert np.allclose(y, yt, rtol=1e-12, atol=1e-12 * np.abs(yt).max()), trial
+
+
+# ---------------------------------------------------------------- og_roi_flags
+def test_roi_flags_are_per_tile_min_max_predicates():
+    """og_roi_flags against numpy's per-tile min / max and the tile table (subtree rule included)"""
+    import helpers as H
+    go, _ = H.build_pair("sphere_empty24", with_device=False)
+    n = go.N
+    x = H.random_fill(n, 21)
+    x[np.arange(n) % 3 == 0] = -np.abs(x[np.arange(n) % 3 == 0])      # some tiles entirely negative
+    go.channel(5)[:] = x
+    lv, _, ty = go.tiles()
+    mn, mx = x.min(axis=1), x.max(axis=1)
+    sub = int(lv.max()) - 1
+    stype = np.where((lv == sub) & (ty == 2), 1, ty)                   # NONLEAF at the subtree level acts as LEAF
+    for pred, a, b, want in [(0, 0.0, 0.0, mx > 0), (1, -0.9, 0.0, mn < -0.9), (2, 0.95, 2.0, (mn <= 2.0) & (mx >= 0.95)),
+                             (3, 1.9, 0.0, (mx - mn) > 1.9)]:
+        for level, types in [(-1, 1), (sub, 1), (-1, 6)]:
+            sel = ((np.where(lv == level, stype, ty) if level >= 0 else ty) & types) != 0
+            fi, tmn, tmx = go.roi_flags(5, pred, a, b, level, types)
+            assert np.array_equal(fi.astype(bool), want & sel), (pred, level, types)
+            assert np.array_equal(tmn[sel], mn[sel]) and np.array_equal(tmx[sel], mx[sel])
+            assert 0 < sel.sum() < n
