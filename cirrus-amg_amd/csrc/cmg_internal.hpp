@@ -29,6 +29,9 @@ struct LaunchList {
     int n = 0;        // workgroups to launch
     int tiles = 0;    // real tiles in the list
     int4* meta = nullptr;   // prolongation lists: per entry (tile, parent, octant | type << 8, 0)
+    // per entry 5 int4: (tile, level, type | octant << 8, parent), the tile's 32-byte halo record
+    // and its ghost-parent record (-1 tile on padding entries); read by the one-tile stencil kernels
+    int4* rec = nullptr;
 };
 
 struct DevGrid {
@@ -67,6 +70,8 @@ struct DevGrid {
     // reduction scratch for the op-level entry points
     double* partial = nullptr;
     double* scal = nullptr;
+    // the scalar block of launches outside a solve: its stop flag is never set
+    double* idle = nullptr;
     // region-of-interest flags and the tile minima / maxima they came from (N each, on first use)
     uint8_t* roi = nullptr;
     uint8_t* roi_locked = nullptr;

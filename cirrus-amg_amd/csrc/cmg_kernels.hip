@@ -61,7 +61,9 @@ __device__ __forceinline__ float derived_diag(const Lds<T>& s, int v, const floa
 
 
 // Per-thread state of the cells a thread owns: with NT threads per tile each
-// thread owns CPT = 512/NT cells v = tid + c*NT (CPT x-planes apart).
+// thread owns CPT = 512/NT cells, v = own_cell<NT>(c) = CPT*tid + c (consecutive
+// along k) for the one-tile-per-workgroup kernels (load_tile), v = tid + c*NT
+// (CPT x-planes apart) for the persistent pipelined forms (issue / stage_lds).
 template <typename T, int NT>
 struct Own {
     static constexpr int CPT = 512 / NT;
@@ -80,6 +82,37 @@ __device__ __forceinline__ const P* uniform_ptr(const P* p)
     const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(u));
     const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(u >> 32));
     return reinterpret_cast<const P*>((uint64_t(hi) << 32) | lo);
+}
+
+// the cells of a thread in the one-tile kernels: CPT = 512/NT consecutive cells, so each
+// per-cell array of a tile is read (and written) with one vector access per thread
+template <int NT>
+__device__ __forceinline__ int own_cell(int c)
+{
+    return (512 / NT) * int(threadIdx.x) + c;
+}
+
+template <int N, typename E>
+struct alignas(N * sizeof(E)) Pack {
+    E v[N];
+};
+
+// N consecutive elements at p (aligned to N elements) in one access
+template <int N, typename E>
+__device__ __forceinline__ void vload(const E* __restrict__ p, E (&out)[N])
+{
+    const Pack<N, E> pk = *reinterpret_cast<const Pack<N, E>*>(p);
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = pk.v[i];
+}
+
+template <int N, typename E>
+__device__ __forceinline__ void vstore(E* __restrict__ p, const E (&in)[N])
+{
+    Pack<N, E> pk;
+#pragma unroll
+    for (int i = 0; i < N; ++i) pk.v[i] = in[i];
+    *reinterpret_cast<Pack<N, E>*>(p) = pk;
 }
 
 // four consecutive FP16 coefficients (quad q of the tile-major layout), back to FP32
@@ -110,6 +143,44 @@ __device__ __forceinline__ int pick6(int4 a, int4 b, int s)
     return r;
 }
 
+// A one-tile stencil workgroup's record (launch-list order, 5 x 16 bytes, read with
+// scalar loads right after the launch): the tile, its level, type | octant << 8 and
+// parent; its halo record (face neighbours, their tile types packed in bytes, own
+// type in byte 7.3); for the operator's fresh ghost values, the parents of GHOST
+// face neighbours and their octants.  One round trip gives the workgroup every
+// address it needs: the tile's own loads and its halo loads follow at once.
+struct TileRec {
+    int t, level, meta, parent;
+    int4 ha, hb, ga, gb;
+};
+
+// t = -1: padding entry, -2: the solve has stopped.  S is never null here (launches
+// outside a solve pass the grid's idle block, whose flag is never set), so the stop
+// flag is read with no branch and its load overlaps the record's.
+template <bool FRESH, bool STOP = true>
+__device__ __forceinline__ TileRec rec_tile(const int4* __restrict__ rec, const double* __restrict__ S)
+{
+    const int4* r = rec + 5 * size_t(blockIdx.x);
+    const int4 r0 = r[0];
+    TileRec x;
+    x.ha = r[1];
+    x.hb = r[2];
+    if (FRESH) {
+        x.ga = r[3];
+        x.gb = r[4];
+    } else {
+        x.ga = make_int4(-1, -1, -1, -1);
+        x.gb = make_int4(-1, -1, 0, 0);
+    }
+    x.t = (STOP && S[S_DONE] != 0.0) ? -2 : r0.x;
+    x.level = r0.y;
+    x.meta = r0.z;
+    x.parent = r0.w;
+    return x;
+}
+
+__device__ __forceinline__ float rec_h(const TopoView& tp, const TileRec& r) { return tp.h0 / float(1 << r.level); }
+
 // LoadAMGLaplacianTileData (src/AMGSolver.cu:395-543).
 // FRESH: the halo is evaluated the way AMGFullNegativeLaplacianOnLeafs sees it
 //   after PropagateToChildren + AccumulateToParentsOneStep: a GHOST cell is its
@@ -119,18 +190,17 @@ __device__ __forceinline__ int pick6(int4 a, int4 b, int s)
 // All global loads (own cells and the 384 halo cells) are issued before the
 // first barrier; only the ghost extrapolation waits for the tile's own values.
 template <bool FRESH, typename T, int NT, bool Q = false>
-__device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, const T* __restrict__ X,
+__device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, float h, const T* __restrict__ X,
                                           const T* __restrict__ G, const float* const* C, Lds<T>& s, Own<T, NT>& o,
                                           const uint16_t* const* Qc = nullptr)
 {
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
     const int tid = threadIdx.x;
-    const size_t base = size_t(t) * 512;
-    // the tile's halo record (uniform): face neighbours, their tile types, the
-    // tile's own type and (FRESH) the parents and octants of GHOST neighbours
-    const int4* hr = reinterpret_cast<const int4*>(tp.halo + size_t(t) * 8);
-    const int4 ha = hr[0], hb = hr[1];
+    const size_t base = size_t(r.t) * 512;
+    // the tile's halo record: face neighbours, their tile types, the tile's own
+    // type and (FRESH) the parents and octants of GHOST neighbours
+    const int4 ha = r.ha, hb = r.hb;
     const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
     const bool dv = !Q && tp.derive && ((ty45 >> 24) & LEAF);   // the Q form reads every diagonal
     // the coefficient planes as values (a runtime-indexed C[axis] would be a load)
@@ -140,43 +210,58 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
     const uint16_t* const qp0 = Q ? uniform_ptr(Qc[0]) : nullptr;
     const uint16_t* const qp1 = Q ? uniform_ptr(Qc[1]) : nullptr;
     const uint16_t* const qp2 = Q ? uniform_ptr(Qc[2]) : nullptr;
-    int4 ga = make_int4(-1, -1, -1, -1), gb = make_int4(-1, -1, 0, 0);
-    if (FRESH) {
-        const int4* gr = reinterpret_cast<const int4*>(tp.ghalo + size_t(t) * 8);
-        ga = gr[0];
-        gb = gr[1];
-    }
+    const int4 ga = r.ga, gb = r.gb;
     const uint32_t goct = uint32_t(gb.z);
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-        const int v = tid + c * NT;
-        o.ct[c] = tp.ctype[base + v];
-        const T xr = X[base + v];
-        o.x[c] = o.ct[c] == INTERIOR ? xr : T(0);
+    {
+        // own cells: one vector load per array
+        const size_t v0 = base + size_t(CPT) * tid;
+        T xr[CPT];
+        vload<CPT>(tp.ctype + v0, o.ct);
+        vload<CPT>(X + v0, xr);
         if constexpr (Q) {
-            o.c[c][0] = coef_q(qp0, base + v, h);
-            o.c[c][1] = coef_q(qp1, base + v, h);
-            o.c[c][2] = coef_q(qp2, base + v, h);
-            o.d[c] = dv ? 0.0f : coef_q(Qc[3], base + v, h);
-        } else {
+            uint16_t q[4][CPT];
+            vload<CPT>(qp0 + v0, q[0]);
+            vload<CPT>(qp1 + v0, q[1]);
+            vload<CPT>(qp2 + v0, q[2]);
+            if (!dv) vload<CPT>(Qc[3] + v0, q[3]);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) o.c[c][a] = C[a][base + v];
-            o.d[c] = dv ? 0.0f : C[3][base + v];
+            for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) o.c[c][a] = float(__builtin_bit_cast(_Float16, q[a][c])) * h;
+                o.d[c] = dv ? 0.0f : float(__builtin_bit_cast(_Float16, q[3][c])) * h;
+            }
+        } else {
+            float cf[4][CPT];
+            vload<CPT>(cp0 + v0, cf[0]);
+            vload<CPT>(cp1 + v0, cf[1]);
+            vload<CPT>(cp2 + v0, cf[2]);
+            if (!dv) vload<CPT>(C[3] + v0, cf[3]);
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) o.c[c][a] = cf[a][c];
+                o.d[c] = dv ? 0.0f : cf[3][c];
+            }
         }
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) o.x[c] = o.ct[c] == INTERIOR ? xr[c] : T(0);
     }
+    // halo, phase 1: every load is issued, unconditionally and with no loaded value used yet
+    // (a use, or a register copy at a branch join, makes the wave wait for all loads in
+    // flight, the tile's own included).  Cells without a neighbour tile read a cell of the
+    // tile itself instead, and the kind decides in phase 2 what the loaded values mean.
     T hraw[HPT];
-    float hcp[HPT];
-    bool hghost[HPT];
-    uint32_t hct[HPT];   // cell type of a same-level neighbour cell; 0xff: no INTERIOR mask
+    uint32_t hcb[HPT];     // the neighbour's + face coefficient: FP16 bits (Q) or FP32 bits
+    uint32_t hct[HPT];     // a same-level neighbour cell's type
+    int hkind[HPT];        // 0 none, 1 same-level cell (INTERIOR mask), 2 GHOST, 3 GHOST without parent (0),
+                           // 4 FRESH NONLEAF (the value is computed below)
+    int hni[HPT];          // the neighbour cell (kind 4)
 #pragma unroll
     for (int m = 0; m < HPT; ++m) {
         const int k = tid + m * NT;
-        hraw[m] = T(0);
-        hcp[m] = h;
-        hghost[m] = false;
-        hct[m] = 0xffu;
-        if (k >= 384) continue;
-        const int f = k >> 6, fi = k & 63;
+        const bool in = k < 384;   // false only at 256 and 512 threads a tile, for whole waves
+        const int kk = in ? k : 0;
+        const int f = kk >> 6, fi = kk & 63;
         const int axis = f >> 1, sgn = (f & 1) ? 1 : -1;
         const int p = fi >> 3, q = fi & 7;
         int nl[3];
@@ -185,32 +270,53 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         else { nl[0] = p; nl[1] = q; nl[2] = sgn < 0 ? 7 : 0; }
         const int slot = sgn < 0 ? axis : axis + 3;
         const int nt = pick6(ha, hb, slot);
-        if (fi == 0) s.empty[f] = nt < 0;
-        if (nt < 0) continue;
-        const size_t ni = size_t(nt) * 512 + cidx(nl[0], nl[1], nl[2]);
-        const uint32_t nty = (slot < 4 ? ty03 >> (8 * slot) : ty45 >> (8 * (slot - 4))) & 0xffu;
-        // only a + face needs the neighbour's coefficient (a - face's is the cell's own)
-        if constexpr (Q) {
-            const uint16_t* qax = axis == 0 ? qp0 : axis == 1 ? qp1 : qp2;
-            if (sgn > 0) hcp[m] = coef_q(qax, ni, h);
-        } else {
-            const float* cax = axis == 0 ? cp0 : axis == 1 ? cp1 : cp2;
-            if (sgn > 0) hcp[m] = cax[ni];
-        }
-        if (nty & GHOST) {
-            hghost[m] = true;
+        if (in && fi == 0) s.empty[f] = nt < 0;
+        const bool ok = in && nt >= 0;
+        const int cl = cidx(nl[0], nl[1], nl[2]);
+        const size_t ni = size_t(ok ? nt : r.t) * 512 + cl;
+        const uint32_t nty = ok ? (slot < 4 ? ty03 >> (8 * slot) : ty45 >> (8 * (slot - 4))) & 0xffu : 0u;
+        // a - face's coefficient is the cell's own: the + face reads the neighbour's, a - face
+        // re-reads the tile's own first 64 (cached) so that the load is unconditional
+        const size_t ci = sgn > 0 ? ni : base + fi;
+        if constexpr (Q) hcb[m] = (axis == 0 ? qp0 : axis == 1 ? qp1 : qp2)[ci];
+        else hcb[m] = __float_as_uint((axis == 0 ? cp0 : axis == 1 ? cp1 : cp2)[ci]);
+        const bool ghost = nty & GHOST;
+        size_t xi = ni;
+        const T* xs = X;
+        int kind = ok ? 1 : 0;
+        if (ghost) {
             if (FRESH) {
                 const int pt = pick6(ga, gb, slot);
                 const int oc = int((goct >> (3 * slot)) & 7u);
-                hraw[m] = pt < 0 ? T(0)
-                                 : X[size_t(pt) * 512 + cidx(4 * (oc & 1) + (nl[0] >> 1), 4 * ((oc >> 1) & 1) + (nl[1] >> 1),
-                                                              4 * ((oc >> 2) & 1) + (nl[2] >> 1))];
+                kind = pt >= 0 ? 2 : 3;
+                xi = pt >= 0 ? size_t(pt) * 512 + cidx(4 * (oc & 1) + (nl[0] >> 1), 4 * ((oc >> 1) & 1) + (nl[1] >> 1),
+                                                       4 * ((oc >> 2) & 1) + (nl[2] >> 1))
+                             : ni;
             } else {
-                hraw[m] = G[ni];
+                kind = 2;
+                xs = G;
             }
         } else if (FRESH && (nty & NONLEAF)) {
-            if (tp.ctype[ni] == INTERIOR) {
-                const int oct = (nl[0] >> 2) | ((nl[1] >> 2) << 1) | ((nl[2] >> 2) << 2);
+            kind = 4;
+        }
+        hraw[m] = xs[xi];
+        // the type is needed for same-level and NONLEAF neighbours; other cells re-read the tile's own
+        hct[m] = tp.ctype[(kind == 1 || kind == 4) ? ni : base + fi];
+        hkind[m] = kind;
+        hni[m] = int(ni - size_t(ok ? nt : r.t) * 512) | ((ok ? nt : r.t) << 9);
+    }
+    if (FRESH) {
+        // NONLEAF neighbours (coarse LEAF tiles next to finer ones only): the mean of the 8
+        // children of an INTERIOR cell, dependent loads
+#pragma unroll
+        for (int m = 0; m < HPT; ++m) {
+            if (hkind[m] != 4) continue;
+            const int nt = hni[m] >> 9, cl = hni[m] & 511;
+            const size_t ni = size_t(nt) * 512 + cl;
+            T val = T(0);
+            if (hct[m] == INTERIOR) {
+                const int n0 = cl >> 6, n1 = (cl >> 3) & 7, n2 = cl & 7;
+                const int oct = (n0 >> 2) | ((n1 >> 2) << 1) | ((n2 >> 2) << 2);
                 const int ctile = tp.child[size_t(nt) * 8 + oct];
                 if (ctile >= 0 && (tp.ttype[ctile] & LEAF)) {
                     T sum = 0;
@@ -221,24 +327,26 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
                         for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
                             for (int k2 = 0; k2 < 2; ++k2)
-                                sum += X[cb + cidx(2 * (nl[0] & 3) + ii, 2 * (nl[1] & 3) + jj, 2 * (nl[2] & 3) + k2)];
-                    hraw[m] = sum * T(0.125);
+                                sum += X[cb + cidx(2 * (n0 & 3) + ii, 2 * (n1 & 3) + jj, 2 * (n2 & 3) + k2)];
+                    val = sum * T(0.125);
                 } else {
-                    hraw[m] = X[ni];
+                    val = X[ni];
                 }
             }
-        } else {
-            // masked to INTERIOR after the barrier, so that no load waits here
-            hraw[m] = X[ni];
-            hct[m] = tp.ctype[ni];
+            hraw[m] = val;
         }
     }
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-        const int v = tid + c * NT;
+        const int v = own_cell<NT>(c);
         s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
+    }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) s.off[a][v] = o.c[c][a];
+    for (int a = 0; a < 3; ++a) {
+        float oc[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) oc[c] = o.c[c][a];
+        vstore<CPT>(&s.off[a][CPT * tid], oc);
     }
     __syncthreads();
 #pragma unroll
@@ -253,8 +361,9 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
         if (axis == 0) { fl[0] = axk; fl[1] = p; fl[2] = q; }
         else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
         else { fl[0] = p; fl[1] = q; fl[2] = axk; }
-        T hv = (hct[m] != 0xffu && hct[m] != INTERIOR) ? T(0) : hraw[m];
-        if (hghost[m]) {
+        // halo, phase 2: the loaded values
+        T hv = hkind[m] == 0 || hkind[m] == 3 || (hkind[m] == 1 && hct[m] != INTERIOR) ? T(0) : hraw[m];
+        if (hkind[m] == 2 || hkind[m] == 3) {
             int cl[3] = {fl[0], fl[1], fl[2]};
             cl[axis] -= sgn;
             T avg = 0;
@@ -266,18 +375,25 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, int t, float h, co
                     for (int dk = 0; dk < 2; ++dk) avg += s.x[SX(cl[0] ^ di, cl[1] ^ dj, cl[2] ^ dk)];
             avg /= 8;
             if constexpr (sizeof(T) == 4)
-                hv = float(double(s.x[SX(cl[0], cl[1], cl[2])]) + 0.5 * double(hraw[m] - avg));
+                hv = float(double(s.x[SX(cl[0], cl[1], cl[2])]) + 0.5 * double(hv - avg));
             else
-                hv = s.x[SX(cl[0], cl[1], cl[2])] + 0.5 * (hraw[m] - avg);
+                hv = s.x[SX(cl[0], cl[1], cl[2])] + 0.5 * (hv - avg);
         }
         s.x[SX(fl[0], fl[1], fl[2])] = hv;
-        if (sgn > 0) s.off[axis][512 + fi] = hcp[m];
+        if (sgn > 0) {
+            float hcp = h;   // no neighbour tile: h (the halo value there is 0)
+            if (hkind[m] != 0) {
+                if constexpr (Q) hcp = float(__builtin_bit_cast(_Float16, uint16_t(hcb[m]))) * h;
+                else hcp = __uint_as_float(hcb[m]);
+            }
+            s.off[axis][512 + fi] = hcp;
+        }
     }
     __syncthreads();
     if (dv) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-            const int v = tid + c * NT;
+            const int v = own_cell<NT>(c);
             o.d[c] = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
         }
     }
@@ -821,28 +937,32 @@ __global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int4* __
 // ------------------------------------------------- one tile per workgroup ----
 // y = A x on LEAF tiles, one tile per workgroup (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
 template <bool DOT, int NT>
-__global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+__global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
                                                  Coef C, float* __restrict__ Y, double* __restrict__ partial,
                                                  const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
-    const int t = list_tile(list, S);   // -1: no tile, -2: the solve has stopped
+    const TileRec r = rec_tile<true>(rec, S);
+    const int t = r.t;   // -1: no tile, -2: the solve has stopped
     if (t < 0) {
         if (DOT && t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
     Own<float, NT> o;
-    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    load_tile<true>(tp, r, rec_h(tp, r), X, X, C.c, s, o);
+    constexpr int CPT = 512 / NT;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    float xr[CPT], y[CPT];
+    if (DOT) vload<CPT>(X + v0, xr);
     double acc = 0;
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        const size_t idx = size_t(t) * 512 + v;
-        const float y = o.ct[c] == INTERIOR ? neg_lap(s, v, o.x[c], o.c[c], o.d[c]) : 0.0f;
-        Y[idx] = y;
-        if (DOT) acc += double(X[idx]) * double(y);
+    for (int c = 0; c < CPT; ++c) {
+        const int v = own_cell<NT>(c);
+        y[c] = o.ct[c] == INTERIOR ? neg_lap(s, v, o.x[c], o.c[c], o.d[c]) : 0.0f;
+        if (DOT) acc += double(xr[c]) * double(y[c]);
     }
+    vstore<CPT>(Y + v0, y);
     if (DOT) {
         double b = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
@@ -851,21 +971,21 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int32_t* __r
 
 // per tile: the number of cells whose stored diagonal is not bitwise the derived one
 template <int NT>
-__global__ __launch_bounds__(NT) void diag_check(TopoView tp, const int32_t* __restrict__ list, Coef C,
+__global__ __launch_bounds__(NT) void diag_check(TopoView tp, const int4* __restrict__ rec, Coef C,
                                                  double* __restrict__ partial)
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
-    const int t = list[blockIdx.x];
+    const TileRec r = rec_tile<true, false>(rec, nullptr);
     double bad = 0;
-    if (t >= 0) {
+    if (r.t >= 0) {
         Own<float, NT> o;
         tp.derive = 0;
-        const float h = tile_h(tp, t);
-        load_tile<true>(tp, t, h, C.c[0], C.c[0], C.c, s, o);
+        const float h = rec_h(tp, r);
+        load_tile<true>(tp, r, h, C.c[0], C.c[0], C.c, s, o);
 #pragma unroll
         for (int c = 0; c < 512 / NT; ++c) {
-            const int v = threadIdx.x + c * NT;
+            const int v = own_cell<NT>(c);
             const float d = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
             bad += __float_as_uint(d) != __float_as_uint(o.d[c]) ? 1.0 : 0.0;
         }
@@ -883,24 +1003,26 @@ __global__ __launch_bounds__(NT) void diag_check(TopoView tp, const int32_t* __r
 // plain first damped-Jacobi sweep x = w b / D.  NONLEAF tiles of the level take
 // the Jacobi sweep.
 template <int NT>
-__global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Y,
+__global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ Y,
                                                      float* __restrict__ X, const float* __restrict__ Gz, Coef C,
                                                      const float* __restrict__ B, float w, const double* __restrict__ S,
                                                      unsigned int* __restrict__ count)
 {
     __shared__ Lds<float> s;
     __shared__ double red[8];
-    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
+    const TileRec r = rec_tile<false>(rec, S);
+    const int t = r.t;   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
-    float b[512 / NT];
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false>(tp, t, tile_h(tp, t), Y, Gz, C.c, s, o);
+    constexpr int CPT = 512 / NT;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    float b[CPT];
+    vload<CPT>(B + v0, b);
+    load_tile<false>(tp, r, rec_h(tp, r), Y, Gz, C.c, s, o);
     double rr = 0, bb = 0;
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
+    for (int c = 0; c < CPT; ++c) {
+        const int v = own_cell<NT>(c);
         if (!(o.ct[c] & INTERIOR)) continue;
         const float r = b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
         rr += double(r) * double(r);
@@ -911,7 +1033,7 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
     const double bs = block_sum(bb, red);
     __shared__ int keep;
     if (threadIdx.x == 0) {
-        const bool leaf = tp.ttype[t] & LEAF;
+        const bool leaf = r.meta & LEAF;
         keep = leaf && rs < bs;
         // tiles judged and tiles that kept the net's correction
         if (count && leaf) {
@@ -920,13 +1042,14 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
         }
     }
     __syncthreads();
+    float yv[CPT], xo[CPT];
+    vload<CPT>(Y + v0, yv);
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        const size_t idx = size_t(t) * 512 + v;
+    for (int c = 0; c < CPT; ++c) {
         const float jac = 0.0f + w * ((o.ct[c] & INTERIOR) ? (b[c] - 0.0f) / o.d[c] : 0.0f);
-        X[idx] = keep ? ((o.ct[c] & INTERIOR) ? Y[idx] : 0.0f) : jac;
+        xo[c] = keep ? ((o.ct[c] & INTERIOR) ? yv[c] : 0.0f) : jac;
     }
+    vstore<CPT>(X + v0, xo);
 }
 
 // damped Jacobi sweep x_out = x_in + w (b - A x_in) / D on subtree-LEAF tiles of a level;
@@ -936,29 +1059,31 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int32_t*
 // elsewhere; X0 = x), with the A x the sweep computes anyway; the sweep then
 // uses the updated B.  Same values as fas_save followed by the plain sweep.
 template <int NT, bool DOT = false, bool Q = false, bool SAVE = false>
-__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ Xin,
+__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
                                              const float* __restrict__ B, float w, const double* __restrict__ S,
                                              double* __restrict__ partial, float* __restrict__ X0 = nullptr)
 {
     __shared__ Lds<float> s;
     __shared__ double red[DOT ? 8 : 1];
-    const int t = list_tile(list, S);   // -1: no tile, -2: the solve has stopped
+    const TileRec r = rec_tile<false>(rec, S);
+    const int t = r.t;   // -1: no tile, -2: the solve has stopped
     if (t < 0) {
         if (DOT && t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
     Own<float, NT> o;
-    float b[512 / NT];
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), Xin, G, C.c, s, o, C.q);
-    const bool save = SAVE && (tp.ttype[t] & NONLEAF);
-    const bool dot_tile = DOT && (tp.ttype[t] & LEAF);   // b.x is r.z on LEAF tiles only
+    constexpr int CPT = 512 / NT;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    float b[CPT], xo[CPT];
+    vload<CPT>(B + v0, b);
+    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), Xin, G, C.c, s, o, C.q);
+    const bool save = SAVE && (r.meta & NONLEAF);
+    const bool dot_tile = DOT && (r.meta & LEAF);   // b.x is r.z on LEAF tiles only
     double acc = 0;
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
+    for (int c = 0; c < CPT; ++c) {
+        const int v = own_cell<NT>(c);
         const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
         if (save) {
             const size_t idx = size_t(t) * 512 + v;
@@ -971,10 +1096,10 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
             const_cast<float*>(B)[idx] = b[c];   // B is the level's right-hand side, updated in place
         }
         const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
-        const float xo = o.x[c] + w * delta;
-        Xout[size_t(t) * 512 + v] = xo;
-        if (dot_tile) acc += double(b[c]) * double(xo);
+        xo[c] = o.x[c] + w * delta;
+        if (dot_tile) acc += double(b[c]) * double(xo[c]);
     }
+    vstore<CPT>(Xout + v0, xo);
     if (DOT) {
         const double sum = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = sum;
@@ -983,30 +1108,29 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int32_t* __restr
 
 // ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
 template <int NT, bool Q = false>
-__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X,
+__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X,
                                                         Coef C, float* __restrict__ B, float Rr,
                                                         const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
     __shared__ float res[512];
-    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
+    const TileRec r = rec_tile<false>(rec, S);
+    const int t = r.t;   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
-    float b[512 / NT];
+    constexpr int CPT = 512 / NT;
+    float b[CPT], rv[CPT];
+    vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * threadIdx.x, b);
+    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), X, X, C.c, s, o, C.q);
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) b[c] = B[size_t(t) * 512 + threadIdx.x + c * NT];
-    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), X, X, C.c, s, o, C.q);
-#pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        res[v] = b[c] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
-    }
+    for (int c = 0; c < CPT; ++c) rv[c] = b[c] - neg_lap(s, own_cell<NT>(c), o.x[c], o.c[c], o.d[c]);
+    vstore<CPT>(&res[CPT * threadIdx.x], rv);
     __syncthreads();
     const int v = threadIdx.x;
     if (v >= 64) return;
-    const int pt = tp.parent[t];
+    const int pt = r.parent;
     if (pt < 0) return;
-    const int oc = tp.toct[t];
+    const int oc = (r.meta >> 8) & 7;
     const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
     float rs = 0, xs = 0;
 #pragma unroll
@@ -1025,30 +1149,35 @@ __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int32
 
 // r64 = b64 - A x64 on LEAF tiles in FP64 (FRESH halo), r32 = float(r64), partial |r64|^2
 template <int NT>
-__global__ __launch_bounds__(NT) void residual64(TopoView tp, const int32_t* __restrict__ list, const double* __restrict__ X,
+__global__ __launch_bounds__(NT) void residual64(TopoView tp, const int4* __restrict__ rec, const double* __restrict__ X,
                                                  Coef C, const double* __restrict__ B, double* __restrict__ R64,
                                                  float* __restrict__ R32, double* __restrict__ partial)
 {
     __shared__ Lds<double> s;
     __shared__ double red[8];
-    const int t = list[blockIdx.x];
+    const TileRec r = rec_tile<true, false>(rec, nullptr);
+    const int t = r.t;
     if (t < 0) {
         if (threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
     Own<double, NT> o;
-    load_tile<true>(tp, t, tile_h(tp, t), X, X, C.c, s, o);
+    load_tile<true>(tp, r, rec_h(tp, r), X, X, C.c, s, o);
+    constexpr int CPT = 512 / NT;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    double bv[CPT], rv[CPT];
+    float r32[CPT];
+    vload<CPT>(B + v0, bv);
     double acc = 0;
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        const size_t idx = size_t(t) * 512 + v;
-        double r = 0;
-        if (o.ct[c] == INTERIOR) r = B[idx] - neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
-        R64[idx] = r;
-        R32[idx] = float(r);
-        acc += r * r;
+    for (int c = 0; c < CPT; ++c) {
+        rv[c] = 0;
+        if (o.ct[c] == INTERIOR) rv[c] = bv[c] - neg_lap(s, own_cell<NT>(c), o.x[c], o.c[c], o.d[c]);
+        r32[c] = float(rv[c]);
+        acc += rv[c] * rv[c];
     }
+    vstore<CPT>(R64 + v0, rv);
+    vstore<CPT>(R32 + v0, r32);
     double b = block_sum(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
@@ -1057,21 +1186,25 @@ __global__ __launch_bounds__(NT) void residual64(TopoView tp, const int32_t* __r
 // one colour of red-black SOR on subtree-LEAF tiles of a level, in place
 // (NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel, src/AMGSolver.cu:838-877)
 template <int NT, bool Q = false>
-__global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __restrict__ list, float* __restrict__ X, Coef C,
+__global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X, Coef C,
                                                const float* __restrict__ B, int color, double omega,
                                                const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
-    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
+    const TileRec r = rec_tile<false>(rec, S);
+    const int t = r.t;   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
-    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), X, X, C.c, s, o, C.q);
+    constexpr int CPT = 512 / NT;
+    float bv[CPT];
+    vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * threadIdx.x, bv);
+    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), X, X, C.c, s, o, C.q);
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
+    for (int c = 0; c < CPT; ++c) {
+        const int v = own_cell<NT>(c);
         if ((((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) continue;
         const size_t idx = size_t(t) * 512 + v;
-        const float b = B[idx];
+        const float b = bv[c];
         const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
         const float delta = (o.ct[c] & INTERIOR) ? (b - ax) / o.d[c] : 0.0f;
         X[idx] = float(double(o.x[c]) + double(delta) * omega);
@@ -1081,26 +1214,30 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int32_t* __res
 // ------------------------------------------------------------- transfers ----
 // FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
 template <int NT, bool Q = false>
-__global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int32_t* __restrict__ list, const float* __restrict__ X,
+__global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
                                                Coef C, float* __restrict__ B, float* __restrict__ X0,
                                                const double* __restrict__ S)
 {
     __shared__ Lds<float> s;
-    const int t = list_tile(list, S);   // < 0: no tile, or the solve has stopped
+    const TileRec r = rec_tile<false>(rec, S);
+    const int t = r.t;   // < 0: no tile, or the solve has stopped
     if (t < 0) return;
     Own<float, NT> o;
-    load_tile<false, float, NT, Q>(tp, t, tile_h(tp, t), X, X, C.c, s, o, C.q);
+    constexpr int CPT = 512 / NT;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    float bv[CPT];
+    vload<CPT>(B + v0, bv);
+    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), X, X, C.c, s, o, C.q);
 #pragma unroll
-    for (int c = 0; c < 512 / NT; ++c) {
-        const int v = threadIdx.x + c * NT;
-        const size_t idx = size_t(t) * 512 + v;
+    for (int c = 0; c < CPT; ++c) {
         if (o.ct[c] & INTERIOR) {
-            B[idx] = B[idx] + neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
-            X0[idx] = X[idx];
+            bv[c] = bv[c] + neg_lap(s, own_cell<NT>(c), o.x[c], o.c[c], o.d[c]);
+            X0[v0 + c] = o.x[c];   // x itself on INTERIOR cells
         } else {
-            B[idx] = 0;
+            bv[c] = 0;
         }
     }
+    vstore<CPT>(B + v0, bv);
 }
 
 // PropagateToChildren on GHOST tiles: ghost cell = parent cell (src/PoissonGrid.cu:377-405)
@@ -1672,22 +1809,26 @@ static int apply_launch(const DevGrid& d, const float* X, Coef C, float* Y, doub
     return g;
 }
 
+// the stop block a one-tile stencil launch reads: the solve's, or the grid's idle block
+static const double* run_flag(const DevGrid& d, const double* S) { return S ? S : d.idle; }
+
 int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
 {
     if (!nonempty(d.leafs)) return 0;
     if (!(persistent_mask() & PERSIST_APPLY)) {
         const TopoView tp = view(d);
+        const double* Sr = run_flag(d, S);
         if (partial) {
             switch (apply_threads()) {
-            case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
-            case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
-            default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, partial, S); break;
+            case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
+            case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
+            default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
             }
         } else {
             switch (apply_threads()) {
-            case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
-            case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
-            default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.d, X, C, Y, nullptr, S); break;
+            case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr); break;
+            case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr); break;
+            default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr); break;
             }
         }
         return d.leafs.n;
@@ -1703,7 +1844,7 @@ void launch_residual64(const DevGrid& d, const double* X, Coef C, const double* 
                        hipStream_t st)
 {
     if (!nonempty(d.leafs)) return;
-    CMG_NT_DISPATCH(k::residual64, d.leafs.n, st, view(d), d.leafs.d, X, C, B, R64, R32, partial);
+    CMG_NT_DISPATCH(k::residual64, d.leafs.n, st, view(d), d.leafs.rec, X, C, B, R64, R32, partial);
 }
 
 void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float* B, int color, double omega, const double* S,
@@ -1711,8 +1852,8 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    if (C.q[0]) CMG_NT_DISPATCH_Q(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
-    else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.d, X, C, B, color, omega, S);
+    if (C.q[0]) CMG_NT_DISPATCH_Q(k::gs_color, l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
+    else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
 }
 
 // one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE> with nt threads a tile
@@ -1721,9 +1862,9 @@ static void jacobi_at(int nt, const LaunchList& l, hipStream_t st, const TopoVie
                       const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0)
 {
     switch (nt) {
-    case 128: k::jacobi<128, DOT, Q, SAVE><<<l.n, 128, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial, X0); break;
-    case 512: k::jacobi<512, DOT, Q, SAVE><<<l.n, 512, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial, X0); break;
-    default: k::jacobi<256, DOT, Q, SAVE><<<l.n, 256, 0, st>>>(tp, l.d, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    case 128: k::jacobi<128, DOT, Q, SAVE><<<l.n, 128, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    case 512: k::jacobi<512, DOT, Q, SAVE><<<l.n, 512, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    default: k::jacobi<256, DOT, Q, SAVE><<<l.n, 256, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0); break;
     }
 }
 
@@ -1739,6 +1880,7 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         throw std::invalid_argument("jacobi: b.x partials, the FAS save and FP16 coefficients need the one-tile form");
     const TopoView tp = view(d);
     const int nt = q ? q_jacobi_threads() : stencil_threads();
+    S = one_tile ? run_flag(d, S) : S;
     if (partial) {
         // one partial per list entry
         if (q) jacobi_at<true, true, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
@@ -1772,10 +1914,11 @@ void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, flo
     if (C.q[0]) {
         if (persistent_mask() & PERSIST_RESTRICT)
             throw std::invalid_argument("residual_restrict: FP16 coefficients need the one-tile form");
-        CMG_NT_DISPATCH_Q(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
+        CMG_NT_DISPATCH_Q(k::residual_restrict, l.n, st, view(d), l.rec, X, C, B, Rr, run_flag(d, S));
         return;
     }
-    if (!(persistent_mask() & PERSIST_RESTRICT)) CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.d, X, C, B, Rr, S);
+    if (!(persistent_mask() & PERSIST_RESTRICT))
+        CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.rec, X, C, B, Rr, run_flag(d, S));
     else CMG_NT_PERSISTENT(k::restrict_p, l.n, st, view(d), l.d, l.n, X, C, B, Rr, S);
 }
 
@@ -1783,8 +1926,8 @@ void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float*
 {
     const LaunchList& l = d.nonleaf[level];
     if (!nonempty(l)) return;
-    if (C.q[0]) CMG_NT_DISPATCH_Q(k::fas_save, l.n, st, view(d), l.d, X, C, B, X0, S);
-    else CMG_NT_DISPATCH(k::fas_save, l.n, st, view(d), l.d, X, C, B, X0, S);
+    if (C.q[0]) CMG_NT_DISPATCH_Q(k::fas_save, l.n, st, view(d), l.rec, X, C, B, X0, run_flag(d, S));
+    else CMG_NT_DISPATCH(k::fas_save, l.n, st, view(d), l.rec, X, C, B, X0, run_flag(d, S));
 }
 
 void launch_coef_to_half(const DevGrid& d, Coef C, uint16_t* const q[4], unsigned int* bad, hipStream_t st)
@@ -1985,7 +2128,7 @@ int launch_power_sum(const DevGrid& d, const float* X, int order, bool volume_we
 int launch_diag_check(const DevGrid& d, Coef C, double* partial, hipStream_t st)
 {
     if (!nonempty(d.leafs)) return 0;
-    k::diag_check<256><<<d.leafs.n, 256, 0, st>>>(view(d), d.leafs.d, C, partial);
+    k::diag_check<256><<<d.leafs.n, 256, 0, st>>>(view(d), d.leafs.rec, C, partial);
     return d.leafs.n;
 }
 
@@ -2000,7 +2143,7 @@ void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
     // ghosts of the level are 0 at the start of a cycle: X is zero-filled there
-    k::learned_accept<256><<<l.n, 256, 0, st>>>(view(d), l.d, Y, X, X, C, B, w, S, count);
+    k::learned_accept<256><<<l.n, 256, 0, st>>>(view(d), l.rec, Y, X, X, C, B, w, run_flag(d, S), count);
 }
 
 }  // namespace cmg

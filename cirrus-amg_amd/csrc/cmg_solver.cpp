@@ -86,10 +86,19 @@ uint64_t spread3(uint32_t x)
     return v;
 }
 
+// per tile: the 5-int4 record of the one-tile stencil kernels (see LaunchList::rec)
+using TileRecords = std::vector<int4>;
+
 // Morton order (coordinates scaled to the finest level), then an XCD-aware
-// permutation: workgroup b runs on XCD b % 8, so XCD x gets the x-th contiguous
-// Morton chunk and neighbouring tiles share an L2.
-LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, bool with_meta = false)
+// permutation: workgroup b runs on XCD b % 8.  The Morton sequence is cut into blocks
+// of about 512 tiles (a multiple of 8 blocks) and block k goes to XCD k % 8, which
+// walks its blocks in order: inside a block, face neighbours share the XCD's L2 (halo
+// reuse), and at any moment the 8 XCDs stream 8 adjacent blocks, so each array is read
+// from a narrow window of memory (DRAM locality).  (A 1024^3-equivalent finest-level
+// sweep on MI355X: 512-tile blocks 68 us, one contiguous chunk per XCD 75.5 us, no
+// XCD grouping 90 us.)
+
+LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, const TileRecords& trec, bool with_meta = false)
 {
     LaunchList l;
     l.tiles = int(tiles.size());
@@ -107,19 +116,37 @@ LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, bool with
     std::vector<int32_t> order;
     const int n = int(key.size());
     constexpr int kXcd = 8;
+    constexpr int kBlock = 512;
     if (n < 64) {
         for (auto& k : key) order.push_back(k.second);
     } else {
-        const int chunk = (n + kXcd - 1) / kXcd;
-        order.assign(size_t(chunk) * kXcd, -1);
-        for (int b = 0; b < chunk * kXcd; ++b) {
-            const int pos = (b % kXcd) * chunk + b / kXcd;
+        // rounds of 8 blocks; the block size spreads n over whole rounds, so that padding
+        // (-1 entries) stays below 8 entries a block
+        const int rounds = (n + kXcd * kBlock - 1) / (kXcd * kBlock);
+        const int bs = (n + kXcd * rounds - 1) / (kXcd * rounds);
+        order.assign(size_t(bs) * kXcd * rounds, -1);
+        for (int b = 0; b < int(order.size()); ++b) {
+            const int x = b % kXcd, kk = b / kXcd;            // the kk-th workgroup of XCD x
+            const int pos = (x + kXcd * (kk / bs)) * bs + kk % bs;
             order[b] = pos < n ? key[pos].second : -1;
         }
     }
     l.n = int(order.size());
     l.d = dmalloc<int32_t>(order.size());
     HIPCHK(hipMemcpy(l.d, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    {
+        std::vector<int4> rec(order.size() * 5, make_int4(0, 0, 0, 0));
+        for (size_t i = 0; i < order.size(); ++i) {
+            const int t = order[i];
+            if (t < 0) {
+                rec[i * 5] = make_int4(-1, 0, 0, -1);
+                continue;
+            }
+            for (int k = 0; k < 5; ++k) rec[i * 5 + k] = trec[size_t(t) * 5 + k];
+        }
+        l.rec = dmalloc<int4>(rec.size());
+        HIPCHK(hipMemcpy(l.rec, rec.data(), rec.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
     if (with_meta) {
         // a tile's parent, octant and type next to it, so that one load gives a
         // prolongation workgroup all it needs to address the parent
@@ -141,6 +168,7 @@ void free_list(LaunchList& l)
 {
     if (l.d) (void)hipFree(l.d);
     if (l.meta) (void)hipFree(l.meta);
+    if (l.rec) (void)hipFree(l.rec);
     l = LaunchList{};
 }
 
@@ -169,7 +197,7 @@ void dev_free(DevGrid& d)
 {
     for (void* p : {(void*)d.ttype, (void*)d.toct, (void*)d.tlevel, (void*)d.nbr, (void*)d.parent, (void*)d.child,
                     (void*)d.halo, (void*)d.ghalo, (void*)d.tcoord, (void*)d.acount,
-                    (void*)d.ctype, (void*)d.partial, (void*)d.scal, (void*)d.roi, (void*)d.roi_locked,
+                    (void*)d.ctype, (void*)d.partial, (void*)d.scal, (void*)d.idle, (void*)d.roi, (void*)d.roi_locked,
                     (void*)d.roi_min, (void*)d.roi_max})
         if (p) (void)hipFree(p);
     for (auto& c : d.ch)
@@ -200,6 +228,7 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.h0 = topo.h0();
     d.level_start = topo.level_start;
     std::vector<uint8_t> oct(N);
+    TileRecords trec;
     for (int t = 0; t < N; ++t)
         oct[t] = uint8_t((topo.t_coord[t][0] & 1) | ((topo.t_coord[t][1] & 1) << 1) | ((topo.t_coord[t][2] & 1) << 2));
     auto up = [&](auto*& dst, const auto& src) {
@@ -240,6 +269,16 @@ void dev_upload(DevGrid& d, const Topology& topo)
         }
         up(d.halo, halo);
         up(d.ghalo, ghalo);
+        trec.assign(size_t(N) * 5, make_int4(0, 0, 0, 0));
+        for (int t = 0; t < N; ++t) {
+            const int32_t* hh = &halo[size_t(t) * 8];
+            const int32_t* gg = &ghalo[size_t(t) * 8];
+            trec[size_t(t) * 5 + 0] = make_int4(t, topo.t_level[t], int(topo.t_type[t]) | (int(oct[t]) << 8), topo.parent[t]);
+            trec[size_t(t) * 5 + 1] = make_int4(hh[0], hh[1], hh[2], hh[3]);
+            trec[size_t(t) * 5 + 2] = make_int4(hh[4], hh[5], hh[6], hh[7]);
+            trec[size_t(t) * 5 + 3] = make_int4(gg[0], gg[1], gg[2], gg[3]);
+            trec[size_t(t) * 5 + 4] = make_int4(gg[4], gg[5], gg[6], gg[7]);
+        }
     }
     up(d.parent, topo.parent);
     up(d.child, topo.child);
@@ -260,17 +299,17 @@ void dev_upload(DevGrid& d, const Topology& topo)
         if ((ty & GHOST) || ((ty & LEAF) && l < topo.max_level())) cyclezero.push_back(t);
         pr[l].push_back(t);
     }
-    d.leafs = make_list(topo, leafs);
-    d.all = make_list(topo, all);
-    d.ghosts = make_list(topo, ghosts);
-    d.leafghost = make_list(topo, leafghost);
-    d.cyclezero = make_list(topo, cyclezero);
+    d.leafs = make_list(topo, leafs, trec);
+    d.all = make_list(topo, all, trec);
+    d.ghosts = make_list(topo, ghosts, trec);
+    d.leafghost = make_list(topo, leafghost, trec);
+    d.cyclezero = make_list(topo, cyclezero, trec);
     d.max_list = std::max(d.leafs.n, d.all.n);
     for (int l = 0; l < topo.num_levels(); ++l) {
-        d.subtree[l] = make_list(topo, sub[l]);
-        d.nonleaf[l] = make_list(topo, nl[l]);
-        d.prolong[l] = make_list(topo, pr[l], true);
-        d.leafat[l] = make_list(topo, la[l]);
+        d.subtree[l] = make_list(topo, sub[l], trec);
+        d.nonleaf[l] = make_list(topo, nl[l], trec);
+        d.prolong[l] = make_list(topo, pr[l], trec, true);
+        d.leafat[l] = make_list(topo, la[l], trec);
     }
     {
         int dev = 0, cus = 0;
@@ -281,6 +320,8 @@ void dev_upload(DevGrid& d, const Topology& topo)
     d.partial = dmalloc<double>(size_t(d.max_list) + 1);
     d.scal = dmalloc<double>(S_COUNT);
     HIPCHK(hipMemset(d.scal, 0, S_COUNT * sizeof(double)));
+    d.idle = dmalloc<double>(S_COUNT);
+    HIPCHK(hipMemset(d.idle, 0, S_COUNT * sizeof(double)));
     HIPCHK(hipDeviceSynchronize());
 }
 
