@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--profile-kernel", type=int, default=None, choices=sorted(KERNELS),
                     help="only time one kernel (for rocprofv3 counter runs) and exit")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--no-legs", action="store_true",
+                    help="time the headline solve and its dominant kernel only (no secondary legs)")
     ap.add_argument("--learned", type=int, default=1, help="also time learned-FCG when trained weights exist")
     ap.add_argument("--cycle-coef", choices=("fp16", "fp32"), default="fp16",
                     help="coefficients the preconditioning cycle reads: FP16 copies of c / h, or the FP32 channels "
@@ -307,6 +309,9 @@ def main():
     result["hbm_copy_ceiling"] = ceil
     result["roofline"]["frac_of_copy_ceiling"] = k_gbs / ceil["GBps"]
     result["solve_frac_of_copy_ceiling"] = solve_gbs / ceil["GBps"]
+    if args.no_legs:
+        print(json.dumps(result))
+        return
     if not args.no_fp64:
         # mixed-precision form: FP32 MGPCG inner solves inside FP64 iterative
         # refinement to 1e-10; its solution is the FP64 solution of the same

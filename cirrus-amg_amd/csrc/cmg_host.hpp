@@ -20,6 +20,9 @@ namespace cmg {
 constexpr int kMaxLevels = 12;
 constexpr int kCells = 512;
 constexpr int kChannels = 15;
+// int4 words of a stencil workgroup's tile record: (tile, level, type | octant << 8, parent),
+// halo record (2), ghost-parent record (2), children (2)
+constexpr int kRecWords = 7;
 
 enum : uint8_t { LEAF = 1, NONLEAF = 2, GHOST = 4 };
 enum : uint8_t { DIRICHLET = 1, INTERIOR = 2, NEUMANN = 4 };

@@ -29,9 +29,11 @@ struct LaunchList {
     int n = 0;        // workgroups to launch
     int tiles = 0;    // real tiles in the list
     int4* meta = nullptr;   // prolongation lists: per entry (tile, parent, octant | type << 8, 0)
-    // per entry 5 int4: (tile, level, type | octant << 8, parent), the tile's 32-byte halo record
-    // and its ghost-parent record (-1 tile on padding entries); read by the one-tile stencil kernels
+    // per entry 7 int4: (tile, level, type | octant << 8, parent), the tile's 32-byte halo record,
+    // its ghost-parent record and its children (-1 tile on padding entries); read by the one-tile
+    // stencil kernels
     int4* rec = nullptr;
+    std::vector<int32_t> order;   // the entries on the host (-1: padding)
 };
 
 struct DevGrid {
@@ -61,6 +63,12 @@ struct DevGrid {
     LaunchList nonleaf[kMaxLevels];
     LaunchList leafat[kMaxLevels];      // LEAF tiles at the level
     LaunchList prolong[kMaxLevels];     // LEAF | NONLEAF | GHOST at the level
+    // every tile of the level has a LEAF or NONLEAF parent, so the FAS prolongation to the
+    // level can be done by the parents' last post-sweep (each parent writes its children)
+    bool push_ok[kMaxLevels] = {};
+    bool root_alone = false;   // level 0 is one LEAF or NONLEAF tile without neighbour tiles
+    bool bottom_ok = false;    // root_alone, and level 1 is its 8 children, LEAF or NONLEAF
+    int bottom_entry[8] = {};  // the subtree[1] list entry of the root's child in each octant
     LaunchList ghosts;                  // all GHOST tiles
     LaunchList leafghost;               // all LEAF and GHOST tiles
     LaunchList cyclezero;               // GHOST tiles and LEAF tiles below the finest level

@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <stdexcept>
+#include <type_traits>
 
 #include "cmg_device.hpp"
 #include "cmg_kernels.hpp"
@@ -31,18 +32,26 @@ namespace cmg {
 namespace k {
 
 
-template <typename T>
+// Q: the kernel reads the FP16 coefficient copies (c / h); the coefficient planes are then
+// staged as those FP16 bits (7.5 KB a tile instead of 10.9, so more tiles fit on a CU)
+template <typename T, bool Q = false>
 struct Lds {
     T x[1000];
-    float off[3][576];   // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
+    // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
+    std::conditional_t<Q, uint16_t, float> off[3][576];
     uint8_t empty[6];    // face f = 2 axis + (side > 0) has no neighbour tile
 };
+
+// a staged coefficient as FP32: FP32 as is, FP16 bits of c / h times h (exact: h is a
+// power of two times h0)
+__device__ __forceinline__ float lds_coef(float v, float) { return v; }
+__device__ __forceinline__ float lds_coef(uint16_t v, float h) { return float(__builtin_bit_cast(_Float16, v)) * h; }
 
 // diagonal of a LEAF INTERIOR cell from its face coefficients, in the order the
 // assembly sums them (CoarsenTypesAndAMGCoeffs, src/AMGSolver.cu:129-160):
 // d = 0 - c-(x) - c+(x) - c-(y) - c+(y) - c-(z) - c+(z), c+ = -h without neighbour
 template <typename T>
-__device__ __forceinline__ float derived_diag(const Lds<T>& s, int v, const float c[3], float h)
+__device__ __forceinline__ float derived_diag(const Lds<T, false>& s, int v, const float c[3], float h)
 {
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
     const float cp0 = i < 7 ? s.off[0][v + 64] : (s.empty[1] ? -h : s.off[0][512 + (j * 8 + k)]);
@@ -87,9 +96,14 @@ __device__ __forceinline__ const P* uniform_ptr(const P* p)
 // the cells of a thread in the one-tile kernels: CPT = 512/NT consecutive cells, so each
 // per-cell array of a tile is read (and written) with one vector access per thread
 template <int NT>
+__device__ __forceinline__ int own_cell(int c, int tid)
+{
+    return (512 / NT) * tid + c;
+}
+template <int NT>
 __device__ __forceinline__ int own_cell(int c)
 {
-    return (512 / NT) * int(threadIdx.x) + c;
+    return own_cell<NT>(c, int(threadIdx.x));
 }
 
 template <int N, typename E>
@@ -143,24 +157,36 @@ __device__ __forceinline__ int pick6(int4 a, int4 b, int s)
     return r;
 }
 
-// A one-tile stencil workgroup's record (launch-list order, 5 x 16 bytes, read with
+// element s (0..7, thread-varying) of (a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w) by selects
+__device__ __forceinline__ int pick8(int4 a, int4 b, int s)
+{
+    int r = pick6(a, b, s);
+    r = s == 6 ? b.z : r;
+    r = s == 7 ? b.w : r;
+    return r;
+}
+
+// A one-tile stencil workgroup's record (launch-list order, 7 x 16 bytes, read with
 // scalar loads right after the launch): the tile, its level, type | octant << 8 and
 // parent; its halo record (face neighbours, their tile types packed in bytes, own
 // type in byte 7.3); for the operator's fresh ghost values, the parents of GHOST
-// face neighbours and their octants.  One round trip gives the workgroup every
-// address it needs: the tile's own loads and its halo loads follow at once.
+// face neighbours and their octants; its children by octant (for a sweep that
+// prolongs into them).  One round trip gives the workgroup every address it needs:
+// the tile's own loads and its halo loads follow at once.
 struct TileRec {
     int t, level, meta, parent;
     int4 ha, hb, ga, gb;
+    int4 ca, cb;   // children by octant (CHILD records only)
+    int tid;       // the thread's index among the threads working on the tile
 };
 
 // t = -1: padding entry, -2: the solve has stopped.  S is never null here (launches
 // outside a solve pass the grid's idle block, whose flag is never set), so the stop
 // flag is read with no branch and its load overlaps the record's.
-template <bool FRESH, bool STOP = true>
+template <bool FRESH, bool STOP = true, bool CHILD = false>
 __device__ __forceinline__ TileRec rec_tile(const int4* __restrict__ rec, const double* __restrict__ S)
 {
-    const int4* r = rec + 5 * size_t(blockIdx.x);
+    const int4* r = rec + kRecWords * size_t(blockIdx.x);
     const int4 r0 = r[0];
     TileRec x;
     x.ha = r[1];
@@ -172,7 +198,14 @@ __device__ __forceinline__ TileRec rec_tile(const int4* __restrict__ rec, const 
         x.ga = make_int4(-1, -1, -1, -1);
         x.gb = make_int4(-1, -1, 0, 0);
     }
+    if (CHILD) {
+        x.ca = r[5];
+        x.cb = r[6];
+    } else {
+        x.ca = x.cb = make_int4(-1, -1, -1, -1);
+    }
     x.t = (STOP && S[S_DONE] != 0.0) ? -2 : r0.x;
+    x.tid = threadIdx.x;
     x.level = r0.y;
     x.meta = r0.z;
     x.parent = r0.w;
@@ -189,14 +222,20 @@ __device__ __forceinline__ float rec_h(const TopoView& tp, const TileRec& r) { r
 // Q: the face coefficients and diagonals come from the FP16 copies Qc (c / h).
 // All global loads (own cells and the 384 halo cells) are issued before the
 // first barrier; only the ghost extrapolation waits for the tile's own values.
-template <bool FRESH, typename T, int NT, bool Q = false>
+// `extra` is called right after the halo loads are issued and before any loaded
+// value is used: loads a kernel needs later (not for the stencil) issued there
+// are in flight together with the tile's.
+struct NoExtraLoads {
+    __device__ void operator()() const {}
+};
+template <bool FRESH, typename T, int NT, bool Q = false, typename Extra = NoExtraLoads>
 __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, float h, const T* __restrict__ X,
-                                          const T* __restrict__ G, const float* const* C, Lds<T>& s, Own<T, NT>& o,
-                                          const uint16_t* const* Qc = nullptr)
+                                          const T* __restrict__ G, const float* const* C, Lds<T, Q>& s, Own<T, NT>& o,
+                                          const uint16_t* const* Qc = nullptr, const Extra& extra = Extra{})
 {
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
-    const int tid = threadIdx.x;
+    const int tid = r.tid;
     const size_t base = size_t(r.t) * 512;
     // the tile's halo record: face neighbours, their tile types, the tile's own
     // type and (FRESH) the parents and octants of GHOST neighbours
@@ -305,6 +344,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         hkind[m] = kind;
         hni[m] = int(ni - size_t(ok ? nt : r.t) * 512) | ((ok ? nt : r.t) << 9);
     }
+    extra();
     if (FRESH) {
         // NONLEAF neighbours (coarse LEAF tiles next to finer ones only): the mean of the 8
         // children of an INTERIOR cell, dependent loads
@@ -338,15 +378,24 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     }
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-        const int v = own_cell<NT>(c);
+        const int v = own_cell<NT>(c, tid);
         s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float oc[CPT];
+        if constexpr (Q) {
+            // back to the FP16 bits: c / h is exact, and so is its conversion
+            const float ih = 1.0f / h;
+            uint16_t oc[CPT];
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) oc[c] = o.c[c][a];
-        vstore<CPT>(&s.off[a][CPT * tid], oc);
+            for (int c = 0; c < CPT; ++c) oc[c] = __builtin_bit_cast(uint16_t, _Float16(o.c[c][a] * ih));
+            vstore<CPT>(&s.off[a][CPT * tid], oc);
+        } else {
+            float oc[CPT];
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) oc[c] = o.c[c][a];
+            vstore<CPT>(&s.off[a][CPT * tid], oc);
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -381,33 +430,33 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         }
         s.x[SX(fl[0], fl[1], fl[2])] = hv;
         if (sgn > 0) {
-            float hcp = h;   // no neighbour tile: h (the halo value there is 0)
-            if (hkind[m] != 0) {
-                if constexpr (Q) hcp = float(__builtin_bit_cast(_Float16, uint16_t(hcb[m]))) * h;
-                else hcp = __uint_as_float(hcb[m]);
-            }
-            s.off[axis][512 + fi] = hcp;
+            // no neighbour tile: h (the halo value there is 0); FP16 1.0 times h in the Q form
+            if constexpr (Q) s.off[axis][512 + fi] = hkind[m] != 0 ? uint16_t(hcb[m]) : uint16_t(0x3C00);
+            else s.off[axis][512 + fi] = hkind[m] != 0 ? __uint_as_float(hcb[m]) : h;
         }
     }
     __syncthreads();
-    if (dv) {
+    if constexpr (!Q) {
+        if (dv) {
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) {
-            const int v = own_cell<NT>(c);
-            o.d[c] = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
+            for (int c = 0; c < CPT; ++c) {
+                const int v = own_cell<NT>(c, tid);
+                o.d[c] = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
+            }
         }
     }
 }
 
 // AMGLaplacianTileData::negativeLap (src/AMGSolver.cu:345-382) for cell v
-template <typename T>
-__device__ __forceinline__ T neg_lap(const Lds<T>& s, int v, T x0, const float c[3], float d)
+// (h: the tile spacing, needed only to read FP16 staged coefficients)
+template <typename T, bool Q>
+__device__ __forceinline__ T neg_lap(const Lds<T, Q>& s, int v, T x0, const float c[3], float d, float h = 0.0f)
 {
     if (d == 0) return T(0);
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
-    const float cp0 = i < 7 ? s.off[0][v + 64] : s.off[0][512 + (j * 8 + k)];
-    const float cp1 = j < 7 ? s.off[1][v + 8] : s.off[1][512 + (i * 8 + k)];
-    const float cp2 = k < 7 ? s.off[2][v + 1] : s.off[2][512 + (i * 8 + j)];
+    const float cp0 = lds_coef(i < 7 ? s.off[0][v + 64] : s.off[0][512 + (j * 8 + k)], h);
+    const float cp1 = lds_coef(j < 7 ? s.off[1][v + 8] : s.off[1][512 + (i * 8 + k)], h);
+    const float cp2 = lds_coef(k < 7 ? s.off[2][v + 1] : s.off[2][512 + (i * 8 + j)], h);
     T r = x0 * T(d);
     r = fma(T(c[0]), s.x[SX(i - 1, j, k)], r);
     r = fma(T(cp0), s.x[SX(i + 1, j, k)], r);
@@ -842,10 +891,19 @@ __global__ __launch_bounds__(256) void copy_flat(const f32x4v* __restrict__ A, f
 // q[a] = FP16 bits of c[a] / h on every tile of the list, 128 threads x 4 cells a
 // tile (float4 in, 4 halves out per channel); *bad counts workgroups that produced
 // a non-finite value
+// four coefficients as FP16 bits of c / h; `nf` is set by a value that does not fit: a
+// non-finite result, or (DIAG) a nonzero diagonal that falls below FP16's smallest normal
+// (2^-14), which would divide the Jacobi and Gauss-Seidel updates by 0 or by a few bits
+template <bool DIAG = false>
 __device__ __forceinline__ uint2 half4_of(float4 v, float h, bool& nf)
 {
     const _Float16 a = _Float16(v.x / h), b = _Float16(v.y / h), c = _Float16(v.z / h), d = _Float16(v.w / h);
     nf = nf || !isfinite(float(a)) || !isfinite(float(b)) || !isfinite(float(c)) || !isfinite(float(d));
+    if (DIAG) {
+        constexpr float kMinNormal = 6.103515625e-05f;
+        nf = nf || (v.x != 0.0f && fabsf(float(a)) < kMinNormal) || (v.y != 0.0f && fabsf(float(b)) < kMinNormal) ||
+             (v.z != 0.0f && fabsf(float(c)) < kMinNormal) || (v.w != 0.0f && fabsf(float(d)) < kMinNormal);
+    }
     const uint32_t lo = uint32_t(__builtin_bit_cast(uint16_t, a)) | (uint32_t(__builtin_bit_cast(uint16_t, b)) << 16);
     const uint32_t hi = uint32_t(__builtin_bit_cast(uint16_t, c)) | (uint32_t(__builtin_bit_cast(uint16_t, d)) << 16);
     return make_uint2(lo, hi);
@@ -866,7 +924,7 @@ __global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* 
         reinterpret_cast<uint2*>(q0)[q] = half4_of(c0, h, nf);
         reinterpret_cast<uint2*>(q1)[q] = half4_of(c1, h, nf);
         reinterpret_cast<uint2*>(q2)[q] = half4_of(c2, h, nf);
-        reinterpret_cast<uint2*>(q3)[q] = half4_of(c3, h, nf);
+        reinterpret_cast<uint2*>(q3)[q] = half4_of<true>(c3, h, nf);
     }
     if (__syncthreads_or(nf) && threadIdx.x == 0) atomicAdd(bad, 1u);
 }
@@ -1058,48 +1116,139 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int4* __
 // that level's FAS save on its NONLEAF tiles (B += A x on INTERIOR cells, 0
 // elsewhere; X0 = x), with the A x the sweep computes anyway; the sweep then
 // uses the updated B.  Same values as fas_save followed by the plain sweep.
-template <int NT, bool DOT = false, bool Q = false, bool SAVE = false>
-__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ Xin,
-                                             float* __restrict__ Xout, const float* __restrict__ G, Coef C,
-                                             const float* __restrict__ B, float w, const double* __restrict__ S,
-                                             double* __restrict__ partial, float* __restrict__ X0 = nullptr)
+// PUSH: the level's last post-sweep also does the FAS prolongation to the next finer
+// level (FASProlongateAndUpdateAMG128Kernel, src/AMGSolver.cu:1216-1255) for the
+// children of its tile, from the x it has just computed: a child's INTERIOR cells get
+// x += P (x_parent - X0_parent) (a GHOST child: x = x_parent), its other cells 0.  Every
+// tile has one parent, so each child is written by exactly one workgroup; the sweep
+// reads no cell of the finer level.  Same values as the sweep followed by fas_prolong_v.
+// EARLY: the children's loads go out with the tile's (else after the sweep, with fewer
+// registers held across it)
+template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false, bool EARLY = true>
+__device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
+                                              const float* __restrict__ Xin, float* __restrict__ Xout,
+                                              const float* __restrict__ G, const Coef& C, const float* __restrict__ B,
+                                              float w, float* __restrict__ X0, float P, bool act)
 {
-    __shared__ Lds<float> s;
-    __shared__ double red[DOT ? 8 : 1];
-    const TileRec r = rec_tile<false>(rec, S);
-    const int t = r.t;   // -1: no tile, -2: the solve has stopped
-    if (t < 0) {
-        if (DOT && t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
-        return;
-    }
+    static_assert(!(SAVE && PUSH), "the FAS save rides on a first sweep, the prolongation on a last one");
+    const int t = r.t, tid = r.tid;
     Own<float, NT> o;
     constexpr int CPT = 512 / NT;
-    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * tid;
     float b[CPT], xo[CPT];
     vload<CPT>(B + v0, b);
-    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), Xin, G, C.c, s, o, C.q);
+    const float h = rec_h(tp, r);
+    // PUSH: a thread's cells run along k; G4 of them lie in one child octant and cover
+    // 2 x 2 rows of 2 G4 consecutive cells of that child.  The children's x and types
+    // are loaded together with the tile's own loads (a missing child: the tile's own
+    // cells, unused).
+    constexpr int G4 = CPT < 4 ? CPT : 4;
+    constexpr int NG = PUSH ? CPT / G4 : 1;
+    constexpr int PR = PUSH ? 2 * G4 : 1;
+    float pxf[NG][4][PR];
+    uint8_t pcf[NG][4][PR];
+    float px0[PUSH ? CPT : 1];
+    int pch[NG];
+    uint8_t pty[NG];
+    auto push_loads = [&]() {
+        if constexpr (PUSH && EARLY) {
+            vload<CPT>(X0 + v0, px0);   // the FAS-saved x on NONLEAF tiles (unused for GHOST children)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int v = own_cell<NT>(g * G4, tid);
+                const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
+                const int ch = pick8(r.ca, r.cb, (i >> 2) | ((j >> 2) << 1) | ((k >> 2) << 2));
+                const int cs = ch >= 0 ? ch : t;
+                pch[g] = ch;
+                pty[g] = tp.ttype[cs];
+#pragma unroll
+                for (int ab = 0; ab < 4; ++ab) {
+                    const size_t fb = size_t(cs) * 512 + cidx(2 * (i & 3) + (ab >> 1), 2 * (j & 3) + (ab & 1), 2 * (k & 3));
+                    vload<PR>(Xout + fb, pxf[g][ab]);
+                    vload<PR>(tp.ctype + fb, pcf[g][ab]);
+                }
+            }
+        }
+    };
+    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads);
     const bool save = SAVE && (r.meta & NONLEAF);
     const bool dot_tile = DOT && (r.meta & LEAF);   // b.x is r.z on LEAF tiles only
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-        const int v = own_cell<NT>(c);
-        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        const int v = own_cell<NT>(c, tid);
+        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c], h);
         if (save) {
             const size_t idx = size_t(t) * 512 + v;
             if (o.ct[c] & INTERIOR) {
                 b[c] = b[c] + ax;
-                X0[idx] = o.x[c];
+                if (act) X0[idx] = o.x[c];
             } else {
                 b[c] = 0.0f;
             }
-            const_cast<float*>(B)[idx] = b[c];   // B is the level's right-hand side, updated in place
+            if (act) const_cast<float*>(B)[idx] = b[c];   // B is the level's right-hand side, updated in place
         }
         const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
         xo[c] = o.x[c] + w * delta;
         if (dot_tile) acc += double(b[c]) * double(xo[c]);
     }
-    vstore<CPT>(Xout + v0, xo);
+    if (act) vstore<CPT>(Xout + v0, xo);
+    if constexpr (PUSH) {
+        if constexpr (!EARLY) {
+            vload<CPT>(X0 + v0, px0);
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int v = own_cell<NT>(g * G4, tid);
+                const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
+                const int ch = pick8(r.ca, r.cb, (i >> 2) | ((j >> 2) << 1) | ((k >> 2) << 2));
+                const int cs = ch >= 0 ? ch : t;
+                pch[g] = ch;
+                pty[g] = tp.ttype[cs];
+#pragma unroll
+                for (int ab = 0; ab < 4; ++ab) {
+                    const size_t fb = size_t(cs) * 512 + cidx(2 * (i & 3) + (ab >> 1), 2 * (j & 3) + (ab & 1), 2 * (k & 3));
+                    vload<PR>(Xout + fb, pxf[g][ab]);
+                    vload<PR>(tp.ctype + fb, pcf[g][ab]);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (pch[g] < 0 || !act) continue;
+            const bool ghost = pty[g] & GHOST;
+            const int v = own_cell<NT>(g * G4, tid);
+            const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
+#pragma unroll
+            for (int ab = 0; ab < 4; ++ab) {
+#pragma unroll
+                for (int e = 0; e < PR; ++e) {
+                    const bool in = pcf[g][ab][e] & INTERIOR;
+                    const float c1 = xo[g * G4 + (e >> 1)], c0 = px0[g * G4 + (e >> 1)];
+                    pxf[g][ab][e] = ghost ? (in ? c1 : 0.0f) : (in ? pxf[g][ab][e] + P * (c1 - c0) : 0.0f);
+                }
+                const size_t fb = size_t(pch[g]) * 512 + cidx(2 * (i & 3) + (ab >> 1), 2 * (j & 3) + (ab & 1), 2 * (k & 3));
+                vstore<PR>(Xout + fb, pxf[g][ab]);
+            }
+        }
+    }
+    return acc;
+}
+
+template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false>
+__global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ Xin,
+                                             float* __restrict__ Xout, const float* __restrict__ G, Coef C,
+                                             const float* __restrict__ B, float w, const double* __restrict__ S,
+                                             double* __restrict__ partial, float* __restrict__ X0 = nullptr,
+                                             float P = 0.0f)
+{
+    __shared__ Lds<float, Q> s;
+    __shared__ double red[DOT ? 8 : 1];
+    const TileRec r = rec_tile<false, true, PUSH>(rec, S);
+    if (r.t < 0) {   // -1: no tile, -2: the solve has stopped
+        if (DOT && r.t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
+        return;
+    }
+    const double acc = jacobi_tile<NT, DOT, Q, SAVE, PUSH>(tp, r, s, Xin, Xout, G, C, B, w, X0, P, true);
     if (DOT) {
         const double sum = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = sum;
@@ -1108,28 +1257,24 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict
 
 // ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
 template <int NT, bool Q = false>
-__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X,
-                                                        Coef C, float* __restrict__ B, float Rr,
-                                                        const double* __restrict__ S)
+__device__ __forceinline__ void residual_restrict_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
+                                                       float* __restrict__ res, float* __restrict__ X, const Coef& C,
+                                                       float* __restrict__ B, float Rr, bool act)
 {
-    __shared__ Lds<float> s;
-    __shared__ float res[512];
-    const TileRec r = rec_tile<false>(rec, S);
-    const int t = r.t;   // < 0: no tile, or the solve has stopped
-    if (t < 0) return;
+    const int t = r.t, tid = r.tid;
     Own<float, NT> o;
     constexpr int CPT = 512 / NT;
     float b[CPT], rv[CPT];
-    vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * threadIdx.x, b);
-    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), X, X, C.c, s, o, C.q);
+    vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * tid, b);
+    const float h = rec_h(tp, r);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
 #pragma unroll
-    for (int c = 0; c < CPT; ++c) rv[c] = b[c] - neg_lap(s, own_cell<NT>(c), o.x[c], o.c[c], o.d[c]);
-    vstore<CPT>(&res[CPT * threadIdx.x], rv);
+    for (int c = 0; c < CPT; ++c) rv[c] = b[c] - neg_lap(s, own_cell<NT>(c, tid), o.x[c], o.c[c], o.d[c], h);
+    vstore<CPT>(&res[CPT * tid], rv);
     __syncthreads();
-    const int v = threadIdx.x;
-    if (v >= 64) return;
+    const int v = tid;
     const int pt = r.parent;
-    if (pt < 0) return;
+    if (v >= 64 || pt < 0 || !act) return;
     const int oc = (r.meta >> 8) & 7;
     const int fl0 = v / 16 * 2, fl1 = (v / 4) % 4 * 2, fl2 = v % 4 * 2;
     float rs = 0, xs = 0;
@@ -1145,6 +1290,18 @@ __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int4*
     const size_t pidx = size_t(pt) * 512 + cidx(4 * (oc & 1) + (fl0 >> 1), 4 * ((oc >> 1) & 1) + (fl1 >> 1), 4 * ((oc >> 2) & 1) + (fl2 >> 1));
     B[pidx] = rs * Rr;
     X[pidx] = xs / 8;
+}
+
+template <int NT, bool Q = false>
+__global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X,
+                                                        Coef C, float* __restrict__ B, float Rr,
+                                                        const double* __restrict__ S)
+{
+    __shared__ Lds<float, Q> s;
+    __shared__ float res[512];
+    const TileRec r = rec_tile<false>(rec, S);
+    if (r.t < 0) return;   // no tile, or the solve has stopped
+    residual_restrict_tile<NT, Q>(tp, r, s, res, X, C, B, Rr, true);
 }
 
 // r64 = b64 - A x64 on LEAF tiles in FP64 (FRESH halo), r32 = float(r64), partial |r64|^2
@@ -1186,58 +1343,465 @@ __global__ __launch_bounds__(NT) void residual64(TopoView tp, const int4* __rest
 // one colour of red-black SOR on subtree-LEAF tiles of a level, in place
 // (NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel, src/AMGSolver.cu:838-877)
 template <int NT, bool Q = false>
-__global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X, Coef C,
-                                               const float* __restrict__ B, int color, double omega,
-                                               const double* __restrict__ S)
+__device__ __forceinline__ void gs_color_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s, float* __restrict__ X,
+                                              const Coef& C, const float* __restrict__ B, int color, double omega, bool act)
 {
-    __shared__ Lds<float> s;
-    const TileRec r = rec_tile<false>(rec, S);
-    const int t = r.t;   // < 0: no tile, or the solve has stopped
-    if (t < 0) return;
+    const int t = r.t, tid = r.tid;
     Own<float, NT> o;
     constexpr int CPT = 512 / NT;
     float bv[CPT];
-    vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * threadIdx.x, bv);
-    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), X, X, C.c, s, o, C.q);
+    vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * tid, bv);
+    const float h = rec_h(tp, r);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    if (!act) return;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-        const int v = own_cell<NT>(c);
+        const int v = own_cell<NT>(c, tid);
         if ((((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) continue;
         const size_t idx = size_t(t) * 512 + v;
         const float b = bv[c];
-        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c]);
+        const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c], h);
         const float delta = (o.ct[c] & INTERIOR) ? (b - ax) / o.d[c] : 0.0f;
         X[idx] = float(double(o.x[c]) + double(delta) * omega);
     }
 }
 
+template <int NT, bool Q = false>
+__global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X, Coef C,
+                                               const float* __restrict__ B, int color, double omega,
+                                               const double* __restrict__ S)
+{
+    __shared__ Lds<float, Q> s;
+    const TileRec r = rec_tile<false>(rec, S);
+    if (r.t < 0) return;   // no tile, or the solve has stopped
+    gs_color_tile<NT, Q>(tp, r, s, X, C, B, color, omega, true);
+}
+
+// The coarsest level's smoothing iterations on a level 0 of one tile without neighbour
+// tiles (the root), in one workgroup: the tile is loaded once and every pass runs on its
+// LDS copy (its halo is 0 throughout).  rbgs: `iters` iterations of colour `order` then
+// 1 - order, followed by `iters2` iterations starting with 1 - order (coarsest(): half
+// the coarsest iterations each way, src/AMGSolver.cu:1257-1263 with the red-black SOR of
+// :838-877); else `iters` damped-Jacobi sweeps.  Same values as one launch per pass of
+// gs_color or jacobi (each pass re-reads x masked to INTERIOR cells, which is what the
+// LDS copy holds).
+template <int NT, bool Q = false>
+__device__ __forceinline__ void coarsest_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s, float* __restrict__ X,
+                                              const Coef& C, const float* __restrict__ B, int rbgs, int iters, int iters2,
+                                              int order, double omega, float w, bool act)
+{
+    const int t = r.t, tid = r.tid;
+    Own<float, NT> o;
+    constexpr int CPT = 512 / NT;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * tid;
+    float bv[CPT], xn[CPT];
+    vload<CPT>(B + v0, bv);
+    const float h = rec_h(tp, r);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    const int passes = rbgs ? 2 * (iters + iters2) : iters;
+    if (passes == 0) return;
+    for (int p = 0; p < passes; ++p) {
+        // colour of this pass: order, 1 - order, ... then from 1 - order after 2 iters passes
+        const int color = rbgs ? ((p < 2 * iters ? order : 1 - order) ^ (p & 1)) : -1;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int v = own_cell<NT>(c, tid);
+            if (rbgs && (((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) continue;
+            const float ax = neg_lap(s, v, o.x[c], o.c[c], o.d[c], h);
+            const float delta = (o.ct[c] & INTERIOR) ? (bv[c] - ax) / o.d[c] : 0.0f;
+            xn[c] = rbgs ? float(double(o.x[c]) + double(delta) * omega) : o.x[c] + w * delta;
+        }
+        if (!rbgs) __syncthreads();   // every read of this sweep is done before x changes
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int v = own_cell<NT>(c, tid);
+            if (rbgs && (((v >> 6) + ((v >> 3) & 7) + (v & 7)) & 1) != color) continue;
+            o.x[c] = o.ct[c] == INTERIOR ? xn[c] : 0.0f;
+            s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
+        }
+        __syncthreads();
+    }
+    // every cell was written by a pass (rbgs: both colours ran at least once)
+    if (act) vstore<CPT>(X + v0, xn);
+}
+
+template <int NT, bool Q = false>
+__global__ __launch_bounds__(NT) void coarsest_iterate(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X,
+                                                       Coef C, const float* __restrict__ B, int rbgs, int iters,
+                                                       int iters2, int order, double omega, float w,
+                                                       const double* __restrict__ S)
+{
+    __shared__ Lds<float, Q> s;
+    const TileRec r = rec_tile<false>(rec, S);
+    if (r.t < 0) return;   // no tile, or the solve has stopped
+    coarsest_tile<NT, Q>(tp, r, s, X, C, B, rbgs, iters, iters2, order, omega, w, true);
+}
+
 // ------------------------------------------------------------- transfers ----
 // FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
 template <int NT, bool Q = false>
-__global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
-                                               Coef C, float* __restrict__ B, float* __restrict__ X0,
-                                               const double* __restrict__ S)
+__device__ __forceinline__ void fas_save_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
+                                              const float* __restrict__ X, const Coef& C, float* __restrict__ B,
+                                              float* __restrict__ X0, bool act)
 {
-    __shared__ Lds<float> s;
-    const TileRec r = rec_tile<false>(rec, S);
-    const int t = r.t;   // < 0: no tile, or the solve has stopped
-    if (t < 0) return;
+    const int t = r.t, tid = r.tid;
     Own<float, NT> o;
     constexpr int CPT = 512 / NT;
-    const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
+    const size_t v0 = size_t(t) * 512 + size_t(CPT) * tid;
     float bv[CPT];
     vload<CPT>(B + v0, bv);
-    load_tile<false, float, NT, Q>(tp, r, rec_h(tp, r), X, X, C.c, s, o, C.q);
+    const float h = rec_h(tp, r);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    if (!act) return;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
         if (o.ct[c] & INTERIOR) {
-            bv[c] = bv[c] + neg_lap(s, own_cell<NT>(c), o.x[c], o.c[c], o.d[c]);
+            bv[c] = bv[c] + neg_lap(s, own_cell<NT>(c, tid), o.x[c], o.c[c], o.d[c], h);
             X0[v0 + c] = o.x[c];   // x itself on INTERIOR cells
         } else {
             bv[c] = 0;
         }
     }
     vstore<CPT>(B + v0, bv);
+}
+
+template <int NT, bool Q = false>
+__global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
+                                               Coef C, float* __restrict__ B, float* __restrict__ X0,
+                                               const double* __restrict__ S)
+{
+    __shared__ Lds<float, Q> s;
+    const TileRec r = rec_tile<false>(rec, S);
+    if (r.t < 0) return;   // no tile, or the solve has stopped
+    fas_save_tile<NT, Q>(tp, r, s, X, C, B, X0, true);
+}
+
+// a record read at a given address (not from the launch list position), thread index tid
+__device__ __forceinline__ TileRec rec_tile_at(const int4* __restrict__ r, int tid)
+{
+    TileRec x;
+    const int4 r0 = r[0];
+    x.t = r0.x;
+    x.level = r0.y;
+    x.meta = r0.z;
+    x.parent = r0.w;
+    x.ha = r[1];
+    x.hb = r[2];
+    x.ga = make_int4(-1, -1, -1, -1);
+    x.gb = make_int4(-1, -1, 0, 0);
+    x.ca = r[5];
+    x.cb = r[6];
+    x.tid = tid;
+    return x;
+}
+
+// The bottom of a FAS mu-cycle, FASMuCycleStep at level 1 with everything below it
+// (src/AMGSolver.cu:1257-1277), in one workgroup of 1024 threads, for a level 0 of one
+// tile without neighbours (the root) whose 8 children are the whole of level 1.  Level 1
+// is then a dense 16^3 block: its x (masked to INTERIOR cells, with a zero halo at the
+// domain boundary) and face coefficients stay in LDS for every pass, so a pass costs LDS
+// reads and a barrier instead of a launch and a memory round trip.  Group g (128 threads,
+// 4 cells each) holds the level-1 tile of octant g.  Every step does the arithmetic of
+// the per-pass kernel it replaces (gs_color, jacobi, residual_restrict, fas_save,
+// fas_prolong_v, jacobi's r.z partial and prolongation into level 2), so the results
+// are bit-identical; the root's steps (FAS save, coarsest iterations) are the per-tile
+// functions themselves, on 512 threads (the other 512 redo them without writing).
+// phase 1: pre-smoothing, residual + restriction, the root's FAS save; phase 2: the
+// prolongation to level 1 and its post-smoothing; phase 3: both with the coarsest
+// iterations (mu times) in between.
+struct BottomArgs {
+    const int4* rec0;    // the root's record (its children by octant)
+    int entry[8];        // level-1 list entry of the tile of each octant (r.z partials)
+    int nonleaf0;
+    float *X, *X0, *B;
+    Coef C;
+    float Rr, P, w;
+    double omega;
+    int rbgs, iters, mu, coarsest_iters;
+    int phase, fas_save1, push;
+    double* rz;          // r.z partials of level 1's last post-sweep (Jacobi), or null
+};
+
+template <bool Q>
+struct BottomLds {
+    float x[18 * 18 * 18];                                    // level 1, masked, zero halo
+    std::conditional_t<Q, uint16_t, float> c[3][4096];        // negative-face coefficients
+    float res[4096];                                          // residuals for the restriction
+    Lds<float, Q> root[2];                                    // the root's steps, two copies
+    double red[16];
+};
+
+__device__ __forceinline__ int bx(int i, int j, int k) { return (i + 1) * 324 + (j + 1) * 18 + (k + 1); }
+__device__ __forceinline__ int bc(int i, int j, int k) { return (i << 8) | (j << 4) | k; }
+
+template <bool Q>
+__device__ __forceinline__ float bottom_lap(const BottomLds<Q>& s, int i, int j, int k, float x0, const float c[3], float d,
+                                            float h)
+{
+    if (d == 0) return 0.0f;
+    // the + face coefficient is the + neighbour's own; h where the level ends (its x is 0 there)
+    const float cp0 = i < 15 ? lds_coef(s.c[0][bc(i + 1, j, k)], h) : h;
+    const float cp1 = j < 15 ? lds_coef(s.c[1][bc(i, j + 1, k)], h) : h;
+    const float cp2 = k < 15 ? lds_coef(s.c[2][bc(i, j, k + 1)], h) : h;
+    float r = x0 * d;
+    r = fmaf(c[0], s.x[bx(i - 1, j, k)], r);
+    r = fmaf(cp0, s.x[bx(i + 1, j, k)], r);
+    r = fmaf(c[1], s.x[bx(i, j - 1, k)], r);
+    r = fmaf(cp1, s.x[bx(i, j + 1, k)], r);
+    r = fmaf(c[2], s.x[bx(i, j, k - 1)], r);
+    r = fmaf(cp2, s.x[bx(i, j, k + 1)], r);
+    return r;
+}
+
+template <bool Q>
+__global__ __launch_bounds__(1024) void cycle_bottom(TopoView tp, BottomArgs a, const double* __restrict__ S)
+{
+    __shared__ BottomLds<Q> s;
+    if (S[S_DONE] != 0.0) return;   // the solve has stopped (uniform)
+    // the group is uniform over a wave (two waves a group)
+    const int g = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 7)), tid = threadIdx.x & 127;
+    const int4* r0p = a.rec0;
+    const int root = r0p[0].x;
+    const int t = pick8(r0p[5], r0p[6], g);   // the level-1 tile of octant g
+    const int tmeta = tp.ttype[t];
+    const float h = tp.h0 / 2.0f;
+    const size_t v0 = size_t(t) * 512 + 4 * tid;
+    const int li = tid >> 4, lj = (tid >> 1) & 7, lk0 = 4 * (tid & 1);   // the thread's cells in the tile
+    const int gi = 8 * (g & 1) + li, gj = 8 * ((g >> 1) & 1) + lj, gk0 = 8 * ((g >> 2) & 1) + lk0;
+    float x[4], b[4], cf[4][3], d[4];
+    uint8_t ct[4];
+    // ---- level 1 into registers and LDS
+    {
+        float xr[4];
+        vload<4>(a.X + v0, xr);
+        vload<4>(a.B + v0, b);
+        vload<4>(tp.ctype + v0, ct);
+        if constexpr (Q) {
+            uint16_t q[4][4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vload<4>(a.C.q[e] + v0, q[e]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    cf[c][e] = float(__builtin_bit_cast(_Float16, q[e][c])) * h;
+                    s.c[e][bc(gi, gj, gk0 + c)] = q[e][c];
+                }
+                d[c] = float(__builtin_bit_cast(_Float16, q[3][c])) * h;
+            }
+        } else {
+            float q[4][4];
+            const bool dv = tp.derive && (tmeta & LEAF);
+#pragma unroll
+            for (int e = 0; e < 3; ++e) vload<4>(a.C.c[e] + v0, q[e]);
+            if (!dv) vload<4>(a.C.c[3] + v0, q[3]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    cf[c][e] = q[e][c];
+                    s.c[e][bc(gi, gj, gk0 + c)] = q[e][c];
+                }
+                d[c] = dv ? 0.0f : q[3][c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            x[c] = ct[c] == INTERIOR ? xr[c] : 0.0f;
+            s.x[bx(gi, gj, gk0 + c)] = x[c];
+        }
+        // the zero halo of the 16^3 block
+        for (int m = threadIdx.x; m < 18 * 18 * 18; m += 1024) {
+            const int hi = m / 324, hj = (m / 18) % 18, hk = m % 18;
+            if (hi == 0 || hi == 17 || hj == 0 || hj == 17 || hk == 0 || hk == 17) s.x[m] = 0.0f;
+        }
+        __syncthreads();
+        if constexpr (!Q) {
+            if (tp.derive && (tmeta & LEAF)) {
+                // derived_diag: d = 0 - c-(x) - c+(x) - ..., c+ = -h where the level ends
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int k = gk0 + c;
+                    const float cp0 = gi < 15 ? s.c[0][bc(gi + 1, gj, k)] : -h;
+                    const float cp1 = gj < 15 ? s.c[1][bc(gi, gj + 1, k)] : -h;
+                    const float cp2 = k < 15 ? s.c[2][bc(gi, gj, k + 1)] : -h;
+                    float dd = 0.0f;
+                    dd = dd - cf[c][0];
+                    dd = dd - cp0;
+                    dd = dd - cf[c][1];
+                    dd = dd - cp1;
+                    dd = dd - cf[c][2];
+                    dd = dd - cp2;
+                    d[c] = (ct[c] & INTERIOR) ? dd : 0.0f;
+                }
+            }
+        }
+    }
+    const int n = a.iters;
+    // one smoothing pass on level 1: colour `col` of red-black SOR (col >= 0) or a damped-Jacobi
+    // sweep (col < 0, `save`: with the FAS save of NONLEAF tiles; `dot`: returns sum b x_out)
+    auto pass = [&](int col, bool save, bool dot, float* xo) -> double {
+        double acc = 0;
+        float ax[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ax[c] = bottom_lap(s, gi, gj, gk0 + c, x[c], cf[c], d[c], h);
+        if (col >= 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (((li + lj + lk0 + c) & 1) != col) continue;
+                const float delta = (ct[c] & INTERIOR) ? (b[c] - ax[c]) / d[c] : 0.0f;
+                x[c] = float(double(x[c]) + double(delta) * a.omega);
+                s.x[bx(gi, gj, gk0 + c)] = x[c];   // 0 on other cells already
+            }
+            __syncthreads();
+            return 0;
+        }
+        const bool sv = save && (tmeta & NONLEAF);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (sv) {
+                if (ct[c] & INTERIOR) {
+                    b[c] = b[c] + ax[c];
+                    a.X0[v0 + c] = x[c];
+                } else {
+                    b[c] = 0.0f;
+                }
+            }
+            const float delta = (ct[c] & INTERIOR) ? (b[c] - ax[c]) / d[c] : 0.0f;
+            xo[c] = x[c] + a.w * delta;
+            if (dot && (tmeta & LEAF)) acc += double(b[c]) * double(xo[c]);
+        }
+        if (sv) vstore<4>(a.B + v0, b);
+        __syncthreads();   // every read of this sweep is done before x changes
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            x[c] = ct[c] == INTERIOR ? xo[c] : 0.0f;
+            s.x[bx(gi, gj, gk0 + c)] = x[c];
+        }
+        __syncthreads();
+        return acc;
+    };
+    float xo[4];
+    if (a.phase & 1) {
+        // pre-smoothing (order 0)
+        for (int it = 0; it < n; ++it) {
+            if (a.rbgs) {
+                pass(0, false, false, xo);
+                pass(1, false, false, xo);
+            } else {
+                pass(-1, a.fas_save1 && it == 0, false, xo);
+            }
+        }
+        // residual and restriction into the root (residual_restrict)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            s.res[g * 512 + 4 * tid + c] = b[c] - bottom_lap(s, gi, gj, gk0 + c, x[c], cf[c], d[c], h);
+        __syncthreads();
+        if (tid < 64) {
+            const int fl0 = tid / 16 * 2, fl1 = (tid / 4) % 4 * 2, fl2 = tid % 4 * 2;
+            float rs = 0, xs = 0;
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                    for (int kk = 0; kk < 2; ++kk) {
+                        rs += s.res[g * 512 + cidx(fl0 + ii, fl1 + jj, fl2 + kk)];
+                        xs += s.x[bx(8 * (g & 1) + fl0 + ii, 8 * ((g >> 1) & 1) + fl1 + jj, 8 * ((g >> 2) & 1) + fl2 + kk)];
+                    }
+            const size_t pidx = size_t(root) * 512 + cidx(4 * (g & 1) + (fl0 >> 1), 4 * ((g >> 1) & 1) + (fl1 >> 1),
+                                                         4 * ((g >> 2) & 1) + (fl2 >> 1));
+            a.B[pidx] = rs * a.Rr;
+            a.X[pidx] = xs / 8;
+        }
+        // level 1's x after the pre-smoothing (read back by phase 2 when the coarsest solve
+        // runs in between; the jacobi ping-pong buffer is not written)
+        vstore<4>(a.X + v0, x);
+        __syncthreads();
+        if (a.nonleaf0) {
+            TileRec r = rec_tile_at(r0p, int(threadIdx.x & 511));
+            fas_save_tile<512, Q>(tp, r, s.root[threadIdx.x >> 9], a.X, a.C, a.B, a.X0, threadIdx.x < 512);
+            __syncthreads();
+        }
+    }
+    if (a.phase == 3) {
+        const int it1 = a.rbgs ? a.coarsest_iters / 2 : a.coarsest_iters;
+        for (int i = 0; i < a.mu; ++i) {
+            TileRec r = rec_tile_at(r0p, int(threadIdx.x & 511));
+            coarsest_tile<512, Q>(tp, r, s.root[threadIdx.x >> 9], a.X, a.C, a.B, a.rbgs, it1, a.rbgs ? it1 : 0, 0,
+                                  a.omega, a.w, threadIdx.x < 512);
+            __syncthreads();
+        }
+    }
+    if (a.phase & 2) {
+        if (a.phase == 2) {
+            // phase 1 left level 1's x in memory
+            float xr[4];
+            vload<4>(a.X + v0, xr);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) x[c] = xr[c];   // masked already
+        }
+        // the prolongation to level 1 (fas_prolong_v; no GHOST tile on a complete level 1)
+        {
+            const size_t pb = size_t(root) * 512 + cidx(gi >> 1, gj >> 1, gk0 >> 1);
+            const float2 xc = *reinterpret_cast<const float2*>(a.X + pb), x0c = *reinterpret_cast<const float2*>(a.X0 + pb);
+            const float cx[2] = {xc.x, xc.y}, cx0[2] = {x0c.x, x0c.y};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                x[c] = (ct[c] & INTERIOR) ? x[c] + a.P * (cx[c >> 1] - cx0[c >> 1]) : 0.0f;
+                s.x[bx(gi, gj, gk0 + c)] = x[c];
+            }
+            __syncthreads();
+        }
+        // post-smoothing (order 1); the last Jacobi sweep sums b.x = r.z and prolongs into level 2
+        double acc = 0;
+        for (int it = 0; it < n; ++it) {
+            if (a.rbgs) {
+                pass(1, false, false, xo);
+                pass(0, false, false, xo);
+            } else {
+                acc = pass(-1, false, a.rz && it == n - 1, xo);
+            }
+        }
+        vstore<4>(a.X + v0, x);
+        if (a.rz) {
+            // the tile's b.x partial, summed as block_sum sums a 128-thread workgroup's
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+            if ((threadIdx.x & 63) == 0) s.red[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                double sum = 0;
+                sum += s.red[2 * g];
+                sum += s.red[2 * g + 1];
+                a.rz[a.entry[g]] = sum;
+            }
+        }
+        if (a.push) {
+            // jacobi's prolongation into the children (level 2) from the final x
+            const int oc = (li >> 2) | ((lj >> 2) << 1) | ((lk0 >> 2) << 2);
+            const int ch = tp.child[size_t(t) * 8 + oc];
+            if (ch >= 0) {
+                const bool ghost = tp.ttype[ch] & GHOST;
+                float x0v[4];
+                vload<4>(a.X0 + v0, x0v);
+#pragma unroll
+                for (int ab = 0; ab < 4; ++ab) {
+                    const size_t fb = size_t(ch) * 512 + cidx(2 * (li & 3) + (ab >> 1), 2 * (lj & 3) + (ab & 1), 2 * (lk0 & 3));
+                    float xf[8];
+                    uint8_t cfb[8];
+                    vload<8>(a.X + fb, xf);
+                    vload<8>(tp.ctype + fb, cfb);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const bool in = cfb[e] & INTERIOR;
+                        const float c1 = xo[e >> 1], c0 = x0v[e >> 1];
+                        xf[e] = ghost ? (in ? c1 : 0.0f) : (in ? xf[e] + a.P * (c1 - c0) : 0.0f);
+                    }
+                    vstore<8>(a.X + fb, xf);
+                }
+            }
+        }
+    }
 }
 
 // PropagateToChildren on GHOST tiles: ghost cell = parent cell (src/PoissonGrid.cu:377-405)
@@ -1723,7 +2287,8 @@ bool set_kernel_forms(int mask, int threads)
 // the Jacobi sweep on FP16 coefficient copies reads half the coefficient bytes of the
 // FP32 form, so a tile's latency chain carries fewer bytes; twice the tiles in flight
 // per CU pays there (MI355X, finest level of the 1024^3 torus workload: 72.6 us at 128
-// threads a tile against 77.8 at 256).  It runs 128 unless a thread count was chosen.
+// threads a tile against 77.8 at 256; 64 threads a tile, 90 VGPRs, ran 71.6 against 69.0).
+// It runs 128 unless a thread count was chosen.
 static int q_jacobi_threads()
 {
     const int v = stencil_threads();
@@ -1732,7 +2297,7 @@ static int q_jacobi_threads()
 
 // the operator on LEAF tiles likewise runs 128 threads a tile unless a thread count
 // was chosen (106 us against 108 at 256 on the 1024^3 torus workload)
-static int apply_threads() { return q_jacobi_threads(); }
+static int apply_threads() { return g_threads_chosen.load(std::memory_order_relaxed) ? stencil_threads() : 128; }
 
 int launched_threads_key() { return apply_threads() * 1000 + q_jacobi_threads(); }
 
@@ -1856,22 +2421,34 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
     else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
 }
 
-// one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE> with nt threads a tile
-template <bool DOT, bool Q, bool SAVE>
+// one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE, PUSH> with nt threads a tile
+template <bool DOT, bool Q, bool SAVE, bool PUSH = false>
 static void jacobi_at(int nt, const LaunchList& l, hipStream_t st, const TopoView& tp, const float* Xin, float* Xout,
-                      const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0)
+                      const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0,
+                      float P = 0.0f)
 {
     switch (nt) {
-    case 128: k::jacobi<128, DOT, Q, SAVE><<<l.n, 128, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0); break;
-    case 512: k::jacobi<512, DOT, Q, SAVE><<<l.n, 512, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0); break;
-    default: k::jacobi<256, DOT, Q, SAVE><<<l.n, 256, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0); break;
+    case 128:
+        k::jacobi<128, DOT, Q, SAVE, PUSH><<<l.n, 128, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        break;
+    case 512:
+        k::jacobi<512, DOT, Q, SAVE, PUSH><<<l.n, 512, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        break;
+    default:
+        k::jacobi<256, DOT, Q, SAVE, PUSH><<<l.n, 256, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        break;
     }
 }
 
-int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial, float* X0)
+bool jacobi_push_available(const DevGrid& d, int level)
 {
-    if (X0 && partial) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
+    return level + 1 <= d.max_level && d.push_ok[level + 1] && !(persistent_mask() & (16 | PERSIST_JACOBI));
+}
+
+int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
+                  const double* S, hipStream_t st, double* partial, float* X0, bool push, float P)
+{
+    if (X0 && partial && !push) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return 0;
     const bool q = C.q[0] != nullptr;
@@ -1881,6 +2458,19 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
     const TopoView tp = view(d);
     const int nt = q ? q_jacobi_threads() : stencil_threads();
     S = one_tile ? run_flag(d, S) : S;
+    if (push) {
+        // the last post-sweep with the prolongation to level + 1 (X0: the FAS-saved x)
+        if (!X0 || !jacobi_push_available(d, level))
+            throw std::invalid_argument("jacobi: the prolongation cannot ride on this level's sweep");
+        if (partial) {
+            if (q) jacobi_at<true, true, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+            else jacobi_at<true, false, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+            return l.n;
+        }
+        if (q) jacobi_at<false, true, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+        else jacobi_at<false, false, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+        return 0;
+    }
     if (partial) {
         // one partial per list entry
         if (q) jacobi_at<true, true, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
@@ -2072,6 +2662,49 @@ void launch_finish(const double* partial, const double* partial2, int n, double*
 {
     if (!S || !partial) throw std::invalid_argument("finish needs the partials and the solver scalars");
     k::finish<<<1, 1024, 0, st>>>(partial, partial2, n, S, kind);
+}
+
+bool cycle_bottom_available(const DevGrid& d)
+{
+    return d.bottom_ok && !(persistent_mask() & 16);
+}
+
+void launch_cycle_bottom(const DevGrid& d, int phase, float* X, float* X0, float* B, Coef C, float Rr, float P, bool rbgs,
+                         int iters, double omega, int mu, int coarsest_iters, bool fas_save1, bool push, double* rz,
+                         const double* S, hipStream_t st)
+{
+    if (!cycle_bottom_available(d)) throw std::logic_error("cycle_bottom: levels 0 and 1 do not fit one workgroup");
+    if (push && (rbgs || iters < 1 || !jacobi_push_available(d, 1)))
+        throw std::invalid_argument("cycle_bottom: the prolongation to level 2 cannot ride on level 1's last sweep");
+    if (rz && (rbgs || iters < 1)) throw std::invalid_argument("cycle_bottom: r.z partials come from a Jacobi sweep");
+    k::BottomArgs a;
+    a.rec0 = d.subtree[0].rec;
+    for (int o = 0; o < 8; ++o) a.entry[o] = d.bottom_entry[o];
+    a.nonleaf0 = d.nonleaf[0].tiles;
+    a.X = X; a.X0 = X0; a.B = B; a.C = C;
+    a.Rr = Rr; a.P = P; a.w = float(omega); a.omega = omega;
+    a.rbgs = rbgs; a.iters = iters; a.mu = mu; a.coarsest_iters = coarsest_iters;
+    a.phase = phase; a.fas_save1 = fas_save1; a.push = push; a.rz = rz;
+    if (C.q[0]) k::cycle_bottom<true><<<1, 1024, 0, st>>>(view(d), a, run_flag(d, S));
+    else k::cycle_bottom<false><<<1, 1024, 0, st>>>(view(d), a, run_flag(d, S));
+}
+
+bool coarsest_iterate_available(const DevGrid& d)
+{
+    return d.root_alone && !(persistent_mask() & 16);
+}
+
+void launch_coarsest_iterate(const DevGrid& d, float* X, Coef C, const float* B, bool rbgs, int iters, int iters2, int order,
+                             double omega, float w, const double* S, hipStream_t st)
+{
+    if (!coarsest_iterate_available(d)) throw std::logic_error("coarsest_iterate: level 0 is not a single tile");
+    const LaunchList& l = d.subtree[0];
+    if (C.q[0])
+        k::coarsest_iterate<256, true><<<1, 256, 0, st>>>(view(d), l.rec, X, C, B, rbgs, iters, iters2, order, omega, w,
+                                                        run_flag(d, S));
+    else
+        k::coarsest_iterate<256, false><<<1, 256, 0, st>>>(view(d), l.rec, X, C, B, rbgs, iters, iters2, order, omega, w,
+                                                         run_flag(d, S));
 }
 
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,

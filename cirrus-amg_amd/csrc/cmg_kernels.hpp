@@ -44,8 +44,14 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
                      hipStream_t st);
 // partial: one b.x_out sum per list entry (one-tile form); returns their number
 // X0: the level's FAS save on its NONLEAF tiles in the same pass (B updated in place, x -> X0)
+// push: instead, the FAS prolongation to level + 1 in the same pass (a last post-sweep that
+// writes the cycle's x; X0 is then the FAS-saved x it reads; P the prolongation weight)
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial = nullptr, float* X0 = nullptr);
+                  const double* S, hipStream_t st, double* partial = nullptr, float* X0 = nullptr, bool push = false,
+                  float P = 0.0f);
+// the prolongation to level + 1 can ride on level's last Jacobi sweep (every tile of level + 1
+// has a LEAF or NONLEAF parent, one-tile Jacobi form)
+bool jacobi_push_available(const DevGrid& d, int level);
 // Dq (optional): the diagonal as FP16 bits of D / h instead of D
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
                         hipStream_t st, const uint16_t* Dq = nullptr);
@@ -161,6 +167,20 @@ unsigned int tile_types_io(DevGrid& d, bool gather, const uint64_t* ptrs, void* 
 void launch_roi_flags(const DevGrid& d, const float* X, int subtree_level, int launch_types, int pred, float a, float b,
                       bool calc_locked, uint8_t* interest, uint8_t* locked, float* tmin, float* tmax, hipStream_t st);
 
+// levels 0 and 1 fit one workgroup (level 0 one tile without neighbours, level 1 its 8
+// children) and forms bit 4 is clear: FASMuCycleStep at level 1 can run as
+// launch_cycle_bottom (phase 3; or phase 1, the coarsest solve, phase 2)
+bool cycle_bottom_available(const DevGrid& d);
+void launch_cycle_bottom(const DevGrid& d, int phase, float* X, float* X0, float* B, Coef C, float Rr, float P, bool rbgs,
+                         int iters, double omega, int mu, int coarsest_iters, bool fas_save1, bool push, double* rz,
+                         const double* S, hipStream_t st);
+// level 0 is one tile without neighbours (and forms bit 4 is clear): its smoothing
+// iterations can run in one workgroup (launch_coarsest_iterate)
+bool coarsest_iterate_available(const DevGrid& d);
+// rbgs: `iters` red-black iterations from colour `order`, then `iters2` from 1 - order
+// (w unused); else `iters` damped-Jacobi sweeps with weight w (omega, iters2, order unused)
+void launch_coarsest_iterate(const DevGrid& d, float* X, Coef C, const float* B, bool rbgs, int iters, int iters2, int order,
+                             double omega, float w, const double* S, hipStream_t st);
 void launch_coarse_direct(const double* inv, const int32_t* cells, int n, const int32_t* zero_cells, int nz, float* X,
                           const float* B, const double* S, hipStream_t st);
 

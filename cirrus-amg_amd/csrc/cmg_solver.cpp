@@ -8,6 +8,7 @@
 // convergence are no-ops and the host only polls the flag.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
@@ -35,7 +36,7 @@ struct cmg_solver {
     float* net_w = nullptr;        // learned tile smoother weights (7776 floats)
     unsigned int* lcount = nullptr;   // learned steps of the last solve: tiles judged, tiles kept
     uint16_t* q16 = nullptr;          // FP16 copies of the four coefficient channels / h (cycle_coef_fp16)
-    unsigned int* qbad = nullptr;     // workgroups whose FP16 copy has a non-finite value
+    unsigned int* qbad = nullptr;     // workgroups whose FP16 copy has a value that does not fit
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -98,6 +99,17 @@ using TileRecords = std::vector<int4>;
 // sweep on MI355X: 512-tile blocks 68 us, one contiguous chunk per XCD 75.5 us, no
 // XCD grouping 90 us.)
 
+// tiles a Morton block holds (CMG_XCD_BLOCK, 64..65536, default 512)
+int xcd_block_tiles()
+{
+    static const int v = [] {
+        const char* e = getenv("CMG_XCD_BLOCK");
+        const int b = e ? atoi(e) : 512;
+        return b >= 64 && b <= 65536 ? b : 512;
+    }();
+    return v;
+}
+
 LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, const TileRecords& trec, bool with_meta = false)
 {
     LaunchList l;
@@ -116,7 +128,7 @@ LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, const Til
     std::vector<int32_t> order;
     const int n = int(key.size());
     constexpr int kXcd = 8;
-    constexpr int kBlock = 512;
+    const int kBlock = xcd_block_tiles();
     if (n < 64) {
         for (auto& k : key) order.push_back(k.second);
     } else {
@@ -132,17 +144,18 @@ LaunchList make_list(const Topology& topo, std::vector<int32_t> tiles, const Til
         }
     }
     l.n = int(order.size());
+    l.order = order;
     l.d = dmalloc<int32_t>(order.size());
     HIPCHK(hipMemcpy(l.d, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     {
-        std::vector<int4> rec(order.size() * 5, make_int4(0, 0, 0, 0));
+        std::vector<int4> rec(order.size() * kRecWords, make_int4(0, 0, 0, 0));
         for (size_t i = 0; i < order.size(); ++i) {
             const int t = order[i];
             if (t < 0) {
-                rec[i * 5] = make_int4(-1, 0, 0, -1);
+                rec[i * kRecWords] = make_int4(-1, 0, 0, -1);
                 continue;
             }
-            for (int k = 0; k < 5; ++k) rec[i * 5 + k] = trec[size_t(t) * 5 + k];
+            for (int k = 0; k < kRecWords; ++k) rec[i * kRecWords + k] = trec[size_t(t) * kRecWords + k];
         }
         l.rec = dmalloc<int4>(rec.size());
         HIPCHK(hipMemcpy(l.rec, rec.data(), rec.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -269,15 +282,19 @@ void dev_upload(DevGrid& d, const Topology& topo)
         }
         up(d.halo, halo);
         up(d.ghalo, ghalo);
-        trec.assign(size_t(N) * 5, make_int4(0, 0, 0, 0));
+        trec.assign(size_t(N) * kRecWords, make_int4(0, 0, 0, 0));
         for (int t = 0; t < N; ++t) {
             const int32_t* hh = &halo[size_t(t) * 8];
             const int32_t* gg = &ghalo[size_t(t) * 8];
-            trec[size_t(t) * 5 + 0] = make_int4(t, topo.t_level[t], int(topo.t_type[t]) | (int(oct[t]) << 8), topo.parent[t]);
-            trec[size_t(t) * 5 + 1] = make_int4(hh[0], hh[1], hh[2], hh[3]);
-            trec[size_t(t) * 5 + 2] = make_int4(hh[4], hh[5], hh[6], hh[7]);
-            trec[size_t(t) * 5 + 3] = make_int4(gg[0], gg[1], gg[2], gg[3]);
-            trec[size_t(t) * 5 + 4] = make_int4(gg[4], gg[5], gg[6], gg[7]);
+            const int32_t* cc = &topo.child[size_t(t) * 8];
+            int4* r = &trec[size_t(t) * kRecWords];
+            r[0] = make_int4(t, topo.t_level[t], int(topo.t_type[t]) | (int(oct[t]) << 8), topo.parent[t]);
+            r[1] = make_int4(hh[0], hh[1], hh[2], hh[3]);
+            r[2] = make_int4(hh[4], hh[5], hh[6], hh[7]);
+            r[3] = make_int4(gg[0], gg[1], gg[2], gg[3]);
+            r[4] = make_int4(gg[4], gg[5], gg[6], gg[7]);
+            r[5] = make_int4(cc[0], cc[1], cc[2], cc[3]);
+            r[6] = make_int4(cc[4], cc[5], cc[6], cc[7]);
         }
     }
     up(d.parent, topo.parent);
@@ -310,6 +327,38 @@ void dev_upload(DevGrid& d, const Topology& topo)
         d.nonleaf[l] = make_list(topo, nl[l], trec);
         d.prolong[l] = make_list(topo, pr[l], trec, true);
         d.leafat[l] = make_list(topo, la[l], trec);
+    }
+    // The prolongation rides on the parents' last post-sweep only into levels of at most
+    // 16384 tiles: there a launch of its own is latency-bound.  Into larger levels the
+    // streaming prolongation kernel is faster than the sweep with its children's traffic
+    // (MI355X, 1024^3-equivalent torus workload: 60 us fused against 18.7 + 36.4 into the
+    // finest level, 18.1 against 8.5 + 12.3 into the next).
+    for (int l = 0; l < topo.num_levels(); ++l) {
+        bool ok = l >= 2 && !pr[l].empty() && pr[l].size() <= 16384;
+        for (int t : pr[l]) {
+            const int p = topo.parent[t];
+            ok = ok && p >= 0 && (topo.t_type[p] & (LEAF | NONLEAF)) && topo.t_level[p] == l - 1;
+        }
+        d.push_ok[l] = ok;
+    }
+    if (sub[0].size() == 1) {
+        const int t = sub[0][0];
+        bool alone = true;
+        for (int f = 0; f < 6; ++f) alone = alone && topo.nbr[size_t(t) * 6 + f] < 0;
+        d.root_alone = alone;
+        if (alone && topo.num_levels() > 1 && sub[1].size() == 8 && pr[1].size() == 8) {
+            bool kids = true;
+            for (int o = 0; o < 8; ++o) {
+                const int c = topo.child[size_t(t) * 8 + o];
+                kids = kids && c >= 0 && topo.parent[c] == t && (topo.t_type[c] & (LEAF | NONLEAF));
+                int e = -1;
+                for (int i = 0; i < int(d.subtree[1].order.size()); ++i)
+                    if (d.subtree[1].order[i] == c) e = i;
+                d.bottom_entry[o] = e;
+                kids = kids && e >= 0;
+            }
+            d.bottom_ok = kids;
+        }
     }
     {
         int dev = 0, cus = 0;
@@ -743,8 +792,11 @@ int rz_offset(const DevGrid& d, int level)
 }
 
 // `x_zero`: every x of the cycle is still zero (first sweep at the finest level);
-// `fas_save`: the level's FAS save is still due and rides on the first sweep
-void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false)
+// `fas_save`: the level's FAS save is still due and rides on the first sweep;
+// `push`: the FAS prolongation to level + 1 comes right after these sweeps and may ride
+// on the last one.  Returns whether it did.
+bool smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false,
+            bool push = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
@@ -753,10 +805,13 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
             launch_gs_color(d, level, c.X, c.C, c.B, order, c.o->omega, c.S, st);
             launch_gs_color(d, level, c.X, c.C, c.B, 1 - order, c.o->omega, c.S, st);
         }
-        return;
+        return false;
     }
     float* cur = c.X;
     float* other = c.s->tmp;
+    // an even number of ping-pong sweeps ends in X, where the finer level's x is; a single
+    // sweep would also carry the FAS save
+    push = push && iters >= 2 && iters % 2 == 0 && !x_zero && jacobi_push_available(d, level);
     for (int it = 0; it < iters; ++it) {
         if (it == 0 && x_zero) {
             if (!(c.zero_done && level == d.max_level))
@@ -765,13 +820,15 @@ void smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
         else {
             // the level's last post-sweep sums b.x = r.z over its LEAF tiles into its slice of c.rz
             const bool last = c.rz && order == 1 && it == iters - 1;
+            const bool pu = push && it == iters - 1;
             launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st,
                           last ? c.rz + rz_offset(d, level) : nullptr,
-                          fas_save && it == 0 ? c.X0 : nullptr);
+                          (fas_save && it == 0) || pu ? c.X0 : nullptr, pu, c.s->P);
         }
         std::swap(cur, other);
     }
     if (cur != c.X) launch_copy_list(d, d.subtree[level], cur, c.X, c.S, st);
+    return push;
 }
 
 // repetitions of the coarse visit (AMGSolver::mu_cycle_repeat_times)
@@ -786,6 +843,11 @@ void coarsest(const CycleCtx& c)
     DevGrid& d = c.s->g->dev;
     if (c.o->coarsest_direct) {
         launch_coarse_direct(c.s->inv0, c.s->cells0, c.s->n0, c.s->zero0, c.s->nz0, c.X, c.B, c.S, d.stream);
+    } else if (coarsest_iterate_available(d)) {
+        // every pass in one workgroup on the LDS copy of the root tile
+        const bool rbgs = c.o->smoother == CMG_SMOOTH_RBGS;
+        const int it = rbgs ? c.o->coarsest_iters / 2 : c.o->coarsest_iters;
+        launch_coarsest_iterate(d, c.X, c.C, c.B, rbgs, it, rbgs ? it : 0, 0, c.o->omega, float(c.o->omega), c.S, d.stream);
     } else if (c.o->smoother == CMG_SMOOTH_RBGS) {
         smooth(c, 0, c.o->coarsest_iters / 2, 0);
         smooth(c, 0, c.o->coarsest_iters / 2, 1);
@@ -797,14 +859,36 @@ void coarsest(const CycleCtx& c)
 // FASMuCycleStep (src/AMGSolver.cu:1257-1277); x_zero: x is still 0 everywhere
 // `fas_save`: the FAS save of this level (after the restriction from level + 1)
 // has not run yet; the first pre-sweep does it
-void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false)
+// `push_up`: the caller prolongs to level + 1 next; the last post-sweep may do it
+// (returns whether it did)
+bool cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false, bool push_up = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
     if (level == 0) {
         if (fas_save) launch_fas_save(d, 0, c.X, c.C, c.B, c.X0, c.S, st);
         coarsest(c);
-        return;
+        return false;
+    }
+    if (level == 1 && level < d.max_level && !x_zero && cycle_bottom_available(d)) {
+        // levels 1 and 0 in one workgroup (the coarsest direct solve keeps its own launch)
+        const bool rbgs = c.o->smoother == CMG_SMOOTH_RBGS;
+        const int n = c.o->level_iters;
+        // (the final x is written in place whatever the sweep count, so any count can prolong)
+        const bool push = push_up && !rbgs && n >= 1 && jacobi_push_available(d, 1);
+        double* rz = c.rz && !rbgs && n >= 1 ? c.rz + rz_offset(d, 1) : nullptr;
+        const bool fas1 = fas_save && !rbgs;
+        if (c.o->coarsest_direct) {
+            launch_cycle_bottom(d, 1, c.X, c.X0, c.B, c.C, c.s->Rr, c.s->P, rbgs, n, c.o->omega, 0, 0, fas1, false, nullptr,
+                                c.S, st);
+            for (int i = 0; i < cycle_mu(c.o); ++i) coarsest(c);
+            launch_cycle_bottom(d, 2, c.X, c.X0, c.B, c.C, c.s->Rr, c.s->P, rbgs, n, c.o->omega, 0, 0, false, push, rz, c.S,
+                                st);
+        } else {
+            launch_cycle_bottom(d, 3, c.X, c.X0, c.B, c.C, c.s->Rr, c.s->P, rbgs, n, c.o->omega, cycle_mu(c.o),
+                                c.o->coarsest_iters, fas1, push, rz, c.S, st);
+        }
+        return push;
     }
     if (x_zero && level == d.max_level && c.o->learned && c.s->net_w) {
         // the learned tile smoother replaces the pre-smoothing of the finest level
@@ -821,9 +905,10 @@ void cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false
     const bool fused = c.o->smoother == CMG_SMOOTH_JACOBI && c.o->level_iters >= 1 && fuse_fas_save_allowed();
     if (!fused) launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
     const int mu = cycle_mu(c.o);
-    for (int i = 0; i < mu; ++i) cycle_step(c, level - 1, false, fused && i == 0);
-    launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
-    smooth(c, level, c.o->level_iters, 1);
+    bool pushed = false;
+    for (int i = 0; i < mu; ++i) pushed = cycle_step(c, level - 1, false, fused && i == 0, i == mu - 1);
+    if (!pushed) launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
+    return smooth(c, level, c.o->level_iters, 1, false, false, push_up);
 }
 
 // AMGSolver::FASFCycle (src/AMGSolver.cu:1358-1387)
@@ -1444,7 +1529,9 @@ void prepare_coef_fp16(cmg_solver* s)
     unsigned int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, s->qbad, sizeof(bad), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
-    if (bad) throw std::invalid_argument("cycle_coef_fp16: a coefficient divided by the tile spacing is outside the FP16 range");
+    if (bad)
+        throw std::invalid_argument("cycle_coef_fp16: a coefficient divided by the tile spacing is outside the FP16 range "
+                                    "(or a nonzero diagonal below its smallest normal)");
     s->q16_stamp = stamp;
 }
 

@@ -451,9 +451,10 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * checking that every stored LEAF diagonal equals the derived one bit for bit);
  * bit 4 set keeps the preconditioner's boundary work in launches of its own (by
  * default PCG iterations with FAS mu-cycles and damped Jacobi fuse the first
- * finest-level sweep from x = 0 into the x/r update, and sum r.z over each
- * level's LEAF tiles in that level's last post-sweep, which regroups the FP64
- * partial sums of r.z).
+ * finest-level sweep from x = 0 into the x/r update, sum r.z over each level's
+ * LEAF tiles in that level's last post-sweep, which regroups the FP64 partial
+ * sums of r.z, do a coarse level's FAS save in its first pre-sweep and the FAS
+ * prolongation to the next finer level in its last post-sweep).
  * stencil_threads: 128, 256 or 512 threads per tile.  Stencil results are
  * bit-identical across forms; PCG scalars differ only by the grouping of FP64
  * partial sums.  Default: mask 0, 256 threads (the Jacobi sweep on FP16 coefficient

@@ -105,3 +105,24 @@ def test_fp16_copies_out_of_range_are_refused():
         solver.solve(opts=P.default_opts(cycle_coef_fp16=1))
     assert e.value.status == 1
     assert "FP16" in str(e.value)
+
+
+def test_fp16_copies_of_tiny_diagonals_are_refused():
+    """Coefficients scaled by 1e-6 (a dt / rho of that size): the diagonals' c / h
+    (about 6e-6) fall below FP16's smallest normal, which would make the cycle divide
+    by 0 or by a few bits; the solve is refused up front.  The same operator solves
+    with the FP32 cycle."""
+    go, gp = H.build_pair("uniform3")
+    go.rhs_uniform(1, 2, True)
+    b = go.channel(1).copy()
+    gp.set_channel(1, b)
+    for a in range(4):
+        gp.set_channel(H.C0 + a, gp.channel(H.C0 + a) * np.float32(1e-6))
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    with pytest.raises(P.CmgError) as e:
+        solver.solve(opts=P.default_opts(cycle_coef_fp16=1))
+    assert e.value.status == 1
+    assert "smallest normal" in str(e.value)
+    gp.set_channel(1, b * np.float32(1e-6))
+    it, rel = solver.solve(relative_tolerance=1e-6, opts=P.default_opts(cycle_coef_fp16=0))
+    assert rel < 1e-6

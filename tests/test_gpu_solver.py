@@ -21,6 +21,9 @@ def _opts(kind):
     if kind == "v_jacobi_gs":  # V(2,2) damped Jacobi, RB-GS coarsest (deterministic end to end)
         return (P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_iters=8),
                 O.make_cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_iters=8))
+    if kind == "w_jacobi_gs":  # W(2,2) damped Jacobi: the prolongation rides on the second coarse visit only
+        return (P.default_opts(smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_iters=8),
+                O.make_cycle(mu=2, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_iters=8))
     if kind == "v_jacobi_direct":
         return (P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1),
                 O.make_cycle(mu=1, smoother=1, omega=2.0 / 3.0, level_iters=2, coarsest_direct=1))
@@ -28,15 +31,25 @@ def _opts(kind):
 
 
 @pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c3_small"])
-@pytest.mark.parametrize("kind", ["ref_w", "v_jacobi_gs"])
-def test_cycle_bitexact(case, kind):
-    print("\ncycle", case, kind, flush=True)
+@pytest.mark.parametrize("kind,mask", [("ref_w", 0), ("ref_w", 16), ("v_jacobi_gs", 0), ("v_jacobi_gs", 16),
+                                       ("w_jacobi_gs", 0), ("w_jacobi_gs", 16)])
+def test_cycle_bitexact(case, kind, mask):
+    """mask 16 keeps the FAS save and the prolongation in launches of their own and the
+    coarsest iterations one launch per pass; by default a coarse level's first pre-sweep
+    does the save, its last post-sweep the prolongation to the finer level, and a
+    single-tile level 0 iterates in one workgroup.  Both give the oracle's bits."""
+    print("\ncycle", case, kind, mask, flush=True)
     go, gp = H.build_pair(case)
     _rhs(go, gp, 5)
     popts, ocyc = _opts(kind)
-    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
-    go.fas_cycle(ocyc, 4, 1, 3, H.C0)
-    solver.cycle(popts, 4, 1, 3)
+    saved = P.get_kernel_forms()
+    try:
+        P.set_kernel_forms(mask, saved[1])
+        solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+        go.fas_cycle(ocyc, 4, 1, 3, H.C0)
+        solver.cycle(popts, 4, 1, 3)
+    finally:
+        P.set_kernel_forms(*saved)
     zo, zp = go.channel(4), gp.channel(4)
     assert np.isfinite(zp).all()
     assert np.array_equal(H.bits(zo), H.bits(zp)), (case, kind, np.abs(zo - zp).max())
