@@ -68,7 +68,12 @@ struct DevGrid {
     bool push_ok[kMaxLevels] = {};
     bool root_alone = false;   // level 0 is one LEAF or NONLEAF tile without neighbour tiles
     bool bottom_ok = false;    // root_alone, and level 1 is its 8 children, LEAF or NONLEAF
-    int bottom_entry[8] = {};  // the subtree[1] list entry of the root's child in each octant
+    int bottom_root = -1;      // the root; by octant its children: tile, type, subtree[1] list entry,
+    int bottom_tile[8] = {};   // and their children (8 per child, -1 none; bit 8 o + c of
+    int bottom_type[8] = {};   // bottom_ghost: that grandchild is GHOST)
+    int bottom_entry[8] = {};
+    int bottom_child[64] = {};
+    unsigned long long bottom_ghost = 0;
     LaunchList ghosts;                  // all GHOST tiles
     LaunchList leafghost;               // all LEAF and GHOST tiles
     LaunchList cyclezero;               // GHOST tiles and LEAF tiles below the finest level

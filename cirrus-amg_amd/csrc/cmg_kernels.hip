@@ -1510,8 +1510,13 @@ __device__ __forceinline__ TileRec rec_tile_at(const int4* __restrict__ r, int t
 // prolongation to level 1 and its post-smoothing; phase 3: both with the coarsest
 // iterations (mu times) in between.
 struct BottomArgs {
-    const int4* rec0;    // the root's record (its children by octant)
-    int entry[8];        // level-1 list entry of the tile of each octant (r.z partials)
+    const int4* rec0;    // the root's record
+    int root;            // the root tile
+    int tile[8];         // the level-1 tile of each octant, its type, its list entry (r.z partials)
+    int type[8];
+    int entry[8];
+    int child[64];       // their children (level 2) by octant, -1 none; bit c of ghost: child c is GHOST
+    unsigned long long ghost;
     int nonleaf0;
     float *X, *X0, *B;
     Coef C;
@@ -1561,9 +1566,9 @@ __global__ __launch_bounds__(1024) void cycle_bottom(TopoView tp, BottomArgs a, 
     // the group is uniform over a wave (two waves a group)
     const int g = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 7)), tid = threadIdx.x & 127;
     const int4* r0p = a.rec0;
-    const int root = r0p[0].x;
-    const int t = pick8(r0p[5], r0p[6], g);   // the level-1 tile of octant g
-    const int tmeta = tp.ttype[t];
+    const int root = a.root;
+    const int t = a.tile[g];   // the level-1 tile of octant g (kernel arguments: no dependent load)
+    const int tmeta = a.type[g];
     const float h = tp.h0 / 2.0f;
     const size_t v0 = size_t(t) * 512 + 4 * tid;
     const int li = tid >> 4, lj = (tid >> 1) & 7, lk0 = 4 * (tid & 1);   // the thread's cells in the tile
@@ -1734,13 +1739,6 @@ __global__ __launch_bounds__(1024) void cycle_bottom(TopoView tp, BottomArgs a, 
         }
     }
     if (a.phase & 2) {
-        if (a.phase == 2) {
-            // phase 1 left level 1's x in memory
-            float xr[4];
-            vload<4>(a.X + v0, xr);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) x[c] = xr[c];   // masked already
-        }
         // the prolongation to level 1 (fas_prolong_v; no GHOST tile on a complete level 1)
         {
             const size_t pb = size_t(root) * 512 + cidx(gi >> 1, gj >> 1, gk0 >> 1);
@@ -1753,13 +1751,28 @@ __global__ __launch_bounds__(1024) void cycle_bottom(TopoView tp, BottomArgs a, 
             }
             __syncthreads();
         }
-        // post-smoothing (order 1); the last Jacobi sweep sums b.x = r.z and prolongs into level 2
+        // post-smoothing (order 1); the last Jacobi sweep sums b.x = r.z and prolongs into level 2,
+        // whose x and types are loaded before that sweep
+        const int oc = (li >> 2) | ((lj >> 2) << 1) | ((lk0 >> 2) << 2);
+        const int ch = a.push ? a.child[8 * g + oc] : -1;
+        float pxf[4][8];
+        uint8_t pcf[4][8];
+        float px0[4];
         double acc = 0;
         for (int it = 0; it < n; ++it) {
             if (a.rbgs) {
                 pass(1, false, false, xo);
                 pass(0, false, false, xo);
             } else {
+                if (it == n - 1 && ch >= 0) {
+                    vload<4>(a.X0 + v0, px0);
+#pragma unroll
+                    for (int ab = 0; ab < 4; ++ab) {
+                        const size_t fb = size_t(ch) * 512 + cidx(2 * (li & 3) + (ab >> 1), 2 * (lj & 3) + (ab & 1), 0);
+                        vload<8>(a.X + fb, pxf[ab]);
+                        vload<8>(tp.ctype + fb, pcf[ab]);
+                    }
+                }
                 acc = pass(-1, false, a.rz && it == n - 1, xo);
             }
         }
@@ -1776,29 +1789,19 @@ __global__ __launch_bounds__(1024) void cycle_bottom(TopoView tp, BottomArgs a, 
                 a.rz[a.entry[g]] = sum;
             }
         }
-        if (a.push) {
-            // jacobi's prolongation into the children (level 2) from the final x
-            const int oc = (li >> 2) | ((lj >> 2) << 1) | ((lk0 >> 2) << 2);
-            const int ch = tp.child[size_t(t) * 8 + oc];
-            if (ch >= 0) {
-                const bool ghost = tp.ttype[ch] & GHOST;
-                float x0v[4];
-                vload<4>(a.X0 + v0, x0v);
+        if (ch >= 0) {
+            // jacobi's prolongation into the children (level 2) from the last sweep's x
+            const bool ghost = (a.ghost >> (8 * g + oc)) & 1ull;
 #pragma unroll
-                for (int ab = 0; ab < 4; ++ab) {
-                    const size_t fb = size_t(ch) * 512 + cidx(2 * (li & 3) + (ab >> 1), 2 * (lj & 3) + (ab & 1), 2 * (lk0 & 3));
-                    float xf[8];
-                    uint8_t cfb[8];
-                    vload<8>(a.X + fb, xf);
-                    vload<8>(tp.ctype + fb, cfb);
+            for (int ab = 0; ab < 4; ++ab) {
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const bool in = cfb[e] & INTERIOR;
-                        const float c1 = xo[e >> 1], c0 = x0v[e >> 1];
-                        xf[e] = ghost ? (in ? c1 : 0.0f) : (in ? xf[e] + a.P * (c1 - c0) : 0.0f);
-                    }
-                    vstore<8>(a.X + fb, xf);
+                for (int e = 0; e < 8; ++e) {
+                    const bool in = pcf[ab][e] & INTERIOR;
+                    const float c1 = xo[e >> 1], c0 = px0[e >> 1];
+                    pxf[ab][e] = ghost ? (in ? c1 : 0.0f) : (in ? pxf[ab][e] + a.P * (c1 - c0) : 0.0f);
                 }
+                const size_t fb = size_t(ch) * 512 + cidx(2 * (li & 3) + (ab >> 1), 2 * (lj & 3) + (ab & 1), 0);
+                vstore<8>(a.X + fb, pxf[ab]);
             }
         }
     }
@@ -2679,7 +2682,14 @@ void launch_cycle_bottom(const DevGrid& d, int phase, float* X, float* X0, float
     if (rz && (rbgs || iters < 1)) throw std::invalid_argument("cycle_bottom: r.z partials come from a Jacobi sweep");
     k::BottomArgs a;
     a.rec0 = d.subtree[0].rec;
-    for (int o = 0; o < 8; ++o) a.entry[o] = d.bottom_entry[o];
+    a.root = d.bottom_root;
+    a.ghost = d.bottom_ghost;
+    for (int o = 0; o < 8; ++o) {
+        a.tile[o] = d.bottom_tile[o];
+        a.type[o] = d.bottom_type[o];
+        a.entry[o] = d.bottom_entry[o];
+    }
+    for (int c = 0; c < 64; ++c) a.child[c] = d.bottom_child[c];
     a.nonleaf0 = d.nonleaf[0].tiles;
     a.X = X; a.X0 = X0; a.B = B; a.C = C;
     a.Rr = Rr; a.P = P; a.w = float(omega); a.omega = omega;

@@ -348,6 +348,8 @@ void dev_upload(DevGrid& d, const Topology& topo)
         d.root_alone = alone;
         if (alone && topo.num_levels() > 1 && sub[1].size() == 8 && pr[1].size() == 8) {
             bool kids = true;
+            d.bottom_root = t;
+            d.bottom_ghost = 0;
             for (int o = 0; o < 8; ++o) {
                 const int c = topo.child[size_t(t) * 8 + o];
                 kids = kids && c >= 0 && topo.parent[c] == t && (topo.t_type[c] & (LEAF | NONLEAF));
@@ -355,7 +357,14 @@ void dev_upload(DevGrid& d, const Topology& topo)
                 for (int i = 0; i < int(d.subtree[1].order.size()); ++i)
                     if (d.subtree[1].order[i] == c) e = i;
                 d.bottom_entry[o] = e;
+                d.bottom_tile[o] = c;
+                d.bottom_type[o] = c >= 0 ? topo.t_type[c] : 0;
                 kids = kids && e >= 0;
+                for (int q = 0; q < 8; ++q) {
+                    const int gc = c >= 0 ? topo.child[size_t(c) * 8 + q] : -1;
+                    d.bottom_child[8 * o + q] = gc;
+                    if (gc >= 0 && (topo.t_type[gc] & GHOST)) d.bottom_ghost |= 1ull << (8 * o + q);
+                }
             }
             d.bottom_ok = kids;
         }
