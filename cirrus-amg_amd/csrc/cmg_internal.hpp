@@ -53,7 +53,7 @@ struct DevGrid {
     int32_t* tcoord = nullptr;    // 3N tile coordinates (device assembly)
     unsigned int* acount = nullptr;   // device assembly: closed cells found
     int32_t* halo = nullptr;      // 8N: per tile the 6 face neighbours, their tile types packed in bytes, own type in byte 7.3
-    int32_t* ghalo = nullptr;     // 8N: per tile the parents of GHOST face neighbours (-1 else), octants packed 3 bits a face
+    int32_t* ghalo = nullptr;     // 8N: per tile the parents of its face neighbours (-1: none), octants packed 3 bits a face
     uint8_t* ctype = nullptr;     // 512N
     float* ch[kChannels] = {};
     // launch lists

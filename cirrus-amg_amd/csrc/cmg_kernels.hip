@@ -169,9 +169,9 @@ __device__ __forceinline__ int pick8(int4 a, int4 b, int s)
 // A one-tile stencil workgroup's record (launch-list order, 7 x 16 bytes, read with
 // scalar loads right after the launch): the tile, its level, type | octant << 8 and
 // parent; its halo record (face neighbours, their tile types packed in bytes, own
-// type in byte 7.3); for the operator's fresh ghost values, the parents of GHOST
-// face neighbours and their octants; its children by octant (for a sweep that
-// prolongs into them).  One round trip gives the workgroup every address it needs:
+// type in byte 7.3); the parents of the face neighbours and their octants (the
+// operator's fresh GHOST values, the cycle's pulled prolongation); its children by
+// octant (for a sweep that prolongs into them).  One round trip gives the workgroup every address it needs:
 // the tile's own loads and its halo loads follow at once.
 struct TileRec {
     int t, level, meta, parent;
@@ -228,11 +228,28 @@ __device__ __forceinline__ float rec_h(const TopoView& tp, const TileRec& r) { r
 struct NoExtraLoads {
     __device__ void operator()() const {}
 };
-template <bool FRESH, typename T, int NT, bool Q = false, typename Extra = NoExtraLoads>
+// Prolongation pulled into a sweep (the cycle's finest level after the coarse correction,
+// in place of a fas_prolong_v launch): mode 1 (the first post-sweep) reads every own and
+// same-level neighbour cell as x + P (x_parent - X0_parent) on INTERIOR cells (0 elsewhere)
+// and every GHOST neighbour cell as x_parent on INTERIOR cells (0 elsewhere); mode 2 (the
+// later post-sweeps) reads GHOST neighbour cells that way and the rest as they are.  Parent
+// values come from G (the cycle's x, whose coarser levels hold the corrected solution) and
+// X0; the parents and octants of face neighbours come with the tile record.  The values are
+// those fas_prolong_v writes, computed with the same operations.
+template <int M>
+struct Pull {
+    static constexpr int mode = M;
+    const float* X0;
+    float P;
+};
+template <bool FRESH, typename T, int NT, bool Q = false, typename Extra = NoExtraLoads, typename PullT = Pull<0>>
 __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, float h, const T* __restrict__ X,
                                           const T* __restrict__ G, const float* const* C, Lds<T, Q>& s, Own<T, NT>& o,
-                                          const uint16_t* const* Qc = nullptr, const Extra& extra = Extra{})
+                                          const uint16_t* const* Qc = nullptr, const Extra& extra = Extra{},
+                                          const PullT& pull = PullT{nullptr, 0.0f})
 {
+    constexpr int PM = PullT::mode;
+    static_assert(PM == 0 || (!FRESH && sizeof(T) == 4), "a pulled prolongation is a cycle (FP32) form");
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
     const int tid = r.tid;
@@ -251,6 +268,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     const uint16_t* const qp2 = Q ? uniform_ptr(Qc[2]) : nullptr;
     const int4 ga = r.ga, gb = r.gb;
     const uint32_t goct = uint32_t(gb.z);
+    constexpr int PC = PM == 1 ? CPT : 1;
+    T own_xr[PC], own_px[PC], own_p0[PC];   // PM 1: own x and parent values, combined after the halo loads
     {
         // own cells: one vector load per array
         const size_t v0 = base + size_t(CPT) * tid;
@@ -282,14 +301,31 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
                 o.d[c] = dv ? 0.0f : cf[3][c];
             }
         }
+        if constexpr (PM == 1) {
+            const int pt = r.parent, poc = (r.meta >> 8) & 7;
+            const size_t pb = size_t(pt >= 0 ? pt : r.t) * 512;
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) o.x[c] = o.ct[c] == INTERIOR ? xr[c] : T(0);
+            for (int c = 0; c < CPT; ++c) {
+                const int v = own_cell<NT>(c, tid);
+                const size_t pc = pb + cidx(4 * (poc & 1) + ((v >> 6) >> 1), 4 * ((poc >> 1) & 1) + (((v >> 3) & 7) >> 1),
+                                            4 * ((poc >> 2) & 1) + ((v & 7) >> 1));
+                own_px[c] = G[pc];
+                own_p0[c] = pull.X0[pc];
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) own_xr[c] = xr[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) o.x[c] = o.ct[c] == INTERIOR ? xr[c] : T(0);
+        }
     }
     // halo, phase 1: every load is issued, unconditionally and with no loaded value used yet
     // (a use, or a register copy at a branch join, makes the wave wait for all loads in
     // flight, the tile's own included).  Cells without a neighbour tile read a cell of the
     // tile itself instead, and the kind decides in phase 2 what the loaded values mean.
     T hraw[HPT];
+    constexpr int PH = PM == 1 ? HPT : 1;
+    T hpx[PH], hp0[PH];    // PM 1: a same-level neighbour cell's parent values
     uint32_t hcb[HPT];     // the neighbour's + face coefficient: FP16 bits (Q) or FP32 bits
     uint32_t hct[HPT];     // a same-level neighbour cell's type
     int hkind[HPT];        // 0 none, 1 same-level cell (INTERIOR mask), 2 GHOST, 3 GHOST without parent (0),
@@ -323,7 +359,21 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         size_t xi = ni;
         const T* xs = X;
         int kind = ok ? 1 : 0;
-        if (ghost) {
+        // PM: the face neighbour's parent cell (the record holds every neighbour's parent)
+        const int npt = PM ? pick6(ga, gb, slot) : -1;
+        const int noc = PM ? int((goct >> (3 * slot)) & 7u) : 0;
+        const size_t npc = size_t(npt >= 0 ? npt : r.t) * 512 +
+                           cidx(4 * (noc & 1) + (nl[0] >> 1), 4 * ((noc >> 1) & 1) + (nl[1] >> 1), 4 * ((noc >> 2) & 1) + (nl[2] >> 1));
+        if constexpr (PM == 1) {
+            hpx[m] = G[npc];
+            hp0[m] = pull.X0[npc];
+        }
+        if (ghost && PM) {
+            // the GHOST cell as the prolongation leaves it: its parent's value, masked below by its type
+            kind = npt >= 0 ? 2 : 3;
+            xi = npc;
+            xs = G;
+        } else if (ghost) {
             if (FRESH) {
                 const int pt = pick6(ga, gb, slot);
                 const int oc = int((goct >> (3 * slot)) & 7u);
@@ -339,12 +389,21 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
             kind = 4;
         }
         hraw[m] = xs[xi];
-        // the type is needed for same-level and NONLEAF neighbours; other cells re-read the tile's own
-        hct[m] = tp.ctype[(kind == 1 || kind == 4) ? ni : base + fi];
+        // the type is needed for same-level and NONLEAF neighbours (PM: GHOST ones too); other
+        // cells re-read the tile's own
+        hct[m] = tp.ctype[(kind == 1 || kind == 4 || (PM && kind == 2)) ? ni : base + fi];
         hkind[m] = kind;
         hni[m] = int(ni - size_t(ok ? nt : r.t) * 512) | ((ok ? nt : r.t) << 9);
     }
     extra();
+    if constexpr (PM == 1) {
+        const bool hp = r.parent >= 0;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const T c1 = hp ? own_px[c] : T(0), c0 = hp ? own_p0[c] : T(0);
+            o.x[c] = (o.ct[c] & INTERIOR) ? own_xr[c] + pull.P * (c1 - c0) : T(0);
+        }
+    }
     if (FRESH) {
         // NONLEAF neighbours (coarse LEAF tiles next to finer ones only): the mean of the 8
         // children of an INTERIOR cell, dependent loads
@@ -412,6 +471,16 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         else { fl[0] = p; fl[1] = q; fl[2] = axk; }
         // halo, phase 2: the loaded values
         T hv = hkind[m] == 0 || hkind[m] == 3 || (hkind[m] == 1 && hct[m] != INTERIOR) ? T(0) : hraw[m];
+        if constexpr (PM != 0) {
+            if (hkind[m] == 2 && !(hct[m] & INTERIOR)) hv = T(0);
+        }
+        if constexpr (PM == 1) {
+            if (hkind[m] == 1 && hct[m] == INTERIOR) {
+                const int npt = pick6(ga, gb, (f >> 1) + ((f & 1) ? 3 : 0));
+                const T c1 = npt >= 0 ? hpx[m] : T(0), c0 = npt >= 0 ? hp0[m] : T(0);
+                hv = hraw[m] + pull.P * (c1 - c0);
+            }
+        }
         if (hkind[m] == 2 || hkind[m] == 3) {
             int cl[3] = {fl[0], fl[1], fl[2]};
             cl[axis] -= sgn;
@@ -1124,7 +1193,8 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int4* __
 // reads no cell of the finer level.  Same values as the sweep followed by fas_prolong_v.
 // EARLY: the children's loads go out with the tile's (else after the sweep, with fewer
 // registers held across it)
-template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false, bool EARLY = true>
+// PM: the prolongation into this level pulled into the sweep (Pull<PM>, X0 and P read)
+template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false, bool EARLY = true, int PM = 0>
 __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
                                               const float* __restrict__ Xin, float* __restrict__ Xout,
                                               const float* __restrict__ G, const Coef& C, const float* __restrict__ B,
@@ -1170,7 +1240,8 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
             }
         }
     };
-    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads);
+    static_assert(!(PM && (SAVE || PUSH)), "a pulled prolongation is a post-sweep of the finest level");
+    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads, Pull<PM>{X0, P});
     const bool save = SAVE && (r.meta & NONLEAF);
     const bool dot_tile = DOT && (r.meta & LEAF);   // b.x is r.z on LEAF tiles only
     double acc = 0;
@@ -1234,7 +1305,7 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
     return acc;
 }
 
-template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false>
+template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false, int PM = 0>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
                                              const float* __restrict__ B, float w, const double* __restrict__ S,
@@ -1243,12 +1314,12 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict
 {
     __shared__ Lds<float, Q> s;
     __shared__ double red[DOT ? 8 : 1];
-    const TileRec r = rec_tile<false, true, PUSH>(rec, S);
+    const TileRec r = rec_tile<PM != 0, true, PUSH>(rec, S);
     if (r.t < 0) {   // -1: no tile, -2: the solve has stopped
         if (DOT && r.t == -1 && threadIdx.x == 0) partial[blockIdx.x] = 0;
         return;
     }
-    const double acc = jacobi_tile<NT, DOT, Q, SAVE, PUSH>(tp, r, s, Xin, Xout, G, C, B, w, X0, P, true);
+    const double acc = jacobi_tile<NT, DOT, Q, SAVE, PUSH, true, PM>(tp, r, s, Xin, Xout, G, C, B, w, X0, P, true);
     if (DOT) {
         const double sum = block_sum(acc, red);
         if (threadIdx.x == 0) partial[blockIdx.x] = sum;
@@ -2273,14 +2344,14 @@ static int persistent_mask()
     const char* e = getenv("CMG_PERSISTENT");
     // one tile per workgroup everywhere: faster than the persistent forms on MI355X at
     // every thread count (the persistent forms hold two tiles' loads in registers)
-    m = e ? (atoi(e) & 31) : 0;
+    m = e ? (atoi(e) & 63) : 0;
     g_mask.store(m, std::memory_order_relaxed);
     return m;
 }
 
 bool set_kernel_forms(int mask, int threads)
 {
-    if (mask < 0 || mask > 31 || (threads != 128 && threads != 256 && threads != 512)) return false;
+    if (mask < 0 || mask > 63 || (threads != 128 && threads != 256 && threads != 512)) return false;
     g_mask.store(mask, std::memory_order_relaxed);
     g_threads.store(threads, std::memory_order_relaxed);
     g_threads_chosen.store(true, std::memory_order_relaxed);
@@ -2424,21 +2495,21 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
     else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
 }
 
-// one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE, PUSH> with nt threads a tile
-template <bool DOT, bool Q, bool SAVE, bool PUSH = false>
+// one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE, PUSH, PM> with nt threads a tile
+template <bool DOT, bool Q, bool SAVE, bool PUSH = false, int PM = 0>
 static void jacobi_at(int nt, const LaunchList& l, hipStream_t st, const TopoView& tp, const float* Xin, float* Xout,
                       const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0,
                       float P = 0.0f)
 {
     switch (nt) {
     case 128:
-        k::jacobi<128, DOT, Q, SAVE, PUSH><<<l.n, 128, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        k::jacobi<128, DOT, Q, SAVE, PUSH, PM><<<l.n, 128, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
         break;
     case 512:
-        k::jacobi<512, DOT, Q, SAVE, PUSH><<<l.n, 512, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        k::jacobi<512, DOT, Q, SAVE, PUSH, PM><<<l.n, 512, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
         break;
     default:
-        k::jacobi<256, DOT, Q, SAVE, PUSH><<<l.n, 256, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        k::jacobi<256, DOT, Q, SAVE, PUSH, PM><<<l.n, 256, 0, st>>>(tp, l.rec, Xin, Xout, G, C, B, w, S, partial, X0, P);
         break;
     }
 }
@@ -2449,9 +2520,9 @@ bool jacobi_push_available(const DevGrid& d, int level)
 }
 
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
-                  const double* S, hipStream_t st, double* partial, float* X0, bool push, float P)
+                  const double* S, hipStream_t st, double* partial, float* X0, bool push, float P, int pull)
 {
-    if (X0 && partial && !push) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
+    if (X0 && partial && !push && !pull) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return 0;
     const bool q = C.q[0] != nullptr;
@@ -2461,6 +2532,29 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
     const TopoView tp = view(d);
     const int nt = q ? q_jacobi_threads() : stencil_threads();
     S = one_tile ? run_flag(d, S) : S;
+    if (pull) {
+        // a post-sweep of a level whose prolongation it pulls in (X0: the FAS-saved x)
+        if (!X0 || push || !one_tile || pull < 1 || pull > 2)
+            throw std::invalid_argument("jacobi: the pulled prolongation is a one-tile post-sweep reading X0");
+        if (pull == 1) {
+            if (partial) {
+                if (q) jacobi_at<true, true, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+                else jacobi_at<true, false, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+                return l.n;
+            }
+            if (q) jacobi_at<false, true, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+            else jacobi_at<false, false, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+            return 0;
+        }
+        if (partial) {
+            if (q) jacobi_at<true, true, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+            else jacobi_at<true, false, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+            return l.n;
+        }
+        if (q) jacobi_at<false, true, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+        else jacobi_at<false, false, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+        return 0;
+    }
     if (push) {
         // the last post-sweep with the prolongation to level + 1 (X0: the FAS-saved x)
         if (!X0 || !jacobi_push_available(d, level))
@@ -2779,6 +2873,7 @@ bool derive_allowed() { return !(persistent_mask() & 8); }
 bool fuse_first_sweep_allowed() { return !(persistent_mask() & 16); }
 bool fuse_fas_save_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 bool fuse_rz_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
+bool pull_prolong_allowed() { return !(persistent_mask() & (16 | 32 | PERSIST_JACOBI)); }
 
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
                            const double* S, hipStream_t st, unsigned int* count)

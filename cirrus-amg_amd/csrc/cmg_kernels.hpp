@@ -15,7 +15,7 @@ struct TopoView {
     const int32_t* child;
     const uint8_t* ctype;
     const int32_t* halo;    // 8 per tile: face neighbours, packed neighbour tile types
-    const int32_t* ghalo;   // 8 per tile: parents of GHOST face neighbours, packed octants
+    const int32_t* ghalo;   // 8 per tile: parents of the face neighbours, packed octants
     float h0;
     int derive;   // LEAF-tile diagonals are derived from the face coefficients (verified equal)
 };
@@ -46,9 +46,13 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 // X0: the level's FAS save on its NONLEAF tiles in the same pass (B updated in place, x -> X0)
 // push: instead, the FAS prolongation to level + 1 in the same pass (a last post-sweep that
 // writes the cycle's x; X0 is then the FAS-saved x it reads; P the prolongation weight)
+// pull 1 / 2: a post-sweep that pulls the prolongation into this level in (the first one:
+// own, same-level and GHOST neighbour cells; the later ones: GHOST neighbour cells), so that
+// fas_prolong_v need not run; X0 is the FAS-saved x, P the prolongation weight.  The level's
+// GHOST tiles then keep their x (the cycle's output there is not the prolongated one).
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
                   const double* S, hipStream_t st, double* partial = nullptr, float* X0 = nullptr, bool push = false,
-                  float P = 0.0f);
+                  float P = 0.0f, int pull = 0);
 // the prolongation to level + 1 can ride on level's last Jacobi sweep (every tile of level + 1
 // has a LEAF or NONLEAF parent, one-tile Jacobi form)
 bool jacobi_push_available(const DevGrid& d, int level);
@@ -156,6 +160,7 @@ bool derive_allowed();             // forms bit 3 clear
 bool fuse_first_sweep_allowed();   // forms bit 4 clear
 bool fuse_rz_allowed();            // forms bit 4 clear and the one-tile Jacobi form
 bool fuse_fas_save_allowed();      // forms bit 4 clear and the one-tile Jacobi form
+bool pull_prolong_allowed();       // forms bits 4 and 5 clear and the one-tile Jacobi form
 
 // reference tile images (PoissonTile<float,15>) <-> this grid, device to device (cmg_tiles.hip):
 // tile t at ptrs[t] (device array of N addresses) or at base + t * stride; returns the number

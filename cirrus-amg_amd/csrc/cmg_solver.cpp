@@ -272,9 +272,10 @@ void dev_upload(DevGrid& d, const Topology& topo)
                 halo[size_t(t) * 8 + f] = nb;
                 if (f < 4) ty03 |= ty << (8 * f);
                 else ty45 |= ty << (8 * (f - 4));
-                const bool gh = nb >= 0 && (ty & GHOST);
-                ghalo[size_t(t) * 8 + f] = gh ? topo.parent[nb] : -1;
-                if (gh) goct |= uint32_t(oct[nb] & 7) << (3 * f);
+                // every face neighbour's parent and octant: GHOST values of the operator, the
+                // pulled prolongation of the cycle's finest post-sweeps
+                ghalo[size_t(t) * 8 + f] = nb >= 0 ? topo.parent[nb] : -1;
+                if (nb >= 0) goct |= uint32_t(oct[nb] & 7) << (3 * f);
             }
             halo[size_t(t) * 8 + 6] = int32_t(ty03);
             halo[size_t(t) * 8 + 7] = int32_t(ty45 | (uint32_t(topo.t_type[t]) << 24));
@@ -804,8 +805,10 @@ int rz_offset(const DevGrid& d, int level)
 // `fas_save`: the level's FAS save is still due and rides on the first sweep;
 // `push`: the FAS prolongation to level + 1 comes right after these sweeps and may ride
 // on the last one.  Returns whether it did.
+// `pull`: the FAS prolongation into this level has not been written; the sweeps compute it
+// on the fly (the first one for every cell it reads, the later ones for GHOST neighbours)
 bool smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = false, bool fas_save = false,
-            bool push = false)
+            bool push = false, bool pull = false)
 {
     DevGrid& d = c.s->g->dev;
     hipStream_t st = d.stream;
@@ -832,7 +835,7 @@ bool smooth(const CycleCtx& c, int level, int iters, int order, bool x_zero = fa
             const bool pu = push && it == iters - 1;
             launch_jacobi(d, level, cur, other, c.X, c.C, c.B, float(c.o->omega), c.S, st,
                           last ? c.rz + rz_offset(d, level) : nullptr,
-                          (fas_save && it == 0) || pu ? c.X0 : nullptr, pu, c.s->P);
+                          (fas_save && it == 0) || pu || pull ? c.X0 : nullptr, pu, c.s->P, pull ? (it == 0 ? 1 : 2) : 0);
         }
         std::swap(cur, other);
     }
@@ -913,11 +916,17 @@ bool cycle_step(const CycleCtx& c, int level, bool x_zero, bool fas_save = false
     // damped Jacobi: the coarser level's FAS save rides on its first pre-sweep
     const bool fused = c.o->smoother == CMG_SMOOTH_JACOBI && c.o->level_iters >= 1 && fuse_fas_save_allowed();
     if (!fused) launch_fas_save(d, level - 1, c.X, c.C, c.B, c.X0, c.S, st);
+    // In a PCG solve (the r.z fusion is on) the cycle's output is read on LEAF tiles only, so
+    // the prolongation into the finest level is left to its post-sweeps, which compute it on
+    // the fly: no launch of its own and no sweep of the coarser level carrying it; the
+    // finest GHOST tiles then keep their zeros.
+    const bool pull = c.rz && level == d.max_level && c.o->smoother == CMG_SMOOTH_JACOBI && c.o->level_iters >= 1 &&
+                      pull_prolong_allowed();
     const int mu = cycle_mu(c.o);
     bool pushed = false;
-    for (int i = 0; i < mu; ++i) pushed = cycle_step(c, level - 1, false, fused && i == 0, i == mu - 1);
-    if (!pushed) launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
-    return smooth(c, level, c.o->level_iters, 1, false, false, push_up);
+    for (int i = 0; i < mu; ++i) pushed = cycle_step(c, level - 1, false, fused && i == 0, !pull && i == mu - 1);
+    if (!pushed && !pull) launch_fas_prolong(d, level, c.X, c.X0, c.s->P, c.S, st);
+    return smooth(c, level, c.o->level_iters, 1, false, false, push_up, pull);
 }
 
 // AMGSolver::FASFCycle (src/AMGSolver.cu:1358-1387)

@@ -454,14 +454,17 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * finest-level sweep from x = 0 into the x/r update, sum r.z over each level's
  * LEAF tiles in that level's last post-sweep, which regroups the FP64 partial
  * sums of r.z, do a coarse level's FAS save in its first pre-sweep and the FAS
- * prolongation to the next finer level in its last post-sweep).
+ * prolongation to the next finer level in its last post-sweep); bit 5 set keeps
+ * the prolongation into the finest level out of the finest post-sweeps (by
+ * default a PCG solve's finest post-sweeps compute it on the fly; the cycle's
+ * output on GHOST tiles of the finest level, which PCG never reads, then keeps 0).
  * stencil_threads: 128, 256 or 512 threads per tile.  Stencil results are
  * bit-identical across forms; PCG scalars differ only by the grouping of FP64
  * partial sums.  Default: mask 0, 256 threads (the Jacobi sweep on FP16 coefficient
  * copies: 128 until a thread count is set here or by CMG_STENCIL_THREADS).  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
-cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..31 */
+cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..63 */
 cmg_status cmg_get_kernel_forms(int* persistent_mask, int* stencil_threads);
 
 /* hipDeviceSynchronize on the current device (timing brackets) */

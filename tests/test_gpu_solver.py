@@ -427,3 +427,30 @@ def test_true_residual_can_be_skipped():
     assert np.array_equal(H.bits(out[0][3]), H.bits(out[1][3]))
     assert np.array_equal(H.bits(out[0][3]), H.bits(out[2][3]))
     assert 0 < out[0][2] < 2e-6 and out[1][2] == -1 and out[2][2] == out[0][2]
+
+
+@pytest.mark.parametrize("case", ["sphere_empty24", "c2_small", "c3_small"])
+@pytest.mark.parametrize("q16", [0, 1])
+def test_pulled_prolongation_changes_no_bit(case, q16):
+    """By default a PCG solve's finest post-sweeps compute the FAS prolongation into the
+    finest level on the fly (forms bit 5 set: fas_prolong_v writes it first).  The LEAF
+    values the sweeps see are the same bits either way, so the whole solve is: the same
+    iterations, the same residual, the same solution bits."""
+    print("\npulled prolongation", case, q16, flush=True)
+    go, gp = H.build_pair(case)
+    b = _rhs(go, gp, 27)
+    opts = P.default_opts(cycle=P.CYCLE_FAS_V, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1,
+                          cycle_coef_fp16=q16)
+    saved = P.get_kernel_forms()
+    out = []
+    try:
+        solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+        for mask in (0, 32):
+            P.set_kernel_forms(mask, saved[1])
+            gp.set_channel(1, b)
+            it, rel = solver.solve(max_iters=100, relative_tolerance=1e-6, coarsest_iters=0, opts=opts)
+            out.append((it, rel, gp.channel(0).copy()))
+    finally:
+        P.set_kernel_forms(*saved)
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1], (out[0][:2], out[1][:2])
+    assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
