@@ -20,6 +20,8 @@ CASES = {
     "center_point14": (O.TARGET_CENTER_POINT, 1, 4, 0, (), None, "plain"),
     "c2_small": (O.TARGET_SDF_BANDS, 2, 4, O.SDF_SPHERE, SPHERE_C2, {4: 0.05, 3: 0.15}, "solid"),
     "c3_small": (O.TARGET_SDF_BANDS, 2, 5, O.SDF_TORUS, TORUS_C3, {5: 0.02, 4: 0.04, 3: 0.08}, "supersampled"),
+    # the C2 octree itself (8,217 tiles; its finest level is large enough for sibling blocks)
+    "c2_full": (O.TARGET_SDF_BANDS, 3, 5, O.SDF_SPHERE, SPHERE_C2, {5: 0.05, 4: 0.15}, "solid"),
 }
 
 

@@ -11,7 +11,7 @@ import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-KERNEL_CASES = ["uniform3", "sphere_empty24", "sphere_solid24", "c2_small", "c3_small", "center_point14"]
+KERNEL_CASES = ["uniform3", "sphere_empty24", "sphere_solid24", "c2_small", "c3_small", "center_point14", "c2_full"]
 
 
 @pytest.fixture(scope="module", params=KERNEL_CASES)

@@ -30,7 +30,7 @@ def _opts(kind):
     raise KeyError(kind)
 
 
-@pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c3_small"])
+@pytest.mark.parametrize("case", ["uniform3", "sphere_empty24", "c3_small", "c2_full"])
 @pytest.mark.parametrize("kind,mask", [("ref_w", 0), ("ref_w", 16), ("v_jacobi_gs", 0), ("v_jacobi_gs", 16),
                                        ("w_jacobi_gs", 0), ("w_jacobi_gs", 16)])
 def test_cycle_bitexact(case, kind, mask):
