@@ -77,6 +77,9 @@ struct DevGrid {
     LaunchList ghosts;                  // all GHOST tiles
     LaunchList leafghost;               // all LEAF and GHOST tiles
     LaunchList cyclezero;               // GHOST tiles and LEAF tiles below the finest level
+    LaunchList cyclezero_pull;          // the same without the finest GHOST tiles (a cycle that pulls its
+                                        // finest prolongation never writes them)
+    LaunchList finest_ghosts;           // GHOST tiles of the finest level
     int max_list = 0;
     int wg_cap = 2048;                  // persistent grid cap: 8 workgroups per CU
     int derive = 0;                     // set by a solve whose operator passed the diagonal check

@@ -454,3 +454,30 @@ def test_pulled_prolongation_changes_no_bit(case, q16):
         P.set_kernel_forms(*saved)
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1], (out[0][:2], out[1][:2])
     assert np.array_equal(H.bits(out[0][2]), H.bits(out[1][2]))
+
+
+@pytest.mark.parametrize("case", ["uniform3", "c3_small"])
+@pytest.mark.parametrize("cycle,iters", [(P.CYCLE_FAS_V, 1), (P.CYCLE_FAS_V, 3), (P.CYCLE_FAS_W, 2), (P.CYCLE_FAS_W, 3)])
+def test_fused_cycle_forms_match_unfused(case, cycle, iters):
+    """Every fused form of the FAS cycle with damped Jacobi (the FAS save in a coarse level's
+    first sweep, the prolongation carried by the coarser level's last sweep, levels 1-0 in one
+    workgroup around the direct solve) against the unfused launches (forms bit 4), bit for bit,
+    for odd sweep counts (the ping-pong ends in the scratch buffer) and for W-cycles (two coarse
+    visits, the direct solve run twice between the bottom workgroup's phases)."""
+    go, gp = H.build_pair(case)
+    _rhs(go, gp, 31)
+    popts = P.default_opts(cycle=cycle, smoother=P.SMOOTH_JACOBI, omega=2.0 / 3.0, coarsest_direct=1)
+    popts.level_iters = iters
+    saved = P.get_kernel_forms()
+    out = []
+    try:
+        solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+        for mask in (0, 16):
+            P.set_kernel_forms(mask, saved[1])
+            gp.set_channel(1, go.channel(1))
+            solver.cycle(popts, 4, 1, 3)
+            out.append(gp.channel(4).copy())
+    finally:
+        P.set_kernel_forms(*saved)
+    assert np.isfinite(out[0]).all()
+    assert np.array_equal(H.bits(out[0]), H.bits(out[1])), (case, cycle, iters)
