@@ -141,7 +141,11 @@ KERNELS = {0: ("apply (y = A p on LEAF tiles + p.y partials)", 25.0),
            8: ("plain copy", 8.0),
            9: ("learned smoother inference, finest level (BF16 MFMA)", 8.0),
            10: ("flat copy 1 GiB -> 1 GiB", 8.0),
-           11: ("damped-Jacobi sweep, finest level, FP16 coefficient copies", 21.0)}
+           11: ("damped-Jacobi sweep, finest level, FP16 coefficient copies", 21.0),
+           # coded coefficients, LEAF diagonals derived: x, b, x_out and one word (type + 3 codes)
+           12: ("damped-Jacobi sweep, finest level, coded coefficients", 16.0),
+           13: ("apply (y = A p on LEAF tiles + p.y partials), coded coefficients", 12.0),
+           14: ("residual + restrict, finest level, coded coefficients", 13.0)}
 # learned smoother: 2 * 27 taps * (1*16 + 16*16 + 16*1) channel pairs * 1000 volume
 # positions per tile ("same" padding on the halo'd 10^3 volume)
 NET_FLOP_PER_TILE = 2.0 * 27 * (16 + 256 + 16) * 1000

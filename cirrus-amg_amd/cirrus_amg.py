@@ -86,7 +86,7 @@ EXPORTS = [
     "cmg_grid_build_device", "cmg_op_norm_noalloc",
     "cmg_grid_set_channel_device", "cmg_grid_get_channel_device", "cmg_grid_gather_tile_channel",
     "cmg_grid_scatter_tile_channel", "cmg_grid_gather_tile_types", "cmg_grid_scatter_tile_types",
-    "cmg_mark_region_of_interest", "cmg_grid_get_roi",
+    "cmg_mark_region_of_interest", "cmg_grid_get_roi", "cmg_solver_coded_levels",
 ]
 
 _lib = None
@@ -121,6 +121,7 @@ def lib():
             "cmg_solver_create": (st, [vp, i, f, f, f, C.POINTER(vp)]), "cmg_solver_destroy": (None, [vp]),
             "cmg_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_solver_get_x64": (st, [vp, vp]),
+            "cmg_solver_coded_levels": (st, [vp, C.POINTER(C.c_uint32)]),
             "cmg_op_full_neg_lap": (st, [vp, i, i, i]), "cmg_op_propagate_ghosts": (st, [vp, i]),
             "cmg_op_accumulate_leafs": (st, [vp, i]), "cmg_op_gs_color": (st, [vp, i, i, i, i, i, d]),
             "cmg_op_jacobi": (st, [vp, i, i, i, i, i, i, f]), "cmg_op_residual_restrict": (st, [vp, i, i, i, i, f]),
@@ -534,6 +535,12 @@ class AMGSolver:
         a = np.zeros((self.grid.N, 512), np.float64)
         _check(lib().cmg_solver_get_x64(self.h, _ptr(a)), "get_x64")
         return a
+
+    def coded_levels(self):
+        """bit mask of the levels whose face coefficients the stencil kernels read as codes"""
+        m = C.c_uint32()
+        _check(lib().cmg_solver_coded_levels(self.h, C.byref(m)), "cmg_solver_coded_levels")
+        return m.value
 
     def apply(self, x, y):
         """y = A x with this solver's operator on LEAF tiles."""

@@ -729,6 +729,7 @@ cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats*
 }
 
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host) { return solver_get_x64(s, host); }
+cmg_status cmg_solver_coded_levels(cmg_solver* s, uint32_t* mask) { return solver_coded_levels(s, mask); }
 
 cmg_status cmg_op_full_neg_lap(cmg_grid* g, int x, int c0, int y)
 {
@@ -825,7 +826,7 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
 cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads)
 {
     if (!set_kernel_forms(persistent_mask, stencil_threads)) {
-        set_error("kernel forms: mask must be 0..63 and threads 128, 256 or 512");
+        set_error("kernel forms: mask must be 0..127 and threads 128, 256 or 512");
         return CMG_ERR_BAD_ARG;
     }
     return CMG_OK;

@@ -190,6 +190,7 @@ cmg_status trainer_weights(cmg_trainer* t, float* get, const float* set);
 cmg_status trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad);
 cmg_status trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss);
 cmg_status solver_get_x64(cmg_solver* s, double* host);
+cmg_status solver_coded_levels(cmg_solver* s, uint32_t* mask);
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 cmg_status op_copy(cmg_grid* g, int src, int dst);
 }  // namespace cmg

@@ -32,14 +32,20 @@ namespace cmg {
 namespace k {
 
 
-// Q: the kernel reads the FP16 coefficient copies (c / h); the coefficient planes are then
-// staged as those FP16 bits (7.5 KB a tile instead of 10.9, so more tiles fit on a CU)
-template <typename T, bool Q = false>
+// Coefficient forms of the one-tile stencil kernels (template argument Q):
+//   0: the FP32 channels;
+//   1: the FP16 copies (c / h); the coefficient planes are then staged as those FP16 bits
+//      (7.5 KB a tile instead of 10.9, so more tiles fit on a CU);
+//   2: the exact per-level codes (Coef::w, Coef::lut): a cell's type and three face codes
+//      in one 32-bit word, decoded through the level's table, which is staged in LDS; the
+//      staged planes and every value computed are those of form 0.
+template <typename T, int Q = 0>
 struct Lds {
     T x[1000];
     // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
-    std::conditional_t<Q, uint16_t, float> off[3][576];
+    std::conditional_t<Q == 1, uint16_t, float> off[3][576];
     uint8_t empty[6];    // face f = 2 axis + (side > 0) has no neighbour tile
+    float lut[Q == 2 ? kCodes : 1];   // form 2: the tile level's code table
 };
 
 // a staged coefficient as FP32: FP32 as is, FP16 bits of c / h times h (exact: h is a
@@ -50,9 +56,10 @@ __device__ __forceinline__ float lds_coef(uint16_t v, float h) { return float(__
 // diagonal of a LEAF INTERIOR cell from its face coefficients, in the order the
 // assembly sums them (CoarsenTypesAndAMGCoeffs, src/AMGSolver.cu:129-160):
 // d = 0 - c-(x) - c+(x) - c-(y) - c+(y) - c-(z) - c+(z), c+ = -h without neighbour
-template <typename T>
-__device__ __forceinline__ float derived_diag(const Lds<T, false>& s, int v, const float c[3], float h)
+template <typename T, int LQ>
+__device__ __forceinline__ float derived_diag(const Lds<T, LQ>& s, int v, const float c[3], float h)
 {
+    static_assert(LQ != 1, "the FP16 form reads every diagonal");
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
     const float cp0 = i < 7 ? s.off[0][v + 64] : (s.empty[1] ? -h : s.off[0][512 + (j * 8 + k)]);
     const float cp1 = j < 7 ? s.off[1][v + 8] : (s.empty[3] ? -h : s.off[1][512 + (i * 8 + k)]);
@@ -242,13 +249,18 @@ struct Pull {
     const float* X0;
     float P;
 };
-template <bool FRESH, typename T, int NT, bool Q = false, typename Extra = NoExtraLoads, typename PullT = Pull<0>>
+// Q 2: W and L are the coded form's cell words and tables (Coef::w, Coef::lut); the LDS
+// may be a form-2 one for a form-0 load (a kernel that takes either form per tile).
+template <bool FRESH, typename T, int NT, int Q = 0, typename Extra = NoExtraLoads, typename PullT = Pull<0>, int LQ>
 __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, float h, const T* __restrict__ X,
-                                          const T* __restrict__ G, const float* const* C, Lds<T, Q>& s, Own<T, NT>& o,
+                                          const T* __restrict__ G, const float* const* C, Lds<T, LQ>& s, Own<T, NT>& o,
                                           const uint16_t* const* Qc = nullptr, const Extra& extra = Extra{},
-                                          const PullT& pull = PullT{nullptr, 0.0f})
+                                          const PullT& pull = PullT{nullptr, 0.0f}, const uint32_t* __restrict__ W = nullptr,
+                                          const float* __restrict__ L = nullptr)
 {
     constexpr int PM = PullT::mode;
+    constexpr bool K = Q == 2;   // coded coefficients
+    static_assert((Q == 1) == (LQ == 1) && (!K || LQ == 2), "the LDS layout follows the coefficient form");
     static_assert(PM == 0 || (!FRESH && sizeof(T) == 4), "a pulled prolongation is a cycle (FP32) form");
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
@@ -258,25 +270,44 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     // type and (FRESH) the parents and octants of GHOST neighbours
     const int4 ha = r.ha, hb = r.hb;
     const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
-    const bool dv = !Q && tp.derive && ((ty45 >> 24) & LEAF);   // the Q form reads every diagonal
+    const bool dv = Q != 1 && tp.derive && ((ty45 >> 24) & LEAF);   // the FP16 form reads every diagonal
+    // K: the level's code table goes out first (its wait then covers no other load)
+    constexpr int LPT = NT >= kCodes ? 1 : kCodes / NT;
+    float lutv[K ? LPT : 1];
+    const bool lut_lane = NT <= kCodes || tid < kCodes;
+    if constexpr (K) {
+        if (lut_lane) vload<LPT>(L + size_t(r.level) * kCodes + size_t(LPT) * tid, lutv);
+    }
     // the coefficient planes as values (a runtime-indexed C[axis] would be a load)
     const float* const cp0 = uniform_ptr(C[0]);
     const float* const cp1 = uniform_ptr(C[1]);
     const float* const cp2 = uniform_ptr(C[2]);
-    const uint16_t* const qp0 = Q ? uniform_ptr(Qc[0]) : nullptr;
-    const uint16_t* const qp1 = Q ? uniform_ptr(Qc[1]) : nullptr;
-    const uint16_t* const qp2 = Q ? uniform_ptr(Qc[2]) : nullptr;
+    const uint16_t* const qp0 = Q == 1 ? uniform_ptr(Qc[0]) : nullptr;
+    const uint16_t* const qp1 = Q == 1 ? uniform_ptr(Qc[1]) : nullptr;
+    const uint16_t* const qp2 = Q == 1 ? uniform_ptr(Qc[2]) : nullptr;
     const int4 ga = r.ga, gb = r.gb;
     const uint32_t goct = uint32_t(gb.z);
     constexpr int PC = PM == 1 ? CPT : 1;
     T own_xr[PC], own_px[PC], own_p0[PC];   // PM 1: own x and parent values, combined after the halo loads
+    uint32_t ow[K ? CPT : 1];                // K: own cell words, decoded once the table is in LDS
     {
         // own cells: one vector load per array
         const size_t v0 = base + size_t(CPT) * tid;
         T xr[CPT];
-        vload<CPT>(tp.ctype + v0, o.ct);
+        if constexpr (K) {
+            vload<CPT>(W + v0, ow);
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) o.ct[c] = uint8_t(ow[c]);
+        } else {
+            vload<CPT>(tp.ctype + v0, o.ct);
+        }
         vload<CPT>(X + v0, xr);
-        if constexpr (Q) {
+        if constexpr (K) {
+            float d3[CPT];
+            if (!dv) vload<CPT>(C[3] + v0, d3);
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) o.d[c] = dv ? 0.0f : d3[c];
+        } else if constexpr (Q == 1) {
             uint16_t q[4][CPT];
             vload<CPT>(qp0 + v0, q[0]);
             vload<CPT>(qp1 + v0, q[1]);
@@ -353,7 +384,9 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         // a - face's coefficient is the cell's own: the + face reads the neighbour's, a - face
         // re-reads the tile's own first 64 (cached) so that the load is unconditional
         const size_t ci = sgn > 0 ? ni : base + fi;
-        if constexpr (Q) hcb[m] = (axis == 0 ? qp0 : axis == 1 ? qp1 : qp2)[ci];
+        // K: the neighbour cell's word, its type and its + face code in one load
+        if constexpr (K) hcb[m] = W[ni];
+        else if constexpr (Q == 1) hcb[m] = (axis == 0 ? qp0 : axis == 1 ? qp1 : qp2)[ci];
         else hcb[m] = __float_as_uint((axis == 0 ? cp0 : axis == 1 ? cp1 : cp2)[ci]);
         const bool ghost = nty & GHOST;
         size_t xi = ni;
@@ -390,11 +423,16 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         }
         hraw[m] = xs[xi];
         // the type is needed for same-level and NONLEAF neighbours (PM: GHOST ones too); other
-        // cells re-read the tile's own
-        hct[m] = tp.ctype[(kind == 1 || kind == 4 || (PM && kind == 2)) ? ni : base + fi];
+        // cells re-read the tile's own (K: it came with the cell word)
+        if constexpr (!K) hct[m] = tp.ctype[(kind == 1 || kind == 4 || (PM && kind == 2)) ? ni : base + fi];
         hkind[m] = kind;
         hni[m] = int(ni - size_t(ok ? nt : r.t) * 512) | ((ok ? nt : r.t) << 9);
     }
+    // a halo cell's type, read once its load is complete
+    auto htype = [&](int m) -> uint32_t {
+        if constexpr (K) return hcb[m] & 0xffu;
+        else return hct[m];
+    };
     extra();
     if constexpr (PM == 1) {
         const bool hp = r.parent >= 0;
@@ -413,7 +451,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
             const int nt = hni[m] >> 9, cl = hni[m] & 511;
             const size_t ni = size_t(nt) * 512 + cl;
             T val = T(0);
-            if (hct[m] == INTERIOR) {
+            if (htype(m) == INTERIOR) {
                 const int n0 = cl >> 6, n1 = (cl >> 3) & 7, n2 = cl & 7;
                 const int oct = (n0 >> 2) | ((n1 >> 2) << 1) | ((n2 >> 2) << 2);
                 const int ctile = tp.child[size_t(nt) * 8 + oct];
@@ -440,23 +478,42 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         const int v = own_cell<NT>(c, tid);
         s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
     }
+    if constexpr (K) {
+        // the table now; the own coefficient planes once it is visible
+        if (lut_lane) vstore<LPT>(&s.lut[LPT * tid], lutv);
+    } else {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if constexpr (Q) {
-            // back to the FP16 bits: c / h is exact, and so is its conversion
-            const float ih = 1.0f / h;
-            uint16_t oc[CPT];
+        for (int a = 0; a < 3; ++a) {
+            if constexpr (Q == 1) {
+                // back to the FP16 bits: c / h is exact, and so is its conversion
+                const float ih = 1.0f / h;
+                uint16_t oc[CPT];
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) oc[c] = __builtin_bit_cast(uint16_t, _Float16(o.c[c][a] * ih));
-            vstore<CPT>(&s.off[a][CPT * tid], oc);
-        } else {
-            float oc[CPT];
+                for (int c = 0; c < CPT; ++c) oc[c] = __builtin_bit_cast(uint16_t, _Float16(o.c[c][a] * ih));
+                vstore<CPT>(&s.off[a][CPT * tid], oc);
+            } else {
+                float oc[CPT];
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) oc[c] = o.c[c][a];
-            vstore<CPT>(&s.off[a][CPT * tid], oc);
+                for (int c = 0; c < CPT; ++c) oc[c] = o.c[c][a];
+                vstore<CPT>(&s.off[a][CPT * tid], oc);
+            }
         }
     }
     __syncthreads();
+    if constexpr (K) {
+        // own face coefficients from the codes (bytes 1..3 of the cell word); phase 2 below
+        // reads no coefficient plane, and the planes are read after the next barrier
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float oc[CPT];
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+                oc[c] = s.lut[(ow[c] >> (8 * a + 8)) & 0xffu];
+                o.c[c][a] = oc[c];
+            }
+            vstore<CPT>(&s.off[a][CPT * tid], oc);
+        }
+    }
 #pragma unroll
     for (int m = 0; m < HPT; ++m) {
         const int k = tid + m * NT;
@@ -470,12 +527,12 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         else if (axis == 1) { fl[0] = p; fl[1] = axk; fl[2] = q; }
         else { fl[0] = p; fl[1] = q; fl[2] = axk; }
         // halo, phase 2: the loaded values
-        T hv = hkind[m] == 0 || hkind[m] == 3 || (hkind[m] == 1 && hct[m] != INTERIOR) ? T(0) : hraw[m];
+        T hv = hkind[m] == 0 || hkind[m] == 3 || (hkind[m] == 1 && htype(m) != INTERIOR) ? T(0) : hraw[m];
         if constexpr (PM != 0) {
-            if (hkind[m] == 2 && !(hct[m] & INTERIOR)) hv = T(0);
+            if (hkind[m] == 2 && !(htype(m) & INTERIOR)) hv = T(0);
         }
         if constexpr (PM == 1) {
-            if (hkind[m] == 1 && hct[m] == INTERIOR) {
+            if (hkind[m] == 1 && htype(m) == INTERIOR) {
                 const int npt = pick6(ga, gb, (f >> 1) + ((f & 1) ? 3 : 0));
                 const T c1 = npt >= 0 ? hpx[m] : T(0), c0 = npt >= 0 ? hp0[m] : T(0);
                 hv = hraw[m] + pull.P * (c1 - c0);
@@ -500,12 +557,13 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         s.x[SX(fl[0], fl[1], fl[2])] = hv;
         if (sgn > 0) {
             // no neighbour tile: h (the halo value there is 0); FP16 1.0 times h in the Q form
-            if constexpr (Q) s.off[axis][512 + fi] = hkind[m] != 0 ? uint16_t(hcb[m]) : uint16_t(0x3C00);
+            if constexpr (Q == 1) s.off[axis][512 + fi] = hkind[m] != 0 ? uint16_t(hcb[m]) : uint16_t(0x3C00);
+            else if constexpr (K) s.off[axis][512 + fi] = hkind[m] != 0 ? s.lut[(hcb[m] >> (8 * axis + 8)) & 0xffu] : h;
             else s.off[axis][512 + fi] = hkind[m] != 0 ? __uint_as_float(hcb[m]) : h;
         }
     }
     __syncthreads();
-    if constexpr (!Q) {
+    if constexpr (Q != 1) {
         if (dv) {
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
@@ -518,8 +576,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
 
 // AMGLaplacianTileData::negativeLap (src/AMGSolver.cu:345-382) for cell v
 // (h: the tile spacing, needed only to read FP16 staged coefficients)
-template <typename T, bool Q>
-__device__ __forceinline__ T neg_lap(const Lds<T, Q>& s, int v, T x0, const float c[3], float d, float h = 0.0f)
+template <typename T, int LQ>
+__device__ __forceinline__ T neg_lap(const Lds<T, LQ>& s, int v, T x0, const float c[3], float d, float h = 0.0f)
 {
     if (d == 0) return T(0);
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
@@ -998,6 +1056,143 @@ __global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* 
     if (__syncthreads_or(nf) && threadIdx.x == 0) atomicAdd(bad, 1u);
 }
 
+// ------------------------------------------------- exact coefficient codes ----
+// Per level, the distinct FP32 bit patterns of the three face coefficients are gathered in
+// an open-addressing set of kCodeSlots keys (code_collect).  A level with at most kCodes of
+// them numbers them in slot order and writes its table of the values (code_number); every
+// cell of such a level then gets its word type | code(c0) << 8 | code(c1) << 16 |
+// code(c2) << 24 (code_encode).  A table entry holds the coefficient's own bits, so a kernel
+// that decodes computes exactly what it computes from the FP32 channels.
+// Workspace (uint32): keys [kMaxLevels][kCodeSlots], slot codes [kMaxLevels][kCodeSlots],
+// then per level the key count, a "not codable" flag and the result (1: coded).
+constexpr int kCodeSlots = 1024;
+constexpr uint32_t kNoKey = 0xffffffffu;   // a NaN pattern; a coefficient with these bits leaves its level uncoded
+
+__device__ __forceinline__ uint32_t code_slot(uint32_t key) { return (key * 2654435761u) >> 22; }
+
+__device__ __forceinline__ void code_insert(uint32_t* __restrict__ set, unsigned int* count, unsigned int* bad,
+                                            uint32_t key)
+{
+    if (key == kNoKey) {
+        *bad = 1u;
+        return;
+    }
+    uint32_t sl = code_slot(key);
+    for (int p = 0; p < kCodeSlots; ++p, sl = (sl + 1) & (kCodeSlots - 1)) {
+        // a slot goes from kNoKey to its key once; a stale kNoKey read is settled by the CAS
+        const uint32_t cur = set[sl];
+        if (cur == key) return;
+        if (cur != kNoKey) continue;
+        const uint32_t prev = atomicCAS(&set[sl], kNoKey, key);
+        if (prev == kNoKey) {
+            atomicAdd(count, 1u);
+            return;
+        }
+        if (prev == key) return;
+    }
+    *bad = 1u;   // more distinct values than slots
+}
+
+__global__ __launch_bounds__(256) void code_collect(TopoView tp, const int32_t* __restrict__ list, int n, Coef C,
+                                                    uint32_t* __restrict__ ws)
+{
+    uint32_t* keys = ws;
+    unsigned int* count = ws + 2 * kMaxLevels * kCodeSlots;
+    unsigned int* bad = count + kMaxLevels;
+    CMG_TILE_LOOP(list, n)
+    {
+        const int l = tp.tlevel[t_];
+        // a level known to be uncodable needs no more work (a stale read only costs time)
+        if (bad[l] || count[l] > unsigned(kCodes)) continue;
+        const size_t q = quad_index(t_);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint4 v = reinterpret_cast<const uint4*>(C.c[a])[q];
+            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.x);
+            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.y);
+            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.z);
+            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.w);
+        }
+    }
+}
+
+// one workgroup per level: codes 0..n-1 to the occupied slots in slot order, the table
+// (unused entries 0), and the level's result
+__global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, float* __restrict__ lut)
+{
+    const int l = blockIdx.x;
+    const uint32_t* keys = ws + l * kCodeSlots;
+    uint32_t* slotcode = ws + kMaxLevels * kCodeSlots + l * kCodeSlots;
+    const unsigned int* count = ws + 2 * kMaxLevels * kCodeSlots;
+    const unsigned int* bad = count + kMaxLevels;
+    unsigned int* ok = ws + 2 * kMaxLevels * kCodeSlots + 2 * kMaxLevels;
+    const bool coded = !bad[l] && count[l] <= unsigned(kCodes);
+    for (int e = threadIdx.x; e < kCodes; e += blockDim.x) lut[l * kCodes + e] = 0.0f;
+    __syncthreads();
+    if (!coded) {
+        if (threadIdx.x == 0) ok[l] = 0u;
+        return;
+    }
+    __shared__ unsigned int part[256];
+    constexpr int SPT = kCodeSlots / 256;   // slots a thread numbers, consecutive
+    uint32_t k[SPT];
+    unsigned int mine = 0;
+#pragma unroll
+    for (int e = 0; e < SPT; ++e) {
+        k[e] = keys[threadIdx.x * SPT + e];
+        mine += k[e] != kNoKey;
+    }
+    part[threadIdx.x] = mine;
+    __syncthreads();
+    unsigned int first = 0;   // occupied slots before this thread's (256 partial sums, read directly)
+    for (int i = 0; i < int(threadIdx.x); ++i) first += part[i];
+#pragma unroll
+    for (int e = 0; e < SPT; ++e) {
+        if (k[e] == kNoKey) continue;
+        slotcode[threadIdx.x * SPT + e] = first;
+        lut[l * kCodes + first] = __uint_as_float(k[e]);
+        ++first;
+    }
+    if (threadIdx.x == 0) ok[l] = 1u;
+}
+
+// the cell words of the coded levels; a key missing from its set (it cannot be, the
+// collection inserted every key of a coded level) clears the level's result
+__global__ __launch_bounds__(256) void code_encode(TopoView tp, const int32_t* __restrict__ list, int n, Coef C,
+                                                   uint32_t* __restrict__ ws, uint32_t* __restrict__ W)
+{
+    const uint32_t* keys = ws;
+    const uint32_t* slotcode = ws + kMaxLevels * kCodeSlots;
+    unsigned int* ok = ws + 2 * kMaxLevels * kCodeSlots + 2 * kMaxLevels;
+    CMG_TILE_LOOP(list, n)
+    {
+        const int l = tp.tlevel[t_];
+        if (!ok[l]) continue;
+        const size_t q = quad_index(t_);
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        uint32_t w[4] = {ct.x, ct.y, ct.z, ct.w};
+        bool missing = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint4 v = reinterpret_cast<const uint4*>(C.c[a])[q];
+            const uint32_t key[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint32_t sl = code_slot(key[e]);
+                int p = 0;
+                while (p < kCodeSlots && keys[l * kCodeSlots + sl] != key[e]) {
+                    sl = (sl + 1) & (kCodeSlots - 1);
+                    ++p;
+                }
+                if (p == kCodeSlots) missing = true;
+                else w[e] |= slotcode[l * kCodeSlots + sl] << (8 * a + 8);
+            }
+        }
+        if (missing) ok[l] = 0u;
+        reinterpret_cast<uint4*>(W)[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 __global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
                                               float* __restrict__ B, const double* __restrict__ S)
 {
@@ -1063,12 +1258,13 @@ __global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int4* __
 
 // ------------------------------------------------- one tile per workgroup ----
 // y = A x on LEAF tiles, one tile per workgroup (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
-template <bool DOT, int NT>
+// K: tiles of the levels whose bit is set in C.coded read the coded coefficients (the same values)
+template <bool DOT, int NT, bool K = false>
 __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
                                                  Coef C, float* __restrict__ Y, double* __restrict__ partial,
                                                  const double* __restrict__ S)
 {
-    __shared__ Lds<float> s;
+    __shared__ Lds<float, K ? 2 : 0> s;
     __shared__ double red[8];
     const TileRec r = rec_tile<true>(rec, S);
     const int t = r.t;   // -1: no tile, -2: the solve has stopped
@@ -1077,7 +1273,11 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __rest
         return;
     }
     Own<float, NT> o;
-    load_tile<true>(tp, r, rec_h(tp, r), X, X, C.c, s, o);
+    if (K && ((C.coded >> r.level) & 1u))
+        load_tile<true, float, NT, K ? 2 : 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o, nullptr, NoExtraLoads{},
+                                             Pull<0>{nullptr, 0.0f}, C.w, C.lut);
+    else
+        load_tile<true, float, NT, 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o);
     constexpr int CPT = 512 / NT;
     const size_t v0 = size_t(t) * 512 + size_t(CPT) * threadIdx.x;
     float xr[CPT], y[CPT];
@@ -1194,7 +1394,7 @@ __global__ __launch_bounds__(NT) void learned_accept(TopoView tp, const int4* __
 // EARLY: the children's loads go out with the tile's (else after the sweep, with fewer
 // registers held across it)
 // PM: the prolongation into this level pulled into the sweep (Pull<PM>, X0 and P read)
-template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false, bool EARLY = true, int PM = 0>
+template <int NT, bool DOT = false, int Q = 0, bool SAVE = false, bool PUSH = false, bool EARLY = true, int PM = 0>
 __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
                                               const float* __restrict__ Xin, float* __restrict__ Xout,
                                               const float* __restrict__ G, const Coef& C, const float* __restrict__ B,
@@ -1241,7 +1441,7 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
         }
     };
     static_assert(!(PM && (SAVE || PUSH)), "a pulled prolongation is a post-sweep of the finest level");
-    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads, Pull<PM>{X0, P});
+    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads, Pull<PM>{X0, P}, C.w, C.lut);
     const bool save = SAVE && (r.meta & NONLEAF);
     const bool dot_tile = DOT && (r.meta & LEAF);   // b.x is r.z on LEAF tiles only
     double acc = 0;
@@ -1305,7 +1505,7 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
     return acc;
 }
 
-template <int NT, bool DOT = false, bool Q = false, bool SAVE = false, bool PUSH = false, int PM = 0>
+template <int NT, bool DOT = false, int Q = 0, bool SAVE = false, bool PUSH = false, int PM = 0>
 __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ Xin,
                                              float* __restrict__ Xout, const float* __restrict__ G, Coef C,
                                              const float* __restrict__ B, float w, const double* __restrict__ S,
@@ -1327,7 +1527,7 @@ __global__ __launch_bounds__(NT) void jacobi(TopoView tp, const int4* __restrict
 }
 
 // ResidualAndRestrictAMG128Kernel (src/AMGSolver.cu:896-964): parent b = R*sum8 r, parent x = sum8 x / 8
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __device__ __forceinline__ void residual_restrict_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
                                                        float* __restrict__ res, float* __restrict__ X, const Coef& C,
                                                        float* __restrict__ B, float Rr, bool act)
@@ -1338,7 +1538,7 @@ __device__ __forceinline__ void residual_restrict_tile(const TopoView& tp, const
     float b[CPT], rv[CPT];
     vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * tid, b);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
 #pragma unroll
     for (int c = 0; c < CPT; ++c) rv[c] = b[c] - neg_lap(s, own_cell<NT>(c, tid), o.x[c], o.c[c], o.d[c], h);
     vstore<CPT>(&res[CPT * tid], rv);
@@ -1363,7 +1563,7 @@ __device__ __forceinline__ void residual_restrict_tile(const TopoView& tp, const
     X[pidx] = xs / 8;
 }
 
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __global__ __launch_bounds__(NT) void residual_restrict(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X,
                                                         Coef C, float* __restrict__ B, float Rr,
                                                         const double* __restrict__ S)
@@ -1413,7 +1613,7 @@ __global__ __launch_bounds__(NT) void residual64(TopoView tp, const int4* __rest
 // ------------------------------------------------------------- smoothers ----
 // one colour of red-black SOR on subtree-LEAF tiles of a level, in place
 // (NegativeLaplacianAndGaussSeidelSameLevelAMG128Kernel, src/AMGSolver.cu:838-877)
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __device__ __forceinline__ void gs_color_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s, float* __restrict__ X,
                                               const Coef& C, const float* __restrict__ B, int color, double omega, bool act)
 {
@@ -1423,7 +1623,7 @@ __device__ __forceinline__ void gs_color_tile(const TopoView& tp, const TileRec&
     float bv[CPT];
     vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * tid, bv);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
     if (!act) return;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
@@ -1437,7 +1637,7 @@ __device__ __forceinline__ void gs_color_tile(const TopoView& tp, const TileRec&
     }
 }
 
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X, Coef C,
                                                const float* __restrict__ B, int color, double omega,
                                                const double* __restrict__ S)
@@ -1456,7 +1656,7 @@ __global__ __launch_bounds__(NT) void gs_color(TopoView tp, const int4* __restri
 // :838-877); else `iters` damped-Jacobi sweeps.  Same values as one launch per pass of
 // gs_color or jacobi (each pass re-reads x masked to INTERIOR cells, which is what the
 // LDS copy holds).
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __device__ __forceinline__ void coarsest_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s, float* __restrict__ X,
                                               const Coef& C, const float* __restrict__ B, int rbgs, int iters, int iters2,
                                               int order, double omega, float w, bool act)
@@ -1468,7 +1668,7 @@ __device__ __forceinline__ void coarsest_tile(const TopoView& tp, const TileRec&
     float bv[CPT], xn[CPT];
     vload<CPT>(B + v0, bv);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
     const int passes = rbgs ? 2 * (iters + iters2) : iters;
     if (passes == 0) return;
     for (int p = 0; p < passes; ++p) {
@@ -1496,7 +1696,7 @@ __device__ __forceinline__ void coarsest_tile(const TopoView& tp, const TileRec&
     if (act) vstore<CPT>(X + v0, xn);
 }
 
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __global__ __launch_bounds__(NT) void coarsest_iterate(TopoView tp, const int4* __restrict__ rec, float* __restrict__ X,
                                                        Coef C, const float* __restrict__ B, int rbgs, int iters,
                                                        int iters2, int order, double omega, float w,
@@ -1510,7 +1710,7 @@ __global__ __launch_bounds__(NT) void coarsest_iterate(TopoView tp, const int4* 
 
 // ------------------------------------------------------------- transfers ----
 // FASSaveSolutionAndUpdateSource128Kernel (src/AMGSolver.cu:1176-1214) on NONLEAF tiles
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __device__ __forceinline__ void fas_save_tile(const TopoView& tp, const TileRec& r, Lds<float, Q>& s,
                                               const float* __restrict__ X, const Coef& C, float* __restrict__ B,
                                               float* __restrict__ X0, bool act)
@@ -1522,7 +1722,7 @@ __device__ __forceinline__ void fas_save_tile(const TopoView& tp, const TileRec&
     float bv[CPT];
     vload<CPT>(B + v0, bv);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
     if (!act) return;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
@@ -1536,7 +1736,7 @@ __device__ __forceinline__ void fas_save_tile(const TopoView& tp, const TileRec&
     vstore<CPT>(B + v0, bv);
 }
 
-template <int NT, bool Q = false>
+template <int NT, int Q = 0>
 __global__ __launch_bounds__(NT) void fas_save(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
                                                Coef C, float* __restrict__ B, float* __restrict__ X0,
                                                const double* __restrict__ S)
@@ -2344,14 +2544,14 @@ static int persistent_mask()
     const char* e = getenv("CMG_PERSISTENT");
     // one tile per workgroup everywhere: faster than the persistent forms on MI355X at
     // every thread count (the persistent forms hold two tiles' loads in registers)
-    m = e ? (atoi(e) & 63) : 0;
+    m = e ? (atoi(e) & 127) : 0;
     g_mask.store(m, std::memory_order_relaxed);
     return m;
 }
 
 bool set_kernel_forms(int mask, int threads)
 {
-    if (mask < 0 || mask > 63 || (threads != 128 && threads != 256 && threads != 512)) return false;
+    if (mask < 0 || mask > 127 || (threads != 128 && threads != 256 && threads != 512)) return false;
     g_mask.store(mask, std::memory_order_relaxed);
     g_threads.store(threads, std::memory_order_relaxed);
     g_threads_chosen.store(true, std::memory_order_relaxed);
@@ -2429,15 +2629,31 @@ static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((
         }                                                                                     \
     } while (0)
 
-// the same with the FP16-coefficient form (second template argument true)
-#define CMG_NT_DISPATCH_Q(KERNEL, GRID, STREAM, ...)                                          \
+// the same with coefficient form F (second template argument: 1 FP16 copies, 2 codes)
+#define CMG_NT_DISPATCH_F(KERNEL, F, GRID, STREAM, ...)                                       \
     do {                                                                                      \
         switch (stencil_threads()) {                                                          \
-        case 128: KERNEL<128, true><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;       \
-        case 512: KERNEL<512, true><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;       \
-        default: KERNEL<256, true><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;       \
+        case 128: KERNEL<128, F><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;          \
+        case 512: KERNEL<512, F><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;          \
+        default: KERNEL<256, F><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         }                                                                                     \
     } while (0)
+
+// every form of a one-tile stencil kernel KERNEL<NT, Q>
+#define CMG_NT_DISPATCH_ANY(KERNEL, FORM, GRID, STREAM, ...)                                  \
+    do {                                                                                      \
+        if ((FORM) == 1) CMG_NT_DISPATCH_F(KERNEL, 1, GRID, STREAM, __VA_ARGS__);            \
+        else if ((FORM) == 2) CMG_NT_DISPATCH_F(KERNEL, 2, GRID, STREAM, __VA_ARGS__);       \
+        else CMG_NT_DISPATCH(KERNEL, GRID, STREAM, __VA_ARGS__);                             \
+    } while (0)
+
+// the coefficient form of a one-tile stencil launch on `level`: 1 the FP16 copies, 2 the
+// codes (set for that level), 0 the FP32 channels
+static int coef_form(const Coef& C, int level)
+{
+    if (C.q[0]) return 1;
+    return (C.w && level >= 0 && level < 32 && ((C.coded >> level) & 1u)) ? 2 : 0;
+}
 
 template <bool DOT, int NT>
 static int apply_launch(const DevGrid& d, const float* X, Coef C, float* Y, double* partial, const double* S, hipStream_t st)
@@ -2457,7 +2673,24 @@ int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* par
     if (!(persistent_mask() & PERSIST_APPLY)) {
         const TopoView tp = view(d);
         const double* Sr = run_flag(d, S);
-        if (partial) {
+        if (C.w && C.coded) {
+            // tiles of coded levels decode their coefficients (the same values)
+            double* const pp = partial;
+            switch (apply_threads()) {
+            case 128:
+                if (pp) k::apply_leaf<true, 128, true><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, pp, Sr);
+                else k::apply_leaf<false, 128, true><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr);
+                break;
+            case 512:
+                if (pp) k::apply_leaf<true, 512, true><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, pp, Sr);
+                else k::apply_leaf<false, 512, true><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr);
+                break;
+            default:
+                if (pp) k::apply_leaf<true, 256, true><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.rec, X, C, Y, pp, Sr);
+                else k::apply_leaf<false, 256, true><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr);
+                break;
+            }
+        } else if (partial) {
             switch (apply_threads()) {
             case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
             case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
@@ -2491,12 +2724,11 @@ void launch_gs_color(const DevGrid& d, int level, float* X, Coef C, const float*
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    if (C.q[0]) CMG_NT_DISPATCH_Q(k::gs_color, l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
-    else CMG_NT_DISPATCH(k::gs_color, l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
+    CMG_NT_DISPATCH_ANY(k::gs_color, coef_form(C, level), l.n, st, view(d), l.rec, X, C, B, color, omega, run_flag(d, S));
 }
 
 // one launch of the one-tile Jacobi form jacobi<NT, DOT, Q, SAVE, PUSH, PM> with nt threads a tile
-template <bool DOT, bool Q, bool SAVE, bool PUSH = false, int PM = 0>
+template <bool DOT, int Q, bool SAVE, bool PUSH = false, int PM = 0>
 static void jacobi_at(int nt, const LaunchList& l, hipStream_t st, const TopoView& tp, const float* Xin, float* Xout,
                       const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0,
                       float P = 0.0f)
@@ -2519,71 +2751,58 @@ bool jacobi_push_available(const DevGrid& d, int level)
     return level + 1 <= d.max_level && d.push_ok[level + 1] && !(persistent_mask() & (16 | PERSIST_JACOBI));
 }
 
+// the one-tile Jacobi launch of coefficient form Q (arguments checked by launch_jacobi)
+template <int Q>
+static int jacobi_one_tile(const LaunchList& l, int nt, hipStream_t st, const TopoView& tp, const float* Xin, float* Xout,
+                           const float* G, Coef C, const float* B, float w, const double* S, double* partial, float* X0,
+                           bool push, float P, int pull)
+{
+    if (pull == 1) {
+        if (partial) jacobi_at<true, Q, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        else jacobi_at<false, Q, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+    } else if (pull == 2) {
+        if (partial) jacobi_at<true, Q, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        else jacobi_at<false, Q, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+    } else if (push) {
+        if (partial) jacobi_at<true, Q, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
+        else jacobi_at<false, Q, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+    } else if (partial) {
+        jacobi_at<true, Q, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
+    } else if (X0) {
+        jacobi_at<false, Q, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0);
+    } else {
+        jacobi_at<false, Q, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, nullptr);
+    }
+    return partial ? l.n : 0;   // one partial per list entry
+}
+
 int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, const float* G, Coef C, const float* B, float w,
                   const double* S, hipStream_t st, double* partial, float* X0, bool push, float P, int pull)
 {
     if (X0 && partial && !push && !pull) throw std::invalid_argument("jacobi: the FAS-save form writes no partials");
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return 0;
-    const bool q = C.q[0] != nullptr;
+    const int form = coef_form(C, level);
     const bool one_tile = !(persistent_mask() & PERSIST_JACOBI);
-    if ((partial || X0 || q) && !one_tile)
-        throw std::invalid_argument("jacobi: b.x partials, the FAS save and FP16 coefficients need the one-tile form");
+    if ((partial || X0 || form) && !one_tile)
+        throw std::invalid_argument("jacobi: b.x partials, the FAS save and FP16 or coded coefficients need the one-tile form");
+    if (pull && (!X0 || push || pull < 1 || pull > 2))
+        throw std::invalid_argument("jacobi: the pulled prolongation is a one-tile post-sweep reading X0");
+    if (push && (!X0 || !jacobi_push_available(d, level)))
+        throw std::invalid_argument("jacobi: the prolongation cannot ride on this level's sweep");
     const TopoView tp = view(d);
-    const int nt = q ? q_jacobi_threads() : stencil_threads();
-    S = one_tile ? run_flag(d, S) : S;
-    if (pull) {
-        // a post-sweep of a level whose prolongation it pulls in (X0: the FAS-saved x)
-        if (!X0 || push || !one_tile || pull < 1 || pull > 2)
-            throw std::invalid_argument("jacobi: the pulled prolongation is a one-tile post-sweep reading X0");
-        if (pull == 1) {
-            if (partial) {
-                if (q) jacobi_at<true, true, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
-                else jacobi_at<true, false, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
-                return l.n;
-            }
-            if (q) jacobi_at<false, true, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
-            else jacobi_at<false, false, false, false, 1>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
-            return 0;
-        }
-        if (partial) {
-            if (q) jacobi_at<true, true, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
-            else jacobi_at<true, false, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
-            return l.n;
-        }
-        if (q) jacobi_at<false, true, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
-        else jacobi_at<false, false, false, false, 2>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
+    // pull: a post-sweep of a level whose prolongation it pulls in; push: the last post-sweep
+    // with the prolongation to level + 1 (X0: the FAS-saved x either way); X0 alone: the first
+    // coarse-level sweep with the level's FAS save
+    if (!one_tile) {
+        CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, tp, l.d, l.n, Xin, Xout, G, C, B, w, S);
         return 0;
     }
-    if (push) {
-        // the last post-sweep with the prolongation to level + 1 (X0: the FAS-saved x)
-        if (!X0 || !jacobi_push_available(d, level))
-            throw std::invalid_argument("jacobi: the prolongation cannot ride on this level's sweep");
-        if (partial) {
-            if (q) jacobi_at<true, true, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
-            else jacobi_at<true, false, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, P);
-            return l.n;
-        }
-        if (q) jacobi_at<false, true, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
-        else jacobi_at<false, false, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0, P);
-        return 0;
-    }
-    if (partial) {
-        // one partial per list entry
-        if (q) jacobi_at<true, true, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
-        else jacobi_at<true, false, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, partial, nullptr);
-        return l.n;
-    }
-    if (X0) {
-        // the first coarse-level sweep with the level's FAS save
-        if (q) jacobi_at<false, true, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0);
-        else jacobi_at<false, false, true>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, X0);
-        return 0;
-    }
-    if (q) jacobi_at<false, true, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, nullptr);
-    else if (one_tile) jacobi_at<false, false, false>(nt, l, st, tp, Xin, Xout, G, C, B, w, S, nullptr, nullptr);
-    else CMG_NT_PERSISTENT(k::jacobi_p, l.n, st, tp, l.d, l.n, Xin, Xout, G, C, B, w, S);
-    return 0;
+    S = run_flag(d, S);
+    const int nt = form ? q_jacobi_threads() : stencil_threads();
+    if (form == 1) return jacobi_one_tile<1>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
+    if (form == 2) return jacobi_one_tile<2>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
+    return jacobi_one_tile<0>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
 }
 
 void launch_jacobi_zero(const DevGrid& d, int level, float* Xout, const float* D, const float* B, float w, const double* S,
@@ -2598,10 +2817,10 @@ void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, flo
 {
     const LaunchList& l = d.subtree[level];
     if (!nonempty(l)) return;
-    if (C.q[0]) {
+    if (const int f = coef_form(C, level)) {
         if (persistent_mask() & PERSIST_RESTRICT)
-            throw std::invalid_argument("residual_restrict: FP16 coefficients need the one-tile form");
-        CMG_NT_DISPATCH_Q(k::residual_restrict, l.n, st, view(d), l.rec, X, C, B, Rr, run_flag(d, S));
+            throw std::invalid_argument("residual_restrict: FP16 or coded coefficients need the one-tile form");
+        CMG_NT_DISPATCH_ANY(k::residual_restrict, f, l.n, st, view(d), l.rec, X, C, B, Rr, run_flag(d, S));
         return;
     }
     if (!(persistent_mask() & PERSIST_RESTRICT))
@@ -2613,14 +2832,29 @@ void launch_fas_save(const DevGrid& d, int level, const float* X, Coef C, float*
 {
     const LaunchList& l = d.nonleaf[level];
     if (!nonempty(l)) return;
-    if (C.q[0]) CMG_NT_DISPATCH_Q(k::fas_save, l.n, st, view(d), l.rec, X, C, B, X0, run_flag(d, S));
-    else CMG_NT_DISPATCH(k::fas_save, l.n, st, view(d), l.rec, X, C, B, X0, run_flag(d, S));
+    CMG_NT_DISPATCH_ANY(k::fas_save, coef_form(C, level), l.n, st, view(d), l.rec, X, C, B, X0, run_flag(d, S));
 }
 
 void launch_coef_to_half(const DevGrid& d, Coef C, uint16_t* const q[4], unsigned int* bad, hipStream_t st)
 {
     if (!nonempty(d.all)) return;
     k::coef_to_half<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, q[0], q[1], q[2], q[3], bad);
+}
+
+size_t coef_code_words() { return size_t(2) * kMaxLevels * k::kCodeSlots + 3 * kMaxLevels; }
+size_t coef_code_result_offset() { return size_t(2) * kMaxLevels * k::kCodeSlots + 2 * kMaxLevels; }
+
+void launch_coef_codes(const DevGrid& d, Coef C, uint32_t* ws, float* lut, uint32_t* W, hipStream_t st)
+{
+    // empty key sets (all bits set), zero counts, flags and results
+    if (hipMemsetAsync(ws, 0xff, sizeof(uint32_t) * kMaxLevels * k::kCodeSlots, st) != hipSuccess ||
+        hipMemsetAsync(ws + 2 * kMaxLevels * k::kCodeSlots, 0, sizeof(uint32_t) * 3 * kMaxLevels, st) != hipSuccess)
+        throw std::runtime_error("coefficient codes: hipMemsetAsync failed");
+    if (nonempty(d.all))
+        k::code_collect<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, ws);
+    k::code_number<<<kMaxLevels, 256, 0, st>>>(ws, lut);
+    if (nonempty(d.all))
+        k::code_encode<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, ws, W);
 }
 
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)
@@ -2874,6 +3108,9 @@ bool fuse_first_sweep_allowed() { return !(persistent_mask() & 16); }
 bool fuse_fas_save_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 bool fuse_rz_allowed() { return !(persistent_mask() & (16 | PERSIST_JACOBI)); }
 bool pull_prolong_allowed() { return !(persistent_mask() & (16 | 32 | PERSIST_JACOBI)); }
+
+// the coded coefficients are read by the one-tile kernels only
+bool coef_codes_allowed() { return !(persistent_mask() & (64 | PERSIST_APPLY | PERSIST_JACOBI | PERSIST_RESTRICT)); }
 
 void launch_learned_accept(const DevGrid& d, int level, const float* Y, float* X, Coef C, const float* B, float w,
                            const double* S, hipStream_t st, unsigned int* count)

@@ -22,11 +22,19 @@ struct TopoView {
 
 // the four operator channels c0..c0+3; q (optional): FP16 copies of c[a] / h
 // (h = the tile's spacing) that the cycle kernels (damped Jacobi, residual and
-// restriction, FAS save) read in place of c when set
+// restriction, FAS save) read in place of c when set.
+// w, lut, coded (optional, exact): on a level whose bit is set in `coded`, every cell's
+// face coefficients c[0..2] are FP32 values of that level's table lut[level * 256 + code];
+// w holds per cell its type (byte 0) and the codes of c[0], c[1], c[2] (bytes 1..3).  The
+// kernels that read them compute the same values as from c (the table holds the FP32 bits).
 struct Coef {
     const float* c[4];
     const uint16_t* q[4];
+    const uint32_t* w;
+    const float* lut;
+    uint32_t coded;
 };
+constexpr int kCodes = 256;   // table entries per level
 
 inline TopoView view(const DevGrid& d)
 {
@@ -161,6 +169,15 @@ bool fuse_first_sweep_allowed();   // forms bit 4 clear
 bool fuse_rz_allowed();            // forms bit 4 clear and the one-tile Jacobi form
 bool fuse_fas_save_allowed();      // forms bit 4 clear and the one-tile Jacobi form
 bool pull_prolong_allowed();       // forms bits 4 and 5 clear and the one-tile Jacobi form
+bool coef_codes_allowed();         // forms bit 6 clear and the one-tile forms of apply, Jacobi and restrict
+
+// exact coefficient codes (Coef::w / lut / coded): ws is a workspace of coef_code_words()
+// uint32 words, lut kMaxLevels * kCodes floats, W one word per cell.  Enqueues the build;
+// once the stream has reached it, the kMaxLevels words at ws + coef_code_result_offset()
+// hold 1 for each level whose cells are coded.
+size_t coef_code_words();
+size_t coef_code_result_offset();
+void launch_coef_codes(const DevGrid& d, Coef C, uint32_t* ws, float* lut, uint32_t* W, hipStream_t st);
 
 // reference tile images (PoissonTile<float,15>) <-> this grid, device to device (cmg_tiles.hip):
 // tile t at ptrs[t] (device array of N addresses) or at base + t * stride; returns the number

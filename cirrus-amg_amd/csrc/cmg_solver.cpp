@@ -37,6 +37,12 @@ struct cmg_solver {
     unsigned int* lcount = nullptr;   // learned steps of the last solve: tiles judged, tiles kept
     uint16_t* q16 = nullptr;          // FP16 copies of the four coefficient channels / h (cycle_coef_fp16)
     unsigned int* qbad = nullptr;     // workgroups whose FP16 copy has a value that does not fit
+    // exact per-level codes of the face coefficients (Coef::w / lut): cell words, tables,
+    // build workspace, and the levels coded by the last build
+    uint32_t* cw = nullptr;
+    float* clut = nullptr;
+    uint32_t* cws = nullptr;
+    uint32_t coded = 0;
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -54,7 +60,7 @@ struct cmg_solver {
             return t == o.t && c[0] == o.c[0] && c[1] == o.c[1] && c[2] == o.c[2] && c[3] == o.c[3];
         }
     };
-    Stamp direct_stamp, q16_stamp, derive_stamp;
+    Stamp direct_stamp, q16_stamp, derive_stamp, code_stamp;
     bool derive_ok = false;
     // captured iteration
     hipGraphExec_t gexec = nullptr;
@@ -662,6 +668,46 @@ cmg_solver::Stamp stamp_now(const cmg_solver* s)
     return st;
 }
 
+// exact per-level codes of the face coefficients for the one-tile stencil kernels (Coef::w):
+// rebuilt whenever the coefficient channels or the cell types have been written since the
+// last build; a level whose face coefficients take more than kCodes distinct values keeps
+// the FP32 channels
+void prepare_coef_codes(cmg_solver* s)
+{
+    if (s->family != CMG_SOLVER_AMG || !coef_codes_allowed()) return;
+    DevGrid& d = s->g->dev;
+    const cmg_solver::Stamp stamp = stamp_now(s);
+    if (s->cw && s->code_stamp == stamp) return;
+    s->code_stamp = cmg_solver::Stamp{};
+    s->coded = 0;
+    if (!s->cw) {
+        s->cw = dmalloc<uint32_t>(size_t(d.N) * kCells);
+        s->clut = dmalloc<float>(size_t(kMaxLevels) * kCodes);
+        s->cws = dmalloc<uint32_t>(coef_code_words());
+    }
+    launch_coef_codes(d, coef(d, s->c0), s->cws, s->clut, s->cw, d.stream);
+    uint32_t ok[kMaxLevels];
+    HIPCHK(hipMemcpyAsync(ok, s->cws + coef_code_result_offset(), sizeof(ok), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    for (int l = 0; l <= d.max_level && l < kMaxLevels; ++l)
+        if (ok[l]) s->coded |= 1u << l;
+    s->code_stamp = stamp;
+}
+
+// the solver's operator for the stencil kernels: the FP32 channels, and the codes of the
+// levels that have them when they are current (built from the channels as they are now)
+Coef solver_coef(const cmg_solver* s)
+{
+    DevGrid& d = s->g->dev;
+    Coef c = coef(d, s->c0);
+    if (s->family == CMG_SOLVER_AMG && s->cw && s->coded && coef_codes_allowed() && s->code_stamp == stamp_now(s)) {
+        c.w = s->cw;
+        c.lut = s->clut;
+        c.coded = s->coded;
+    }
+    return c;
+}
+
 // dense FP64 inverse of the level-0 operator on its INTERIOR cells, probed
 // cell by cell with the same halo rules as the device stencil (level 0 has no
 // ghosts); Gauss-Jordan with partial pivoting.  For a pure-Neumann system the
@@ -1045,7 +1091,7 @@ void geo_full_apply(cmg_solver* s, float* X, float* Y, bool diag, const double* 
 int enqueue_apply(cmg_solver* s, float* X, float* Y, double* partial, const double* S)
 {
     DevGrid& d = s->g->dev;
-    if (s->family == CMG_SOLVER_AMG) return launch_apply(d, X, coef(d, s->c0), Y, partial, S, d.stream);
+    if (s->family == CMG_SOLVER_AMG) return launch_apply(d, X, solver_coef(s), Y, partial, S, d.stream);
     geo_full_apply(s, X, Y, false, S);
     return partial ? launch_dot(d, d.leafs, X, Y, partial, S, d.stream) : 0;
 }
@@ -1061,7 +1107,7 @@ void enqueue_cycle_ptr(cmg_solver* s, const cmg_solve_opts* o, float* Xp, float*
         geo_vcycle(s, o, Xp, RHSp, X0p, S);
         return;
     }
-    Coef C = coef(d, s->c0);
+    Coef C = solver_coef(s);
     if (o->cycle_coef_fp16) {
         if (!s->q16) throw std::logic_error("cycle_coef_fp16: coefficient copies not prepared");
         const size_t n = size_t(d.N) * kCells;
@@ -1187,14 +1233,14 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     launch_update_p(d, P, Z, s->S, st);
 }
 
-std::string graph_key(const cmg_solve_opts* o, int derive)
+std::string graph_key(const cmg_solve_opts* o, int derive, uint32_t coded)
 {
     std::ostringstream k;
     k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->learned << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann << ',' << o->cycle_coef_fp16;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
-    k << ',' << mask << ',' << threads << ',' << launched_threads_key() << ',' << derive;
+    k << ',' << mask << ',' << threads << ',' << launched_threads_key() << ',' << derive << ',' << coded;
     return k.str();
 }
 
@@ -1309,7 +1355,7 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     }
     // capture (or reuse) the iteration graph
     if (o->use_graph) {
-        const std::string key = graph_key(o, d.derive);
+        const std::string key = graph_key(o, d.derive, solver_coef(s).coded);
         if (!s->gexec || s->gkey != key) {
             if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;
             hipGraph_t graph;
@@ -1502,7 +1548,7 @@ void solver_destroy(cmg_solver* s)
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->net_w,
-                    (void*)s->lcount, (void*)s->q16, (void*)s->qbad,
+                    (void*)s->lcount, (void*)s->q16, (void*)s->qbad, (void*)s->cw, (void*)s->clut, (void*)s->cws,
                     (void*)s->x64, (void*)s->b64,
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
@@ -1538,6 +1584,7 @@ cmg_status solver_cycle(cmg_solver* s, const cmg_solve_opts* o, int x, int rhs, 
         ensure_buffers(s);
         if (s->family == CMG_SOLVER_AMG && o->coarsest_direct) ensure_direct(s, o->is_pure_neumann != 0);
         if (s->family == CMG_SOLVER_AMG && o->cycle_coef_fp16) prepare_coef_fp16(s);
+        if (s->family == CMG_SOLVER_AMG) prepare_coef_codes(s);
         enqueue_cycle(s, o, x, rhs, x0, nullptr);
         HIPCHK(hipStreamSynchronize(s->g->dev.stream));
     });
@@ -1615,6 +1662,7 @@ cmg_status solver_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_stats*
         if (o->learned) HIPCHK(hipMemset(s->lcount, 0, 2 * sizeof(unsigned int)));   // counts of this solve
         if (amg && o->coarsest_direct) ensure_direct(s, o->is_pure_neumann != 0);
         if (amg && o->cycle_coef_fp16) prepare_coef_fp16(s);
+        if (amg) prepare_coef_codes(s);
         DeriveScope derive(d, amg && derive_allowed() && diagonal_derivable(s));
         // keep b in FP64 for the true residual / refinement (not needed when an FP32 solve skips it)
         if (amg && !(o->precision == CMG_PREC_FP32 && o->no_true_residual))
@@ -1703,8 +1751,9 @@ cmg_status solver_mg_solve(cmg_solver* s, const cmg_solve_opts* o, cmg_solve_sta
         hipStream_t stream = d.stream;
         ensure_buffers(s);
         if (o->coarsest_direct) ensure_direct(s, o->is_pure_neumann != 0);
+        prepare_coef_codes(s);
         float *X = dev_channel(d, CH_X), *B = dev_channel(d, CH_R), *X0 = dev_channel(d, CH_P);
-        const Coef C = coef(d, s->c0);
+        const Coef C = solver_coef(s);
         set_scalars(s, 0.0, o->max_iters);
         double h[S_COUNT];
         HIPCHK(hipEventRecord(s->e0, stream));
@@ -2004,7 +2053,7 @@ cmg_status op_copy(cmg_grid* g, int src, int dst)
 
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles)
 {
-    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 11 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
+    if (!s || !avg_ms || reps <= 0 || kernel < 0 || kernel > 14 || s->family != CMG_SOLVER_AMG) return CMG_ERR_BAD_ARG;
     if (auto e = check_solver(s)) return e;
     if (kernel == 9 && !s->net_w) return CMG_ERR_STATE;
     CMG_TRY({
@@ -2021,6 +2070,16 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
             const size_t n = size_t(d.N) * kCells;
             for (int a = 0; a < 4; ++a) Cq.q[a] = s->q16 + size_t(a) * n;
         }
+        // kernels 12-14: the coded coefficients (the finest level must be coded), with LEAF
+        // diagonals derived from the faces when a solve would derive them
+        Coef Ck = C;
+        if (kernel >= 12) {
+            if (!coef_codes_allowed()) throw std::invalid_argument("bench_kernel: the coded coefficients are switched off");
+            prepare_coef_codes(s);
+            Ck = solver_coef(s);
+            if (!((Ck.coded >> L) & 1u)) throw std::invalid_argument("bench_kernel: the finest level is not coded");
+        }
+        DeriveScope derive(d, kernel >= 12 && derive_allowed() && diagonal_derivable(s));
         // live scalars for the kernels that read alpha / beta: not stopped, alpha = beta = 1/2
         double h[S_COUNT] = {};
         h[S_ALPHA] = 0.5;
@@ -2047,6 +2106,9 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
             case 9: launch_net_infer(d, d.subtree[L], R, C.c[3], s->net_w, s->tmp, S, st); nt = d.subtree[L].tiles; break;
             case 10: launch_copy_flat(d, fa, fb, nflat, st); nt = int(nflat / 512); break;
             case 11: launch_jacobi(d, L, Z, s->tmp, Z, Cq, R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
+            case 12: launch_jacobi(d, L, Z, s->tmp, Z, Ck, R, 2.0f / 3.0f, S, st); nt = d.subtree[L].tiles; break;
+            case 13: launch_apply(d, Pp, Ck, AP, s->partial, S, st); nt = d.leafs.tiles; break;
+            case 14: launch_residual_restrict(d, L, Z, Ck, AP, 1.0f, S, st); nt = d.subtree[L].tiles; break;
             }
         };
         one();
@@ -2061,6 +2123,15 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
         if (fa) (void)hipFree(fa);
         if (fb) (void)hipFree(fb);
     });
+    return CMG_OK;
+}
+
+// levels whose face coefficients the stencil kernels read as codes now (bit l: level l)
+cmg_status solver_coded_levels(cmg_solver* s, uint32_t* mask)
+{
+    if (!s || !mask) return CMG_ERR_BAD_ARG;
+    if (auto e = check_solver(s)) return e;
+    *mask = solver_coef(s).coded;
     return CMG_OK;
 }
 

@@ -345,6 +345,11 @@ cmg_status cmg_gmg_damped_jacobi_solve(cmg_solver* s, const cmg_solve_opts* opts
 cmg_status cmg_solver_apply(cmg_solver* s, int x, int y);
 /* the FP64 solution of the last FP64-refinement solve (N*512 doubles) */
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host);
+/* levels (bit l: level l) whose face coefficients the solver's stencil kernels read as
+ * exact one-byte codes (forms bit 6 of cmg_set_kernel_forms), as built by the last solve,
+ * cycle or multigrid solve; 0 before the first, after the coefficient channels or cell
+ * types were written since, or with bit 6 set.  No reference counterpart. */
+cmg_status cmg_solver_coded_levels(cmg_solver* s, uint32_t* mask);
 
 /* ------------------------------------------------------- learned smoother -- */
 /* No reference counterpart.  A 3-layer 3D conv net on one halo'd 10^3 tile
@@ -437,7 +442,9 @@ cmg_status cmg_op_cycle(cmg_solver* s, const cmg_solve_opts* opts, int x, int rh
  * otherwise), 10 = flat copy of 1 GiB into a second 1 GiB buffer (the on-box HBM
  * copy ceiling; allocates both buffers for the call; tiles = 2^28 / 512),
  * 11 = the damped-Jacobi sweep of kernel 1 reading the FP16 coefficient copies
- * of cycle_coef_fp16 (made by the call).
+ * of cycle_coef_fp16 (made by the call), 12 / 13 / 14 = kernels 1 / 0 / 2 reading
+ * the coded coefficients of forms bit 6 (made by the call; CMG_ERR_BAD_ARG when
+ * bit 6 is set or the finest level is not coded).
  * Overwrites the solver's work channels (0-4) and scalars.  No reference counterpart (the reference times whole solves
  * with CPUTimer, cirrus_cutcell/FluidEuler.cu:705-707). */
 cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
@@ -457,14 +464,23 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * prolongation to the next finer level in its last post-sweep); bit 5 set keeps
  * the prolongation into the finest level out of the finest post-sweeps (by
  * default a PCG solve's finest post-sweeps compute it on the fly; the cycle's
- * output on GHOST tiles of the finest level, which PCG never reads, then keeps 0).
+ * output on GHOST tiles of the finest level, which PCG never reads, then keeps 0);
+ * bit 6 set keeps the FP32 coefficient channels in every stencil kernel (by default
+ * an AMG solve, cycle or multigrid solve first codes, per level, the face
+ * coefficients of every cell as one-byte indices into a table of that level's
+ * distinct FP32 values, when there are at most 256 of them, and the one-tile
+ * operator, Jacobi, Gauss-Seidel, residual-restriction and FAS-save kernels read
+ * the cell's type and three codes as one 32-bit word: the values, and so every
+ * result, are those of the FP32 channels; the codes are rebuilt after the
+ * coefficient channels or the cell types are written; the persistent forms of
+ * bits 0-2 read the FP32 channels).
  * stencil_threads: 128, 256 or 512 threads per tile.  Stencil results are
  * bit-identical across forms; PCG scalars differ only by the grouping of FP64
  * partial sums.  Default: mask 0, 256 threads (the Jacobi sweep on FP16 coefficient
  * copies: 128 until a thread count is set here or by CMG_STENCIL_THREADS).  No reference counterpart (the reference
  * kernels have one fixed form, src/AMGSolver.cu:589-632, 838-964).  Returns
  * CMG_ERR_BAD_ARG for an invalid mask or thread count. */
-cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..63 */
+cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads);   /* mask 0..127 */
 cmg_status cmg_get_kernel_forms(int* persistent_mask, int* stencil_threads);
 
 /* hipDeviceSynchronize on the current device (timing brackets) */

@@ -55,7 +55,7 @@ def test_hash_full_is_reported():
 def test_kernel_forms_validation():
     saved = P.get_kernel_forms()
     try:
-        for bad in ((64, 256), (-1, 256), (7, 64), (0, 1024)):
+        for bad in ((128, 256), (-1, 256), (7, 64), (0, 1024)):
             with pytest.raises(P.CmgError) as e:
                 P.set_kernel_forms(*bad)
             assert e.value.status == P.lib().cmg_set_kernel_forms(*bad)
