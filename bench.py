@@ -58,17 +58,29 @@ def build(P, cfg):
     return g
 
 
-def canonical_bytes_per_iter(lv, ty, max_level, coef_bytes=16):
+def canonical_bytes_per_iter(lv, ty, max_level, coef_bytes=16, coded=0):
     """B_iter = 81 N_leaf + sum_l n_l (21+29+29.5+8.5+2*29) (tile-dense, FP32 vectors, four FP32
-    coefficients = 16 B).  coef_bytes = 8: the cycle reads FP16 coefficient copies, so each of the
-    five cycle passes that read coefficients (first sweep, second sweep, residual + restriction,
-    two post-sweeps) moves 8 B less per cell: sum_l n_l (13+21+21.5+8.5+2*21).  The CG part (81 B)
-    keeps the FP32 operator in both forms."""
+    coefficients = 16 B and the cell type 1 B per pass).  coef_bytes = 8: the cycle reads FP16
+    coefficient copies, so each of the five cycle passes that read coefficients (first sweep,
+    second sweep, residual + restriction, two post-sweeps) moves 8 B less per cell:
+    sum_l n_l (13+21+21.5+8.5+2*21).  The CG part (81 B) keeps the FP32 operator.
+    coded (bit mask of levels, FP32 form): on a coded level the four passes with a halo and the
+    operator read one 4-B word (type and three face codes) in place of 17 B, with LEAF diagonals
+    derived from the faces (NONLEAF tiles still read their 4-B diagonal): 13 B less per LEAF cell
+    and pass, 9 B less per NONLEAF cell and pass."""
     n_leaf = int((ty == 1).sum()) * 512
     per_level = 21 + 29 + 29.5 + 8.5 + 2 * 29 - 5 * (16 - coef_bytes)
     levels = sum(int(((lv == l) & ((ty == 1) | (ty == 2))).sum()) * 512 * per_level for l in range(1, max_level + 1))
     coarsest = 512 * 8 * 512 + 512 * 8   # dense FP64 inverse of level 0 + its vectors
-    return 81.0 * n_leaf + levels + coarsest
+    saved = 0.0
+    for l in range(max_level + 1):
+        if coded >> l & 1:
+            n_lf = int(((lv == l) & (ty == 1)).sum()) * 512
+            n_nl = int(((lv == l) & (ty == 2)).sum()) * 512
+            if l >= 1:
+                saved += 4 * (13.0 * n_lf + 9.0 * n_nl)
+            saved += 13.0 * n_lf
+    return 81.0 * n_leaf + levels + coarsest - saved
 
 
 def oracle_grid(cfg):
@@ -194,9 +206,11 @@ def main():
     ap.add_argument("--no-legs", action="store_true",
                     help="time the headline solve and its dominant kernel only (no secondary legs)")
     ap.add_argument("--learned", type=int, default=1, help="also time learned-FCG when trained weights exist")
-    ap.add_argument("--cycle-coef", choices=("fp16", "fp32"), default="fp16",
-                    help="coefficients the preconditioning cycle reads: FP16 copies of c / h, or the FP32 channels "
-                         "(the PCG operator is FP32 either way); the other form is timed as a secondary leg")
+    ap.add_argument("--cycle-coef", choices=("fp16", "fp32"), default="fp32",
+                    help="coefficients the preconditioning cycle reads: the FP32 channels (read as exact per-level "
+                         "codes where a level has at most 255 distinct face values: the same values), or FP16 "
+                         "copies of c / h (the PCG operator is FP32 either way); the other form is timed as a "
+                         "secondary leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs the single-GPU solve only: this build has no multi-GPU path "
@@ -251,24 +265,39 @@ def main():
     step(checked(opts))   # untimed: the FP64 true residual of the same solve
     true_rel = solver.last_stats.true_rel_res
     x32 = g.channel(0).astype(np.float64) if not args.no_fp64 else None
+    m_in = (types == 2) & (ty == 1)[:, None]   # LEAF INTERIOR cells
 
+    # levels whose coefficients the kernels read as exact codes (forms bit 6 clear)
+    coded = solver.coded_levels()
+    fine_coded = bool(coded >> g.max_level & 1)
     # dominant kernel: damped-Jacobi sweep on the finest level (3 per PCG iteration plus the
-    # first sweep from zero); FP32 form: x_in, b, 3 face coefficients, diagonal (4 B each),
-    # type (1 B), x_out = 29 B per cell; FP16-coefficient form: coefficients and diagonal 2 B = 21 B
-    kid = 11 if q16 else 1
+    # first sweep from zero); FP32 channels: x_in, b, 3 face coefficients, diagonal (4 B each),
+    # type (1 B), x_out = 29 B per cell; FP16 copies: coefficients and diagonal 2 B = 21 B; coded
+    # (LEAF diagonals derived): x_in, b, x_out and one 4-B word (type and three codes) = 16 B
+    kid = 11 if q16 else (12 if fine_coded else 1)
     k_ms, k_tiles = solver.bench_kernel(kid, 50)
     alg_bytes = KERNELS[kid][1] * 512 * k_tiles
     k_gbs = alg_bytes / (k_ms * 1e-3) / 1e9
-    traffic = load_traffic(("jacobi_fp16coef_finest_%s_pmc.txt" if q16 else "jacobi_finest_%s_pmc.txt") % args.config)
-    o_ms, o_tiles = solver.bench_kernel(1 if q16 else 11, 50)
-    o_bytes = KERNELS[1 if q16 else 11][1] * 512 * o_tiles
-    other_form = {"kernel": KERNELS[1 if q16 else 11][0], "ms": o_ms, "GBps": o_bytes / (o_ms * 1e-3) / 1e9,
-                  "bytes_per_launch": o_bytes}
+    traffic = load_traffic({11: "jacobi_fp16coef_finest_%s_pmc.txt", 12: "jacobi_coded_finest_%s_pmc.txt",
+                            1: "jacobi_finest_%s_pmc.txt"}[kid] % args.config)
+    forms = {}
+    for fk in (1, 11, 12):
+        if fk == 12 and not fine_coded:
+            continue
+        f_ms, f_tiles = (k_ms, k_tiles) if fk == kid else solver.bench_kernel(fk, 50)
+        f_bytes = KERNELS[fk][1] * 512 * f_tiles
+        forms[KERNELS[fk][0]] = {"ms": f_ms, "GBps": f_bytes / (f_ms * 1e-3) / 1e9, "bytes_per_launch": f_bytes}
     apply_ms, apply_tiles = solver.bench_kernel(0, 50)
-    apply_gbs = 25.0 * 512 * apply_tiles / (apply_ms * 1e-3) / 1e9
+    apply_kernel = {"fp32_channels": {"GBps": 25.0 * 512 * apply_tiles / (apply_ms * 1e-3) / 1e9, "ms": apply_ms,
+                                      "tiles": apply_tiles}}
+    if fine_coded:
+        a_ms, a_tiles = solver.bench_kernel(13, 50)
+        apply_kernel["coded"] = {"GBps": KERNELS[13][1] * 512 * a_tiles / (a_ms * 1e-3) / 1e9, "ms": a_ms,
+                                 "tiles": a_tiles}
 
-    b_iter = canonical_bytes_per_iter(lv, ty, g.max_level, 8 if q16 else 16)       # the headline form's own bytes
-    b_iter_other = canonical_bytes_per_iter(lv, ty, g.max_level, 16 if q16 else 8)
+    # the headline form's own bytes, and the other cycle form's
+    b_iter = canonical_bytes_per_iter(lv, ty, g.max_level, 8 if q16 else 16, 0 if q16 else coded)
+    b_iter_other = canonical_bytes_per_iter(lv, ty, g.max_level, 16 if q16 else 8, coded if q16 else 0)
     solve_ms = float(np.mean(dev_ms))
     it_mean = float(np.mean(iters))
     solve_gbs = b_iter * it_mean / (solve_ms * 1e-3) / 1e9
@@ -285,12 +314,14 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": ("f32 (f16 cycle coefficients, f64 dots, f64 level-0 direct solve)" if q16
-                  else "f32 (f64 dots, f64 level-0 direct solve)"),
+                  else "f32 (f64 dots, f64 level-0 direct solve; FP32 coefficients read as exact per-level codes)"),
         "data": "synthetic (seeded)",
         "config": {"workload": cfg["name"], "tiles": int(g.N), "leaf_tiles": int((ty == 1).sum()), "dof": dof,
                    "levels": int(g.max_level) + 1,
                    "solver": "PCG (FP32 operator) + FAS V(2,2) damped Jacobi w=2/3, direct level 0"
-                             + (", cycle on FP16 coefficient copies c/h" if q16 else ""),
+                             + (", cycle on FP16 coefficient copies c/h" if q16 else
+                                ", FP32 cycle; coefficients of levels %s read as exact 8-bit codes"
+                                % [l for l in range(g.max_level + 1) if coded >> l & 1]),
                    "parallelism": "single GPU"},
         "iterations": iters,
         "rel_res": rels,
@@ -301,12 +332,15 @@ def main():
         "solve_frac_of_hbm": solve_gbs / HBM_PEAK_GBS,
         "canonical_bytes_per_iter": b_iter,
         "canonical_bytes_note": ("cycle coefficients at 2 B (FP16 copies): sum_l n_l (13+21+21.5+8.5+2*21) + 81 N_leaf"
-                                 if q16 else "cycle coefficients at 4 B: sum_l n_l (21+29+29.5+8.5+2*29) + 81 N_leaf"),
+                                 if q16 else "coefficients at 4 B: sum_l n_l (21+29+29.5+8.5+2*29) + 81 N_leaf, less "
+                                 "13 B per LEAF cell and 9 B per NONLEAF cell for each halo pass and the operator on "
+                                 "coded levels (one 4-B word of type and codes)"),
+        "coded_levels": [l for l in range(g.max_level + 1) if coded >> l & 1],
         "roofline": {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k_gbs / HBM_PEAK_GBS,
                      "traffic": traffic, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
                      "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles},
-        "jacobi_other_form": other_form,
-        "apply_kernel": {"GBps": apply_gbs, "ms": apply_ms, "tiles": apply_tiles},
+        "jacobi_forms": forms,
+        "apply_kernel": apply_kernel,
         "build_s": t_build,
     }
     ceil = stream_copy_ceiling(solver)
@@ -445,6 +479,34 @@ def main():
                                     "mfma_util": tf / MFMA_BF16_PEAK_TFLOPS,
                                     "flop_per_tile": NET_FLOP_PER_TILE}
         result["learned"] = legs
+    if not q16 and coded:
+        # the headline solve with the codes switched off (forms bit 6): the same values from the
+        # FP32 channels; the solution differs only through the grouping of FP64 partial sums
+        # where the two forms launch different thread counts
+        mask0, _ = P.get_kernel_forms()
+        P.set_kernel_forms(mask0 | 64, 0)
+        try:
+            msn, itn, devn = [], [], []
+            for rep in range(args.warmup + args.steps):
+                it, rel = step()
+                if rep >= args.warmup:
+                    itn.append(it)
+                    devn.append(solver.last_stats.ms)
+            P.device_synchronize()
+            xn = g.channel(0).astype(np.float64)
+            t0 = time.perf_counter()
+            for rep in range(args.steps):
+                step()
+            P.device_synchronize()
+            msn = (time.perf_counter() - t0) * 1e3 / args.steps
+        finally:
+            P.set_kernel_forms(mask0, 0)
+        leg = {"ms": msn, "device_ms": float(np.mean(devn)), "iterations": itn, "rel_res": rel,
+               "ms_ratio_to_headline": msn / ms_per_step,
+               "canonical_bytes_per_iter": canonical_bytes_per_iter(lv, ty, g.max_level, 16, 0)}
+        if x32 is not None:
+            leg["rel_l2_diff_vs_headline"] = float(np.linalg.norm((xn - x32)[m_in]) / np.linalg.norm(x32[m_in]))
+        result["fp32_channels_without_codes"] = leg
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(cfg, 200)
     print(json.dumps(result))

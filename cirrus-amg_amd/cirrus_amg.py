@@ -467,7 +467,8 @@ def device_synchronize():
 
 def set_kernel_forms(persistent_mask, stencil_threads):
     """Stencil kernel forms (bit 0 operator, bit 1 Jacobi, bit 2 restrict persistent, bit 3 stored
-    diagonals, bit 4 unfused first sweep; threads per tile)."""
+    diagonals, bit 4 unfused boundary work, bit 5 no pulled prolongation, bit 6 no coefficient
+    codes; threads per tile, 0 for the library's defaults)."""
     _check(lib().cmg_set_kernel_forms(persistent_mask, stencil_threads), "cmg_set_kernel_forms")
 
 

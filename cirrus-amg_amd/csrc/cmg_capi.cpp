@@ -826,7 +826,7 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
 cmg_status cmg_set_kernel_forms(int persistent_mask, int stencil_threads)
 {
     if (!set_kernel_forms(persistent_mask, stencil_threads)) {
-        set_error("kernel forms: mask must be 0..127 and threads 128, 256 or 512");
+        set_error("kernel forms: mask must be 0..127 and threads 0, 128, 256 or 512");
         return CMG_ERR_BAD_ARG;
     }
     return CMG_OK;

@@ -37,16 +37,29 @@ namespace k {
 //   1: the FP16 copies (c / h); the coefficient planes are then staged as those FP16 bits
 //      (7.5 KB a tile instead of 10.9, so more tiles fit on a CU);
 //   2: the exact per-level codes (Coef::w, Coef::lut): a cell's type and three face codes
-//      in one 32-bit word, decoded through the level's table, which is staged in LDS; the
-//      staged planes and every value computed are those of form 0.
+//      in one 32-bit word; the planes are staged as the codes (1.7 KB a tile) and read
+//      through the level's table, which is staged in LDS (entry kCodes - 1 holds the
+//      tile spacing, the plane value of a face without neighbour tile); every value
+//      computed is that of form 0;
+//   3: the same codes, the planes staged as their FP32 values (the LDS layout of form 0
+//      plus the table, so that one kernel can take forms 0 and 3 tile by tile).
 template <typename T, int Q = 0>
 struct Lds {
     T x[1000];
     // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
-    std::conditional_t<Q == 1, uint16_t, float> off[3][576];
+    std::conditional_t<Q == 1, uint16_t, std::conditional_t<Q == 2, uint8_t, float>> off[3][576];
     uint8_t empty[6];    // face f = 2 axis + (side > 0) has no neighbour tile
-    float lut[Q == 2 ? kCodes : 1];   // form 2: the tile level's code table
+    float lut[Q >= 2 ? kCodes : 1];   // forms 2, 3: the tile level's code table
 };
+
+// plane a's staged coefficient at index i as FP32 (h: the tile spacing)
+template <typename T, int LQ>
+__device__ __forceinline__ float staged_coef(const Lds<T, LQ>& s, int a, int i, float h)
+{
+    if constexpr (LQ == 1) return float(__builtin_bit_cast(_Float16, s.off[a][i])) * h;
+    else if constexpr (LQ == 2) return s.lut[s.off[a][i]];
+    else return s.off[a][i];
+}
 
 // a staged coefficient as FP32: FP32 as is, FP16 bits of c / h times h (exact: h is a
 // power of two times h0)
@@ -61,9 +74,13 @@ __device__ __forceinline__ float derived_diag(const Lds<T, LQ>& s, int v, const 
 {
     static_assert(LQ != 1, "the FP16 form reads every diagonal");
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
-    const float cp0 = i < 7 ? s.off[0][v + 64] : (s.empty[1] ? -h : s.off[0][512 + (j * 8 + k)]);
-    const float cp1 = j < 7 ? s.off[1][v + 8] : (s.empty[3] ? -h : s.off[1][512 + (i * 8 + k)]);
-    const float cp2 = k < 7 ? s.off[2][v + 1] : (s.empty[5] ? -h : s.off[2][512 + (i * 8 + j)]);
+    // one plane read per face (a face without neighbour tile reads its staged h, unused)
+    const float t0 = staged_coef(s, 0, i < 7 ? v + 64 : 512 + (j * 8 + k), h);
+    const float t1 = staged_coef(s, 1, j < 7 ? v + 8 : 512 + (i * 8 + k), h);
+    const float t2 = staged_coef(s, 2, k < 7 ? v + 1 : 512 + (i * 8 + j), h);
+    const float cp0 = (i == 7 && s.empty[1]) ? -h : t0;
+    const float cp1 = (j == 7 && s.empty[3]) ? -h : t1;
+    const float cp2 = (k == 7 && s.empty[5]) ? -h : t2;
     float d = 0.0f;
     d = d - c[0];
     d = d - cp0;
@@ -259,8 +276,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
                                           const float* __restrict__ L = nullptr)
 {
     constexpr int PM = PullT::mode;
-    constexpr bool K = Q == 2;   // coded coefficients
-    static_assert((Q == 1) == (LQ == 1) && (!K || LQ == 2), "the LDS layout follows the coefficient form");
+    constexpr bool K = Q >= 2;   // coded coefficients (2: staged as codes, 3: as values)
+    static_assert((Q == 1) == (LQ == 1) && (K ? LQ == Q : LQ != 2), "the LDS layout follows the coefficient form");
     static_assert(PM == 0 || (!FRESH && sizeof(T) == 4), "a pulled prolongation is a cycle (FP32) form");
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
@@ -479,8 +496,17 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         s.x[SX(v >> 6, (v >> 3) & 7, v & 7)] = o.x[c];
     }
     if constexpr (K) {
-        // the table now; the own coefficient planes once it is visible
+        // the table now; form 2 stages the own codes too, form 3 the values once the table is visible
         if (lut_lane) vstore<LPT>(&s.lut[LPT * tid], lutv);
+        if constexpr (Q == 2) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                uint8_t oc[CPT];
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) oc[c] = uint8_t(ow[c] >> (8 * a + 8));
+                vstore<CPT>(&s.off[a][CPT * tid], oc);
+            }
+        }
     } else {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -501,8 +527,9 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     }
     __syncthreads();
     if constexpr (K) {
-        // own face coefficients from the codes (bytes 1..3 of the cell word); phase 2 below
-        // reads no coefficient plane, and the planes are read after the next barrier
+        // own face coefficients from the codes (bytes 1..3 of the cell word); form 3 stages
+        // them (phase 2 below reads no coefficient plane, and the planes are read after the
+        // next barrier)
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float oc[CPT];
@@ -511,7 +538,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
                 oc[c] = s.lut[(ow[c] >> (8 * a + 8)) & 0xffu];
                 o.c[c][a] = oc[c];
             }
-            vstore<CPT>(&s.off[a][CPT * tid], oc);
+            if constexpr (Q == 3) vstore<CPT>(&s.off[a][CPT * tid], oc);
         }
     }
 #pragma unroll
@@ -558,7 +585,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         if (sgn > 0) {
             // no neighbour tile: h (the halo value there is 0); FP16 1.0 times h in the Q form
             if constexpr (Q == 1) s.off[axis][512 + fi] = hkind[m] != 0 ? uint16_t(hcb[m]) : uint16_t(0x3C00);
-            else if constexpr (K) s.off[axis][512 + fi] = hkind[m] != 0 ? s.lut[(hcb[m] >> (8 * axis + 8)) & 0xffu] : h;
+            else if constexpr (Q == 2) s.off[axis][512 + fi] = hkind[m] != 0 ? uint8_t(hcb[m] >> (8 * axis + 8)) : uint8_t(kCodes - 1);
+            else if constexpr (Q == 3) s.off[axis][512 + fi] = hkind[m] != 0 ? s.lut[(hcb[m] >> (8 * axis + 8)) & 0xffu] : h;
             else s.off[axis][512 + fi] = hkind[m] != 0 ? __uint_as_float(hcb[m]) : h;
         }
     }
@@ -568,7 +596,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
                 const int v = own_cell<NT>(c, tid);
-                o.d[c] = (o.ct[c] & INTERIOR) ? derived_diag(s, v, o.c[c], h) : 0.0f;
+                const float dd = derived_diag(s, v, o.c[c], h);
+                o.d[c] = (o.ct[c] & INTERIOR) ? dd : 0.0f;
             }
         }
     }
@@ -581,9 +610,9 @@ __device__ __forceinline__ T neg_lap(const Lds<T, LQ>& s, int v, T x0, const flo
 {
     if (d == 0) return T(0);
     const int i = v >> 6, j = (v >> 3) & 7, k = v & 7;
-    const float cp0 = lds_coef(i < 7 ? s.off[0][v + 64] : s.off[0][512 + (j * 8 + k)], h);
-    const float cp1 = lds_coef(j < 7 ? s.off[1][v + 8] : s.off[1][512 + (i * 8 + k)], h);
-    const float cp2 = lds_coef(k < 7 ? s.off[2][v + 1] : s.off[2][512 + (i * 8 + j)], h);
+    const float cp0 = staged_coef(s, 0, i < 7 ? v + 64 : 512 + (j * 8 + k), h);
+    const float cp1 = staged_coef(s, 1, j < 7 ? v + 8 : 512 + (i * 8 + k), h);
+    const float cp2 = staged_coef(s, 2, k < 7 ? v + 1 : 512 + (i * 8 + j), h);
     T r = x0 * T(d);
     r = fma(T(c[0]), s.x[SX(i - 1, j, k)], r);
     r = fma(T(cp0), s.x[SX(i + 1, j, k)], r);
@@ -1058,8 +1087,9 @@ __global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* 
 
 // ------------------------------------------------- exact coefficient codes ----
 // Per level, the distinct FP32 bit patterns of the three face coefficients are gathered in
-// an open-addressing set of kCodeSlots keys (code_collect).  A level with at most kCodes of
-// them numbers them in slot order and writes its table of the values (code_number); every
+// an open-addressing set of kCodeSlots keys (code_collect).  A level with at most kCodes - 1
+// of them numbers them in slot order and writes its table of the values, entry kCodes - 1
+// being the level's spacing (code_number); every
 // cell of such a level then gets its word type | code(c0) << 8 | code(c1) << 16 |
 // code(c2) << 24 (code_encode).  A table entry holds the coefficient's own bits, so a kernel
 // that decodes computes exactly what it computes from the FP32 channels.
@@ -1118,7 +1148,7 @@ __global__ __launch_bounds__(256) void code_collect(TopoView tp, const int32_t* 
 
 // one workgroup per level: codes 0..n-1 to the occupied slots in slot order, the table
 // (unused entries 0), and the level's result
-__global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, float* __restrict__ lut)
+__global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, float* __restrict__ lut, float h0)
 {
     const int l = blockIdx.x;
     const uint32_t* keys = ws + l * kCodeSlots;
@@ -1126,8 +1156,9 @@ __global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, fl
     const unsigned int* count = ws + 2 * kMaxLevels * kCodeSlots;
     const unsigned int* bad = count + kMaxLevels;
     unsigned int* ok = ws + 2 * kMaxLevels * kCodeSlots + 2 * kMaxLevels;
-    const bool coded = !bad[l] && count[l] <= unsigned(kCodes);
-    for (int e = threadIdx.x; e < kCodes; e += blockDim.x) lut[l * kCodes + e] = 0.0f;
+    const bool coded = !bad[l] && count[l] <= unsigned(kCodes - 1);
+    // entry kCodes - 1: the level's spacing, as the tile kernels compute it (rec_h)
+    for (int e = threadIdx.x; e < kCodes; e += blockDim.x) lut[l * kCodes + e] = e == kCodes - 1 ? h0 / float(1 << l) : 0.0f;
     __syncthreads();
     if (!coded) {
         if (threadIdx.x == 0) ok[l] = 0u;
@@ -1264,7 +1295,7 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __rest
                                                  Coef C, float* __restrict__ Y, double* __restrict__ partial,
                                                  const double* __restrict__ S)
 {
-    __shared__ Lds<float, K ? 2 : 0> s;
+    __shared__ Lds<float, K ? 3 : 0> s;
     __shared__ double red[8];
     const TileRec r = rec_tile<true>(rec, S);
     const int t = r.t;   // -1: no tile, -2: the solve has stopped
@@ -1274,7 +1305,7 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __rest
     }
     Own<float, NT> o;
     if (K && ((C.coded >> r.level) & 1u))
-        load_tile<true, float, NT, K ? 2 : 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o, nullptr, NoExtraLoads{},
+        load_tile<true, float, NT, K ? 3 : 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o, nullptr, NoExtraLoads{},
                                              Pull<0>{nullptr, 0.0f}, C.w, C.lut);
     else
         load_tile<true, float, NT, 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o);
@@ -1459,7 +1490,8 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
             }
             if (act) const_cast<float*>(B)[idx] = b[c];   // B is the level's right-hand side, updated in place
         }
-        const float delta = (o.ct[c] & INTERIOR) ? (b[c] - ax) / o.d[c] : 0.0f;
+        const float dq = (b[c] - ax) / o.d[c];   // masked below (non-INTERIOR cells: d = 0)
+        const float delta = (o.ct[c] & INTERIOR) ? dq : 0.0f;
         xo[c] = o.x[c] + w * delta;
         if (dot_tile) acc += double(b[c]) * double(xo[c]);
     }
@@ -2551,10 +2583,11 @@ static int persistent_mask()
 
 bool set_kernel_forms(int mask, int threads)
 {
-    if (mask < 0 || mask > 127 || (threads != 128 && threads != 256 && threads != 512)) return false;
+    if (mask < 0 || mask > 127 || (threads != 0 && threads != 128 && threads != 256 && threads != 512)) return false;
     g_mask.store(mask, std::memory_order_relaxed);
+    // 0: the library's thread counts again (CMG_STENCIL_THREADS if set, else the per-kernel defaults)
+    g_threads_chosen.store(threads != 0, std::memory_order_relaxed);
     g_threads.store(threads, std::memory_order_relaxed);
-    g_threads_chosen.store(true, std::memory_order_relaxed);
     return true;
 }
 
@@ -2852,7 +2885,7 @@ void launch_coef_codes(const DevGrid& d, Coef C, uint32_t* ws, float* lut, uint3
         throw std::runtime_error("coefficient codes: hipMemsetAsync failed");
     if (nonempty(d.all))
         k::code_collect<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, ws);
-    k::code_number<<<kMaxLevels, 256, 0, st>>>(ws, lut);
+    k::code_number<<<kMaxLevels, 256, 0, st>>>(ws, lut, d.h0);
     if (nonempty(d.all))
         k::code_encode<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, ws, W);
 }

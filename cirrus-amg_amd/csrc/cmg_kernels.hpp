@@ -24,7 +24,8 @@ struct TopoView {
 // (h = the tile's spacing) that the cycle kernels (damped Jacobi, residual and
 // restriction, FAS save) read in place of c when set.
 // w, lut, coded (optional, exact): on a level whose bit is set in `coded`, every cell's
-// face coefficients c[0..2] are FP32 values of that level's table lut[level * 256 + code];
+// face coefficients c[0..2] are FP32 values of that level's table lut[level * 256 + code]
+// (codes 0..254; entry 255 holds the level's tile spacing);
 // w holds per cell its type (byte 0) and the codes of c[0], c[1], c[2] (bytes 1..3).  The
 // kernels that read them compute the same values as from c (the table holds the FP32 bits).
 struct Coef {

@@ -468,13 +468,14 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * bit 6 set keeps the FP32 coefficient channels in every stencil kernel (by default
  * an AMG solve, cycle or multigrid solve first codes, per level, the face
  * coefficients of every cell as one-byte indices into a table of that level's
- * distinct FP32 values, when there are at most 256 of them, and the one-tile
+ * distinct FP32 values, when there are at most 255 of them, and the one-tile
  * operator, Jacobi, Gauss-Seidel, residual-restriction and FAS-save kernels read
  * the cell's type and three codes as one 32-bit word: the values, and so every
  * result, are those of the FP32 channels; the codes are rebuilt after the
  * coefficient channels or the cell types are written; the persistent forms of
  * bits 0-2 read the FP32 channels).
- * stencil_threads: 128, 256 or 512 threads per tile.  Stencil results are
+ * stencil_threads: 128, 256 or 512 threads per tile, or 0 for the library's
+ * defaults (CMG_STENCIL_THREADS when set).  Stencil results are
  * bit-identical across forms; PCG scalars differ only by the grouping of FP64
  * partial sums.  Default: mask 0, 256 threads (the Jacobi sweep on FP16 coefficient
  * copies: 128 until a thread count is set here or by CMG_STENCIL_THREADS).  No reference counterpart (the reference

@@ -4,7 +4,7 @@ Per level, the stencil kernels may read a cell's face coefficients as one-byte i
 into a table of that level's distinct FP32 values (with the cell type in the same 32-bit
 word).  The table holds the values' own bits, so every result must equal, bit for bit,
 the one computed from the FP32 channels (bit 6 set); the levels coded must be exactly
-those whose face coefficients take at most 256 distinct bit patterns.
+those whose face coefficients take at most 255 distinct bit patterns.
 """
 import numpy as np
 import pytest
@@ -18,13 +18,13 @@ CODES_OFF = 64
 
 
 def _expected_levels(go):
-    """bit l set when level l's three face-coefficient channels hold at most 256 distinct bit patterns"""
+    """bit l set when level l's three face-coefficient channels hold at most 255 distinct bit patterns"""
     lv = go.tiles()[0]
     mask = 0
     for level in range(int(lv.max()) + 1):
         rows = lv == level
         keys = np.unique(np.concatenate([go.channel(H.C0 + a)[rows].view(np.uint32).ravel() for a in range(3)]))
-        if len(keys) <= 256:
+        if len(keys) <= 255:
             mask |= 1 << level
     return mask
 
