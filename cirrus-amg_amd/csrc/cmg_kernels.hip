@@ -1123,6 +1123,27 @@ __device__ __forceinline__ void code_insert(uint32_t* __restrict__ set, unsigned
     *bad = 1u;   // more distinct values than slots
 }
 
+// f(key) for this lane's key, with f called once per distinct key of the wave (by the first
+// lane holding it; the lanes of the wave must all be active): most waves hold one to three
+// distinct coefficient values
+template <typename F>
+__device__ __forceinline__ uint32_t per_distinct_key(uint32_t key, F f)
+{
+    uint32_t out = 0;
+    unsigned long long pending = __ballot(1);
+    while (pending) {
+        const int first = __ffsll(pending) - 1;
+        const uint32_t k0 = __builtin_amdgcn_readlane(key, first);
+        uint32_t r = 0;
+        if (int(__lane_id()) == first) r = f(k0);
+        r = __builtin_amdgcn_readlane(r, first);
+        const bool mine = key == k0;
+        if (mine) out = r;
+        pending &= ~__ballot(mine);
+    }
+    return out;
+}
+
 __global__ __launch_bounds__(256) void code_collect(TopoView tp, const int32_t* __restrict__ list, int n, Coef C,
                                                     uint32_t* __restrict__ ws)
 {
@@ -1135,13 +1156,17 @@ __global__ __launch_bounds__(256) void code_collect(TopoView tp, const int32_t* 
         // a level known to be uncodable needs no more work (a stale read only costs time)
         if (bad[l] || count[l] > unsigned(kCodes)) continue;
         const size_t q = quad_index(t_);
+        auto ins = [&](uint32_t k) -> uint32_t {
+            code_insert(keys + l * kCodeSlots, count + l, bad + l, k);
+            return 0u;
+        };
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const uint4 v = reinterpret_cast<const uint4*>(C.c[a])[q];
-            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.x);
-            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.y);
-            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.z);
-            code_insert(keys + l * kCodeSlots, count + l, bad + l, v.w);
+            per_distinct_key(v.x, ins);
+            per_distinct_key(v.y, ins);
+            per_distinct_key(v.z, ins);
+            per_distinct_key(v.w, ins);
         }
     }
 }
@@ -2591,11 +2616,12 @@ bool set_kernel_forms(int mask, int threads)
     return true;
 }
 
-// the Jacobi sweep on FP16 coefficient copies reads half the coefficient bytes of the
-// FP32 form, so a tile's latency chain carries fewer bytes; twice the tiles in flight
-// per CU pays there (MI355X, finest level of the 1024^3 torus workload: 72.6 us at 128
-// threads a tile against 77.8 at 256; 64 threads a tile, 90 VGPRs, ran 71.6 against 69.0).
-// It runs 128 unless a thread count was chosen.
+// the one-tile kernels on FP16 coefficient copies or codes stage fewer bytes a tile and
+// carry fewer bytes in a tile's latency chain; twice the tiles in flight per CU pays there
+// (MI355X, finest level of the 1024^3 torus workload: FP16 Jacobi sweep 72.6 us at 128
+// threads a tile against 77.8 at 256, 64 threads a tile, 90 VGPRs, 71.6 against 69.0;
+// coded residual + restriction 50.0 against 59.4).  They run 128 unless a thread count was
+// chosen.
 static int q_jacobi_threads()
 {
     const int v = stencil_threads();
@@ -2662,10 +2688,11 @@ static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((
         }                                                                                     \
     } while (0)
 
-// the same with coefficient form F (second template argument: 1 FP16 copies, 2 codes)
+// the same with coefficient form F (second template argument: 1 FP16 copies, 2 codes), at
+// the thread count of the compact forms
 #define CMG_NT_DISPATCH_F(KERNEL, F, GRID, STREAM, ...)                                       \
     do {                                                                                      \
-        switch (stencil_threads()) {                                                          \
+        switch (q_jacobi_threads()) {                                                         \
         case 128: KERNEL<128, F><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         case 512: KERNEL<512, F><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         default: KERNEL<256, F><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;          \
