@@ -338,7 +338,10 @@ def main():
         "coded_levels": [l for l in range(g.max_level + 1) if coded >> l & 1],
         "roofline": {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k_gbs / HBM_PEAK_GBS,
                      "traffic": traffic, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
-                     "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles},
+                     "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles,
+                     # the same values in the FP32 channels' bytes (29 B a cell): what the sweep delivers
+                     "fp32_channel_bytes_GBps": KERNELS[1][1] * 512 * k_tiles / (k_ms * 1e-3) / 1e9,
+                     "traffic_GBps": traffic / (k_ms * 1e-3) / 1e9 if traffic else None},
         "jacobi_forms": forms,
         "apply_kernel": apply_kernel,
         "build_s": t_build,
