@@ -2632,6 +2632,15 @@ static int q_jacobi_threads()
 // was chosen (106 us against 108 at 256 on the 1024^3 torus workload)
 static int apply_threads() { return g_threads_chosen.load(std::memory_order_relaxed) ? stencil_threads() : 128; }
 
+// a level launch of at most kSmallLaunch tiles is one round of workgroups at 512 threads a
+// tile, and its time is one tile's latency chain, which fewer cells a thread shorten: such
+// launches run 512 threads a tile unless a thread count was chosen
+constexpr int kSmallLaunch = 1024;
+static int level_threads(int def, int tiles)
+{
+    return (!g_threads_chosen.load(std::memory_order_relaxed) && tiles <= kSmallLaunch) ? 512 : def;
+}
+
 int launched_threads_key() { return apply_threads() * 1000 + q_jacobi_threads(); }
 
 void get_kernel_forms(int* mask, int* threads)
@@ -2689,22 +2698,22 @@ static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((
     } while (0)
 
 // the same with coefficient form F (second template argument: 1 FP16 copies, 2 codes), at
-// the thread count of the compact forms
+// the thread count of the compact forms (or of a small level launch)
 #define CMG_NT_DISPATCH_F(KERNEL, F, GRID, STREAM, ...)                                       \
     do {                                                                                      \
-        switch (q_jacobi_threads()) {                                                         \
+        switch (level_threads((F) ? q_jacobi_threads() : stencil_threads(), (GRID))) {        \
         case 128: KERNEL<128, F><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         case 512: KERNEL<512, F><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         default: KERNEL<256, F><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         }                                                                                     \
     } while (0)
 
-// every form of a one-tile stencil kernel KERNEL<NT, Q>
+// every form of a one-tile stencil kernel KERNEL<NT, Q> on one level
 #define CMG_NT_DISPATCH_ANY(KERNEL, FORM, GRID, STREAM, ...)                                  \
     do {                                                                                      \
         if ((FORM) == 1) CMG_NT_DISPATCH_F(KERNEL, 1, GRID, STREAM, __VA_ARGS__);            \
         else if ((FORM) == 2) CMG_NT_DISPATCH_F(KERNEL, 2, GRID, STREAM, __VA_ARGS__);       \
-        else CMG_NT_DISPATCH(KERNEL, GRID, STREAM, __VA_ARGS__);                             \
+        else CMG_NT_DISPATCH_F(KERNEL, 0, GRID, STREAM, __VA_ARGS__);                        \
     } while (0)
 
 // the coefficient form of a one-tile stencil launch on `level`: 1 the FP16 copies, 2 the
@@ -2859,7 +2868,7 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         return 0;
     }
     S = run_flag(d, S);
-    const int nt = form ? q_jacobi_threads() : stencil_threads();
+    const int nt = level_threads(form ? q_jacobi_threads() : stencil_threads(), l.n);
     if (form == 1) return jacobi_one_tile<1>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
     if (form == 2) return jacobi_one_tile<2>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
     return jacobi_one_tile<0>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
@@ -2884,7 +2893,7 @@ void launch_residual_restrict(const DevGrid& d, int level, float* X, Coef C, flo
         return;
     }
     if (!(persistent_mask() & PERSIST_RESTRICT))
-        CMG_NT_DISPATCH(k::residual_restrict, l.n, st, view(d), l.rec, X, C, B, Rr, run_flag(d, S));
+        CMG_NT_DISPATCH_F(k::residual_restrict, 0, l.n, st, view(d), l.rec, X, C, B, Rr, run_flag(d, S));
     else CMG_NT_PERSISTENT(k::restrict_p, l.n, st, view(d), l.d, l.n, X, C, B, Rr, S);
 }
 
