@@ -2745,7 +2745,7 @@ int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* par
         if (C.w && C.coded) {
             // tiles of coded levels decode their coefficients (the same values)
             double* const pp = partial;
-            switch (apply_threads()) {
+            switch (level_threads(apply_threads(), d.leafs.n)) {
             case 128:
                 if (pp) k::apply_leaf<true, 128, true><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, pp, Sr);
                 else k::apply_leaf<false, 128, true><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr);
@@ -2760,13 +2760,13 @@ int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* par
                 break;
             }
         } else if (partial) {
-            switch (apply_threads()) {
+            switch (level_threads(apply_threads(), d.leafs.n)) {
             case 128: k::apply_leaf<true, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
             case 512: k::apply_leaf<true, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
             default: k::apply_leaf<true, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.rec, X, C, Y, partial, Sr); break;
             }
         } else {
-            switch (apply_threads()) {
+            switch (level_threads(apply_threads(), d.leafs.n)) {
             case 128: k::apply_leaf<false, 128><<<d.leafs.n, 128, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr); break;
             case 512: k::apply_leaf<false, 512><<<d.leafs.n, 512, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr); break;
             default: k::apply_leaf<false, 256><<<d.leafs.n, 256, 0, st>>>(tp, d.leafs.rec, X, C, Y, nullptr, Sr); break;
