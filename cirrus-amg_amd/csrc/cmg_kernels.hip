@@ -2641,7 +2641,11 @@ static int level_threads(int def, int tiles)
     return (!g_threads_chosen.load(std::memory_order_relaxed) && tiles <= kSmallLaunch) ? 512 : def;
 }
 
-int launched_threads_key() { return apply_threads() * 1000 + q_jacobi_threads(); }
+int launched_threads_key()
+{
+    // the chosen flag decides the small-launch counts as well
+    return (g_threads_chosen.load(std::memory_order_relaxed) ? 1000000 : 0) + apply_threads() * 1000 + q_jacobi_threads();
+}
 
 void get_kernel_forms(int* mask, int* threads)
 {

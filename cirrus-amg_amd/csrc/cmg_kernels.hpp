@@ -159,10 +159,11 @@ int launch_assemble_operator(const DevGrid& d, int c0, float R, int face_rule, i
                              hipStream_t st);
 
 // stencil kernel forms (persistence mask bit 0 apply / 1 Jacobi / 2 restrict, bit 3 stored
-// diagonals, bit 4 unfused first sweep; threads per tile)
+// diagonals, bit 4 unfused boundary work, bit 5 no pulled prolongation, bit 6 no coefficient
+// codes; threads per tile, 0 for the defaults)
 bool set_kernel_forms(int mask, int threads);
-// the thread counts the operator and the FP16-copy Jacobi sweep launch with now (these follow
-// stencil_threads only once a count was chosen): part of a captured graph's identity
+// the thread counts the operator, the compact-form kernels and small level launches use now
+// (they follow stencil_threads only once a count was chosen): part of a captured graph's identity
 int launched_threads_key();
 void get_kernel_forms(int* mask, int* threads);
 bool derive_allowed();             // forms bit 3 clear
