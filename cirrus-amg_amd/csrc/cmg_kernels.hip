@@ -352,13 +352,19 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         if constexpr (PM == 1) {
             const int pt = r.parent, poc = (r.meta >> 8) & 7;
             const size_t pb = size_t(pt >= 0 ? pt : r.t) * 512;
+            // a thread's cells run along k in one row: their parents are CPT / 2 consecutive
+            // cells (aligned), read as one vector per array
+            constexpr int PP = CPT >= 2 ? CPT / 2 : 1;
+            const int v = own_cell<NT>(0, tid);
+            const size_t pc = pb + cidx(4 * (poc & 1) + ((v >> 6) >> 1), 4 * ((poc >> 1) & 1) + (((v >> 3) & 7) >> 1),
+                                        4 * ((poc >> 2) & 1) + ((v & 7) >> 1));
+            T ppx[PP], pp0[PP];
+            vload<PP>(G + pc, ppx);
+            vload<PP>(pull.X0 + pc, pp0);
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
-                const int v = own_cell<NT>(c, tid);
-                const size_t pc = pb + cidx(4 * (poc & 1) + ((v >> 6) >> 1), 4 * ((poc >> 1) & 1) + (((v >> 3) & 7) >> 1),
-                                            4 * ((poc >> 2) & 1) + ((v & 7) >> 1));
-                own_px[c] = G[pc];
-                own_p0[c] = pull.X0[pc];
+                own_px[c] = ppx[CPT >= 2 ? c >> 1 : 0];
+                own_p0[c] = pp0[CPT >= 2 ? c >> 1 : 0];
             }
 #pragma unroll
             for (int c = 0; c < CPT; ++c) own_xr[c] = xr[c];
