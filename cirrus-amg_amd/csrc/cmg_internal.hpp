@@ -19,6 +19,7 @@ enum Scalar : int {
     S_RR = 7, S_RHS2 = 8, S_THR2 = 9, S_MEAN = 10, S_TMP = 11,
     S_MAXIT = 12,
     S_STRIDE = 13,   // convergence tested when S_ITER % S_STRIDE == 0; 0 = never
+    S_XPEND = 14,    // 1: the last x/r update's x += alpha p waits for its p update
     S_COUNT = 16
 };
 

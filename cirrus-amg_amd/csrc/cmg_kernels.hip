@@ -913,30 +913,40 @@ __global__ __launch_bounds__(NT) void restrict_p(TopoView tp, const int32_t* __r
 // Z0: with the first damped-Jacobi sweep from x = 0 of level `zlevel` fused in,
 // z0 = 0 + w (r / D) on INTERIOR cells (0 elsewhere) of the level's tiles, from
 // the updated r (bit-identical to jacobi_zero_v on r)
+// X == nullptr: x += alpha p is deferred to the p update of the same iteration, which reads p
+// anyway (update_p_v with X; x_pending_v after the last iteration); S[S_XPEND] marks it pending
 template <bool Z0>
 __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
                                                    float* __restrict__ R, const float* __restrict__ P,
                                                    const float* __restrict__ AP, double* __restrict__ partial,
-                                                   const double* __restrict__ S, int zlevel, float* __restrict__ Zo,
+                                                   double* __restrict__ S, int zlevel, float* __restrict__ Zo,
                                                    const float* __restrict__ D, float w, const uint16_t* __restrict__ Dq)
 {
     __shared__ double red[8];
     if (stopped(S)) return;
     const float a = float(S[S_ALPHA]);
+    const bool with_x = X != nullptr;
+    if (!with_x && blockIdx.x == 0 && threadIdx.x == 0) S[S_XPEND] = 1.0;
     double acc = 0;
     CMG_TILE_LOOP(list, n)
     {
         const size_t q = quad_index(t_);
         const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
-        float4 x = reinterpret_cast<const float4*>(X)[q];
         float4 r = reinterpret_cast<const float4*>(R)[q];
-        const float4 p = reinterpret_cast<const float4*>(P)[q];
         const float4 ap = reinterpret_cast<const float4*>(AP)[q];
-        if (ct.x & INTERIOR) { x.x = fmaf(a, p.x, x.x); r.x = fmaf(-a, ap.x, r.x); }
-        if (ct.y & INTERIOR) { x.y = fmaf(a, p.y, x.y); r.y = fmaf(-a, ap.y, r.y); }
-        if (ct.z & INTERIOR) { x.z = fmaf(a, p.z, x.z); r.z = fmaf(-a, ap.z, r.z); }
-        if (ct.w & INTERIOR) { x.w = fmaf(a, p.w, x.w); r.w = fmaf(-a, ap.w, r.w); }
-        reinterpret_cast<float4*>(X)[q] = x;
+        if (with_x) {
+            float4 x = reinterpret_cast<const float4*>(X)[q];
+            const float4 p = reinterpret_cast<const float4*>(P)[q];
+            if (ct.x & INTERIOR) x.x = fmaf(a, p.x, x.x);
+            if (ct.y & INTERIOR) x.y = fmaf(a, p.y, x.y);
+            if (ct.z & INTERIOR) x.z = fmaf(a, p.z, x.z);
+            if (ct.w & INTERIOR) x.w = fmaf(a, p.w, x.w);
+            reinterpret_cast<float4*>(X)[q] = x;
+        }
+        if (ct.x & INTERIOR) r.x = fmaf(-a, ap.x, r.x);
+        if (ct.y & INTERIOR) r.y = fmaf(-a, ap.y, r.y);
+        if (ct.z & INTERIOR) r.z = fmaf(-a, ap.z, r.z);
+        if (ct.w & INTERIOR) r.w = fmaf(-a, ap.w, r.w);
         reinterpret_cast<float4*>(R)[q] = r;
         acc += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z) + double(r.w) * double(r.w);
         if (Z0 && tp.tlevel[t_] == zlevel) {
@@ -953,22 +963,54 @@ __global__ __launch_bounds__(256) void update_xr_v(TopoView tp, const int32_t* _
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
+// X: also the iteration's deferred x += alpha p (the p it reads is the iteration's), which
+// clears S[S_XPEND]
 __global__ __launch_bounds__(256) void update_p_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ P,
-                                                  const float* __restrict__ Z, const double* __restrict__ S)
+                                                  const float* __restrict__ Z, double* __restrict__ S, float* __restrict__ X)
 {
     if (stopped(S)) return;
     const float b = float(S[S_BETA]);
+    const float a = float(S[S_ALPHA]);
+    if (X && blockIdx.x == 0 && threadIdx.x == 0) S[S_XPEND] = 0.0;
     CMG_TILE_LOOP(list, n)
     {
         const size_t q = quad_index(t_);
         const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
         float4 p = reinterpret_cast<const float4*>(P)[q];
         const float4 z = reinterpret_cast<const float4*>(Z)[q];
+        if (X) {
+            float4 x = reinterpret_cast<const float4*>(X)[q];
+            if (ct.x & INTERIOR) x.x = fmaf(a, p.x, x.x);
+            if (ct.y & INTERIOR) x.y = fmaf(a, p.y, x.y);
+            if (ct.z & INTERIOR) x.z = fmaf(a, p.z, x.z);
+            if (ct.w & INTERIOR) x.w = fmaf(a, p.w, x.w);
+            reinterpret_cast<float4*>(X)[q] = x;
+        }
         if (ct.x & INTERIOR) p.x = fmaf(b, p.x, z.x);
         if (ct.y & INTERIOR) p.y = fmaf(b, p.y, z.y);
         if (ct.z & INTERIOR) p.z = fmaf(b, p.z, z.z);
         if (ct.w & INTERIOR) p.w = fmaf(b, p.w, z.w);
         reinterpret_cast<float4*>(P)[q] = p;
+    }
+}
+
+// the deferred x += alpha p of a solve's last iteration (its p update did not run), when pending
+__global__ __launch_bounds__(256) void x_pending_v(TopoView tp, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+                                                   const float* __restrict__ P, const double* __restrict__ S)
+{
+    if (S[S_XPEND] == 0.0) return;
+    const float a = float(S[S_ALPHA]);
+    CMG_TILE_LOOP(list, n)
+    {
+        const size_t q = quad_index(t_);
+        const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
+        float4 x = reinterpret_cast<const float4*>(X)[q];
+        const float4 p = reinterpret_cast<const float4*>(P)[q];
+        if (ct.x & INTERIOR) x.x = fmaf(a, p.x, x.x);
+        if (ct.y & INTERIOR) x.y = fmaf(a, p.y, x.y);
+        if (ct.z & INTERIOR) x.z = fmaf(a, p.z, x.z);
+        if (ct.w & INTERIOR) x.w = fmaf(a, p.w, x.w);
+        reinterpret_cast<float4*>(X)[q] = x;
     }
 }
 
@@ -1519,13 +1561,14 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
             } else {
                 b[c] = 0.0f;
             }
-            if (act) const_cast<float*>(B)[idx] = b[c];   // B is the level's right-hand side, updated in place
         }
         const float dq = (b[c] - ax) / o.d[c];   // masked below (non-INTERIOR cells: d = 0)
         const float delta = (o.ct[c] & INTERIOR) ? dq : 0.0f;
         xo[c] = o.x[c] + w * delta;
         if (dot_tile) acc += double(b[c]) * double(xo[c]);
     }
+    // B is the level's right-hand side, updated in place (one vector a thread)
+    if (save && act) vstore<CPT>(const_cast<float*>(B) + v0, b);
     if (act) vstore<CPT>(Xout + v0, xo);
     if constexpr (PUSH) {
         if constexpr (!EARLY) {
@@ -3003,7 +3046,7 @@ int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const flo
     return g;
 }
 
-int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
+int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, double* S,
                      hipStream_t st, float* Z0, const float* D, float w, const uint16_t* Dq)
 {
     if (!S) throw std::invalid_argument("update_xr needs the solver scalars (alpha)");
@@ -3018,11 +3061,18 @@ int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const
     return g;
 }
 
-void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st)
+void launch_update_p(const DevGrid& d, float* P, const float* Z, double* S, hipStream_t st, float* X)
 {
     if (!S) throw std::invalid_argument("update_p needs the solver scalars (beta)");
     if (!nonempty(d.leafs)) return;
-    k::update_p_v<<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, P, Z, S);
+    k::update_p_v<<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, P, Z, S, X);
+}
+
+void launch_x_pending(const DevGrid& d, float* X, const float* P, const double* S, hipStream_t st)
+{
+    if (!S) throw std::invalid_argument("x_pending needs the solver scalars (alpha)");
+    if (!nonempty(d.leafs)) return;
+    k::x_pending_v<<<vector_grid(d, d.leafs.n), 256, 0, st>>>(view(d), d.leafs.d, d.leafs.n, X, P, S);
 }
 
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st)

@@ -85,10 +85,13 @@ int launch_dot(const DevGrid& d, const LaunchList& l, const float* A, const floa
 // partial = sum A.B, partial2 = sum A.C per workgroup (one pass over A)
 int launch_dot2(const DevGrid& d, const LaunchList& l, const float* A, const float* B, const float* C, double* partial,
                 double* partial2, const double* S, hipStream_t st);
-int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, const double* S,
+// X == nullptr: x += alpha p deferred to the p update (launch_update_p with X), and after a solve's
+// last iteration to launch_x_pending (which runs only when it is pending)
+int launch_update_xr(const DevGrid& d, float* X, float* R, const float* P, const float* AP, double* partial, double* S,
                      hipStream_t st, float* Z0 = nullptr, const float* D = nullptr, float w = 0.0f,
                      const uint16_t* Dq = nullptr);
-void launch_update_p(const DevGrid& d, float* P, const float* Z, const double* S, hipStream_t st);
+void launch_update_p(const DevGrid& d, float* P, const float* Z, double* S, hipStream_t st, float* X = nullptr);
+void launch_x_pending(const DevGrid& d, float* X, const float* P, const double* S, hipStream_t st);
 void launch_sum_count(const DevGrid& d, const float* X, double* psum, double* pcnt, const double* S, hipStream_t st);
 void launch_sub_mean(const DevGrid& d, float* X, const double* S, hipStream_t st);
 void launch_dot64(const LaunchList& l, const double* A, const double* B, double* partial, hipStream_t st);
