@@ -58,7 +58,7 @@ def build(P, cfg):
     return g
 
 
-def canonical_bytes_per_iter(lv, ty, max_level, coef_bytes=16, coded=0):
+def canonical_bytes_per_iter(lv, ty, max_level, coef_bytes=16, coded=0, wide=0):
     """B_iter = 81 N_leaf + sum_l n_l (21+29+29.5+8.5+2*29) (tile-dense, FP32 vectors, four FP32
     coefficients = 16 B and the cell type 1 B per pass).  coef_bytes = 8: the cycle reads FP16
     coefficient copies, so each of the five cycle passes that read coefficients (first sweep,
@@ -67,7 +67,9 @@ def canonical_bytes_per_iter(lv, ty, max_level, coef_bytes=16, coded=0):
     coded (bit mask of levels, FP32 form): on a coded level the four passes with a halo and the
     operator read one 4-B word (type and three face codes) in place of 17 B, with LEAF diagonals
     derived from the faces (NONLEAF tiles still read their 4-B diagonal): 13 B less per LEAF cell
-    and pass, 9 B less per NONLEAF cell and pass."""
+    and pass, 9 B less per NONLEAF cell and pass.  wide (bit mask of the coded levels with
+    ten-bit codes): the same for the four cycle passes, while the operator reads the FP32
+    channels there."""
     n_leaf = int((ty == 1).sum()) * 512
     per_level = 21 + 29 + 29.5 + 8.5 + 2 * 29 - 5 * (16 - coef_bytes)
     levels = sum(int(((lv == l) & ((ty == 1) | (ty == 2))).sum()) * 512 * per_level for l in range(1, max_level + 1))
@@ -79,7 +81,8 @@ def canonical_bytes_per_iter(lv, ty, max_level, coef_bytes=16, coded=0):
             n_nl = int(((lv == l) & (ty == 2)).sum()) * 512
             if l >= 1:
                 saved += 4 * (13.0 * n_lf + 9.0 * n_nl)
-            saved += 13.0 * n_lf
+            if not wide >> l & 1:
+                saved += 13.0 * n_lf
     return 81.0 * n_leaf + levels + coarsest - saved
 
 
@@ -268,7 +271,7 @@ def main():
     m_in = (types == 2) & (ty == 1)[:, None]   # LEAF INTERIOR cells
 
     # levels whose coefficients the kernels read as exact codes (forms bit 6 clear)
-    coded = solver.coded_levels()
+    coded, wide = solver.code_levels()
     fine_coded = bool(coded >> g.max_level & 1)
     # dominant kernel: damped-Jacobi sweep on the finest level (3 per PCG iteration plus the
     # first sweep from zero); FP32 channels: x_in, b, 3 face coefficients, diagonal (4 B each),
@@ -296,8 +299,9 @@ def main():
                                  "tiles": a_tiles}
 
     # the headline form's own bytes, and the other cycle form's
-    b_iter = canonical_bytes_per_iter(lv, ty, g.max_level, 8 if q16 else 16, 0 if q16 else coded)
-    b_iter_other = canonical_bytes_per_iter(lv, ty, g.max_level, 16 if q16 else 8, coded if q16 else 0)
+    b_iter = canonical_bytes_per_iter(lv, ty, g.max_level, 8 if q16 else 16, 0 if q16 else coded, 0 if q16 else wide)
+    b_iter_other = canonical_bytes_per_iter(lv, ty, g.max_level, 16 if q16 else 8, coded if q16 else 0,
+                                            wide if q16 else 0)
     solve_ms = float(np.mean(dev_ms))
     it_mean = float(np.mean(iters))
     solve_gbs = b_iter * it_mean / (solve_ms * 1e-3) / 1e9
@@ -320,8 +324,10 @@ def main():
                    "levels": int(g.max_level) + 1,
                    "solver": "PCG (FP32 operator) + FAS V(2,2) damped Jacobi w=2/3, direct level 0"
                              + (", cycle on FP16 coefficient copies c/h" if q16 else
-                                ", FP32 cycle; coefficients of levels %s read as exact 8-bit codes"
-                                % [l for l in range(g.max_level + 1) if coded >> l & 1]),
+                                ", FP32 cycle; coefficients of levels %s read as exact 8-bit codes, of levels %s "
+                                "as exact 10-bit codes"
+                                % ([l for l in range(g.max_level + 1) if (coded & ~wide) >> l & 1],
+                                   [l for l in range(g.max_level + 1) if wide >> l & 1])),
                    "parallelism": "single GPU"},
         "iterations": iters,
         "rel_res": rels,
@@ -334,8 +340,10 @@ def main():
         "canonical_bytes_note": ("cycle coefficients at 2 B (FP16 copies): sum_l n_l (13+21+21.5+8.5+2*21) + 81 N_leaf"
                                  if q16 else "coefficients at 4 B: sum_l n_l (21+29+29.5+8.5+2*29) + 81 N_leaf, less "
                                  "13 B per LEAF cell and 9 B per NONLEAF cell for each halo pass and the operator on "
-                                 "coded levels (one 4-B word of type and codes)"),
+                                 "coded levels (one 4-B word of type and codes; the operator reads FP32 channels on "
+                                 "10-bit-coded levels)"),
         "coded_levels": [l for l in range(g.max_level + 1) if coded >> l & 1],
+        "ten_bit_coded_levels": [l for l in range(g.max_level + 1) if wide >> l & 1],
         "roofline": {"bound": "hbm", "achieved": k_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k_gbs / HBM_PEAK_GBS,
                      "traffic": traffic, "kernel": "jacobi sweep, finest level", "kernel_ms": k_ms,
                      "bytes_per_launch": alg_bytes, "tiles_per_launch": k_tiles,

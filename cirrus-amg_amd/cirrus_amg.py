@@ -121,7 +121,7 @@ def lib():
             "cmg_solver_create": (st, [vp, i, f, f, f, C.POINTER(vp)]), "cmg_solver_destroy": (None, [vp]),
             "cmg_solve": (st, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
             "cmg_solver_get_x64": (st, [vp, vp]),
-            "cmg_solver_coded_levels": (st, [vp, C.POINTER(C.c_uint32)]),
+            "cmg_solver_coded_levels": (st, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
             "cmg_op_full_neg_lap": (st, [vp, i, i, i]), "cmg_op_propagate_ghosts": (st, [vp, i]),
             "cmg_op_accumulate_leafs": (st, [vp, i]), "cmg_op_gs_color": (st, [vp, i, i, i, i, i, d]),
             "cmg_op_jacobi": (st, [vp, i, i, i, i, i, i, f]), "cmg_op_residual_restrict": (st, [vp, i, i, i, i, f]),
@@ -539,9 +539,13 @@ class AMGSolver:
 
     def coded_levels(self):
         """bit mask of the levels whose face coefficients the stencil kernels read as codes"""
-        m = C.c_uint32()
-        _check(lib().cmg_solver_coded_levels(self.h, C.byref(m)), "cmg_solver_coded_levels")
-        return m.value
+        return self.code_levels()[0]
+
+    def code_levels(self):
+        """(coded levels, the subset with ten-bit codes) as bit masks"""
+        m, w = C.c_uint32(), C.c_uint32()
+        _check(lib().cmg_solver_coded_levels(self.h, C.byref(m), C.byref(w)), "cmg_solver_coded_levels")
+        return m.value, w.value
 
     def apply(self, x, y):
         """y = A x with this solver's operator on LEAF tiles."""

@@ -729,7 +729,7 @@ cmg_status cmg_solve(cmg_solver* s, const cmg_solve_opts* opts, cmg_solve_stats*
 }
 
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host) { return solver_get_x64(s, host); }
-cmg_status cmg_solver_coded_levels(cmg_solver* s, uint32_t* mask) { return solver_coded_levels(s, mask); }
+cmg_status cmg_solver_coded_levels(cmg_solver* s, uint32_t* mask, uint32_t* wide) { return solver_coded_levels(s, mask, wide); }
 
 cmg_status cmg_op_full_neg_lap(cmg_grid* g, int x, int c0, int y)
 {

@@ -20,6 +20,7 @@ enum Scalar : int {
     S_MAXIT = 12,
     S_STRIDE = 13,   // convergence tested when S_ITER % S_STRIDE == 0; 0 = never
     S_XPEND = 14,    // 1: the last x/r update's x += alpha p waits for its p update
+    S_POLL = 15,     // stopping-test reductions (FIN_RR) run in this solve, skipped ones included
     S_COUNT = 16
 };
 
@@ -191,7 +192,7 @@ cmg_status trainer_weights(cmg_trainer* t, float* get, const float* set);
 cmg_status trainer_loss_grad(cmg_trainer* t, int first, int n, int bf16, double* loss, float* grad);
 cmg_status trainer_step(cmg_trainer* t, int batch, float lr, int bf16, double* loss);
 cmg_status solver_get_x64(cmg_solver* s, double* host);
-cmg_status solver_coded_levels(cmg_solver* s, uint32_t* mask);
+cmg_status solver_coded_levels(cmg_solver* s, uint32_t* mask, uint32_t* wide);
 cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int* tiles);
 cmg_status op_copy(cmg_grid* g, int src, int dst);
 }  // namespace cmg

@@ -42,14 +42,16 @@ namespace k {
 //      tile spacing, the plane value of a face without neighbour tile); every value
 //      computed is that of form 0;
 //   3: the same codes, the planes staged as their FP32 values (the LDS layout of form 0
-//      plus the table, so that one kernel can take forms 0 and 3 tile by tile).
+//      plus the table, so that one kernel can take forms 0 and 3 tile by tile);
+//   4: ten-bit codes (levels with 256 to 1023 distinct values), staged as 16-bit codes
+//      and read through the level's table of kWideCodes entries.
 template <typename T, int Q = 0>
 struct Lds {
     T x[1000];
     // [a][0..511] own negative faces, [a][512+f] neighbour face on the + side
-    std::conditional_t<Q == 1, uint16_t, std::conditional_t<Q == 2, uint8_t, float>> off[3][576];
+    std::conditional_t<Q == 1 || Q == 4, uint16_t, std::conditional_t<Q == 2, uint8_t, float>> off[3][576];
     uint8_t empty[6];    // face f = 2 axis + (side > 0) has no neighbour tile
-    float lut[Q >= 2 ? kCodes : 1];   // forms 2, 3: the tile level's code table
+    float lut[Q == 4 ? kWideCodes : Q >= 2 ? kCodes : 1];   // forms 2-4: the tile level's code table
 };
 
 // plane a's staged coefficient at index i as FP32 (h: the tile spacing)
@@ -57,8 +59,28 @@ template <typename T, int LQ>
 __device__ __forceinline__ float staged_coef(const Lds<T, LQ>& s, int a, int i, float h)
 {
     if constexpr (LQ == 1) return float(__builtin_bit_cast(_Float16, s.off[a][i])) * h;
-    else if constexpr (LQ == 2) return s.lut[s.off[a][i]];
+    else if constexpr (LQ == 2 || LQ == 4) return s.lut[s.off[a][i]];
     else return s.off[a][i];
+}
+
+// the entries of a coded level's table (the last one holds the tile spacing)
+template <int Q>
+constexpr int code_table() { return Q == 4 ? kWideCodes : kCodes; }
+
+// a cell word's code of the face coefficient on axis a: bits 8a+8.. (forms 2, 3) or 10a+2.. (form 4)
+template <int Q>
+__device__ __forceinline__ uint32_t word_code(uint32_t w, int a)
+{
+    if constexpr (Q == 4) return (w >> (10 * a + 2)) & 0x3ffu;
+    else return (w >> (8 * a + 8)) & 0xffu;
+}
+
+// a cell word's cell type: byte 0 (forms 2, 3), or 0..3 in bits 0-1 for 0, DIRICHLET, INTERIOR, NEUMANN (form 4)
+template <int Q>
+__device__ __forceinline__ uint8_t word_type(uint32_t w)
+{
+    if constexpr (Q == 4) return uint8_t((0x4210u >> (4 * (w & 3u))) & 0xfu);
+    else return uint8_t(w);
 }
 
 // a staged coefficient as FP32: FP32 as is, FP16 bits of c / h times h (exact: h is a
@@ -276,8 +298,8 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
                                           const float* __restrict__ L = nullptr)
 {
     constexpr int PM = PullT::mode;
-    constexpr bool K = Q >= 2;   // coded coefficients (2: staged as codes, 3: as values)
-    static_assert((Q == 1) == (LQ == 1) && (K ? LQ == Q : LQ != 2), "the LDS layout follows the coefficient form");
+    constexpr bool K = Q >= 2;   // coded coefficients (2, 4: staged as codes, 3: as values)
+    static_assert((Q == 1) == (LQ == 1) && (K ? LQ == Q : (LQ != 2 && LQ != 4)), "the LDS layout follows the coefficient form");
     static_assert(PM == 0 || (!FRESH && sizeof(T) == 4), "a pulled prolongation is a cycle (FP32) form");
     constexpr int CPT = 512 / NT;
     constexpr int HPT = (384 + NT - 1) / NT;
@@ -289,11 +311,12 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     const uint32_t ty03 = uint32_t(hb.z), ty45 = uint32_t(hb.w);
     const bool dv = Q != 1 && tp.derive && ((ty45 >> 24) & LEAF);   // the FP16 form reads every diagonal
     // K: the level's code table goes out first (its wait then covers no other load)
-    constexpr int LPT = NT >= kCodes ? 1 : kCodes / NT;
+    constexpr int TAB = code_table<Q>();
+    constexpr int LPT = NT >= TAB ? 1 : TAB / NT;
     float lutv[K ? LPT : 1];
-    const bool lut_lane = NT <= kCodes || tid < kCodes;
+    const bool lut_lane = NT <= TAB || tid < TAB;
     if constexpr (K) {
-        if (lut_lane) vload<LPT>(L + size_t(r.level) * kCodes + size_t(LPT) * tid, lutv);
+        if (lut_lane) vload<LPT>(L + size_t(r.level) * kWideCodes + size_t(LPT) * tid, lutv);
     }
     // the coefficient planes as values (a runtime-indexed C[axis] would be a load)
     const float* const cp0 = uniform_ptr(C[0]);
@@ -314,7 +337,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         if constexpr (K) {
             vload<CPT>(W + v0, ow);
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) o.ct[c] = uint8_t(ow[c]);
+            for (int c = 0; c < CPT; ++c) o.ct[c] = word_type<Q>(ow[c]);
         } else {
             vload<CPT>(tp.ctype + v0, o.ct);
         }
@@ -453,7 +476,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     }
     // a halo cell's type, read once its load is complete
     auto htype = [&](int m) -> uint32_t {
-        if constexpr (K) return hcb[m] & 0xffu;
+        if constexpr (K) return word_type<Q>(hcb[m]);
         else return hct[m];
     };
     extra();
@@ -504,12 +527,13 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     if constexpr (K) {
         // the table now; form 2 stages the own codes too, form 3 the values once the table is visible
         if (lut_lane) vstore<LPT>(&s.lut[LPT * tid], lutv);
-        if constexpr (Q == 2) {
+        if constexpr (Q == 2 || Q == 4) {
+            using CodeT = std::conditional_t<Q == 4, uint16_t, uint8_t>;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                uint8_t oc[CPT];
+                CodeT oc[CPT];
 #pragma unroll
-                for (int c = 0; c < CPT; ++c) oc[c] = uint8_t(ow[c] >> (8 * a + 8));
+                for (int c = 0; c < CPT; ++c) oc[c] = CodeT(word_code<Q>(ow[c], a));
                 vstore<CPT>(&s.off[a][CPT * tid], oc);
             }
         }
@@ -533,7 +557,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
     }
     __syncthreads();
     if constexpr (K) {
-        // own face coefficients from the codes (bytes 1..3 of the cell word); form 3 stages
+        // own face coefficients from the codes of the cell word; form 3 stages
         // them (phase 2 below reads no coefficient plane, and the planes are read after the
         // next barrier)
 #pragma unroll
@@ -541,7 +565,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
             float oc[CPT];
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
-                oc[c] = s.lut[(ow[c] >> (8 * a + 8)) & 0xffu];
+                oc[c] = s.lut[word_code<Q>(ow[c], a)];
                 o.c[c][a] = oc[c];
             }
             if constexpr (Q == 3) vstore<CPT>(&s.off[a][CPT * tid], oc);
@@ -591,8 +615,9 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         if (sgn > 0) {
             // no neighbour tile: h (the halo value there is 0); FP16 1.0 times h in the Q form
             if constexpr (Q == 1) s.off[axis][512 + fi] = hkind[m] != 0 ? uint16_t(hcb[m]) : uint16_t(0x3C00);
-            else if constexpr (Q == 2) s.off[axis][512 + fi] = hkind[m] != 0 ? uint8_t(hcb[m] >> (8 * axis + 8)) : uint8_t(kCodes - 1);
-            else if constexpr (Q == 3) s.off[axis][512 + fi] = hkind[m] != 0 ? s.lut[(hcb[m] >> (8 * axis + 8)) & 0xffu] : h;
+            else if constexpr (Q == 2) s.off[axis][512 + fi] = hkind[m] != 0 ? uint8_t(word_code<Q>(hcb[m], axis)) : uint8_t(kCodes - 1);
+            else if constexpr (Q == 4) s.off[axis][512 + fi] = hkind[m] != 0 ? uint16_t(word_code<Q>(hcb[m], axis)) : uint16_t(kWideCodes - 1);
+            else if constexpr (Q == 3) s.off[axis][512 + fi] = hkind[m] != 0 ? s.lut[word_code<Q>(hcb[m], axis)] : h;
             else s.off[axis][512 + fi] = hkind[m] != 0 ? __uint_as_float(hcb[m]) : h;
         }
     }
@@ -1135,18 +1160,21 @@ __global__ __launch_bounds__(256) void coef_to_half(TopoView tp, const int32_t* 
 
 // ------------------------------------------------- exact coefficient codes ----
 // Per level, the distinct FP32 bit patterns of the three face coefficients are gathered in
-// an open-addressing set of kCodeSlots keys (code_collect).  A level with at most kCodes - 1
-// of them numbers them in slot order and writes its table of the values, entry kCodes - 1
-// being the level's spacing (code_number); every
-// cell of such a level then gets its word type | code(c0) << 8 | code(c1) << 16 |
-// code(c2) << 24 (code_encode).  A table entry holds the coefficient's own bits, so a kernel
-// that decodes computes exactly what it computes from the FP32 channels.
+// an open-addressing set of kCodeSlots keys (code_collect).  A level with at most
+// kWideCodes - 1 of them numbers them in slot order and writes its table of the values,
+// the level's spacing in entry kCodes - 1 (at most kCodes - 1 values: one-byte codes) or
+// kWideCodes - 1 (ten-bit codes) (code_number); every cell of such a level then gets its
+// word, type | code(c0) << 8 | code(c1) << 16 | code(c2) << 24 or, with ten-bit codes,
+// type2 | code(c0) << 2 | code(c1) << 12 | code(c2) << 22 with type2 = 0..3 for the cell
+// types 0, DIRICHLET, INTERIOR, NEUMANN (a level with another type value is not coded)
+// (code_encode).  A table entry holds the coefficient's own bits, so a kernel that
+// decodes computes exactly what it computes from the FP32 channels.
 // Workspace (uint32): keys [kMaxLevels][kCodeSlots], slot codes [kMaxLevels][kCodeSlots],
 // then per level the key count, a "not codable" flag and the result (1: coded).
-constexpr int kCodeSlots = 1024;
+constexpr int kCodeSlots = 4096;   // a quarter full at most
 constexpr uint32_t kNoKey = 0xffffffffu;   // a NaN pattern; a coefficient with these bits leaves its level uncoded
 
-__device__ __forceinline__ uint32_t code_slot(uint32_t key) { return (key * 2654435761u) >> 22; }
+__device__ __forceinline__ uint32_t code_slot(uint32_t key) { return (key * 2654435761u) >> 20; }
 
 __device__ __forceinline__ void code_insert(uint32_t* __restrict__ set, unsigned int* count, unsigned int* bad,
                                             uint32_t key)
@@ -1202,7 +1230,7 @@ __global__ __launch_bounds__(256) void code_collect(TopoView tp, const int32_t* 
     {
         const int l = tp.tlevel[t_];
         // a level known to be uncodable needs no more work (a stale read only costs time)
-        if (bad[l] || count[l] > unsigned(kCodes)) continue;
+        if (bad[l] || count[l] > unsigned(kWideCodes)) continue;
         const size_t q = quad_index(t_);
         auto ins = [&](uint32_t k) -> uint32_t {
             code_insert(keys + l * kCodeSlots, count + l, bad + l, k);
@@ -1220,7 +1248,7 @@ __global__ __launch_bounds__(256) void code_collect(TopoView tp, const int32_t* 
 }
 
 // one workgroup per level: codes 0..n-1 to the occupied slots in slot order, the table
-// (unused entries 0), and the level's result
+// (kWideCodes entries, unused ones 0), and the level's result (1 one-byte, 2 ten-bit codes)
 __global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, float* __restrict__ lut, float h0)
 {
     const int l = blockIdx.x;
@@ -1229,9 +1257,12 @@ __global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, fl
     const unsigned int* count = ws + 2 * kMaxLevels * kCodeSlots;
     const unsigned int* bad = count + kMaxLevels;
     unsigned int* ok = ws + 2 * kMaxLevels * kCodeSlots + 2 * kMaxLevels;
-    const bool coded = !bad[l] && count[l] <= unsigned(kCodes - 1);
-    // entry kCodes - 1: the level's spacing, as the tile kernels compute it (rec_h)
-    for (int e = threadIdx.x; e < kCodes; e += blockDim.x) lut[l * kCodes + e] = e == kCodes - 1 ? h0 / float(1 << l) : 0.0f;
+    const bool coded = !bad[l] && count[l] <= unsigned(kWideCodes - 1);
+    const bool narrow = count[l] <= unsigned(kCodes - 1);
+    // the last entry of the code range: the level's spacing, as the tile kernels compute it (rec_h)
+    const int last = narrow ? kCodes - 1 : kWideCodes - 1;
+    for (int e = threadIdx.x; e < kWideCodes; e += blockDim.x)
+        lut[l * kWideCodes + e] = e == last ? h0 / float(1 << l) : 0.0f;
     __syncthreads();
     if (!coded) {
         if (threadIdx.x == 0) ok[l] = 0u;
@@ -1254,10 +1285,10 @@ __global__ __launch_bounds__(256) void code_number(uint32_t* __restrict__ ws, fl
     for (int e = 0; e < SPT; ++e) {
         if (k[e] == kNoKey) continue;
         slotcode[threadIdx.x * SPT + e] = first;
-        lut[l * kCodes + first] = __uint_as_float(k[e]);
+        lut[l * kWideCodes + first] = __uint_as_float(k[e]);
         ++first;
     }
-    if (threadIdx.x == 0) ok[l] = 1u;
+    if (threadIdx.x == 0) ok[l] = narrow ? 1u : 2u;
 }
 
 // the cell words of the coded levels; a key missing from its set (it cannot be, the
@@ -1271,11 +1302,21 @@ __global__ __launch_bounds__(256) void code_encode(TopoView tp, const int32_t* _
     CMG_TILE_LOOP(list, n)
     {
         const int l = tp.tlevel[t_];
-        if (!ok[l]) continue;
+        const unsigned int form = ok[l];
+        if (!form) continue;
+        const bool wide = form == 2u;
         const size_t q = quad_index(t_);
         const uchar4 ct = reinterpret_cast<const uchar4*>(tp.ctype)[q];
         uint32_t w[4] = {ct.x, ct.y, ct.z, ct.w};
         bool missing = false;
+        if (wide) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t t = w[e];
+                missing |= t != 0u && t != DIRICHLET && t != INTERIOR && t != NEUMANN;
+                w[e] = t == NEUMANN ? 3u : t;
+            }
+        }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const uint4 v = reinterpret_cast<const uint4*>(C.c[a])[q];
@@ -1289,7 +1330,7 @@ __global__ __launch_bounds__(256) void code_encode(TopoView tp, const int32_t* _
                     ++p;
                 }
                 if (p == kCodeSlots) missing = true;
-                else w[e] |= slotcode[l * kCodeSlots + sl] << (8 * a + 8);
+                else w[e] |= slotcode[l * kCodeSlots + sl] << (wide ? 10 * a + 2 : 8 * a + 8);
             }
         }
         if (missing) ok[l] = 0u;
@@ -1362,7 +1403,8 @@ __global__ __launch_bounds__(256) void fas_prolong_v(TopoView tp, const int4* __
 
 // ------------------------------------------------- one tile per workgroup ----
 // y = A x on LEAF tiles, one tile per workgroup (AMGFullNegativeLaplacianOnLeafs); optional x.y partial
-// K: tiles of the levels whose bit is set in C.coded read the coded coefficients (the same values)
+// K: tiles of the levels with one-byte codes (bit set in C.coded, clear in C.wide) read the
+// coded coefficients (the same values)
 template <bool DOT, int NT, bool K = false>
 __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __restrict__ rec, const float* __restrict__ X,
                                                  Coef C, float* __restrict__ Y, double* __restrict__ partial,
@@ -1377,7 +1419,7 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __rest
         return;
     }
     Own<float, NT> o;
-    if (K && ((C.coded >> r.level) & 1u))
+    if (K && (((C.coded & ~C.wide) >> r.level) & 1u))
         load_tile<true, float, NT, K ? 3 : 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o, nullptr, NoExtraLoads{},
                                              Pull<0>{nullptr, 0.0f}, C.w, C.lut);
     else
@@ -2330,28 +2372,11 @@ __device__ __forceinline__ void finish_terms(const double* __restrict__ p, const
     }
 }
 
-__global__ __launch_bounds__(1024) void finish(const double* __restrict__ partial, const double* __restrict__ partial2, int n,
-                                               double* __restrict__ S, int kind)
+// thread 0 of finish (1024 threads): the 16 wave sums, in wave order, into the scalars of `kind`
+__device__ __forceinline__ void finish_apply(double* __restrict__ S, int kind, const double* red, const double* red2)
 {
-    __shared__ double red[16];
-    __shared__ double red2[16];
-    // the stop flag is read first and tested only before the writes, so that its load
-    // overlaps the partials' (a stopped solve's partials are stale but in bounds)
-    const bool skip = kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0;
-    // each thread adds its strided partials in index order
-    double v = 0, v2 = 0;
-    if (partial2) finish_terms<16, true>(partial, partial2, n, v, v2);
-    else finish_terms<40, false>(partial, nullptr, n, v, v2);
-    for (int o = 32; o > 0; o >>= 1) {
-        v += __shfl_xor(v, o, 64);
-        v2 += __shfl_xor(v2, o, 64);
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { red[w] = v; red2[w] = v2; }
-    __syncthreads();
-    if (threadIdx.x != 0 || skip) return;
     double s = 0, s2 = 0;
-    for (int i = 0; i < int(blockDim.x >> 6); ++i) { s += red[i]; s2 += red2[i]; }
+    for (int i = 0; i < 16; ++i) { s += red[i]; s2 += red2[i]; }
     switch (kind) {
     case FIN_SUM: S[S_TMP] = s; break;
     case FIN_GAMMA0: S[S_GAMMA] = s; break;
@@ -2392,6 +2417,44 @@ __global__ __launch_bounds__(1024) void finish(const double* __restrict__ partia
         if (!isfinite(S[S_BETA])) S[S_DONE] = 3;
         break;
     }
+}
+
+// FIN_RR with sig (host-mapped coherent memory): one 64-bit word, the count of FIN_RR launches
+// of the solve (S_POLL) times 8 plus the stop code S_DONE (0-4), stored as a system-scope
+// atomic: the host sees the outcome of each stopping test without a copy or a stream
+// synchronisation (the word is all it reads there, so no release of other data is needed)
+__global__ __launch_bounds__(1024) void finish(const double* __restrict__ partial, const double* __restrict__ partial2, int n,
+                                               double* __restrict__ S, int kind, unsigned long long* __restrict__ sig)
+{
+    __shared__ double red[16];
+    __shared__ double red2[16];
+    // the stop flag is read first and tested only before the writes, so that its load
+    // overlaps the partials' (a stopped solve's partials are stale but in bounds)
+    const bool skip = kind != FIN_SUM && kind != FIN_MEAN && S[S_DONE] != 0.0;
+    // each thread adds its strided partials in index order
+    double v = 0, v2 = 0;
+    if (partial2) finish_terms<16, true>(partial, partial2, n, v, v2);
+    else finish_terms<40, false>(partial, nullptr, n, v, v2);
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, 64);
+        v2 += __shfl_xor(v2, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[w] = v; red2[w] = v2; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (kind == FIN_RR) {
+        // the stopping test (skipped once the solve has stopped) and its outcome
+        S[S_POLL] += 1;
+        if (!skip) finish_apply(S, kind, red, red2);
+        if (sig) {
+            const unsigned long long word = (static_cast<unsigned long long>(S[S_POLL]) << 3) |
+                                            static_cast<unsigned long long>(S[S_DONE]);
+            __hip_atomic_store(sig, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
+    if (!skip) finish_apply(S, kind, red, red2);
 }
 
 // level-0 FP64 dense solve: x = A0^-1 b on the level's INTERIOR cells.
@@ -2677,6 +2740,13 @@ static int q_jacobi_threads()
     return g_threads_chosen.load(std::memory_order_relaxed) ? v : 128;
 }
 
+// the thread count of a one-tile stencil launch of coefficient form F (before the small-launch
+// rule): the compact forms (FP16 copies, one-byte codes) run q_jacobi_threads; the FP32
+// channels and the ten-bit codes, whose table makes a tile's LDS 11.5 KB, stencil_threads
+// (MI355X, the 2,048 tiles of the torus workload's level 5 with ten-bit codes: the sweep
+// carrying the prolongation to level 6 18.8 us at 128 threads a tile, 17.2 with FP32 at 256)
+static int form_threads(int F) { return (F == 1 || F == 2) ? q_jacobi_threads() : stencil_threads(); }
+
 // the operator on LEAF tiles likewise runs 128 threads a tile unless a thread count
 // was chosen (106 us against 108 at 256 on the 1024^3 torus workload)
 static int apply_threads() { return g_threads_chosen.load(std::memory_order_relaxed) ? stencil_threads() : 128; }
@@ -2750,11 +2820,11 @@ static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((
         }                                                                                     \
     } while (0)
 
-// the same with coefficient form F (second template argument: 1 FP16 copies, 2 codes), at
-// the thread count of the compact forms (or of a small level launch)
+// the same with coefficient form F (second template argument: 1 FP16 copies, 2 one-byte
+// codes, 4 ten-bit codes), at the thread count of its form (or of a small level launch)
 #define CMG_NT_DISPATCH_F(KERNEL, F, GRID, STREAM, ...)                                       \
     do {                                                                                      \
-        switch (level_threads((F) ? q_jacobi_threads() : stencil_threads(), (GRID))) {        \
+        switch (level_threads(form_threads(F), (GRID))) {                                     \
         case 128: KERNEL<128, F><<<(GRID), 128, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         case 512: KERNEL<512, F><<<(GRID), 512, 0, (STREAM)>>>(__VA_ARGS__); break;          \
         default: KERNEL<256, F><<<(GRID), 256, 0, (STREAM)>>>(__VA_ARGS__); break;          \
@@ -2766,15 +2836,17 @@ static inline int vector_grid(const DevGrid&, int n) { return std::max(8, 8 * ((
     do {                                                                                      \
         if ((FORM) == 1) CMG_NT_DISPATCH_F(KERNEL, 1, GRID, STREAM, __VA_ARGS__);            \
         else if ((FORM) == 2) CMG_NT_DISPATCH_F(KERNEL, 2, GRID, STREAM, __VA_ARGS__);       \
+        else if ((FORM) == 4) CMG_NT_DISPATCH_F(KERNEL, 4, GRID, STREAM, __VA_ARGS__);       \
         else CMG_NT_DISPATCH_F(KERNEL, 0, GRID, STREAM, __VA_ARGS__);                        \
     } while (0)
 
 // the coefficient form of a one-tile stencil launch on `level`: 1 the FP16 copies, 2 the
-// codes (set for that level), 0 the FP32 channels
+// one-byte codes, 4 the ten-bit codes (set for that level), 0 the FP32 channels
 static int coef_form(const Coef& C, int level)
 {
     if (C.q[0]) return 1;
-    return (C.w && level >= 0 && level < 32 && ((C.coded >> level) & 1u)) ? 2 : 0;
+    if (!C.w || level < 0 || level >= 32 || !((C.coded >> level) & 1u)) return 0;
+    return ((C.wide >> level) & 1u) ? 4 : 2;
 }
 
 template <bool DOT, int NT>
@@ -2795,8 +2867,9 @@ int launch_apply(const DevGrid& d, const float* X, Coef C, float* Y, double* par
     if (!(persistent_mask() & PERSIST_APPLY)) {
         const TopoView tp = view(d);
         const double* Sr = run_flag(d, S);
-        if (C.w && C.coded) {
-            // tiles of coded levels decode their coefficients (the same values)
+        if (C.w && (C.coded & ~C.wide)) {
+            // tiles of levels with one-byte codes decode their coefficients (the same values);
+            // ten-bit-coded levels read the FP32 channels here
             double* const pp = partial;
             switch (level_threads(apply_threads(), d.leafs.n)) {
             case 128:
@@ -2921,9 +2994,10 @@ int launch_jacobi(const DevGrid& d, int level, const float* Xin, float* Xout, co
         return 0;
     }
     S = run_flag(d, S);
-    const int nt = level_threads(form ? q_jacobi_threads() : stencil_threads(), l.n);
+    const int nt = level_threads(form_threads(form), l.n);
     if (form == 1) return jacobi_one_tile<1>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
     if (form == 2) return jacobi_one_tile<2>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
+    if (form == 4) return jacobi_one_tile<4>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
     return jacobi_one_tile<0>(l, nt, st, tp, Xin, Xout, G, C, B, w, S, partial, X0, push, P, pull);
 }
 
@@ -3118,10 +3192,11 @@ void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t
     k::to_float<<<l.n, 512, 0, st>>>(l.d, A, B);
 }
 
-void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st)
+void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st,
+                   unsigned long long* sig)
 {
     if (!S || !partial) throw std::invalid_argument("finish needs the partials and the solver scalars");
-    k::finish<<<1, 1024, 0, st>>>(partial, partial2, n, S, kind);
+    k::finish<<<1, 1024, 0, st>>>(partial, partial2, n, S, kind, sig);
 }
 
 bool cycle_bottom_available(const DevGrid& d)

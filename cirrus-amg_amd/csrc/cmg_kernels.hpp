@@ -23,19 +23,25 @@ struct TopoView {
 // the four operator channels c0..c0+3; q (optional): FP16 copies of c[a] / h
 // (h = the tile's spacing) that the cycle kernels (damped Jacobi, residual and
 // restriction, FAS save) read in place of c when set.
-// w, lut, coded (optional, exact): on a level whose bit is set in `coded`, every cell's
-// face coefficients c[0..2] are FP32 values of that level's table lut[level * 256 + code]
-// (codes 0..254; entry 255 holds the level's tile spacing);
-// w holds per cell its type (byte 0) and the codes of c[0], c[1], c[2] (bytes 1..3).  The
-// kernels that read them compute the same values as from c (the table holds the FP32 bits).
+// w, lut, coded, wide (optional, exact): on a level whose bit is set in `coded`, every
+// cell's face coefficients c[0..2] are FP32 values of that level's table
+// lut[level * kWideCodes + code].  One-byte codes (bit clear in `wide`): codes 0..254,
+// entry 255 holds the level's tile spacing, and w holds per cell its type (byte 0) and
+// the codes of c[0], c[1], c[2] (bytes 1..3).  Ten-bit codes (bit set in `wide`, for
+// levels with 256 to 1023 distinct values): codes 0..1022, entry 1023 the spacing, and w
+// holds the type as 0..3 (none, DIRICHLET, INTERIOR, NEUMANN; bits 0-1) and the codes in
+// bits 2-11, 12-21 and 22-31.  The kernels that read them compute the same values as
+// from c (the table holds the FP32 bits).
 struct Coef {
     const float* c[4];
     const uint16_t* q[4];
     const uint32_t* w;
     const float* lut;
     uint32_t coded;
+    uint32_t wide;
 };
-constexpr int kCodes = 256;   // table entries per level
+constexpr int kCodes = 256;        // table entries of a one-byte-coded level
+constexpr int kWideCodes = 1024;   // table entries of a ten-bit-coded level; the table's stride per level
 
 inline TopoView view(const DevGrid& d)
 {
@@ -99,7 +105,10 @@ void launch_axpy(const DevGrid& d, const LaunchList& l, float* Y, const float* X
 void launch_update_x64(const DevGrid& d, double* X64, const float* E, hipStream_t st);
 void launch_to_double(const DevGrid& d, const LaunchList& l, const float* A, double* B, bool masked, hipStream_t st);
 void launch_to_float(const LaunchList& l, const double* A, float* B, hipStream_t st);
-void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st);
+// sig (FIN_RR only; host-mapped coherent memory, may be null): receives S_POLL * 8 + S_DONE,
+// the count of the solve's FIN_RR launches and the stop code, as one 64-bit word
+void launch_finish(const double* partial, const double* partial2, int n, double* S, int kind, hipStream_t st,
+                   unsigned long long* sig = nullptr);
 // non-FAS cycles and FASFCycle / FASMuCycleSolve
 void launch_restrict_source(const DevGrid& d, int level, float* X, float coeff, const double* S, hipStream_t st);
 // leaf_only: LEAF|NONLEAF tiles of the level (CMG), else GHOST tiles too
@@ -176,10 +185,11 @@ bool fuse_fas_save_allowed();      // forms bit 4 clear and the one-tile Jacobi 
 bool pull_prolong_allowed();       // forms bits 4 and 5 clear and the one-tile Jacobi form
 bool coef_codes_allowed();         // forms bit 6 clear and the one-tile forms of apply, Jacobi and restrict
 
-// exact coefficient codes (Coef::w / lut / coded): ws is a workspace of coef_code_words()
-// uint32 words, lut kMaxLevels * kCodes floats, W one word per cell.  Enqueues the build;
-// once the stream has reached it, the kMaxLevels words at ws + coef_code_result_offset()
-// hold 1 for each level whose cells are coded.
+// exact coefficient codes (Coef::w / lut / coded / wide): ws is a workspace of
+// coef_code_words() uint32 words, lut kMaxLevels * kWideCodes floats, W one word per cell.
+// Enqueues the build; once the stream has reached it, the kMaxLevels words at
+// ws + coef_code_result_offset() hold per level 1 (one-byte codes), 2 (ten-bit codes) or
+// 0 (not coded).
 size_t coef_code_words();
 size_t coef_code_result_offset();
 void launch_coef_codes(const DevGrid& d, Coef C, uint32_t* ws, float* lut, uint32_t* W, hipStream_t st);

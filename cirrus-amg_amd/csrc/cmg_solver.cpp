@@ -29,6 +29,10 @@ struct cmg_solver {
     float Rm = 0.5f, Rr = 1.0f, P = 1.0f;
     double* S = nullptr;
     double* hS = nullptr;          // pinned ring of scalar snapshots
+    // host-mapped coherent word the PCG iteration's stopping test publishes: tests run so far
+    // times 8 plus the stop code; sig is its device address
+    unsigned long long* hsig = nullptr;
+    unsigned long long* sig = nullptr;
     double* partial = nullptr;
     double* partial2 = nullptr;
     float* tmp = nullptr;          // Jacobi ping-pong buffer
@@ -42,7 +46,7 @@ struct cmg_solver {
     uint32_t* cw = nullptr;
     float* clut = nullptr;
     uint32_t* cws = nullptr;
-    uint32_t coded = 0;
+    uint32_t coded = 0, wide = 0;   // levels with codes; those with ten-bit codes
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -670,8 +674,9 @@ cmg_solver::Stamp stamp_now(const cmg_solver* s)
 
 // exact per-level codes of the face coefficients for the one-tile stencil kernels (Coef::w):
 // rebuilt whenever the coefficient channels or the cell types have been written since the
-// last build; a level whose face coefficients take more than kCodes distinct values keeps
-// the FP32 channels
+// last build; a level whose face coefficients take at most kCodes - 1 distinct values gets
+// one-byte codes, one with at most kWideCodes - 1 ten-bit codes, any other keeps the FP32
+// channels
 void prepare_coef_codes(cmg_solver* s)
 {
     if (s->family != CMG_SOLVER_AMG || !coef_codes_allowed()) return;
@@ -680,17 +685,20 @@ void prepare_coef_codes(cmg_solver* s)
     if (s->cw && s->code_stamp == stamp) return;
     s->code_stamp = cmg_solver::Stamp{};
     s->coded = 0;
+    s->wide = 0;
     if (!s->cw) {
         s->cw = dmalloc<uint32_t>(size_t(d.N) * kCells);
-        s->clut = dmalloc<float>(size_t(kMaxLevels) * kCodes);
+        s->clut = dmalloc<float>(size_t(kMaxLevels) * kWideCodes);
         s->cws = dmalloc<uint32_t>(coef_code_words());
     }
     launch_coef_codes(d, coef(d, s->c0), s->cws, s->clut, s->cw, d.stream);
     uint32_t ok[kMaxLevels];
     HIPCHK(hipMemcpyAsync(ok, s->cws + coef_code_result_offset(), sizeof(ok), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
-    for (int l = 0; l <= d.max_level && l < kMaxLevels; ++l)
+    for (int l = 0; l <= d.max_level && l < kMaxLevels; ++l) {
         if (ok[l]) s->coded |= 1u << l;
+        if (ok[l] == 2) s->wide |= 1u << l;
+    }
     s->code_stamp = stamp;
 }
 
@@ -704,6 +712,7 @@ Coef solver_coef(const cmg_solver* s)
         c.w = s->cw;
         c.lut = s->clut;
         c.coded = s->coded;
+        c.wide = s->wide;
     }
     return c;
 }
@@ -1214,7 +1223,7 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     const int n2 = fuse ? launch_update_xr(d, nullptr, R, P, AP, s->partial, s->S, st, s->tmp, coef(d, s->c0).c[3],
                                            float(o->omega), dq)
                         : launch_update_xr(d, nullptr, R, P, AP, s->partial, s->S, st);
-    launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st);
+    launch_finish(s->partial, nullptr, n2, s->S, FIN_RR, st, s->sig);
     if (o->is_pure_neumann) enqueue_recenter(s, R);
     const bool frz = rz_fusable(s, o);
     enqueue_precondition(s, o, s->S, fuse, frz ? s->partial : nullptr);
@@ -1235,14 +1244,15 @@ void enqueue_iteration(cmg_solver* s, const cmg_solve_opts* o)
     launch_update_p(d, P, Z, s->S, st, X);
 }
 
-std::string graph_key(const cmg_solve_opts* o, int derive, uint32_t coded)
+// code_forms: the coded levels (bits 0-15) and those with ten-bit codes (bits 16-31)
+std::string graph_key(const cmg_solve_opts* o, int derive, uint32_t code_forms)
 {
     std::ostringstream k;
     k << o->cycle << ',' << cycle_mu(o) << ',' << o->flexible << ',' << o->learned << ',' << o->smoother << ',' << o->omega << ',' << o->level_iters << ',' << o->coarsest_iters << ','
       << o->coarsest_direct << ',' << o->is_pure_neumann << ',' << o->cycle_coef_fp16;
     int mask = 0, threads = 0;
     get_kernel_forms(&mask, &threads);
-    k << ',' << mask << ',' << threads << ',' << launched_threads_key() << ',' << derive << ',' << coded;
+    k << ',' << mask << ',' << threads << ',' << launched_threads_key() << ',' << derive << ',' << code_forms;
     return k.str();
 }
 
@@ -1289,6 +1299,25 @@ void read_scalars(cmg_solver* s, double* h)
     HIPCHK(hipMemcpyAsync(s->hS, s->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     std::memcpy(h, s->hS, S_COUNT * sizeof(double));
+}
+
+// waits until the solve's stopping test has run `tests` times (the count the finish kernel
+// publishes in host-mapped memory) and returns its stop code, or -1 if the stream drained first
+static int wait_stop_test(cmg_solver* s, int tests)
+{
+    const unsigned long long want = static_cast<unsigned long long>(tests);
+    for (unsigned int spin = 1;; ++spin) {
+        unsigned long long w = __atomic_load_n(s->hsig, __ATOMIC_ACQUIRE);
+        if ((w >> 3) >= want) return int(w & 7u);
+        if ((spin & 255u) == 0) {
+            const hipError_t e = hipStreamQuery(s->g->dev.stream);
+            if (e == hipSuccess) {
+                w = __atomic_load_n(s->hsig, __ATOMIC_ACQUIRE);
+                return (w >> 3) >= want ? int(w & 7u) : -1;
+            }
+            if (e != hipErrorNotReady) HIPCHK(e);
+        }
+    }
 }
 
 // one FP32 PCG solve on channels 0-4; returns the device status code
@@ -1357,7 +1386,8 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
     }
     // capture (or reuse) the iteration graph
     if (o->use_graph) {
-        const std::string key = graph_key(o, d.derive, solver_coef(s).coded);
+        const Coef sc = solver_coef(s);
+        const std::string key = graph_key(o, d.derive, sc.coded | (sc.wide << 16));
         if (!s->gexec || s->gkey != key) {
             if (s->gexec) (void)hipGraphExecDestroy(s->gexec), s->gexec = nullptr;
             hipGraph_t graph;
@@ -1369,21 +1399,29 @@ int pcg32(cmg_solver* s, const cmg_solve_opts* o, double tol, int max_iters, boo
             s->gkey = key;
         }
     }
-    const int poll = stride > 0 ? stride : std::max(1, max_iters);
+    // Iteration k + 1 is launched as soon as iteration k's stopping test (the r.r reduction,
+    // early in the iteration; every stride-th iteration tests, and breakdowns stop at any) has
+    // published that the solve goes on: the device never waits for the host, and no iteration
+    // runs that the test has ruled out (AMGSolver.cu:1487-1497 reads the residual norm back
+    // every sync_stride iterations).
+    __atomic_store_n(s->hsig, 0ull, __ATOMIC_RELEASE);
     int launched = 0;
-    for (;;) {
-        const int k = std::min(poll, max_iters - launched);
-        for (int i = 0; i < k; ++i) {
-            if (o->use_graph) HIPCHK(hipGraphLaunch(s->gexec, st));
-            else enqueue_iteration(s, o);
+    while (launched < max_iters) {
+        if (o->use_graph) HIPCHK(hipGraphLaunch(s->gexec, st));
+        else enqueue_iteration(s, o);
+        ++launched;
+        int stop = wait_stop_test(s, launched);
+        if (stop < 0) {
+            // the stream drained without the publication: the scalars decide
+            read_scalars(s, h);
+            stop = int(h[S_DONE]);
         }
-        launched += k;
-        read_scalars(s, h);
-        if (h[S_DONE] != 0 || launched >= max_iters) break;
+        if (stop != 0) break;
     }
     // the last iteration's x += alpha p when its p update did not run (the solve stopped after
     // the x/r update)
     launch_x_pending(d, X, P, s->S, st);
+    read_scalars(s, h);
     *iters = int(h[S_ITER]);
     const double rhs2 = h[S_RHS2];
     if (!(rhs2 > 0) || !std::isfinite(rhs2)) return rhs2 == 0 ? 1 : 3;   // zero b: x = 0; non-finite b
@@ -1449,6 +1487,9 @@ cmg_status solver_create(cmg_grid* g, int c0, float Rm, float Rr, float P, cmg_s
         s->S = dmalloc<double>(S_COUNT);
         HIPCHK(hipMemset(s->S, 0, S_COUNT * sizeof(double)));
         HIPCHK(hipHostMalloc((void**)&s->hS, S_COUNT * sizeof(double), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&s->hsig, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent));
+        *s->hsig = 0;
+        HIPCHK(hipHostGetDevicePointer((void**)&s->sig, s->hsig, 0));
         // room for a fused r.z: one partial per finest-level list entry plus a vector grid
         const size_t np = size_t(g->dev.max_list) + size_t(g->dev.wg_cap) + 64;
         s->partial = dmalloc<double>(np);
@@ -1558,6 +1599,7 @@ void solver_destroy(cmg_solver* s)
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
     if (s->hS) (void)hipHostFree(s->hS);
+    if (s->hsig) (void)hipHostFree(s->hsig);
     if (s->e0) (void)hipEventDestroy(s->e0);
     if (s->e1) (void)hipEventDestroy(s->e1);
     delete s;
@@ -2132,11 +2174,13 @@ cmg_status bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms, int
 }
 
 // levels whose face coefficients the stencil kernels read as codes now (bit l: level l)
-cmg_status solver_coded_levels(cmg_solver* s, uint32_t* mask)
+cmg_status solver_coded_levels(cmg_solver* s, uint32_t* mask, uint32_t* wide)
 {
     if (!s || !mask) return CMG_ERR_BAD_ARG;
     if (auto e = check_solver(s)) return e;
-    *mask = solver_coef(s).coded;
+    const Coef c = solver_coef(s);
+    *mask = c.coded;
+    if (wide) *wide = c.wide;
     return CMG_OK;
 }
 

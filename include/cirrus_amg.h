@@ -346,10 +346,12 @@ cmg_status cmg_solver_apply(cmg_solver* s, int x, int y);
 /* the FP64 solution of the last FP64-refinement solve (N*512 doubles) */
 cmg_status cmg_solver_get_x64(cmg_solver* s, double* host);
 /* levels (bit l: level l) whose face coefficients the solver's stencil kernels read as
- * exact one-byte codes (forms bit 6 of cmg_set_kernel_forms), as built by the last solve,
- * cycle or multigrid solve; 0 before the first, after the coefficient channels or cell
- * types were written since, or with bit 6 set.  No reference counterpart. */
-cmg_status cmg_solver_coded_levels(cmg_solver* s, uint32_t* mask);
+ * exact codes (forms bit 6 of cmg_set_kernel_forms), as built by the last solve, cycle or
+ * multigrid solve; 0 before the first, after the coefficient channels or cell types were
+ * written since, or with bit 6 set.  wide (may be NULL): the subset with ten-bit codes
+ * (256 to 1023 distinct values; the operator reads the FP32 channels on those levels).
+ * No reference counterpart. */
+cmg_status cmg_solver_coded_levels(cmg_solver* s, uint32_t* mask, uint32_t* wide);
 
 /* ------------------------------------------------------- learned smoother -- */
 /* No reference counterpart.  A 3-layer 3D conv net on one halo'd 10^3 tile
@@ -467,10 +469,12 @@ cmg_status cmg_bench_kernel(cmg_solver* s, int kernel, int reps, double* avg_ms,
  * output on GHOST tiles of the finest level, which PCG never reads, then keeps 0);
  * bit 6 set keeps the FP32 coefficient channels in every stencil kernel (by default
  * an AMG solve, cycle or multigrid solve first codes, per level, the face
- * coefficients of every cell as one-byte indices into a table of that level's
- * distinct FP32 values, when there are at most 255 of them, and the one-tile
- * operator, Jacobi, Gauss-Seidel, residual-restriction and FAS-save kernels read
- * the cell's type and three codes as one 32-bit word: the values, and so every
+ * coefficients of every cell as indices into a table of that level's distinct
+ * FP32 values: one-byte codes when there are at most 255 of them, ten-bit codes
+ * when there are at most 1023 (and the cell types are 0, DIRICHLET, INTERIOR or
+ * NEUMANN); the one-tile Jacobi, Gauss-Seidel, residual-restriction and FAS-save
+ * kernels, and the one-tile operator on one-byte-coded levels, read the cell's
+ * type and three codes as one 32-bit word: the values, and so every
  * result, are those of the FP32 channels; the codes are rebuilt after the
  * coefficient channels or the cell types are written; the persistent forms of
  * bits 0-2 read the FP32 channels).
