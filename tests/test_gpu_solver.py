@@ -162,6 +162,27 @@ def test_warm_start_and_zero_rhs():
     del x0
 
 
+def test_nonfinite_rhs_stops_the_solve():
+    """A non-finite b stops the solve on the device (the |b|^2 reduction sets the stop flag)
+    and the solve reports it (AMGSolver.cu:1410-1425 tests |b| first); the iterations launched
+    on the published stopping test all see the flag.  The next solve with a finite b converges."""
+    go, gp = H.build_pair("sphere_empty24")
+    b = _rhs(go, gp, 13)
+    popts, _ = _opts("v_jacobi_direct")
+    solver = P.AMGSolver(gp, H.C0, H.R_MATRIX, 1.0, 1.0)
+    leaf = np.where(go.tiles()[2] == 1)[0]
+    interior = np.where(go.cell_types()[leaf[0]] == 2)[0]
+    bad = b.copy()
+    bad[leaf[0], interior[0]] = np.nan
+    gp.set_channel(1, bad)
+    with pytest.raises(P.CmgError) as e:
+        solver.solve(max_iters=50, relative_tolerance=1e-6, opts=popts)
+    assert e.value.status == 4, e.value
+    gp.set_channel(1, b)
+    it, rel = solver.solve(max_iters=50, relative_tolerance=1e-6, opts=popts)
+    assert 0 < it < 50 and rel < 1e-6
+
+
 def test_pure_neumann():
     """All-Neumann box (tests/SolverTests.cu:789-805): singular operator, recentred residual."""
     go, gp = H.build_pair("uniform2")
