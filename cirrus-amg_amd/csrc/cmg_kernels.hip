@@ -295,7 +295,7 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
                                           const T* __restrict__ G, const float* const* C, Lds<T, LQ>& s, Own<T, NT>& o,
                                           const uint16_t* const* Qc = nullptr, const Extra& extra = Extra{},
                                           const PullT& pull = PullT{nullptr, 0.0f}, const uint32_t* __restrict__ W = nullptr,
-                                          const float* __restrict__ L = nullptr)
+                                          const float* __restrict__ L = nullptr, const uint32_t* __restrict__ ZW = nullptr)
 {
     constexpr int PM = PullT::mode;
     constexpr bool K = Q >= 2;   // coded coefficients (2, 4: staged as codes, 3: as values)
@@ -430,8 +430,9 @@ __device__ __forceinline__ void load_tile(const TopoView& tp, const TileRec& r, 
         // a - face's coefficient is the cell's own: the + face reads the neighbour's, a - face
         // re-reads the tile's own first 64 (cached) so that the load is unconditional
         const size_t ci = sgn > 0 ? ni : base + fi;
-        // K: the neighbour cell's word, its type and its + face code in one load
-        if constexpr (K) hcb[m] = W[ni];
+        // K: the neighbour cell's word, its type and its + face code in one load (z faces from
+        // the tile's contiguous copy of those words when there is one: the same word)
+        if constexpr (K) hcb[m] = (ZW && axis == 2) ? ZW[base / 4 + (f - 4) * 64 + fi] : W[ni];
         else if constexpr (Q == 1) hcb[m] = (axis == 0 ? qp0 : axis == 1 ? qp1 : qp2)[ci];
         else hcb[m] = __float_as_uint((axis == 0 ? cp0 : axis == 1 ? cp1 : cp2)[ci]);
         const bool ghost = nty & GHOST;
@@ -1338,6 +1339,21 @@ __global__ __launch_bounds__(256) void code_encode(TopoView tp, const int32_t* _
     }
 }
 
+// Coef::zw: thread f * 64 + (8 i + j) of a tile's workgroup copies the word of the cell across
+// face f (0: -z, the z-neighbour's cell (i, j, 7); 1: +z, its cell (i, j, 0)), or the tile's own
+// cell there when it has no z-neighbour (a word the stencil kernels then do not use)
+__global__ __launch_bounds__(128) void code_zfaces(TopoView tp, const int32_t* __restrict__ list, int n,
+                                                   const uint32_t* __restrict__ W, uint32_t* __restrict__ ZW)
+{
+    if (int(blockIdx.x) >= n) return;
+    const int t = list[blockIdx.x];
+    if (t < 0) return;
+    const int f = threadIdx.x >> 6, fi = threadIdx.x & 63;
+    const int nt = tp.halo[size_t(t) * 8 + (f ? 5 : 2)];
+    const int src = nt >= 0 ? nt : t;
+    ZW[size_t(t) * 128 + threadIdx.x] = W[size_t(src) * 512 + cidx(fi >> 3, fi & 7, f ? 0 : 7)];
+}
+
 __global__ __launch_bounds__(256) void copy_v(const int32_t* __restrict__ list, int n, const float* __restrict__ A,
                                               float* __restrict__ B, const double* __restrict__ S)
 {
@@ -1421,7 +1437,7 @@ __global__ __launch_bounds__(NT) void apply_leaf(TopoView tp, const int4* __rest
     Own<float, NT> o;
     if (K && (((C.coded & ~C.wide) >> r.level) & 1u))
         load_tile<true, float, NT, K ? 3 : 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o, nullptr, NoExtraLoads{},
-                                             Pull<0>{nullptr, 0.0f}, C.w, C.lut);
+                                             Pull<0>{nullptr, 0.0f}, C.w, C.lut, C.zw);
     else
         load_tile<true, float, NT, 0>(tp, r, rec_h(tp, r), X, X, C.c, s, o);
     constexpr int CPT = 512 / NT;
@@ -1587,7 +1603,7 @@ __device__ __forceinline__ double jacobi_tile(const TopoView& tp, const TileRec&
         }
     };
     static_assert(!(PM && (SAVE || PUSH)), "a pulled prolongation is a post-sweep of the finest level");
-    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads, Pull<PM>{X0, P}, C.w, C.lut);
+    load_tile<false, float, NT, Q>(tp, r, h, Xin, G, C.c, s, o, C.q, push_loads, Pull<PM>{X0, P}, C.w, C.lut, C.zw);
     const bool save = SAVE && (r.meta & NONLEAF);
     const bool dot_tile = DOT && (r.meta & LEAF);   // b.x is r.z on LEAF tiles only
     double acc = 0;
@@ -1686,7 +1702,7 @@ __device__ __forceinline__ void residual_restrict_tile(const TopoView& tp, const
     float b[CPT], rv[CPT];
     vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * tid, b);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut, C.zw);
 #pragma unroll
     for (int c = 0; c < CPT; ++c) rv[c] = b[c] - neg_lap(s, own_cell<NT>(c, tid), o.x[c], o.c[c], o.d[c], h);
     vstore<CPT>(&res[CPT * tid], rv);
@@ -1771,7 +1787,7 @@ __device__ __forceinline__ void gs_color_tile(const TopoView& tp, const TileRec&
     float bv[CPT];
     vload<CPT>(B + size_t(t) * 512 + size_t(CPT) * tid, bv);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut, C.zw);
     if (!act) return;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
@@ -1816,7 +1832,7 @@ __device__ __forceinline__ void coarsest_tile(const TopoView& tp, const TileRec&
     float bv[CPT], xn[CPT];
     vload<CPT>(B + v0, bv);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut, C.zw);
     const int passes = rbgs ? 2 * (iters + iters2) : iters;
     if (passes == 0) return;
     for (int p = 0; p < passes; ++p) {
@@ -1870,7 +1886,7 @@ __device__ __forceinline__ void fas_save_tile(const TopoView& tp, const TileRec&
     float bv[CPT];
     vload<CPT>(B + v0, bv);
     const float h = rec_h(tp, r);
-    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut);
+    load_tile<false, float, NT, Q>(tp, r, h, X, X, C.c, s, o, C.q, NoExtraLoads{}, Pull<0>{nullptr, 0.0f}, C.w, C.lut, C.zw);
     if (!act) return;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
@@ -3051,6 +3067,12 @@ void launch_coef_codes(const DevGrid& d, Coef C, uint32_t* ws, float* lut, uint3
     k::code_number<<<kMaxLevels, 256, 0, st>>>(ws, lut, d.h0);
     if (nonempty(d.all))
         k::code_encode<<<vector_grid(d, d.all.n), 256, 0, st>>>(view(d), d.all.d, d.all.n, C, ws, W);
+}
+
+void launch_code_zfaces(const DevGrid& d, const uint32_t* W, uint32_t* ZW, hipStream_t st)
+{
+    if (!nonempty(d.all)) return;
+    k::code_zfaces<<<d.all.n, 128, 0, st>>>(view(d), d.all.d, d.all.n, W, ZW);
 }
 
 void launch_fas_prolong(const DevGrid& d, int level, float* X, const float* X0, float P, const double* S, hipStream_t st)

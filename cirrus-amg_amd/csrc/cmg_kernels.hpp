@@ -31,7 +31,10 @@ struct TopoView {
 // levels with 256 to 1023 distinct values): codes 0..1022, entry 1023 the spacing, and w
 // holds the type as 0..3 (none, DIRICHLET, INTERIOR, NEUMANN; bits 0-1) and the codes in
 // bits 2-11, 12-21 and 22-31.  The kernels that read them compute the same values as
-// from c (the table holds the FP32 bits).
+// from c (the table holds the FP32 bits).  zw (optional, with w): per tile 128 words, the
+// words of the cells across its -z face (the z-neighbour's k = 7 plane) then across its +z
+// face (k = 0), 64 each in (i, j) order, so that a tile reads them contiguously instead of
+// one 4-byte word per 32-byte sector (the tile's own cells where it has no z-neighbour).
 struct Coef {
     const float* c[4];
     const uint16_t* q[4];
@@ -39,6 +42,7 @@ struct Coef {
     const float* lut;
     uint32_t coded;
     uint32_t wide;
+    const uint32_t* zw;
 };
 constexpr int kCodes = 256;        // table entries of a one-byte-coded level
 constexpr int kWideCodes = 1024;   // table entries of a ten-bit-coded level; the table's stride per level
@@ -191,6 +195,8 @@ bool coef_codes_allowed();         // forms bit 6 clear and the one-tile forms o
 // ws + coef_code_result_offset() hold per level 1 (one-byte codes), 2 (ten-bit codes) or
 // 0 (not coded).
 size_t coef_code_words();
+// Coef::zw from the cell words W (N * 128 words), for every tile
+void launch_code_zfaces(const DevGrid& d, const uint32_t* W, uint32_t* ZW, hipStream_t st);
 size_t coef_code_result_offset();
 void launch_coef_codes(const DevGrid& d, Coef C, uint32_t* ws, float* lut, uint32_t* W, hipStream_t st);
 

@@ -47,6 +47,7 @@ struct cmg_solver {
     float* clut = nullptr;
     uint32_t* cws = nullptr;
     uint32_t coded = 0, wide = 0;   // levels with codes; those with ten-bit codes
+    uint32_t* czw = nullptr;        // per tile, the words across its z faces (Coef::zw)
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // level-0 dense inverse
     double* inv0 = nullptr;
@@ -690,8 +691,10 @@ void prepare_coef_codes(cmg_solver* s)
         s->cw = dmalloc<uint32_t>(size_t(d.N) * kCells);
         s->clut = dmalloc<float>(size_t(kMaxLevels) * kWideCodes);
         s->cws = dmalloc<uint32_t>(coef_code_words());
+        s->czw = dmalloc<uint32_t>(size_t(d.N) * 128);
     }
     launch_coef_codes(d, coef(d, s->c0), s->cws, s->clut, s->cw, d.stream);
+    launch_code_zfaces(d, s->cw, s->czw, d.stream);
     uint32_t ok[kMaxLevels];
     HIPCHK(hipMemcpyAsync(ok, s->cws + coef_code_result_offset(), sizeof(ok), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
@@ -713,6 +716,7 @@ Coef solver_coef(const cmg_solver* s)
         c.lut = s->clut;
         c.coded = s->coded;
         c.wide = s->wide;
+        c.zw = s->czw;
     }
     return c;
 }
@@ -1594,7 +1598,7 @@ void solver_destroy(cmg_solver* s)
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     for (void* p : {(void*)s->S, (void*)s->partial, (void*)s->partial2, (void*)s->tmp, (void*)s->scratch, (void*)s->net_w,
-                    (void*)s->lcount, (void*)s->q16, (void*)s->qbad, (void*)s->cw, (void*)s->clut, (void*)s->cws,
+                    (void*)s->lcount, (void*)s->q16, (void*)s->qbad, (void*)s->cw, (void*)s->clut, (void*)s->cws, (void*)s->czw,
                     (void*)s->x64, (void*)s->b64,
                     (void*)s->r64, (void*)s->inv0, (void*)s->cells0, (void*)s->zero0})
         if (p) (void)hipFree(p);
